@@ -1,0 +1,75 @@
+"""Pins the CPU oracle against the reference's only golden data (Docs/Example).
+
+Bit-exact: EuclideanDistanceSquares, euclideanDistanceSquareCounts.txt, max EDT square,
+ContainingBallRadiiSquares, ShortestPathDistances, CavityIds, cavityDescriptions.txt.
+Partition (up to relabelling, fed the fixture's own maximum-ball mask): PoreThroatIds and
+PoreThroatIdsWatershed.  The fixture was made with 7 OpenMP threads and MSVC's STL, so
+coordinates, the maximum-ball mask and ID numbering are only invariants there.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from tests import example_fixture as ex
+from tests.volume_utils import canonical_relabel, sha256_array
+
+
+@pytest.fixture(scope="module")
+def run():
+    solid = ex.solid()
+    e, max_e, hist = oracle.edt(solid, threads=4)
+    cr, cx, cy, cz, mask, nmax = oracle.containing_balls(e, threads=4)
+    return dict(solid=solid, e=e, max_e=max_e, hist=hist, cr=cr, cx=cx, cy=cy, cz=cz, mask=mask, nmax=nmax)
+
+
+def test_edt_bit_exact(run):
+    assert sha256_array(run["e"]) == ex.digests()["edt2"]
+
+
+def test_edt_statistics(run):
+    assert run["max_e"] == 2561
+    assert ex.text("intermediateStatistics.txt").startswith("Max Euclidean distance square: 2561\n")
+    assert oracle.histogram_text(run["hist"]) == ex.text("euclideanDistanceSquareCounts.txt")
+
+
+def test_containing_radii_bit_exact(run):
+    assert sha256_array(run["cr"]) == ex.digests()["cr2"]
+
+
+def test_containing_centres_are_valid(run):
+    """The fixture's coordinates are thread-race dependent; check the single-thread rule's
+    invariants instead: the centre covers the voxel and its EDT equals the containing radius."""
+    e, cr = run["e"], run["cr"]
+    Z, X, Y = e.shape
+    void = e > 0
+    cx, cy, cz = run["cx"][void].astype(np.int64), run["cy"][void].astype(np.int64), run["cz"][void].astype(np.int64)
+    zz, xx, yy = np.nonzero(void)
+    d2 = (cx - xx) ** 2 + (cy - yy) ** 2 + (cz - zz) ** 2
+    assert np.all(d2 <= cr[void])
+    assert np.all(e[cz, cx, cy] == cr[void])
+    assert np.all(run["cx"][~void] == 0xFFFFFFFF)
+    # 1-thread maximum-ball count (the fixture's 377746 came from a 7-thread race)
+    assert run["nmax"] == 380507
+    assert int((run["mask"] == 255).sum()) == run["nmax"]
+
+
+def test_pore_partition_matches_fixture(run):
+    pt, ptxt, ktxt = oracle.pores(ex.mask(), run["e"], run["cr"])
+    pt, ptxt, ktxt = oracle.remove_empty_pores(pt, ptxt, ktxt)
+    assert sha256_array(canonical_relabel(pt)) == ex.digests()["pt_partition"]
+    assert len(ptxt.splitlines()) == len(ex.text("pores.txt").splitlines())
+    assert len(ktxt.splitlines()) == len(ex.text("sharedPixelTypes.txt").splitlines())
+    ptw, wtxt, wktxt = oracle.watershed(run["cr"], run["max_e"], ptxt)
+    assert sha256_array(canonical_relabel(ptw)) == ex.digests()["ptw_partition"]
+    assert len(wtxt.splitlines()) == len(ex.text("poresWatershed.txt").splitlines())
+    # pore seeds: same multiset of (seed radius, boundary flag) rows as the fixture
+    ours = sorted(tuple(l.split()[4:6]) for l in ptxt.splitlines()[1:])
+    theirs = sorted(tuple(l.split()[4:6]) for l in ex.text("pores.txt").splitlines()[1:])
+    assert ours == theirs
+
+
+def test_geodesic_bit_exact():
+    dist, cav, text = oracle.geodesic(ex.solid())
+    assert sha256_array(dist) == ex.digests()["spd"]
+    assert sha256_array(cav) == ex.digests()["cavity"]
+    assert text == ex.text("cavityDescriptions.txt")
