@@ -1,0 +1,160 @@
+/*
+ * pma.h -- C ABI of the MI355X pore-throat + geodesic pipeline.
+ *
+ * One context owns one HIP stream, one device-resident volume set and the scratch of
+ * every stage.  Volumes are laid out [z][x][y]: voxel (x, y, z) is element
+ * (z * X + x) * Y + y, with X = TIFF rows, Y = TIFF columns, Z = slice files, exactly the
+ * reference's DataArray layout (ImageProcessing/Model/Headers/DataArray.h:85-94).
+ * Calls are blocking, a context is not shared between threads, and no C++ exception
+ * crosses this boundary: every entry point returns a pma_status and leaves a message for
+ * pma_last_error().
+ *
+ * The stage entry points replace the reference's service calls, which exchange data
+ * through folders of TIFF stacks; here the data stays in HBM between stages and can be
+ * loaded or read back per volume (pma_load_volume / pma_read_volume) and per text file
+ * (pma_set_text / pma_get_text) in the reference's own text formats.
+ */
+#ifndef PMA_PMA_H
+#define PMA_PMA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pma_ctx pma_ctx;
+
+typedef enum pma_status {
+    PMA_OK = 0,
+    PMA_ERR_INVALID_ARGUMENT = 1, /* bad pointer, size or enum value */
+    PMA_ERR_HIP = 2,              /* a HIP runtime call or kernel launch failed */
+    PMA_ERR_OUT_OF_MEMORY = 3,    /* device allocation failed */
+    PMA_ERR_STATE = 4,            /* a stage's input volume or table has not been produced */
+    PMA_ERR_CAPACITY = 5,         /* caller buffer too small; the needed size is returned */
+    PMA_ERR_INTERNAL = 6          /* an internal bound was exceeded (message says which) */
+} pma_status;
+
+typedef struct pma_dims {
+    int32_t x, y, z; /* image size along X (rows), Y (columns), Z (slices) */
+} pma_dims;
+
+/* Stencil of the pore, watershed and geodesic stages: "Use D3Q7 stencil" of config.txt
+ * (Serializer.cpp:40-54; ModelUtilities.h:22-29). */
+typedef enum pma_stencil { PMA_STENCIL_D3Q27 = 0, PMA_STENCIL_D3Q7 = 1 } pma_stencil;
+
+/* The volumes of the reference's output folders (Constants.h:28-39). */
+typedef enum pma_volume {
+    PMA_VOL_SOLID = 0,      /* u8, non-zero = solid              Images/                      */
+    PMA_VOL_EDT2 = 1,       /* u32 squared EDT                    EuclideanDistanceSquares/    */
+    PMA_VOL_CR2 = 2,        /* u32 containing-ball radius square  ContainingBallRadiiSquares/  */
+    PMA_VOL_CX = 3,         /* u32 containing-ball centre x       ContainingBallCoordinatesX/  */
+    PMA_VOL_CY = 4,         /* u32 containing-ball centre y       ContainingBallCoordinatesY/  */
+    PMA_VOL_CZ = 5,         /* u32 containing-ball centre z       ContainingBallCoordinatesZ/  */
+    PMA_VOL_MASK = 6,       /* u8 0 solid, 127 inner, 255 centre  IsMaximumBall/               */
+    PMA_VOL_PT = 7,         /* u32 pore / throat IDs              PoreThroatIds/               */
+    PMA_VOL_PTW = 8,        /* u32 watershed pore / throat IDs    PoreThroatIdsWatershed/      */
+    PMA_VOL_SPD = 9,        /* f32 geodesic distance              ShortestPathDistances/       */
+    PMA_VOL_CAVITY = 10,    /* u32 cavity IDs                     CavityIds/                   */
+    PMA_VOL_COUNT = 11
+} pma_volume;
+
+/* The reference's text outputs (Constants.h:15-25; formats of Serializer.cpp:139-221). */
+typedef enum pma_text {
+    PMA_TEXT_INTERMEDIATE_STATISTICS = 0,  /* intermediateStatistics.txt        */
+    PMA_TEXT_EDT_COUNTS = 1,               /* euclideanDistanceSquareCounts.txt */
+    PMA_TEXT_PORES = 2,                    /* pores.txt                         */
+    PMA_TEXT_THROATS = 3,                  /* sharedPixelTypes.txt              */
+    PMA_TEXT_PORES_WATERSHED = 4,          /* poresWatershed.txt                */
+    PMA_TEXT_THROATS_WATERSHED = 5,        /* sharedPixelTypesWatershed.txt     */
+    PMA_TEXT_CAVITIES = 6,                 /* cavityDescriptions.txt            */
+    PMA_TEXT_COUNT = 7
+} pma_text;
+
+/* Per-stage device time of the last call of each stage, in milliseconds (HIP events on
+ * the context's stream), and the counters the roofline figures are computed from. */
+typedef enum pma_stage {
+    PMA_STAGE_EDT = 0,
+    PMA_STAGE_CONTAINING_BALLS = 1,
+    PMA_STAGE_PORES = 2,
+    PMA_STAGE_REMOVE_EMPTY_PORES = 3,
+    PMA_STAGE_WATERSHED = 4,
+    PMA_STAGE_GEODESIC = 5,
+    PMA_STAGE_COUNT = 6
+} pma_stage;
+
+typedef struct pma_stage_info {
+    double device_ms;     /* events around the stage on the context's stream            */
+    double host_ms;       /* wall time of the entry point                                */
+    uint64_t launches;    /* kernel launches issued by the stage                          */
+    uint64_t work_items;  /* stage-specific work counter (see DESIGN.md of the project)  */
+} pma_stage_info;
+
+/* ---- context --------------------------------------------------------------------- */
+pma_status pma_create(pma_ctx** ctx, int device);
+void pma_destroy(pma_ctx* ctx);
+/* Message of the last failure on this thread ("" if none). */
+const char* pma_last_error(void);
+
+/* Stencil and the reference's "Number of threads"-independent knobs.  Replaces the
+ * Config fields read by Serializer::ReadConfig (Serializer.cpp:31-64) that the path uses. */
+pma_status pma_set_stencil(pma_ctx* ctx, pma_stencil stencil);
+
+/* ---- volumes ------------------------------------------------------------------------ */
+/* Sets the image size (allocating nothing yet) and invalidates every held volume. */
+pma_status pma_set_dims(pma_ctx* ctx, pma_dims dims);
+pma_status pma_get_dims(pma_ctx* ctx, pma_dims* dims);
+/* Copies a whole volume in (src_on_device != 0: src is a device pointer on the context's
+ * device).  Loading a stage input lets a stage run on data produced elsewhere, like the
+ * reference's -rib / -apt modes reading earlier folders (CommandLineDispatcher.cpp:91-106). */
+pma_status pma_load_volume(pma_ctx* ctx, pma_volume which, const void* src, int src_on_device);
+/* Copies a whole volume out; bytes must be the volume's size. */
+pma_status pma_read_volume(pma_ctx* ctx, pma_volume which, void* dst, size_t bytes, int dst_on_device);
+/* Device address of a held volume (valid until pma_set_dims or pma_destroy). */
+pma_status pma_volume_device_ptr(pma_ctx* ctx, pma_volume which, void** dptr);
+
+/* ---- stages (each replaces one reference service entry point) ------------------------ */
+/* EuclideanDistanceComputer::ComputeEuclideanDistances (EuclideanDistanceComputer.h:42):
+ * SOLID -> EDT2, max EDT square, histogram of non-boundary-crossing squares. */
+pma_status pma_edt(pma_ctx* ctx);
+/* ContainingBallsAssigner::SetContainingBalls (ContainingBallsAssigner.h:20):
+ * EDT2 -> CR2, CX, CY, CZ, MASK and the maximum-ball count. */
+pma_status pma_containing_balls(pma_ctx* ctx);
+/* PoreAndThroatComputer::AssignPoreAndThroatIds (PoreAndThroatComputer.h:21):
+ * MASK, EDT2, CR2 -> PT, pores.txt, sharedPixelTypes.txt. */
+pma_status pma_pores(pma_ctx* ctx);
+/* EmptyPoresRemover::RemoveEmptyPores (EmptyPoresRemover.h:27): PT + tables in place. */
+pma_status pma_remove_empty_pores(pma_ctx* ctx);
+/* WatershedComputer::AssignPoreAndThroatIds (WatershedComputer.h:22):
+ * CR2 + pores.txt -> PTW, poresWatershed.txt, sharedPixelTypesWatershed.txt. */
+pma_status pma_watershed(pma_ctx* ctx);
+/* ShortestPathComputer::ComputeShortestPathDistances (ShortestPathComputer.h:35):
+ * SOLID -> SPD, CAVITY, cavityDescriptions.txt. */
+pma_status pma_geodesic(pma_ctx* ctx);
+/* The reference's -all mode followed by -spd (ImageProcessingManager.cpp:63-108). */
+pma_status pma_run_all(pma_ctx* ctx);
+
+/* ---- statistics and tables ------------------------------------------------------------ */
+pma_status pma_get_statistics(pma_ctx* ctx, uint32_t* max_edt2, uint64_t* maximum_balls_count);
+/* counts[0..maxE] of non-boundary-crossing EDT squares; *len receives maxE + 1. */
+pma_status pma_get_histogram(pma_ctx* ctx, uint64_t* dst, size_t cap, size_t* len);
+/* Text of one of the reference's files, NUL-terminated; *len receives strlen + 1. */
+pma_status pma_get_text(pma_ctx* ctx, pma_text which, char* dst, size_t cap, size_t* len);
+/* Replaces a held table by parsing the reference's text (pores.txt, sharedPixelTypes.txt,
+ * intermediateStatistics.txt), as the reference's later stages read earlier files. */
+pma_status pma_set_text(pma_ctx* ctx, pma_text which, const char* text);
+pma_status pma_get_stage_info(pma_ctx* ctx, pma_stage stage, pma_stage_info* info);
+
+/* ---- synthetic inputs (written into SOLID; 0 = void, 255 = solid) --------------------- */
+/* Union of n spheres with centres uniform in the box and radii uniform in [rmin, rmax]. */
+pma_status pma_generate_spheres(pma_ctx* ctx, int32_t n, float rmin, float rmax, uint64_t seed);
+/* i.i.d. N(0,1) noise, separable Gaussian blur (sigma, truncated at 4 sigma, reflected
+ * boundary), void = the lowest `porosity` fraction of the field. */
+pma_status pma_generate_gaussian(pma_ctx* ctx, float sigma, float porosity, uint64_t seed);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PMA_PMA_H */

@@ -49,6 +49,7 @@ def lib():
                                        ctypes.POINTER(ctypes.c_void_p)]
         L.oracle_geodesic.argtypes = [_u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p, _u32p]
         L.oracle_geodesic.restype = ctypes.c_void_p
+        L.oracle_sort_desc.argtypes = [_u64p, ctypes.c_size_t]
         _lib = L
     return _lib
 
@@ -141,6 +142,13 @@ def geodesic(solid, d3q7=False):
     cav = np.empty(solid.shape, np.uint32)
     t = lib().oracle_geodesic(_ptr(solid, _u8p), X, Y, Z, int(d3q7), _ptr(dist, _f32p), _ptr(cav, _u32p))
     return dist, cav, _take_string(t)
+
+
+def sort_desc(records):
+    """libstdc++ std::sort of u64 records by their high 32 bits, descending."""
+    a = np.array(records, dtype=np.uint64, copy=True)
+    lib().oracle_sort_desc(_ptr(a, _u64p), a.size)
+    return a
 
 
 def pipeline(solid, d3q7=False, threads=1):

@@ -200,6 +200,12 @@ extern "C" {
 
 void oracle_free(void* p) { std::free(p); }
 
+// std::sort of u64 records by their high 32 bits, descending: the comparator shape of
+// MaximumBallComparer (Types.h:189-195) applied through StlUtilities::Sort (StlUtilities.h:95-98).
+void oracle_sort_desc(uint64_t* a, size_t n) {
+    std::sort(a, a + n, [](uint64_t x, uint64_t y) { return (x >> 32) > (y >> 32); });
+}
+
 // ---------------------------------------------------------------------------------------
 // EuclideanDistanceComputer (EuclideanDistanceComputer.cpp:149-343): X two-scan with the
 // start value X-1 (:164,:180), Y and Z brute-force windows of floor(sqrt(g))+1 (:234,:295),

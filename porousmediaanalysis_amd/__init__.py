@@ -1,0 +1,211 @@
+"""MI355X-native pore-throat + geodesic pipeline (Python view of the C ABI in include/pma/pma.h).
+
+The product is libpma.so: hand-written HIP kernels for gfx950 behind an extern "C" API.
+This module only loads that library with ctypes and moves numpy arrays in and out; it has
+no compute path of its own, and importing it fails loudly when the library is missing.
+
+Volumes are numpy arrays of shape (Z, X, Y), C order: the reference's [z][x][y] layout.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpma.so")
+
+VOL_SOLID, VOL_EDT2, VOL_CR2, VOL_CX, VOL_CY, VOL_CZ, VOL_MASK, VOL_PT, VOL_PTW, VOL_SPD, VOL_CAVITY = range(11)
+TEXT_STATS, TEXT_EDT_COUNTS, TEXT_PORES, TEXT_THROATS, TEXT_PORES_WS, TEXT_THROATS_WS, TEXT_CAVITIES = range(7)
+STAGE_EDT, STAGE_BALLS, STAGE_PORES, STAGE_REMOVE, STAGE_WATERSHED, STAGE_GEODESIC = range(6)
+STAGE_NAMES = ["edt", "containing_balls", "pores", "remove_empty_pores", "watershed", "geodesic"]
+_DTYPES = {VOL_SOLID: np.uint8, VOL_MASK: np.uint8, VOL_SPD: np.float32}
+
+STATUS = {0: "PMA_OK", 1: "PMA_ERR_INVALID_ARGUMENT", 2: "PMA_ERR_HIP", 3: "PMA_ERR_OUT_OF_MEMORY",
+          4: "PMA_ERR_STATE", 5: "PMA_ERR_CAPACITY", 6: "PMA_ERR_INTERNAL"}
+
+# Every symbol the header declares (checked by the CPU test of the library).
+EXPORTS = [
+    "pma_create", "pma_destroy", "pma_last_error", "pma_set_stencil", "pma_set_dims", "pma_get_dims",
+    "pma_load_volume", "pma_read_volume", "pma_volume_device_ptr", "pma_edt", "pma_containing_balls",
+    "pma_pores", "pma_remove_empty_pores", "pma_watershed", "pma_geodesic", "pma_run_all",
+    "pma_get_statistics", "pma_get_histogram", "pma_get_text", "pma_set_text", "pma_get_stage_info",
+    "pma_generate_spheres", "pma_generate_gaussian",
+]
+
+
+class PmaError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{STATUS.get(status, status)}: {message}")
+        self.status = status
+
+
+class Dims(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_int32), ("y", ctypes.c_int32), ("z", ctypes.c_int32)]
+
+
+class StageInfo(ctypes.Structure):
+    _fields_ = [("device_ms", ctypes.c_double), ("host_ms", ctypes.c_double), ("launches", ctypes.c_uint64),
+                ("work_items", ctypes.c_uint64)]
+
+
+def build(jobs=8):
+    """Compiles libpma.so for gfx950 with hipcc (Makefile in csrc/)."""
+    subprocess.run(["make", "-s", f"-j{jobs}", "-C", os.path.join(_HERE, "csrc")], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build() (hipcc) first")
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        L.pma_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int]
+        L.pma_destroy.argtypes = [vp]
+        L.pma_destroy.restype = None
+        L.pma_last_error.restype = ctypes.c_char_p
+        L.pma_set_stencil.argtypes = [vp, ctypes.c_int]
+        L.pma_set_dims.argtypes = [vp, Dims]
+        L.pma_get_dims.argtypes = [vp, ctypes.POINTER(Dims)]
+        L.pma_load_volume.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int]
+        L.pma_read_volume.argtypes = [vp, ctypes.c_int, vp, ctypes.c_size_t, ctypes.c_int]
+        L.pma_volume_device_ptr.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp)]
+        for f in ["pma_edt", "pma_containing_balls", "pma_pores", "pma_remove_empty_pores", "pma_watershed",
+                  "pma_geodesic", "pma_run_all"]:
+            getattr(L, f).argtypes = [vp]
+        L.pma_get_statistics.argtypes = [vp, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint64)]
+        L.pma_get_histogram.argtypes = [vp, vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+        L.pma_get_text.argtypes = [vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+        L.pma_set_text.argtypes = [vp, ctypes.c_int, ctypes.c_char_p]
+        L.pma_get_stage_info.argtypes = [vp, ctypes.c_int, ctypes.POINTER(StageInfo)]
+        L.pma_generate_spheres.argtypes = [vp, ctypes.c_int32, ctypes.c_float, ctypes.c_float, ctypes.c_uint64]
+        L.pma_generate_gaussian.argtypes = [vp, ctypes.c_float, ctypes.c_float, ctypes.c_uint64]
+        _lib = L
+    return _lib
+
+
+def _check(status):
+    if status != 0:
+        raise PmaError(status, lib().pma_last_error().decode())
+
+
+class Context:
+    """One device context (one HIP stream, one resident volume set)."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        _check(lib().pma_create(ctypes.byref(h), device))
+        self._h = h
+        self.shape = None
+
+    def close(self):
+        if self._h:
+            lib().pma_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- configuration and volumes --------------------------------------------------------
+    def set_stencil(self, d3q7):
+        _check(lib().pma_set_stencil(self._h, 1 if d3q7 else 0))
+
+    def set_shape(self, shape):
+        """shape = (Z, X, Y)."""
+        Z, X, Y = (int(v) for v in shape)
+        _check(lib().pma_set_dims(self._h, Dims(X, Y, Z)))
+        self.shape = (Z, X, Y)
+
+    def load(self, which, array):
+        dt = _DTYPES.get(which, np.uint32)
+        a = np.ascontiguousarray(array, dtype=dt)
+        if self.shape != a.shape:
+            self.set_shape(a.shape)
+        _check(lib().pma_load_volume(self._h, which, a.ctypes.data_as(ctypes.c_void_p), 0))
+
+    def read(self, which):
+        dt = _DTYPES.get(which, np.uint32)
+        out = np.empty(self.shape, dt)
+        _check(lib().pma_read_volume(self._h, which, out.ctypes.data_as(ctypes.c_void_p), out.nbytes, 0))
+        return out
+
+    def device_ptr(self, which):
+        p = ctypes.c_void_p()
+        _check(lib().pma_volume_device_ptr(self._h, which, ctypes.byref(p)))
+        return p.value
+
+    # -- stages ---------------------------------------------------------------------------
+    def edt(self):
+        _check(lib().pma_edt(self._h))
+
+    def containing_balls(self):
+        _check(lib().pma_containing_balls(self._h))
+
+    def pores(self):
+        _check(lib().pma_pores(self._h))
+
+    def remove_empty_pores(self):
+        _check(lib().pma_remove_empty_pores(self._h))
+
+    def watershed(self):
+        _check(lib().pma_watershed(self._h))
+
+    def geodesic(self):
+        _check(lib().pma_geodesic(self._h))
+
+    def run_all(self):
+        _check(lib().pma_run_all(self._h))
+
+    # -- statistics and text ------------------------------------------------------------
+    def statistics(self):
+        m, n = ctypes.c_uint32(), ctypes.c_uint64()
+        _check(lib().pma_get_statistics(self._h, ctypes.byref(m), ctypes.byref(n)))
+        return int(m.value), int(n.value)
+
+    def histogram(self):
+        n = ctypes.c_size_t()
+        st = lib().pma_get_histogram(self._h, None, 0, ctypes.byref(n))
+        if st not in (0, 5):
+            _check(st)
+        out = np.zeros(n.value, np.uint64)
+        _check(lib().pma_get_histogram(self._h, out.ctypes.data_as(ctypes.c_void_p), out.size, ctypes.byref(n)))
+        return out
+
+    def text(self, which):
+        n = ctypes.c_size_t()
+        st = lib().pma_get_text(self._h, which, None, 0, ctypes.byref(n))
+        if st not in (0, 5):
+            _check(st)
+        buf = ctypes.create_string_buffer(n.value)
+        _check(lib().pma_get_text(self._h, which, buf, n.value, ctypes.byref(n)))
+        return buf.value.decode()
+
+    def set_text(self, which, text):
+        _check(lib().pma_set_text(self._h, which, text.encode()))
+
+    def stage_info(self, stage):
+        s = StageInfo()
+        _check(lib().pma_get_stage_info(self._h, stage, ctypes.byref(s)))
+        return dict(device_ms=s.device_ms, host_ms=s.host_ms, launches=s.launches, work_items=s.work_items)
+
+    # -- synthetic inputs ---------------------------------------------------------------
+    def generate_spheres(self, shape, n, rmin, rmax, seed):
+        self.set_shape(shape)
+        _check(lib().pma_generate_spheres(self._h, n, rmin, rmax, seed))
+
+    def generate_gaussian(self, shape, sigma, porosity, seed):
+        self.set_shape(shape)
+        _check(lib().pma_generate_gaussian(self._h, sigma, porosity, seed))

@@ -1,0 +1,149 @@
+// Squared Euclidean distance transform (EuclideanDistanceComputer.cpp:149-343).
+//
+// The reference computes, per void voxel, the exact squared distance to the nearest solid
+// voxel or to the virtual solid planes x = -X and x = 2X-1 that its X-pass start value
+// X-1 implies (:164, :180).  The same value is produced here by three separable passes:
+//   pass X: one lane per (y, z) column, forward + backward scan of the 1-D distance,
+//   pass Y: one workgroup per (x, z) row staged in LDS, exact windowed minimum of
+//           g(y') + (y-y')^2 over |y-y'| <= floor(sqrt(g(y))) (the reference's window),
+//   pass Z: one lane per (x, y) column, lanes along y so every load is coalesced, the same
+//           windowed minimum along z, fused with the maximum and the histogram of squares
+//           whose ball does not cross the image boundary (:310-327).
+#include "pma_internal.h"
+
+namespace pma {
+namespace {
+
+constexpr int kLocalBins = 2048;
+
+__device__ __forceinline__ uint32_t isqrt_u32(uint32_t v) {
+    uint32_t s = (uint32_t)sqrtf((float)v);
+    while ((uint64_t)s * s > v) --s;
+    while ((uint64_t)(s + 1) * (s + 1) <= v) ++s;
+    return s;
+}
+
+__global__ void __launch_bounds__(256) edt_pass_x(const uint8_t* __restrict__ solid,
+                                                  uint32_t* __restrict__ g, int X, int Y) {
+    int y = blockIdx.x * blockDim.x + threadIdx.x;
+    if (y >= Y) return;
+    size_t base = (size_t)blockIdx.y * X * Y + y;
+    uint32_t d = (uint32_t)(X - 1);
+    for (int x = 0; x < X; ++x) {
+        size_t i = base + (size_t)x * Y;
+        d = solid[i] ? 0u : d + 1u;
+        g[i] = d;
+    }
+    d = (uint32_t)(X - 1);
+    for (int x = X - 1; x >= 0; --x) {
+        size_t i = base + (size_t)x * Y;
+        d = solid[i] ? 0u : d + 1u;
+        uint32_t f = min(g[i], d);
+        g[i] = f * f;
+    }
+}
+
+__global__ void __launch_bounds__(256) edt_pass_y(const uint32_t* __restrict__ g,
+                                                  uint32_t* __restrict__ h, int X, int Y) {
+    extern __shared__ uint32_t line[];
+    size_t base = ((size_t)blockIdx.y * X + blockIdx.x) * Y;
+    for (int y = threadIdx.x; y < Y; y += blockDim.x) line[y] = g[base + y];
+    __syncthreads();
+    for (int y = threadIdx.x; y < Y; y += blockDim.x) {
+        uint32_t best = line[y];
+        if (best != 0) {
+            int w = (int)isqrt_u32(best) + 1;
+            int lo = min(w, y), hi = min(w, Y - y);
+            for (int dy = -lo; dy < hi; ++dy) best = min(best, line[y + dy] + (uint32_t)(dy * dy));
+        }
+        h[base + y] = best;
+    }
+}
+
+__global__ void __launch_bounds__(256) edt_pass_z(const uint32_t* __restrict__ h,
+                                                  uint32_t* __restrict__ e, int X, int Y, int Z,
+                                                  unsigned long long* __restrict__ hist,
+                                                  uint32_t hist_len, uint32_t* __restrict__ max_out) {
+    __shared__ uint32_t lhist[kLocalBins];
+    __shared__ uint32_t wave_max[4];
+    for (int i = threadIdx.x; i < kLocalBins; i += blockDim.x) lhist[i] = 0;
+    __syncthreads();
+    int y = blockIdx.x * blockDim.x + threadIdx.x;
+    int x = blockIdx.y;
+    uint32_t my_max = 0;
+    if (y < Y) {
+        size_t stride = (size_t)X * Y;
+        size_t base = (size_t)x * Y + y;
+        int mxy = min(min(x, y), min(X - 1 - x, Y - 1 - y));
+        for (int z = 0; z < Z; ++z) {
+            uint32_t best = h[base + (size_t)z * stride];
+            if (best != 0) {
+                int w = (int)isqrt_u32(best) + 1;
+                int lo = min(w, z), hi = min(w, Z - z);
+                for (int dz = -lo; dz < hi; ++dz)
+                    best = min(best, h[base + (size_t)(z + dz) * stride] + (uint32_t)(dz * dz));
+                my_max = max(my_max, best);
+                // sqrt(best) > m  <=>  best > m*m for integer m >= 0
+                int m = min(mxy, min(z, Z - 1 - z));
+                if ((uint64_t)best <= (uint64_t)m * (uint64_t)m) {
+                    if (best < kLocalBins) atomicAdd(&lhist[best], 1u);
+                    else if (best < hist_len) atomicAdd(&hist[best], 1ull);
+                }
+            }
+            e[base + (size_t)z * stride] = best;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) my_max = max(my_max, (uint32_t)__shfl_xor((int)my_max, off));
+    if ((threadIdx.x & 63) == 0) wave_max[threadIdx.x >> 6] = my_max;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t m = 0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) m = max(m, wave_max[i]);
+        atomicMax(max_out, m);
+    }
+    int lim = min((uint32_t)kLocalBins, hist_len);
+    for (int i = threadIdx.x; i < lim; i += blockDim.x)
+        if (lhist[i]) atomicAdd(&hist[i], (unsigned long long)lhist[i]);
+}
+
+}  // namespace
+
+void run_edt(pma_ctx* c) {
+    PMA_REQUIRE(c->valid[PMA_VOL_SOLID], PMA_ERR_STATE, "pma_edt: the solid volume has not been loaded");
+    const int X = c->dims.x, Y = c->dims.y, Z = c->dims.z;
+    PMA_REQUIRE(Z <= 65535 && X <= 65535, PMA_ERR_INVALID_ARGUMENT, "pma_edt: X and Z must be <= 65535");
+    PMA_REQUIRE((size_t)Y * 4 <= 64 * 1024, PMA_ERR_INVALID_ARGUMENT, "pma_edt: Y must be <= 16384");
+    c->ensure_volume(PMA_VOL_EDT2);
+    uint32_t* e = c->v<uint32_t>(PMA_VOL_EDT2);
+    uint32_t* tmp = c->buf<uint32_t>("edt_tmp", c->n);
+    const uint8_t* solid = c->v<uint8_t>(PMA_VOL_SOLID);
+
+    // Counted squares satisfy e <= m*m with m <= (min(X,Y,Z)-1)/2.
+    int mind = std::min(X, std::min(Y, Z));
+    uint32_t mmax = (uint32_t)std::max(0, (mind - 1) / 2);
+    uint32_t hist_len = mmax * mmax + 1;
+    unsigned long long* hist = c->buf<unsigned long long>("edt_hist", hist_len);
+    uint32_t* dmax = c->buf<uint32_t>("edt_max", 1);
+    PMA_HIP(hipMemsetAsync(hist, 0, hist_len * sizeof(unsigned long long), c->stream));
+    PMA_HIP(hipMemsetAsync(dmax, 0, sizeof(uint32_t), c->stream));
+
+    edt_pass_x<<<dim3(grid_for(Y, 256), Z), 256, 0, c->stream>>>(solid, e, X, Y);
+    PMA_LAUNCHED(c);
+    edt_pass_y<<<dim3(X, Z), 256, (size_t)Y * sizeof(uint32_t), c->stream>>>(e, tmp, X, Y);
+    PMA_LAUNCHED(c);
+    edt_pass_z<<<dim3(grid_for(Y, 256), X), 256, 0, c->stream>>>(tmp, e, X, Y, Z, hist, hist_len, dmax);
+    PMA_LAUNCHED(c);
+
+    uint32_t m = read_scalar(c, dmax);
+    c->max_e = m;
+    c->n_max_balls = 0;
+    c->histogram.assign((size_t)m + 1, 0);
+    size_t copy = std::min<size_t>((size_t)m + 1, hist_len);
+    PMA_HIP(hipMemcpyAsync(c->histogram.data(), hist, copy * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+    PMA_HIP(hipStreamSynchronize(c->stream));
+    c->have_edt_stats = true;
+    c->valid[PMA_VOL_EDT2] = true;
+    c->stage[PMA_STAGE_EDT].work_items = c->n;
+}
+
+}  // namespace pma

@@ -1,0 +1,269 @@
+// Geodesic distances and cavities (ShortestPathComputer.cpp:39-511).
+//
+// The reference runs a float32 Dijkstra: d(v) = min over settled neighbours u of
+// fl(d(u) + w) with w = sqrtf(1|2|3) by |offset|_1 (1.0f with D3Q7).  fl(a + w) is isotone
+// in a and never below a, so the result is the least fixed point
+//     d(v) = min(d(v), min_u fl(d(u) + w)),  i.e. the minimum over paths of the left-fold
+// float32 sum, whatever the settling order.  It is computed here by frontier relaxation:
+// an atomicMin on the bit pattern of non-negative floats, re-queueing improved voxels
+// until no distance changes.  Every add is one correctly rounded fp32 add (no contraction).
+//
+// Start voxels follow SelectStartingPosition / FindVoidPixelOnBoxBoundaries (:396-511):
+// the centre min((int)(0.5 * size + 0.5), size - 1) if void and unreached, else the first
+// void unreached voxel on box shells of growing half size h, scanned face by face
+// (constant x low/high, then y, then z; the other two axes outer-to-inner).  The scan
+// position of a voxel is computed in closed form and the first one is found by atomicMin.
+#include "pma_internal.h"
+
+#include <cfloat>
+#include <cmath>
+
+namespace pma {
+namespace {
+
+constexpr uint32_t kFltMaxBits = 0x7F7FFFFFu;
+
+__global__ void geo_init(const uint8_t* __restrict__ solid, float* __restrict__ dist, uint32_t* __restrict__ cav,
+                         size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    bool s = solid[i] != 0;
+    dist[i] = s ? -1.0f : FLT_MAX;
+    cav[i] = s ? 0u : 0xFFFFFFFFu;
+}
+
+// Smallest scan key of voxel v among the shells h in [lo_h, hi_h), or ~0 if none.
+__device__ unsigned long long shell_key(const int v[3], const int c[3], const int S[3], int lo_h, int hi_h) {
+    unsigned long long best = ~0ull;
+    for (int k = 0; k < 3; ++k) {
+        const int a0 = k == 0 ? 1 : 0, a1 = k == 2 ? 1 : 2;
+        int lower = max(lo_h, max(abs(v[a0] - c[a0]), abs(v[a1] - c[a1])));
+        int hexact = abs(v[k] - c[k]);
+        int h = -1;
+        if (hexact >= lower) h = hexact;
+        else {
+            // a face clamped to the image border stays on v[k] for every larger h
+            if (v[k] == 0 && lower >= c[k]) h = lower;
+            if (v[k] == S[k] - 1 && lower >= S[k] - 1 - c[k]) h = lower;
+        }
+        if (h < 0 || h >= hi_h) continue;
+        int f0 = max(c[k] - h, 0);
+        int fi = v[k] == f0 ? 0 : 1;
+        unsigned long long key = ((unsigned long long)h << 44) | ((unsigned long long)k << 42) |
+                                 ((unsigned long long)fi << 41) | ((unsigned long long)v[a0] << 20) |
+                                 (unsigned long long)v[a1];
+        best = min(best, key);
+    }
+    return best;
+}
+
+__global__ void find_start(const uint8_t* __restrict__ solid, const float* __restrict__ dist, int X, int Y, int Z,
+                           int cx, int cy, int cz, int lo_h, int hi_h, unsigned long long* __restrict__ out) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long best = ~0ull;
+    if (i < (size_t)X * Y * Z && solid[i] == 0 && __float_as_uint(dist[i]) == kFltMaxBits) {
+        int v[3];
+        v[1] = (int)(i % Y);
+        size_t r = i / Y;
+        v[0] = (int)(r % X);
+        v[2] = (int)(r / X);
+        const int c[3] = {cx, cy, cz}, S[3] = {X, Y, Z};
+        best = shell_key(v, c, S, lo_h, hi_h);
+    }
+    for (int off = 32; off > 0; off >>= 1) best = min(best, (unsigned long long)__shfl_xor(best, off));
+    if ((threadIdx.x & 63) == 0 && best != ~0ull) atomicMin(out, best);
+}
+
+__global__ void set_source(float* dist, size_t i, uint32_t* frontier, uint32_t* count) {
+    dist[i] = 0.0f;
+    frontier[0] = (uint32_t)i;
+    *count = 1;
+}
+
+__global__ void __launch_bounds__(256) relax(const uint8_t* __restrict__ solid, float* __restrict__ dist,
+                                             const uint32_t* __restrict__ frontier, uint32_t nf,
+                                             uint32_t* __restrict__ next, uint32_t* __restrict__ next_count,
+                                             uint32_t* __restrict__ stamp, uint32_t round, int X, int Y, int Z,
+                                             int d3q7) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nf) return;
+    const uint32_t i = frontier[t];
+    const int y = (int)(i % (uint32_t)Y);
+    const uint32_t r = i / (uint32_t)Y;
+    const int x = (int)(r % (uint32_t)X), z = (int)(r / (uint32_t)X);
+    const float d = dist[i];
+    const float w1 = 1.0f, w2 = sqrtf(2.0f), w3 = sqrtf(3.0f);
+    for (int dz = -1; dz <= 1; ++dz) {
+        const int nz = z + dz;
+        if (nz < 0 || nz >= Z) continue;
+        for (int dx = -1; dx <= 1; ++dx) {
+            const int nx = x + dx;
+            if (nx < 0 || nx >= X) continue;
+            for (int dy = -1; dy <= 1; ++dy) {
+                const int m = abs(dz) + abs(dx) + abs(dy);
+                if (m == 0 || (d3q7 && m > 1)) continue;
+                const int ny = y + dy;
+                if (ny < 0 || ny >= Y) continue;
+                const uint32_t j = ((uint32_t)nz * (uint32_t)X + (uint32_t)nx) * (uint32_t)Y + (uint32_t)ny;
+                if (solid[j]) continue;
+                const float w = d3q7 ? 1.0f : (m == 1 ? w1 : (m == 2 ? w2 : w3));
+                const float nd = __fadd_rn(d, w);
+                const uint32_t nb = __float_as_uint(nd);
+                if (nb < __float_as_uint(dist[j])) {
+                    const uint32_t old = atomicMin(reinterpret_cast<unsigned int*>(dist) + j, nb);
+                    if (nb < old && atomicExch(&stamp[j], round) != round) next[atomicAdd(next_count, 1u)] = j;
+                }
+            }
+        }
+    }
+}
+
+// Voxels reached in this cavity (void, cavity id unset, distance set) take the id; walls
+// touched are OR-ed as bits (axis * 2 + side).
+__global__ void finish_cavity(const uint8_t* __restrict__ solid, const float* __restrict__ dist,
+                              uint32_t* __restrict__ cav, uint32_t id, int X, int Y, int Z,
+                              unsigned long long* __restrict__ count, uint32_t* __restrict__ walls) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long mine = 0;
+    uint32_t wb = 0;
+    if (i < (size_t)X * Y * Z && solid[i] == 0 && cav[i] == 0xFFFFFFFFu &&
+        __float_as_uint(dist[i]) != kFltMaxBits) {
+        cav[i] = id;
+        mine = 1;
+        int y = (int)(i % Y);
+        size_t r = i / Y;
+        int x = (int)(r % X), z = (int)(r / X);
+        if (x == 0) wb |= 1u;
+        if (x == X - 1) wb |= 2u;
+        if (y == 0) wb |= 4u;
+        if (y == Y - 1) wb |= 8u;
+        if (z == 0) wb |= 16u;
+        if (z == Z - 1) wb |= 32u;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        mine += __shfl_xor(mine, off);
+        wb |= (uint32_t)__shfl_xor((int)wb, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (mine) atomicAdd(count, mine);
+        if (wb) atomicOr(walls, wb);
+    }
+}
+
+// Distinct boundary planes as bits: a size-1 axis has one plane (FillAllBoundaryPlanes).
+uint32_t plane_bits(uint32_t wb, const int S[3]) {
+    uint32_t out = 0;
+    for (int a = 0; a < 3; ++a) {
+        uint32_t lo = (wb >> (2 * a)) & 1u, hi = (wb >> (2 * a + 1)) & 1u;
+        if (S[a] == 1) out |= (lo | hi) << (2 * a);
+        else out |= (lo << (2 * a)) | (hi << (2 * a + 1));
+    }
+    return out;
+}
+
+}  // namespace
+
+void run_geodesic(pma_ctx* c) {
+    PMA_REQUIRE(c->valid[PMA_VOL_SOLID], PMA_ERR_STATE, "pma_geodesic: the solid volume has not been loaded");
+    const int X = c->dims.x, Y = c->dims.y, Z = c->dims.z;
+    PMA_REQUIRE((double)X * Y * Z < 4294967295.0 && X < (1 << 20) && Y < (1 << 20) && Z < (1 << 20),
+                PMA_ERR_INVALID_ARGUMENT, "pma_geodesic: volume too large");
+    c->ensure_volume(PMA_VOL_SPD);
+    c->ensure_volume(PMA_VOL_CAVITY);
+    const uint8_t* solid = c->v<uint8_t>(PMA_VOL_SOLID);
+    float* dist = c->v<float>(PMA_VOL_SPD);
+    uint32_t* cav = c->v<uint32_t>(PMA_VOL_CAVITY);
+    const size_t n = c->n;
+    geo_init<<<grid_for(n, 256), 256, 0, c->stream>>>(solid, dist, cav, n);
+    PMA_LAUNCHED(c);
+    uint32_t* fa = c->buf<uint32_t>("geo_frontier_a", n);
+    uint32_t* fb = c->buf<uint32_t>("geo_frontier_b", n);
+    uint32_t* stamp = c->buf<uint32_t>("geo_stamp", n);
+    uint32_t* cnt = c->buf<uint32_t>("geo_count", 2);
+    unsigned long long* key = c->buf<unsigned long long>("geo_key", 1);
+    unsigned long long* vcount = c->buf<unsigned long long>("geo_vcount", 1);
+    uint32_t* walls = c->buf<uint32_t>("geo_walls", 1);
+    PMA_HIP(hipMemsetAsync(stamp, 0xFF, n * sizeof(uint32_t), c->stream));
+
+    const int S[3] = {X, Y, Z};
+    int ctr[3];
+    for (int a = 0; a < 3; ++a) ctr[a] = std::min((int)(0.5 * S[a] + 0.5), S[a] - 1);
+    int max_half = 0;
+    for (int a = 0; a < 3; ++a) max_half = std::max(max_half, std::max(S[a] - 1 - ctr[a], ctr[a]));
+    uint32_t all_walls = plane_bits(0x3Fu, S);
+    const int d3q7 = c->stencil == PMA_STENCIL_D3Q7;
+
+    c->cavities.clear();
+    int half = 0;
+    uint32_t id = 0, round = 0;
+    uint64_t relaxations = 0;
+    while (true) {
+        ++half;
+        int start[3];
+        bool found = false;
+        const size_t ci = ((size_t)ctr[2] * X + ctr[0]) * Y + ctr[1];
+        uint8_t cs = read_scalar(c, solid + ci);
+        float cd = read_scalar(c, dist + ci);
+        if (cs == 0 && cd == FLT_MAX) {
+            for (int a = 0; a < 3; ++a) start[a] = ctr[a];
+            found = true;
+        } else if (half < max_half) {
+            PMA_HIP(hipMemsetAsync(key, 0xFF, sizeof(unsigned long long), c->stream));
+            find_start<<<grid_for(n, 256), 256, 0, c->stream>>>(solid, dist, X, Y, Z, ctr[0], ctr[1], ctr[2], half,
+                                                                  max_half, key);
+            PMA_LAUNCHED(c);
+            unsigned long long k = read_scalar(c, key);
+            if (k != ~0ull) {
+                int h = (int)(k >> 44), ax = (int)((k >> 42) & 3), fi = (int)((k >> 41) & 1);
+                int a0 = ax == 0 ? 1 : 0, a1 = ax == 2 ? 1 : 2;
+                start[a0] = (int)((k >> 20) & 0xFFFFF);
+                start[a1] = (int)(k & 0xFFFFF);
+                start[ax] = fi == 0 ? std::max(ctr[ax] - h, 0) : std::min(ctr[ax] + h, S[ax] - 1);
+                half = h;
+                found = true;
+            } else {
+                half = max_half;
+            }
+        }
+        if (!found) break;
+        id += 127;
+        const size_t si = ((size_t)start[2] * X + start[0]) * Y + start[1];
+        set_source<<<1, 1, 0, c->stream>>>(dist, si, fa, cnt);
+        PMA_LAUNCHED(c);
+        uint32_t nf = 1;
+        uint32_t* cur = fa;
+        uint32_t* nxt = fb;
+        while (nf > 0) {
+            PMA_HIP(hipMemsetAsync(cnt + 1, 0, sizeof(uint32_t), c->stream));
+            relax<<<grid_for(nf, 256), 256, 0, c->stream>>>(solid, dist, cur, nf, nxt, cnt + 1, stamp, round++, X, Y,
+                                                            Z, d3q7);
+            PMA_LAUNCHED(c);
+            relaxations += nf;
+            nf = read_scalar(c, cnt + 1);
+            std::swap(cur, nxt);
+        }
+        PMA_HIP(hipMemsetAsync(vcount, 0, sizeof(unsigned long long), c->stream));
+        PMA_HIP(hipMemsetAsync(walls, 0, sizeof(uint32_t), c->stream));
+        finish_cavity<<<grid_for(n, 256), 256, 0, c->stream>>>(solid, dist, cav, id, X, Y, Z, vcount, walls);
+        PMA_LAUNCHED(c);
+        CavityRow row;
+        row.id = id;
+        row.x = start[0];
+        row.y = start[1];
+        row.z = start[2];
+        int d2 = 0;
+        for (int a = 0; a < 3; ++a) d2 += (start[a] - ctr[a]) * (start[a] - ctr[a]);
+        row.distance_to_center = std::sqrt((double)d2);
+        row.void_count = read_scalar(c, vcount);
+        uint32_t wb = plane_bits(read_scalar(c, walls), S);
+        row.reached_walls = __builtin_popcount(wb);
+        row.reaches_all = wb == all_walls;
+        c->cavities.push_back(row);
+        if (row.reaches_all) break;
+    }
+    c->have_cavities = true;
+    c->valid[PMA_VOL_SPD] = c->valid[PMA_VOL_CAVITY] = true;
+    c->stage[PMA_STAGE_GEODESIC].work_items = relaxations;
+}
+
+}  // namespace pma

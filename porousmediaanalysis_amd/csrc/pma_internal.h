@@ -1,0 +1,177 @@
+// Internal declarations shared by the host orchestration and the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "pma/pma.h"
+
+namespace pma {
+
+// Raised inside the library, turned into a pma_status at the C boundary.
+struct Error : std::runtime_error {
+    pma_status status;
+    Error(pma_status s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+#define PMA_HIP(expr)                                                                         \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            throw ::pma::Error(e_ == hipErrorOutOfMemory ? PMA_ERR_OUT_OF_MEMORY : PMA_ERR_HIP, \
+                               std::string(#expr) + ": " + hipGetErrorString(e_));            \
+    } while (0)
+
+#define PMA_LAUNCHED(ctx)                                                                     \
+    do {                                                                                      \
+        hipError_t e_ = hipGetLastError();                                                    \
+        if (e_ != hipSuccess)                                                                 \
+            throw ::pma::Error(PMA_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e_)); \
+        (ctx)->launches++;                                                                    \
+    } while (0)
+
+#define PMA_REQUIRE(cond, status, msg)                                                        \
+    do {                                                                                      \
+        if (!(cond)) throw ::pma::Error((status), (msg));                                     \
+    } while (0)
+
+// Host tables (Types.h:157-180 of the reference).
+struct PoreRow {
+    uint32_t id = 0;
+    int32_t x = 0, y = 0, z = 0;
+    uint32_t r2 = 0;
+    bool boundary = false;
+    uint64_t volume = 0;
+};
+struct ThroatRow {
+    uint32_t id = 0;
+    uint64_t volume = 0;
+    std::vector<uint32_t> pores;
+};
+struct CavityRow {
+    uint32_t id = 0;
+    int32_t x = 0, y = 0, z = 0;
+    double distance_to_center = 0;
+    uint64_t void_count = 0;
+    int32_t reached_walls = 0;
+    bool reaches_all = false;
+};
+
+// A device allocation that only grows.
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    void reserve(size_t n);
+    void release();
+    template <class T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+}  // namespace pma
+
+struct pma_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    pma_dims dims{0, 0, 0};
+    size_t n = 0;
+    pma_stencil stencil = PMA_STENCIL_D3Q27;
+
+    pma::DeviceBuffer vol[PMA_VOL_COUNT];
+    bool valid[PMA_VOL_COUNT] = {};
+
+    // named scratch buffers, kept between calls
+    std::map<std::string, pma::DeviceBuffer> scratch;
+    // small pinned host staging buffer for scalar read-backs
+    void* pinned = nullptr;
+    size_t pinned_bytes = 0;
+
+    // statistics of the EDT and containing-ball stages (intermediateStatistics.txt)
+    bool have_edt_stats = false;
+    uint32_t max_e = 0;
+    uint64_t n_max_balls = 0;
+    std::vector<uint64_t> histogram;
+
+    // tables
+    bool have_pores = false, have_pores_ws = false, have_cavities = false;
+    std::vector<pma::PoreRow> pores, pores_ws;
+    std::vector<pma::ThroatRow> throats, throats_ws;
+    std::vector<pma::CavityRow> cavities;
+
+    pma_stage_info stage[PMA_STAGE_COUNT] = {};
+    uint64_t launches = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    template <class T> T* v(pma_volume w) { return vol[w].as<T>(); }
+    void* buf(const std::string& name, size_t bytes) {
+        pma::DeviceBuffer& b = scratch[name];
+        b.reserve(bytes);
+        return b.ptr;
+    }
+    template <class T> T* buf(const std::string& name, size_t count) {
+        return static_cast<T*>(buf(name, count * sizeof(T)));
+    }
+    void* host_staging(size_t bytes);
+    void ensure_volume(pma_volume w);
+    void sync();
+};
+
+namespace pma {
+
+size_t volume_element_size(pma_volume w);
+
+// Stage implementations (one .hip file each).
+void run_edt(pma_ctx* c);
+void run_containing_balls(pma_ctx* c);
+void run_pores(pma_ctx* c);
+void run_remove_empty_pores(pma_ctx* c);
+void run_watershed(pma_ctx* c);
+void run_geodesic(pma_ctx* c);
+void run_generate_spheres(pma_ctx* c, int32_t n, float rmin, float rmax, uint64_t seed);
+void run_generate_gaussian(pma_ctx* c, float sigma, float porosity, uint64_t seed);
+
+// Device-wide helpers (scan.hip).
+// Exclusive scan of n u32 values into out (out may alias in); returns the total.
+uint64_t exclusive_scan_u32(pma_ctx* c, const uint32_t* in, uint32_t* out, size_t n);
+void exclusive_scan_u64(pma_ctx* c, const uint64_t* in, uint64_t* out, size_t n);
+// Stable LSD radix sorts over bits [begin_bit, end_bit) of the keys (rsort.hip).
+void radix_sort_pairs_u64(pma_ctx* c, const uint64_t* keys_in, uint64_t* keys_out, const uint32_t* vals_in,
+                          uint32_t* vals_out, size_t n, int begin_bit = 0, int end_bit = 64);
+void radix_sort_pairs_u64_u64(pma_ctx* c, const uint64_t* keys_in, uint64_t* keys_out, const uint64_t* vals_in,
+                              uint64_t* vals_out, size_t n, int begin_bit = 0, int end_bit = 64);
+void radix_sort_pairs_u32(pma_ctx* c, const uint32_t* keys_in, uint32_t* keys_out, const uint32_t* vals_in,
+                          uint32_t* vals_out, size_t n, int begin_bit = 0, int end_bit = 32);
+// Exact emulation of libstdc++ std::sort with comp(a, b) = (a >> 32) > (b >> 32) (sort.hip).
+void sort_like_libstdcxx_desc(pma_ctx* c, uint64_t* records, size_t n);
+
+// Ordered pore lists (lists.hip).  A list holds up to kListMax pore indices, stored as
+// index + 1 in 21-bit slots, three slots per word (0 = end of list).
+constexpr int kListMax = 9;
+struct ListTable {
+    uint32_t distinct = 0;           // number of distinct lists
+    std::vector<uint64_t> words;     // 3 words per distinct list
+    std::vector<uint64_t> min_time;  // earliest formation time of each distinct list
+    std::vector<uint64_t> finals;    // summed record weights per distinct list
+};
+// Dedupes R records (w0, w1, w2, time, weight); rec_distinct[r] receives the distinct
+// index of record r (device).  Host receives the distinct table.
+ListTable dedupe_lists(pma_ctx* c, const uint64_t* w0, const uint64_t* w1, const uint64_t* w2, const uint64_t* time,
+                       const uint32_t* weight, size_t R, uint32_t* rec_distinct);
+std::vector<uint32_t> list_members(const uint64_t* w);
+
+// Copies one value from device memory through pinned staging.
+template <class T> T read_scalar(pma_ctx* c, const T* dptr) {
+    T* h = static_cast<T*>(c->host_staging(sizeof(T)));
+    PMA_HIP(hipMemcpyAsync(h, dptr, sizeof(T), hipMemcpyDeviceToHost, c->stream));
+    PMA_HIP(hipStreamSynchronize(c->stream));
+    return *h;
+}
+
+inline unsigned grid_for(size_t n, unsigned block) { return (unsigned)((n + block - 1) / block); }
+
+}  // namespace pma

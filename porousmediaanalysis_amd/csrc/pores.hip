@@ -1,0 +1,928 @@
+// Pore seeds and pore propagation (PoreAndThroatComputer.cpp:81-269 with
+// BasePoreAndThroatComputer.cpp:37-133), reproduced exactly.
+//
+// Reference (one active area): maximum balls (mask 255) are listed in z, x, y order and
+// sorted by r2 with std::sort (:93-123); walking that order, a still-unlabelled centre opens
+// pore 127, 128, ... (+127 after a multiple of 255) and a DFS adds the pore to every voxel
+// reachable through void neighbours of non-increasing CR (:192-269).  A voxel reached by
+// several pores carries the key of the ordered list [p2, p1, p3, ...]; keys are created in
+// order of first DFS contact and numbered down from UINT_MAX - 1; zero-volume keys are erased.
+//
+// What determines each output, and how it is computed here:
+//   * seeds: the pores are exactly the regional-maximum plateaus of CR (26- or 6-connected
+//     components of equal CR with no higher neighbour); a plateau's seed is its element that
+//     comes first in the sorted list, and pores are numbered in that order.  (A non-maximal
+//     plateau is reachable, through non-increasing CR, from a regional maximum of higher CR
+//     whose seed sorts earlier; a maximal plateau cannot be entered from outside.)  The sort
+//     is the exact libstdc++ permutation (sort.hip).
+//   * pore sets: set(v) = {p : v reachable from seed(p)} is constant on a plateau and is the
+//     union of the sets of the higher plateaus adjacent to it, plus its own pore; it is
+//     computed over the plateau DAG level by level in descending CR.
+//   * key numbering: the key for set S is created while the largest pore of S runs its DFS,
+//     at the first visit of a voxel whose earlier-pore class is S minus that pore; every such
+//     prefix class is one creation.  The order of first visits inside one pore's DFS is
+//     replayed by one wave per pore (a recursive DFS with neighbours in reverse loop order is
+//     the same visit order as the reference's stack DFS), stopping once all its classes have
+//     been seen.
+//   * volumes = voxels per final set; boundary = some voxel of the pore's reach lies on the
+//     image border.
+#include "pma_internal.h"
+
+#include <algorithm>
+#include <numeric>
+
+namespace pma {
+namespace {
+
+constexpr uint32_t kInv = 0xFFFFFFFFu;
+constexpr int kSetCap = 64;  // largest pore set a plateau may carry
+
+__device__ __forceinline__ void decode(uint32_t i, int X, int Y, int& x, int& y, int& z) {
+    y = (int)(i % (uint32_t)Y);
+    uint32_t r = i / (uint32_t)Y;
+    x = (int)(r % (uint32_t)X);
+    z = (int)(r / (uint32_t)X);
+}
+
+__device__ __forceinline__ bool in_stencil(int k, int d3q7) {
+    const int dx = k / 9 - 1, dy = (k / 3) % 3 - 1, dz = k % 3 - 1;
+    const int m = abs(dx) + abs(dy) + abs(dz);
+    return m != 0 && (!d3q7 || m == 1);
+}
+
+// Neighbour index k in the reference's loop order (x outer, y, z inner).
+__device__ __forceinline__ bool neighbour(int k, int x, int y, int z, int X, int Y, int Z, uint32_t& j) {
+    const int nx = x + k / 9 - 1, ny = y + (k / 3) % 3 - 1, nz = z + k % 3 - 1;
+    if (nx < 0 || nx >= X || ny < 0 || ny >= Y || nz < 0 || nz >= Z) return false;
+    j = ((uint32_t)nz * (uint32_t)X + (uint32_t)nx) * (uint32_t)Y + (uint32_t)ny;
+    return true;
+}
+
+__device__ __forceinline__ uint32_t pore_id_of(uint32_t k) { return 127u + 255u * (k / 129u) + (k % 129u); }
+
+// ---- seeds ------------------------------------------------------------------------------
+constexpr int kChunk = 16;
+
+__global__ void __launch_bounds__(256) count_centres(const uint8_t* __restrict__ mask, size_t n,
+                                                     uint32_t* __restrict__ block_count) {
+    __shared__ uint32_t sum;
+    if (threadIdx.x == 0) sum = 0;
+    __syncthreads();
+    size_t base = (size_t)blockIdx.x * 256 * kChunk + (size_t)threadIdx.x * kChunk;
+    uint32_t c = 0;
+    for (int k = 0; k < kChunk; ++k)
+        if (base + k < n) c += mask[base + k] == 255;
+    atomicAdd(&sum, c);
+    __syncthreads();
+    if (threadIdx.x == 0) block_count[blockIdx.x] = sum;
+}
+
+__global__ void __launch_bounds__(256) write_centres(const uint8_t* __restrict__ mask, const uint32_t* __restrict__ e,
+                                                     size_t n, const uint32_t* __restrict__ block_off,
+                                                     unsigned long long* __restrict__ rec) {
+    __shared__ uint32_t wsum[4];
+    size_t base = (size_t)blockIdx.x * 256 * kChunk + (size_t)threadIdx.x * kChunk;
+    uint32_t c = 0;
+    for (int k = 0; k < kChunk; ++k)
+        if (base + k < n) c += mask[base + k] == 255;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t incl = c;
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t o = __shfl_up(incl, off);
+        if (lane >= off) incl += o;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    uint32_t o = block_off[blockIdx.x] + incl - c;
+    for (int w = 0; w < wave; ++w) o += wsum[w];
+    for (int k = 0; k < kChunk; ++k) {
+        size_t i = base + k;
+        if (i < n && mask[i] == 255) rec[o++] = ((unsigned long long)e[i] << 32) | (unsigned long long)i;
+    }
+}
+
+__global__ void cc_init(const uint32_t* __restrict__ cr, size_t n, uint32_t* __restrict__ parent) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) parent[i] = cr[i] ? (uint32_t)i : kInv;
+}
+
+__device__ uint32_t find_root(uint32_t* parent, uint32_t x) {
+    while (true) {
+        uint32_t p = parent[x];
+        if (p == x) return x;
+        uint32_t g = parent[p];
+        if (g != p) parent[x] = g;  // path halving: g is an ancestor of x
+        x = p;
+    }
+}
+
+__device__ void unite(uint32_t* parent, uint32_t a, uint32_t b) {
+    while (true) {
+        a = find_root(parent, a);
+        b = find_root(parent, b);
+        if (a == b) return;
+        if (a < b) {
+            uint32_t t = a;
+            a = b;
+            b = t;
+        }
+        uint32_t old = atomicCAS(&parent[a], a, b);  // hook the larger root under the smaller
+        if (old == a) return;
+        a = old;
+    }
+}
+
+__global__ void __launch_bounds__(256) cc_union(const uint32_t* __restrict__ cr, uint32_t* parent, int X, int Y, int Z,
+                                                int d3q7) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)X * Y * Z) return;
+    uint32_t c = cr[i];
+    if (!c) return;
+    int x, y, z;
+    decode((uint32_t)i, X, Y, x, y, z);
+    for (int k = 0; k < 13; ++k) {  // the half of the stencil with smaller linear index
+        if (!in_stencil(k, d3q7)) continue;
+        // k < 13 in loop order means dx < 0, or dx == 0 and (dy < 0 or (dy == 0 and dz < 0))
+        uint32_t j;
+        if (!neighbour(k, x, y, z, X, Y, Z, j)) continue;
+        if (cr[j] == c) unite(parent, (uint32_t)i, j);
+    }
+}
+
+__global__ void cc_flatten(uint32_t* parent, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && parent[i] != kInv) parent[i] = find_root(parent, (uint32_t)i);
+}
+
+__global__ void __launch_bounds__(256) cc_not_max(const uint32_t* __restrict__ cr, const uint32_t* __restrict__ root,
+                                                  uint8_t* __restrict__ not_max, int X, int Y, int Z, int d3q7) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)X * Y * Z) return;
+    uint32_t c = cr[i];
+    if (!c) return;
+    int x, y, z;
+    decode((uint32_t)i, X, Y, x, y, z);
+    for (int k = 0; k < 27; ++k) {
+        if (!in_stencil(k, d3q7)) continue;
+        uint32_t j;
+        if (!neighbour(k, x, y, z, X, Y, Z, j)) continue;
+        if (cr[j] > c) {
+            not_max[root[i]] = 1;
+            return;
+        }
+    }
+}
+
+__global__ void first_in_sorted(const unsigned long long* __restrict__ rec, size_t M, const uint32_t* __restrict__ root,
+                                const uint8_t* __restrict__ not_max, uint32_t* __restrict__ seedpos) {
+    size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= M) return;
+    uint32_t r = root[(uint32_t)rec[s]];
+    if (!not_max[r]) atomicMin(&seedpos[r], (uint32_t)s);
+}
+
+__global__ void seed_flags(const unsigned long long* __restrict__ rec, size_t M, const uint32_t* __restrict__ root,
+                           const uint32_t* __restrict__ seedpos, uint32_t* __restrict__ flag) {
+    size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= M) return;
+    flag[s] = seedpos[root[(uint32_t)rec[s]]] == (uint32_t)s;
+}
+
+__global__ void write_seeds(const unsigned long long* __restrict__ rec, size_t M, const uint32_t* __restrict__ flag,
+                            const uint32_t* __restrict__ off, unsigned long long* __restrict__ seeds) {
+    size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < M && flag[s]) seeds[off[s]] = rec[s];
+}
+
+// ---- plateaus ----------------------------------------------------------------------------
+__global__ void root_flags(const uint32_t* __restrict__ root, size_t n, uint32_t* __restrict__ flag) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flag[i] = root[i] == (uint32_t)i;
+}
+
+__global__ void plateau_index(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ off, size_t n,
+                              uint32_t* __restrict__ pidx, const uint32_t* __restrict__ cr,
+                              uint32_t* __restrict__ plat_cr) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && flag[i]) {
+        pidx[i] = off[i];
+        plat_cr[off[i]] = cr[i];
+    }
+}
+
+// root -> plateau index for every void voxel (in place), plus plateau size and border flag.
+__global__ void plateau_of_voxels(uint32_t* root, const uint32_t* __restrict__ pidx, int X, int Y, int Z,
+                                  uint32_t* __restrict__ plat_size, uint32_t* __restrict__ plat_border) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)X * Y * Z) return;
+    uint32_t r = root[i];
+    if (r == kInv) return;
+    uint32_t q = pidx[r];
+    __syncthreads();  // every root read in this block happens before any write below
+    root[i] = q;
+    atomicAdd(&plat_size[q], 1u);
+    int x, y, z;
+    decode((uint32_t)i, X, Y, x, y, z);
+    if (x == 0 || y == 0 || z == 0 || x == X - 1 || y == Y - 1 || z == Z - 1) plat_border[q] = 1;
+}
+
+__device__ __forceinline__ unsigned long long mix64(unsigned long long k) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ull;
+    k ^= k >> 33;
+    return k;
+}
+
+// Distinct plateau edges (higher CR -> lower CR) in an open-addressing set.
+__global__ void __launch_bounds__(256) plateau_edges(const uint32_t* __restrict__ cr, const uint32_t* __restrict__ plat,
+                                                     int X, int Y, int Z, int d3q7, unsigned long long* __restrict__ table,
+                                                     unsigned long long mask, uint32_t* __restrict__ overflow) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)X * Y * Z) return;
+    uint32_t c = cr[i];
+    if (!c) return;
+    const uint32_t dst = plat[i];
+    int x, y, z;
+    decode((uint32_t)i, X, Y, x, y, z);
+    uint32_t done[26];
+    int nd = 0;
+    for (int k = 0; k < 27; ++k) {
+        if (!in_stencil(k, d3q7)) continue;
+        uint32_t j;
+        if (!neighbour(k, x, y, z, X, Y, Z, j)) continue;
+        if (cr[j] <= c) continue;
+        const uint32_t src = plat[j];
+        bool seen = false;
+        for (int q = 0; q < nd; ++q) seen |= done[q] == src;
+        if (seen) continue;
+        done[nd++] = src;
+        const unsigned long long key = ((unsigned long long)src << 32) | dst;
+        unsigned long long h = mix64(key) & mask;
+        for (unsigned long long probe = 0;; ++probe) {
+            if (probe > mask) {
+                atomicOr(overflow, 1u);
+                break;
+            }
+            unsigned long long cur = table[h];
+            if (cur == key) break;
+            if (cur == ~0ull) {
+                unsigned long long old = atomicCAS(&table[h], ~0ull, key);
+                if (old == ~0ull || old == key) break;
+            }
+            h = (h + 1) & mask;
+        }
+    }
+}
+
+__global__ void slot_flags(const unsigned long long* __restrict__ table, size_t cap, uint32_t* __restrict__ flag) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cap) flag[i] = table[i] != ~0ull;
+}
+
+// Edge keys re-packed dst-major so a sort groups them by target.
+__global__ void write_edges(const unsigned long long* __restrict__ table, size_t cap, const uint32_t* __restrict__ flag,
+                            const uint32_t* __restrict__ off, unsigned long long* __restrict__ edges) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cap && flag[i]) {
+        unsigned long long k = table[i];
+        edges[off[i]] = (k << 32) | (k >> 32);
+    }
+}
+
+__global__ void edge_ranges(const unsigned long long* __restrict__ edges, size_t E, uint32_t* __restrict__ in_start,
+                            uint32_t* __restrict__ in_count) {
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    uint32_t dst = (uint32_t)(edges[e] >> 32);
+    if (e == 0 || (uint32_t)(edges[e - 1] >> 32) != dst) in_start[dst] = (uint32_t)e;
+    atomicAdd(&in_count[dst], 1u);
+}
+
+__global__ void level_keys(const uint32_t* __restrict__ plat_cr, size_t P, unsigned long long* __restrict__ keys,
+                           uint32_t* __restrict__ vals) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= P) return;
+    keys[q] = ((unsigned long long)(0xFFFFFFFFu - plat_cr[q]) << 32) | q;
+    vals[q] = (uint32_t)q;
+}
+
+__global__ void level_heads(const unsigned long long* __restrict__ keys, size_t P, uint32_t* __restrict__ head) {
+    size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < P) head[s] = (s == 0 || (keys[s] >> 32) != (keys[s - 1] >> 32));
+}
+
+__global__ void level_starts(const uint32_t* __restrict__ head, const uint32_t* __restrict__ off, size_t P,
+                             uint32_t* __restrict__ start) {
+    size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < P && head[s]) start[off[s]] = (uint32_t)s;
+}
+
+__global__ void set_union_level(const uint32_t* __restrict__ order, uint32_t lo, uint32_t hi,
+                                const uint32_t* __restrict__ in_start, const uint32_t* __restrict__ in_count,
+                                const unsigned long long* __restrict__ edges, const uint32_t* __restrict__ plat_pore,
+                                uint32_t* __restrict__ set_off, uint32_t* __restrict__ set_len, uint32_t* __restrict__ pool,
+                                unsigned long long* __restrict__ pool_top, unsigned long long pool_cap,
+                                uint32_t* __restrict__ overflow) {
+    uint32_t s = lo + blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= hi) return;
+    const uint32_t q = order[s];
+    uint32_t loc[kSetCap];
+    int m = 0;
+    bool of = false;
+    auto insert = [&](uint32_t p) {
+        int pos = 0;
+        while (pos < m && loc[pos] < p) ++pos;
+        if (pos < m && loc[pos] == p) return;
+        if (m == kSetCap) {
+            of = true;
+            return;
+        }
+        for (int t = m; t > pos; --t) loc[t] = loc[t - 1];
+        loc[pos] = p;
+        ++m;
+    };
+    if (plat_pore[q] != kInv) insert(plat_pore[q]);
+    const uint32_t e0 = in_start[q], ne = in_count[q];
+    for (uint32_t e = e0; e < e0 + ne; ++e) {
+        const uint32_t src = (uint32_t)edges[e];
+        const uint32_t so = set_off[src], sl = set_len[src];
+        for (uint32_t t = 0; t < sl; ++t) insert(pool[so + t]);
+    }
+    if (of) atomicOr(overflow, 1u);
+    unsigned long long o = atomicAdd(pool_top, (unsigned long long)m);
+    if (o + m > pool_cap) {
+        atomicOr(overflow, 2u);
+        set_off[q] = 0;
+        set_len[q] = 0;
+        return;
+    }
+    for (int t = 0; t < m; ++t) pool[o + t] = loc[t];
+    set_off[q] = (uint32_t)o;
+    set_len[q] = (uint32_t)m;
+}
+
+// ---- key records -------------------------------------------------------------------------
+__global__ void prefix_counts(const uint32_t* __restrict__ set_len, size_t P, uint32_t* __restrict__ cnt) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < P) cnt[q] = set_len[q] >= 2 ? set_len[q] - 1 : 0;
+}
+
+__global__ void prefix_records(const uint32_t* __restrict__ set_off, const uint32_t* __restrict__ set_len,
+                               const uint32_t* __restrict__ pool, const uint32_t* __restrict__ plat_size, size_t P,
+                               const uint32_t* __restrict__ rbase, uint64_t* __restrict__ w0, uint64_t* __restrict__ w1,
+                               uint64_t* __restrict__ w2, uint64_t* __restrict__ time, uint32_t* __restrict__ weight) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= P) return;
+    const uint32_t m = set_len[q];
+    if (m < 2) return;
+    const uint32_t* s = pool + set_off[q];
+    uint64_t w[3] = {0, 0, 0};
+    w[0] = (uint64_t)(s[0] + 1);
+    uint32_t r = rbase[q];
+    for (uint32_t j = 1; j < m && j < (uint32_t)kListMax; ++j) {
+        w[j / 3] |= (uint64_t)(s[j] + 1) << (21 * (j % 3));
+        w0[r] = w[0];
+        w1[r] = w[1];
+        w2[r] = w[2];
+        time[r] = 0;
+        weight[r] = j == m - 1 ? plat_size[q] : 0;
+        ++r;
+    }
+}
+
+__global__ void reach_sizes(const uint32_t* __restrict__ set_off, const uint32_t* __restrict__ set_len,
+                            const uint32_t* __restrict__ pool, const uint32_t* __restrict__ plat_size,
+                            const uint32_t* __restrict__ plat_border, size_t P, unsigned long long* __restrict__ reach,
+                            uint32_t* __restrict__ boundary) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= P) return;
+    const uint32_t* s = pool + set_off[q];
+    for (uint32_t t = 0; t < set_len[q]; ++t) {
+        atomicAdd(&reach[s[t]], (unsigned long long)plat_size[q]);
+        if (plat_border[q]) boundary[s[t]] = 1;
+    }
+}
+
+// ---- DFS replay: one wave per pore ------------------------------------------------------------
+struct ReplayTask {
+    uint32_t pore, seed, ncls, pad;
+    unsigned long long stack_off;
+    unsigned long long event_local_off;  // first bit of this pore's seen bitmap in LDS (0)
+};
+
+__device__ __forceinline__ int member_index(const uint32_t* __restrict__ pool, uint32_t off, uint32_t len, uint32_t p) {
+    for (uint32_t t = 0; t < len; ++t)
+        if (pool[off + t] == p) return (int)t;
+    return -1;
+}
+
+__global__ void __launch_bounds__(64) dfs_replay(const ReplayTask* __restrict__ tasks, const uint32_t* __restrict__ cr,
+                                                 const uint32_t* __restrict__ plat, const uint32_t* __restrict__ set_off,
+                                                 const uint32_t* __restrict__ set_len, const uint32_t* __restrict__ pool,
+                                                 const uint32_t* __restrict__ rbase,
+                                                 const uint32_t* __restrict__ rec_distinct,
+                                                 const uint32_t* __restrict__ event_local, uint32_t* __restrict__ visited,
+                                                 uint32_t* __restrict__ stack_voxel, uint8_t* __restrict__ stack_k,
+                                                 uint32_t* __restrict__ rank, int X, int Y, int Z, int d3q7,
+                                                 uint32_t* __restrict__ error) {
+    __shared__ uint32_t seen[1024];  // up to 32768 classes per pore
+    const ReplayTask task = tasks[blockIdx.x];
+    const int lane = threadIdx.x;
+    const uint32_t p = task.pore;
+    if (task.ncls > 32768) {
+        if (lane == 0) atomicOr(error, 4u);
+        return;
+    }
+    for (int i = lane; i < 1024; i += 64) seen[i] = 0;
+    __syncthreads();
+    uint32_t* sv = stack_voxel + task.stack_off;
+    uint8_t* sk = stack_k + task.stack_off;
+    uint32_t found = 0;
+    int top = -1;
+
+    // visit(v): mark, record the class, push a frame.  Returns false on a broken invariant.
+    auto visit = [&](uint32_t v) -> bool {
+        const uint32_t q = plat[v];
+        const uint32_t so = set_off[q], sl = set_len[q];
+        const int j = member_index(pool, so, sl, p);
+        if (j < 0) return false;
+        if (lane == 0) {
+            atomicOr(&visited[v], 1u << j);
+            if (j >= 1) {
+                const uint32_t e = rec_distinct[rbase[q] + (uint32_t)j - 1];
+                const uint32_t l = event_local[e];
+                if (!(seen[l >> 5] & (1u << (l & 31)))) {
+                    seen[l >> 5] |= 1u << (l & 31);
+                    rank[e] = found;
+                }
+            }
+        }
+        __syncthreads();
+        // every lane recomputes 'found' from the shared bitmap owner decision
+        return true;
+    };
+    (void)visit;
+
+    // Lane 0 keeps the authoritative state; other lanes evaluate neighbours.
+    auto class_of = [&](uint32_t v, int& j) -> uint32_t {
+        const uint32_t q = plat[v];
+        j = member_index(pool, set_off[q], set_len[q], p);
+        if (j >= 1) return rec_distinct[rbase[q] + (uint32_t)j - 1];
+        return kInv;
+    };
+
+    {
+        int j;
+        uint32_t e = class_of(task.seed, j);
+        if (j < 0) {
+            if (lane == 0) atomicOr(error, 1u);
+            return;
+        }
+        if (lane == 0) atomicOr(&visited[task.seed], 1u << j);
+        if (e != kInv) {
+            uint32_t l = event_local[e];
+            if (lane == 0) {
+                seen[l >> 5] |= 1u << (l & 31);
+                rank[e] = found;
+            }
+            ++found;
+        }
+        top = 0;
+        if (lane == 0) {
+            sv[0] = task.seed;
+            sk[0] = 27;
+        }
+    }
+    __syncthreads();
+    while (top >= 0 && found < task.ncls) {
+        const uint32_t u = sv[top];
+        const int k = sk[top];
+        int x, y, z;
+        decode(u, X, Y, x, y, z);
+        const uint32_t cu = cr[u];
+        bool cand = false;
+        uint32_t j = 0;
+        if (lane < k && lane < 27 && in_stencil(lane, d3q7) && neighbour(lane, x, y, z, X, Y, Z, j)) {
+            const uint32_t cn = cr[j];
+            if (cn != 0 && cn <= cu) {
+                const uint32_t q = plat[j];
+                const int b = member_index(pool, set_off[q], set_len[q], p);
+                cand = b >= 0 && !(visited[j] & (1u << b));
+                if (b < 0) atomicOr(error, 2u);
+            }
+        }
+        const unsigned long long bal = __ballot(cand);
+        if (!bal) {
+            --top;  // pop the frame
+            continue;
+        }
+        const int kk = 63 - __clzll(bal);  // highest index below k: reverse loop order
+        const uint32_t n = (uint32_t)__shfl((int)j, kk);
+        int b;
+        const uint32_t e = class_of(n, b);
+        if (lane == 0) {
+            sk[top] = (uint8_t)kk;
+            atomicOr(&visited[n], 1u << b);
+            sv[top + 1] = n;
+            sk[top + 1] = 27;
+        }
+        if (e != kInv) {
+            const uint32_t l = event_local[e];
+            const bool is_new = !(seen[l >> 5] & (1u << (l & 31)));
+            __syncthreads();
+            if (is_new) {
+                if (lane == 0) {
+                    seen[l >> 5] |= 1u << (l & 31);
+                    rank[e] = found;
+                }
+                ++found;
+            }
+        }
+        ++top;
+        __syncthreads();
+    }
+}
+
+// ---- labels ------------------------------------------------------------------------------------
+__global__ void write_labels(const uint32_t* __restrict__ plat, const uint32_t* __restrict__ set_off,
+                             const uint32_t* __restrict__ set_len, const uint32_t* __restrict__ pool,
+                             const uint32_t* __restrict__ rbase, const uint32_t* __restrict__ rec_distinct,
+                             const uint32_t* __restrict__ key_id, size_t n, uint32_t* __restrict__ pt) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t q = plat[i];
+    if (q == kInv) {
+        pt[i] = 0;
+        return;
+    }
+    uint32_t m = set_len[q];
+    if (m == 0) pt[i] = 0;
+    else if (m == 1) pt[i] = pore_id_of(pool[set_off[q]]);
+    else pt[i] = key_id[rec_distinct[rbase[q] + m - 2]];
+}
+
+__global__ void single_volumes(const uint32_t* __restrict__ set_off, const uint32_t* __restrict__ set_len,
+                               const uint32_t* __restrict__ pool, const uint32_t* __restrict__ plat_size, size_t P,
+                               unsigned long long* __restrict__ vol) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < P && set_len[q] == 1) atomicAdd(&vol[pool[set_off[q]]], (unsigned long long)plat_size[q]);
+}
+
+__global__ void fill_u32(uint32_t* p, uint32_t v, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+}  // namespace
+
+void run_pores(pma_ctx* c) {
+    PMA_REQUIRE(c->valid[PMA_VOL_MASK] && c->valid[PMA_VOL_EDT2] && c->valid[PMA_VOL_CR2], PMA_ERR_STATE,
+                "pma_pores: the maximum-ball mask, EDT and containing radii are needed");
+    const int X = c->dims.x, Y = c->dims.y, Z = c->dims.z;
+    const size_t n = c->n;
+    PMA_REQUIRE(n < 0xFFFFFFFFull, PMA_ERR_INVALID_ARGUMENT, "pma_pores: volume too large");
+    const int d3q7 = c->stencil == PMA_STENCIL_D3Q7;
+    const uint8_t* mask = c->v<uint8_t>(PMA_VOL_MASK);
+    const uint32_t* e = c->v<uint32_t>(PMA_VOL_EDT2);
+    const uint32_t* cr = c->v<uint32_t>(PMA_VOL_CR2);
+    c->ensure_volume(PMA_VOL_PT);
+    uint32_t* pt = c->v<uint32_t>(PMA_VOL_PT);
+    const unsigned nb = grid_for(n, 256);
+
+    // 1. maximum balls in z, x, y order (memory order), then the libstdc++ permutation.
+    const size_t cblocks = (n + 256 * kChunk - 1) / (256 * kChunk);
+    uint32_t* bcount = c->buf<uint32_t>("pt_bcount", cblocks);
+    uint32_t* boff = c->buf<uint32_t>("pt_boff", cblocks);
+    count_centres<<<(unsigned)cblocks, 256, 0, c->stream>>>(mask, n, bcount);
+    PMA_LAUNCHED(c);
+    const uint64_t M = exclusive_scan_u32(c, bcount, boff, cblocks);
+    unsigned long long* rec = c->buf<unsigned long long>("pt_rec", std::max<uint64_t>(M, 1));
+    write_centres<<<(unsigned)cblocks, 256, 0, c->stream>>>(mask, e, n, boff, rec);
+    PMA_LAUNCHED(c);
+    sort_like_libstdcxx_desc(c, reinterpret_cast<uint64_t*>(rec), M);
+
+    // 2. plateaus of CR and regional maxima.
+    uint32_t* root = c->buf<uint32_t>("pt_root", n);
+    uint8_t* not_max = c->buf<uint8_t>("pt_not_max", n);
+    uint32_t* seedpos = c->buf<uint32_t>("pt_seedpos", n);
+    cc_init<<<nb, 256, 0, c->stream>>>(cr, n, root);
+    PMA_LAUNCHED(c);
+    cc_union<<<nb, 256, 0, c->stream>>>(cr, root, X, Y, Z, d3q7);
+    PMA_LAUNCHED(c);
+    cc_flatten<<<nb, 256, 0, c->stream>>>(root, n);
+    PMA_LAUNCHED(c);
+    PMA_HIP(hipMemsetAsync(not_max, 0, n, c->stream));
+    PMA_HIP(hipMemsetAsync(seedpos, 0xFF, n * sizeof(uint32_t), c->stream));
+    cc_not_max<<<nb, 256, 0, c->stream>>>(cr, root, not_max, X, Y, Z, d3q7);
+    PMA_LAUNCHED(c);
+
+    // 3. seeds: the first sorted maximum ball of every regional-maximum plateau.
+    uint32_t npores = 0;
+    unsigned long long* seeds = c->buf<unsigned long long>("pt_seeds", std::max<uint64_t>(M, 1));
+    if (M) {
+        first_in_sorted<<<grid_for(M, 256), 256, 0, c->stream>>>(rec, M, root, not_max, seedpos);
+        PMA_LAUNCHED(c);
+        uint32_t* sflag = c->buf<uint32_t>("pt_sflag", M);
+        uint32_t* soff = c->buf<uint32_t>("pt_soff", M);
+        seed_flags<<<grid_for(M, 256), 256, 0, c->stream>>>(rec, M, root, seedpos, sflag);
+        PMA_LAUNCHED(c);
+        npores = (uint32_t)exclusive_scan_u32(c, sflag, soff, M);
+        write_seeds<<<grid_for(M, 256), 256, 0, c->stream>>>(rec, M, sflag, soff, seeds);
+        PMA_LAUNCHED(c);
+    }
+    std::vector<unsigned long long> seeds_h(npores);
+    if (npores)
+        PMA_HIP(hipMemcpyAsync(seeds_h.data(), seeds, npores * 8ull, hipMemcpyDeviceToHost, c->stream));
+
+    // 4. plateau indices, sizes, border flags.
+    uint32_t* rflag = c->buf<uint32_t>("pt_rflag", n);
+    uint32_t* roff = c->buf<uint32_t>("pt_roff", n);
+    root_flags<<<nb, 256, 0, c->stream>>>(root, n, rflag);
+    PMA_LAUNCHED(c);
+    const uint64_t P = exclusive_scan_u32(c, rflag, roff, n);
+    uint32_t* pidx = seedpos;  // seed positions are no longer needed
+    const size_t P1 = std::max<uint64_t>(P, 1);
+    uint32_t* plat_cr = c->buf<uint32_t>("pt_plat_cr", P1);
+    uint32_t* plat_size = c->buf<uint32_t>("pt_plat_size", P1);
+    uint32_t* plat_border = c->buf<uint32_t>("pt_plat_border", P1);
+    uint32_t* plat_pore = c->buf<uint32_t>("pt_plat_pore", P1);
+    plateau_index<<<nb, 256, 0, c->stream>>>(rflag, roff, n, pidx, cr, plat_cr);
+    PMA_LAUNCHED(c);
+    PMA_HIP(hipMemsetAsync(plat_size, 0, P1 * 4, c->stream));
+    PMA_HIP(hipMemsetAsync(plat_border, 0, P1 * 4, c->stream));
+    fill_u32<<<grid_for(P1, 256), 256, 0, c->stream>>>(plat_pore, kInv, P1);
+    PMA_LAUNCHED(c);
+    // pore of each seed plateau (host knows the seed voxels)
+    PMA_HIP(hipStreamSynchronize(c->stream));
+    std::vector<uint32_t> seed_root(npores), seed_plat(npores);
+    {
+        // read root and plateau index of every seed voxel
+        std::vector<uint32_t> tmp(npores);
+        uint32_t* d_sv = c->buf<uint32_t>("pt_seed_voxels", std::max(npores, 1u));
+        for (uint32_t k = 0; k < npores; ++k) tmp[k] = (uint32_t)seeds_h[k];
+        for (uint32_t k = 0; k < npores; ++k) {
+            seed_root[k] = 0;
+        }
+        (void)d_sv;
+    }
+    plateau_of_voxels<<<nb, 256, 0, c->stream>>>(root, pidx, X, Y, Z, plat_size, plat_border);
+    PMA_LAUNCHED(c);
+    uint32_t* plat = root;  // now the plateau index of every void voxel
+    if (npores) {
+        // plat_pore[plat[seed_k]] = k
+        std::vector<uint32_t> sv(npores);
+        for (uint32_t k = 0; k < npores; ++k) sv[k] = (uint32_t)seeds_h[k];
+        uint32_t* d_sv = c->buf<uint32_t>("pt_seed_voxels", npores);
+        PMA_HIP(hipMemcpyAsync(d_sv, sv.data(), npores * 4ull, hipMemcpyHostToDevice, c->stream));
+        std::vector<uint32_t> sp(npores);
+        uint32_t* d_sp = c->buf<uint32_t>("pt_seed_plat", npores);
+        // gather plat of seeds on the device, then scatter pores
+        struct G {
+            static __global__ void gather(const uint32_t* plat, const uint32_t* sv, uint32_t n, uint32_t* out,
+                                          uint32_t* plat_pore) {
+                uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+                if (k < n) {
+                    out[k] = plat[sv[k]];
+                    plat_pore[out[k]] = k;
+                }
+            }
+        };
+        G::gather<<<grid_for(npores, 256), 256, 0, c->stream>>>(plat, d_sv, npores, d_sp, plat_pore);
+        PMA_LAUNCHED(c);
+        (void)sp;
+    }
+
+    // 5. plateau DAG edges (distinct), grouped by target.
+    size_t cap = 1024;
+    while (cap < P * 8) cap <<= 1;
+    unsigned long long E = 0;
+    unsigned long long* table = nullptr;
+    uint32_t* of = c->buf<uint32_t>("pt_overflow", 1);
+    for (int attempt = 0;; ++attempt) {
+        table = c->buf<unsigned long long>("pt_edge_table", cap);
+        PMA_HIP(hipMemsetAsync(table, 0xFF, cap * 8, c->stream));
+        PMA_HIP(hipMemsetAsync(of, 0, 4, c->stream));
+        plateau_edges<<<nb, 256, 0, c->stream>>>(cr, plat, X, Y, Z, d3q7, table, cap - 1, of);
+        PMA_LAUNCHED(c);
+        if (read_scalar(c, of) == 0) break;
+        PMA_REQUIRE(attempt < 4, PMA_ERR_INTERNAL, "pma_pores: plateau edge table overflow");
+        cap <<= 2;
+    }
+    uint32_t* tflag = c->buf<uint32_t>("pt_tflag", cap);
+    uint32_t* toff = c->buf<uint32_t>("pt_toff", cap);
+    slot_flags<<<grid_for(cap, 256), 256, 0, c->stream>>>(table, cap, tflag);
+    PMA_LAUNCHED(c);
+    E = exclusive_scan_u32(c, tflag, toff, cap);
+    const size_t E1 = std::max<unsigned long long>(E, 1);
+    unsigned long long* edges = c->buf<unsigned long long>("pt_edges", E1);
+    unsigned long long* edges2 = c->buf<unsigned long long>("pt_edges2", E1);
+    write_edges<<<grid_for(cap, 256), 256, 0, c->stream>>>(table, cap, tflag, toff, edges);
+    PMA_LAUNCHED(c);
+    uint32_t* dummy = c->buf<uint32_t>("pt_dummy", E1);
+    uint32_t* dummy2 = c->buf<uint32_t>("pt_dummy2", E1);
+    radix_sort_pairs_u64(c, reinterpret_cast<uint64_t*>(edges), reinterpret_cast<uint64_t*>(edges2), dummy, dummy2, E);
+    if (E) std::swap(edges, edges2);
+    uint32_t* in_start = c->buf<uint32_t>("pt_in_start", P1);
+    uint32_t* in_count = c->buf<uint32_t>("pt_in_count", P1);
+    PMA_HIP(hipMemsetAsync(in_start, 0, P1 * 4, c->stream));
+    PMA_HIP(hipMemsetAsync(in_count, 0, P1 * 4, c->stream));
+    if (E) {
+        edge_ranges<<<grid_for(E, 256), 256, 0, c->stream>>>(edges, E, in_start, in_count);
+        PMA_LAUNCHED(c);
+    }
+
+    // 6. pore sets, level by level in descending CR.
+    unsigned long long* lkeys = c->buf<unsigned long long>("pt_lkeys", P1);
+    unsigned long long* lkeys2 = c->buf<unsigned long long>("pt_lkeys2", P1);
+    uint32_t* lvals = c->buf<uint32_t>("pt_lvals", P1);
+    uint32_t* order = c->buf<uint32_t>("pt_order", P1);
+    uint32_t* set_off = c->buf<uint32_t>("pt_set_off", P1);
+    uint32_t* set_len = c->buf<uint32_t>("pt_set_len", P1);
+    PMA_HIP(hipMemsetAsync(set_len, 0, P1 * 4, c->stream));
+    std::vector<uint32_t> lstart_h;
+    if (P) {
+        level_keys<<<grid_for(P, 256), 256, 0, c->stream>>>(plat_cr, P, lkeys, lvals);
+        PMA_LAUNCHED(c);
+        radix_sort_pairs_u64(c, reinterpret_cast<uint64_t*>(lkeys), reinterpret_cast<uint64_t*>(lkeys2), lvals, order, P);
+        uint32_t* lhead = c->buf<uint32_t>("pt_lhead", P);
+        uint32_t* loff = c->buf<uint32_t>("pt_loff", P);
+        level_heads<<<grid_for(P, 256), 256, 0, c->stream>>>(lkeys2, P, lhead);
+        PMA_LAUNCHED(c);
+        uint32_t L = (uint32_t)exclusive_scan_u32(c, lhead, loff, P);
+        uint32_t* lstart = c->buf<uint32_t>("pt_lstart", L);
+        level_starts<<<grid_for(P, 256), 256, 0, c->stream>>>(lhead, loff, P, lstart);
+        PMA_LAUNCHED(c);
+        lstart_h.resize(L + 1);
+        PMA_HIP(hipMemcpyAsync(lstart_h.data(), lstart, L * 4ull, hipMemcpyDeviceToHost, c->stream));
+        PMA_HIP(hipStreamSynchronize(c->stream));
+        lstart_h[L] = (uint32_t)P;
+    }
+    unsigned long long pool_cap = std::max<unsigned long long>(4 * P, 1024);
+    uint32_t* pool = nullptr;
+    unsigned long long* pool_top = c->buf<unsigned long long>("pt_pool_top", 1);
+    for (int attempt = 0;; ++attempt) {
+        pool = c->buf<uint32_t>("pt_pool", pool_cap);
+        PMA_HIP(hipMemsetAsync(pool_top, 0, 8, c->stream));
+        PMA_HIP(hipMemsetAsync(of, 0, 4, c->stream));
+        for (size_t l = 0; l + 1 < lstart_h.size(); ++l) {
+            uint32_t lo = lstart_h[l], hi = lstart_h[l + 1];
+            set_union_level<<<grid_for(hi - lo, 64), 64, 0, c->stream>>>(order, lo, hi, in_start, in_count, edges,
+                                                                         plat_pore, set_off, set_len, pool, pool_top,
+                                                                         pool_cap, of);
+            PMA_LAUNCHED(c);
+        }
+        uint32_t o = read_scalar(c, of);
+        PMA_REQUIRE(!(o & 1u), PMA_ERR_INTERNAL, "pma_pores: a plateau is reached by more than 64 pores");
+        if (!(o & 2u)) break;
+        PMA_REQUIRE(attempt < 3, PMA_ERR_INTERNAL, "pma_pores: pore set pool overflow");
+        pool_cap = read_scalar(c, pool_top) + 1024;
+    }
+
+    // 7. prefix classes (key creations), their distinct lists and final volumes.
+    uint32_t* pcnt = c->buf<uint32_t>("pt_pcnt", P1);
+    uint32_t* rbase = c->buf<uint32_t>("pt_rbase", P1);
+    uint64_t R = 0;
+    if (P) {
+        prefix_counts<<<grid_for(P, 256), 256, 0, c->stream>>>(set_len, P, pcnt);
+        PMA_LAUNCHED(c);
+        R = exclusive_scan_u32(c, pcnt, rbase, P);
+    }
+    const size_t R1 = std::max<uint64_t>(R, 1);
+    uint64_t* w0 = c->buf<uint64_t>("pt_w0", R1);
+    uint64_t* w1 = c->buf<uint64_t>("pt_w1", R1);
+    uint64_t* w2 = c->buf<uint64_t>("pt_w2", R1);
+    uint64_t* tm = c->buf<uint64_t>("pt_tm", R1);
+    uint32_t* wt = c->buf<uint32_t>("pt_wt", R1);
+    uint32_t* rd = c->buf<uint32_t>("pt_rd", R1);
+    ListTable lt;
+    if (R) {
+        prefix_records<<<grid_for(P, 256), 256, 0, c->stream>>>(set_off, set_len, pool, plat_size, P, rbase, w0, w1, w2,
+                                                                tm, wt);
+        PMA_LAUNCHED(c);
+        lt = dedupe_lists(c, w0, w1, w2, tm, wt, R, rd);
+    }
+    // events per pore (the largest pore of each distinct prefix list creates it)
+    const uint32_t D = lt.distinct;
+    std::vector<uint32_t> ev_owner(D), ev_local(D), ncls(npores, 0);
+    for (uint32_t d = 0; d < D; ++d) {
+        std::vector<uint32_t> mem = list_members(&lt.words[3 * (size_t)d]);
+        PMA_REQUIRE(mem.size() >= 2 && mem.size() <= (size_t)kListMax, PMA_ERR_INTERNAL,
+                    "pma_pores: a throat touches more than 9 pores");
+        ev_owner[d] = mem.back();
+        ev_local[d] = ncls[ev_owner[d]]++;
+    }
+    // reach sizes and boundary flags
+    unsigned long long* reach = c->buf<unsigned long long>("pt_reach", std::max(npores, 1u));
+    uint32_t* bflag = c->buf<uint32_t>("pt_bflag", std::max(npores, 1u));
+    PMA_HIP(hipMemsetAsync(reach, 0, std::max(npores, 1u) * 8ull, c->stream));
+    PMA_HIP(hipMemsetAsync(bflag, 0, std::max(npores, 1u) * 4ull, c->stream));
+    if (P) {
+        reach_sizes<<<grid_for(P, 256), 256, 0, c->stream>>>(set_off, set_len, pool, plat_size, plat_border, P, reach,
+                                                             bflag);
+        PMA_LAUNCHED(c);
+    }
+    std::vector<unsigned long long> reach_h(npores);
+    std::vector<uint32_t> bflag_h(npores);
+    if (npores) {
+        PMA_HIP(hipMemcpyAsync(reach_h.data(), reach, npores * 8ull, hipMemcpyDeviceToHost, c->stream));
+        PMA_HIP(hipMemcpyAsync(bflag_h.data(), bflag, npores * 4ull, hipMemcpyDeviceToHost, c->stream));
+        PMA_HIP(hipStreamSynchronize(c->stream));
+    }
+
+    // 8. DFS replays for pores with two or more classes: rank of each class inside its pore.
+    std::vector<uint32_t> rank_h(D, 0);
+    {
+        std::vector<ReplayTask> tasks;
+        unsigned long long stack_total = 0;
+        for (uint32_t k = 0; k < npores; ++k) {
+            if (ncls[k] < 2) continue;
+            ReplayTask t{};
+            t.pore = k;
+            t.seed = (uint32_t)seeds_h[k];
+            t.ncls = ncls[k];
+            t.stack_off = stack_total;
+            stack_total += reach_h[k] + 1;
+            tasks.push_back(t);
+        }
+        if (!tasks.empty()) {
+            ReplayTask* d_tasks = c->buf<ReplayTask>("pt_tasks", tasks.size());
+            uint32_t* d_local = c->buf<uint32_t>("pt_ev_local", std::max(D, 1u));
+            uint32_t* d_rank = c->buf<uint32_t>("pt_rank", std::max(D, 1u));
+            uint32_t* visited = c->buf<uint32_t>("pt_visited", n);
+            uint32_t* stack_voxel = c->buf<uint32_t>("pt_stack_voxel", stack_total);
+            uint8_t* stack_k = c->buf<uint8_t>("pt_stack_k", stack_total);
+            PMA_HIP(hipMemcpyAsync(d_tasks, tasks.data(), tasks.size() * sizeof(ReplayTask), hipMemcpyHostToDevice,
+                                   c->stream));
+            PMA_HIP(hipMemcpyAsync(d_local, ev_local.data(), D * 4ull, hipMemcpyHostToDevice, c->stream));
+            PMA_HIP(hipMemsetAsync(d_rank, 0, D * 4ull, c->stream));
+            PMA_HIP(hipMemsetAsync(visited, 0, n * 4, c->stream));
+            PMA_HIP(hipMemsetAsync(of, 0, 4, c->stream));
+            dfs_replay<<<(unsigned)tasks.size(), 64, 0, c->stream>>>(d_tasks, cr, plat, set_off, set_len, pool, rbase,
+                                                                    rd, d_local, visited, stack_voxel, stack_k, d_rank,
+                                                                    X, Y, Z, d3q7, of);
+            PMA_LAUNCHED(c);
+            PMA_REQUIRE(read_scalar(c, of) == 0, PMA_ERR_INTERNAL, "pma_pores: DFS replay invariant broken");
+            PMA_HIP(hipMemcpyAsync(rank_h.data(), d_rank, D * 4ull, hipMemcpyDeviceToHost, c->stream));
+            PMA_HIP(hipStreamSynchronize(c->stream));
+        }
+        c->stage[PMA_STAGE_PORES].work_items = stack_total;
+    }
+    // creation order: by pore, then by first visit inside the pore's DFS
+    std::vector<uint32_t> ord(D);
+    std::iota(ord.begin(), ord.end(), 0u);
+    std::sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) {
+        if (ev_owner[a] != ev_owner[b]) return ev_owner[a] < ev_owner[b];
+        return rank_h[a] < rank_h[b];
+    });
+    std::vector<uint32_t> key_id(std::max(D, 1u), 0);
+    for (uint32_t r = 0; r < D; ++r) key_id[ord[r]] = 0xFFFFFFFFu - 1u - r;
+
+    // 9. labels.
+    uint32_t* d_kid = c->buf<uint32_t>("pt_key_id", std::max(D, 1u));
+    if (D) PMA_HIP(hipMemcpyAsync(d_kid, key_id.data(), D * 4ull, hipMemcpyHostToDevice, c->stream));
+    write_labels<<<nb, 256, 0, c->stream>>>(plat, set_off, set_len, pool, rbase, rd, d_kid, n, pt);
+    PMA_LAUNCHED(c);
+    unsigned long long* pvol = c->buf<unsigned long long>("pt_pvol", std::max(npores, 1u));
+    PMA_HIP(hipMemsetAsync(pvol, 0, std::max(npores, 1u) * 8ull, c->stream));
+    if (P) {
+        single_volumes<<<grid_for(P, 256), 256, 0, c->stream>>>(set_off, set_len, pool, plat_size, P, pvol);
+        PMA_LAUNCHED(c);
+    }
+    std::vector<unsigned long long> pvol_h(npores);
+    if (npores) PMA_HIP(hipMemcpyAsync(pvol_h.data(), pvol, npores * 8ull, hipMemcpyDeviceToHost, c->stream));
+    PMA_HIP(hipStreamSynchronize(c->stream));
+
+    // 10. tables: pores in ascending ID, keys with volume in ascending ID, lists in the
+    // reference's order [p2, p1, p3, ...].
+    c->pores.assign(npores, PoreRow{});
+    for (uint32_t k = 0; k < npores; ++k) {
+        PoreRow& r = c->pores[k];
+        r.id = 127u + 255u * (k / 129u) + (k % 129u);
+        uint32_t v = (uint32_t)seeds_h[k];
+        r.y = (int)(v % (uint32_t)Y);
+        r.x = (int)((v / (uint32_t)Y) % (uint32_t)X);
+        r.z = (int)(v / (uint32_t)Y / (uint32_t)X);
+        r.r2 = (uint32_t)(seeds_h[k] >> 32);
+        r.volume = pvol_h[k];
+        r.boundary = bflag_h[k] != 0;
+    }
+    c->throats.clear();
+    for (uint32_t r = D; r-- > 0;) {
+        uint32_t d = ord[r];
+        if (lt.finals[d] == 0) continue;
+        std::vector<uint32_t> mem = list_members(&lt.words[3 * (size_t)d]);
+        ThroatRow t;
+        t.id = key_id[d];
+        t.volume = lt.finals[d];
+        t.pores.push_back(127u + 255u * (mem[1] / 129u) + (mem[1] % 129u));
+        t.pores.push_back(127u + 255u * (mem[0] / 129u) + (mem[0] % 129u));
+        for (size_t q = 2; q < mem.size(); ++q) t.pores.push_back(127u + 255u * (mem[q] / 129u) + (mem[q] % 129u));
+        c->throats.push_back(t);
+    }
+    c->have_pores = true;
+    c->valid[PMA_VOL_PT] = true;
+}
+
+}  // namespace pma

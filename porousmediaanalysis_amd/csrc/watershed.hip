@@ -1,0 +1,553 @@
+// Watershed flood (WatershedComputer.cpp:30-219), reproduced exactly.
+//
+// Reference: one FIFO per depth d (= CR square); seeds are pushed in ascending pore ID with
+// depth = seed r2; depths run from maxE down to 0; a popped node (v, p, d) visits its 26
+// (or 6) neighbours in x, y, z loop order; a neighbour n qualifies when it is in the image,
+// void and CR(n) <= d; an unlabelled n is claimed by p and pushed to FIFO CR(n); a labelled
+// n owned by another pore gets that pore appended to its ordered throat list (without a
+// push).  Keys are never erased; their IDs count down from UINT_MAX - 1 in order of first
+// formation.
+//
+// GPU plan (level-synchronous, exact FIFO order):
+//   * an event's global time is T = (maxE - d + 1) << 40 | pos << 5 | k, where pos is the
+//     popped node's position in FIFO d and k its neighbour index; T increases exactly in the
+//     reference's event order (pushes into FIFO d all happen before d is popped, and FIFO d
+//     is popped in position order).
+//   * FIFO d's content pushed from higher depths is sorted by push time when d starts; the
+//     pops of FIFO d are split into rounds (round r+1 = the nodes pushed into d by round r).
+//   * in a round, every qualifying contact to an unlabelled voxel bids its time with a 64-bit
+//     atomicMin (round-stamped so older bids lose); the minimum wins the claim.  All other
+//     contacts whose pore differs from the owner are logged (voxel, time, pore).
+//   * claims into the same depth are appended in (pos, k) order with a prefix sum; claims into
+//     lower depths are appended in any order with their push time (sorted when that depth
+//     starts).
+//   * after the flood the log is sorted by (voxel, time): the distinct foreign pores in order
+//     of first contact, c2, c3, ..., give the list [c2, owner, c3, ...] and its prefixes, each
+//     formed at the first contact of its last pore; keys are numbered by formation time.
+#include "pma_internal.h"
+
+#include <algorithm>
+
+namespace pma {
+namespace {
+
+constexpr uint64_t kPosMask = (1ull << 35) - 1;
+
+struct LogEntry {
+    unsigned long long t;
+    uint32_t voxel;
+    uint32_t pore;
+};
+
+__device__ __forceinline__ void decode(uint32_t i, int X, int Y, int& x, int& y, int& z) {
+    y = (int)(i % (uint32_t)Y);
+    uint32_t r = i / (uint32_t)Y;
+    x = (int)(r % (uint32_t)X);
+    z = (int)(r / (uint32_t)X);
+}
+
+__global__ void cr_histogram(const uint32_t* __restrict__ cr, size_t n, uint32_t* __restrict__ hist, uint32_t len) {
+    __shared__ uint32_t lh[4096];
+    for (int i = threadIdx.x; i < 4096; i += blockDim.x) lh[i] = 0;
+    __syncthreads();
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        uint32_t v = cr[i];
+        if (v == 0 || v >= len) continue;
+        if (v < 4096) atomicAdd(&lh[v], 1u);
+        else atomicAdd(&hist[v], 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < min(4096u, len); i += blockDim.x)
+        if (lh[i]) atomicAdd(&hist[i], lh[i]);
+}
+
+__global__ void push_seeds(const uint32_t* __restrict__ seed_voxel, const uint32_t* __restrict__ seed_r2,
+                           uint32_t np, uint32_t* __restrict__ label, const uint64_t* __restrict__ qoff,
+                           uint32_t* __restrict__ qcount, unsigned long long* __restrict__ qitem,
+                           unsigned long long* __restrict__ qkey) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= np) return;
+    uint32_t v = seed_voxel[k], lvl = seed_r2[k];
+    label[v] = k + 1;
+    uint32_t slot = atomicAdd(&qcount[lvl], 1u);
+    qitem[qoff[lvl] + slot] = (unsigned long long)v | ((unsigned long long)k << 32);
+    qkey[qoff[lvl] + slot] = k;  // initial pushes precede every flood push (top field 0)
+}
+
+__global__ void __launch_bounds__(256) ws_bid(const unsigned long long* __restrict__ items, uint32_t a, uint32_t b,
+                                              const uint32_t* __restrict__ cr, const uint32_t* __restrict__ label,
+                                              unsigned long long* __restrict__ claim, unsigned long long hi,
+                                              uint32_t d, int X, int Y, int Z, int d3q7) {
+    uint32_t pos = a + blockIdx.x * blockDim.x + threadIdx.x;
+    if (pos >= b) return;
+    uint32_t v = (uint32_t)items[pos];
+    int x, y, z;
+    decode(v, X, Y, x, y, z);
+    for (int k = 0; k < 27; ++k) {
+        const int dx = k / 9 - 1, dy = (k / 3) % 3 - 1, dz = k % 3 - 1;
+        const int m = abs(dx) + abs(dy) + abs(dz);
+        if (m == 0 || (d3q7 && m > 1)) continue;
+        const int nx = x + dx, ny = y + dy, nz = z + dz;
+        if (nx < 0 || nx >= X || ny < 0 || ny >= Y || nz < 0 || nz >= Z) continue;
+        const uint32_t j = ((uint32_t)nz * (uint32_t)X + (uint32_t)nx) * (uint32_t)Y + (uint32_t)ny;
+        const uint32_t c = cr[j];
+        if (c == 0 || c > d || label[j] != 0) continue;
+        atomicMin(&claim[j], hi | ((unsigned long long)pos << 5) | (unsigned long long)k);
+    }
+}
+
+__global__ void __launch_bounds__(256) ws_resolve(const unsigned long long* __restrict__ items, uint32_t a, uint32_t b,
+                                                  const uint32_t* __restrict__ cr, uint32_t* __restrict__ label,
+                                                  const unsigned long long* __restrict__ claim, unsigned long long hi,
+                                                  unsigned long long tlevel, uint32_t d, int X, int Y, int Z, int d3q7,
+                                                  uint32_t* __restrict__ same_mask, uint32_t* __restrict__ same_cnt,
+                                                  const uint64_t* __restrict__ qoff, uint32_t* __restrict__ qcount,
+                                                  unsigned long long* __restrict__ qitem,
+                                                  unsigned long long* __restrict__ qkey, LogEntry* __restrict__ log,
+                                                  unsigned long long* __restrict__ log_count, unsigned long long log_cap) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t pos = a + t;
+    if (pos >= b) return;
+    unsigned long long it = items[pos];
+    uint32_t v = (uint32_t)it, p = (uint32_t)(it >> 32);
+    int x, y, z;
+    decode(v, X, Y, x, y, z);
+    uint32_t mask = 0;
+    for (int k = 0; k < 27; ++k) {
+        const int dx = k / 9 - 1, dy = (k / 3) % 3 - 1, dz = k % 3 - 1;
+        const int m = abs(dx) + abs(dy) + abs(dz);
+        if (m == 0 || (d3q7 && m > 1)) continue;
+        const int nx = x + dx, ny = y + dy, nz = z + dz;
+        if (nx < 0 || nx >= X || ny < 0 || ny >= Y || nz < 0 || nz >= Z) continue;
+        const uint32_t j = ((uint32_t)nz * (uint32_t)X + (uint32_t)nx) * (uint32_t)Y + (uint32_t)ny;
+        const uint32_t c = cr[j];
+        if (c == 0 || c > d) continue;
+        const unsigned long long mine = hi | ((unsigned long long)pos << 5) | (unsigned long long)k;
+        const uint32_t lab = label[j];
+        uint32_t owner;
+        if (lab == 0) {
+            const unsigned long long w = claim[j];
+            if (w == mine) {
+                label[j] = p + 1;
+                if (c == d) {
+                    mask |= 1u << k;
+                } else {
+                    uint32_t slot = atomicAdd(&qcount[c], 1u);
+                    qitem[qoff[c] + slot] = (unsigned long long)j | ((unsigned long long)p << 32);
+                    qkey[qoff[c] + slot] = tlevel | ((unsigned long long)pos << 5) | (unsigned long long)k;
+                }
+                continue;
+            }
+            owner = (uint32_t)(items[(w >> 5) & kPosMask] >> 32);
+        } else {
+            owner = lab - 1;
+        }
+        if (owner != p) {
+            unsigned long long s = atomicAdd(log_count, 1ull);
+            if (s < log_cap) log[s] = LogEntry{tlevel | ((unsigned long long)pos << 5) | (unsigned long long)k, j, p};
+        }
+    }
+    same_mask[t] = mask;
+    same_cnt[t] = (uint32_t)__popc(mask);
+}
+
+__global__ void __launch_bounds__(256) ws_append(unsigned long long* __restrict__ items, uint32_t a, uint32_t b,
+                                                 const uint32_t* __restrict__ same_mask,
+                                                 const uint32_t* __restrict__ same_off, int X, int Y) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t pos = a + t;
+    if (pos >= b) return;
+    uint32_t mask = same_mask[t];
+    if (!mask) return;
+    unsigned long long it = items[pos];
+    uint32_t v = (uint32_t)it;
+    unsigned long long p = it >> 32;
+    int x, y, z;
+    decode(v, X, Y, x, y, z);
+    uint32_t o = b + same_off[t];
+    while (mask) {
+        int k = __ffs(mask) - 1;
+        mask &= mask - 1;
+        const int dx = k / 9 - 1, dy = (k / 3) % 3 - 1, dz = k % 3 - 1;
+        const uint32_t j = ((uint32_t)(z + dz) * (uint32_t)X + (uint32_t)(x + dx)) * (uint32_t)Y + (uint32_t)(y + dy);
+        items[o++] = (unsigned long long)j | (p << 32);
+    }
+}
+
+__global__ void log_split(const LogEntry* __restrict__ log, size_t n, unsigned long long* __restrict__ t,
+                          uint32_t* __restrict__ idx) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    t[i] = log[i].t;
+    idx[i] = (uint32_t)i;
+}
+
+__global__ void gather_voxel(const LogEntry* __restrict__ log, const uint32_t* __restrict__ perm,
+                             uint32_t* __restrict__ key, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) key[i] = log[perm[i]].voxel;
+}
+
+__global__ void run_heads(const LogEntry* __restrict__ log, const uint32_t* __restrict__ perm,
+                          uint32_t* __restrict__ head, size_t n) {
+    size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    head[s] = (s == 0 || log[perm[s]].voxel != log[perm[s - 1]].voxel) ? 1u : 0u;
+}
+
+__global__ void run_starts(const uint32_t* __restrict__ head, const uint32_t* __restrict__ hscan,
+                           uint32_t* __restrict__ start, size_t n) {
+    size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n && head[s]) start[hscan[s]] = (uint32_t)s;
+}
+
+// Distinct foreign pores of one voxel in order of first contact.
+__device__ int foreign_list(const LogEntry* __restrict__ log, const uint32_t* __restrict__ perm, uint32_t s0,
+                            uint32_t s1, uint32_t owner, uint32_t* pores, unsigned long long* times, int* overflow) {
+    int m = 0;
+    for (uint32_t s = s0; s < s1; ++s) {
+        const LogEntry e = log[perm[s]];
+        if (e.pore == owner) continue;
+        bool seen = false;
+        for (int q = 0; q < m; ++q) seen |= pores[q] == e.pore;
+        if (seen) continue;
+        if (m == kListMax - 1) {
+            *overflow = 1;
+            break;
+        }
+        pores[m] = e.pore;
+        times[m] = e.t;
+        ++m;
+    }
+    return m;
+}
+
+__global__ void count_prefixes(const LogEntry* __restrict__ log, const uint32_t* __restrict__ perm,
+                               const uint32_t* __restrict__ start, uint32_t runs, size_t n,
+                               const uint32_t* __restrict__ label, uint32_t* __restrict__ cnt,
+                               uint32_t* __restrict__ overflow) {
+    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= runs) return;
+    uint32_t s0 = start[r], s1 = r + 1 < runs ? start[r + 1] : (uint32_t)n;
+    uint32_t owner = label[log[perm[s0]].voxel] - 1;
+    uint32_t pores[kListMax];
+    unsigned long long times[kListMax];
+    int of = 0;
+    cnt[r] = (uint32_t)foreign_list(log, perm, s0, s1, owner, pores, times, &of);
+    if (of) atomicOr(overflow, 1u);
+}
+
+__global__ void write_prefixes(const LogEntry* __restrict__ log, const uint32_t* __restrict__ perm,
+                               const uint32_t* __restrict__ start, uint32_t runs, size_t n,
+                               const uint32_t* __restrict__ label, const uint32_t* __restrict__ off,
+                               uint64_t* __restrict__ w0, uint64_t* __restrict__ w1, uint64_t* __restrict__ w2,
+                               uint64_t* __restrict__ time, uint32_t* __restrict__ final_flag,
+                               uint32_t* __restrict__ voxel, uint32_t* __restrict__ shared_owned) {
+    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= runs) return;
+    uint32_t s0 = start[r], s1 = r + 1 < runs ? start[r + 1] : (uint32_t)n;
+    uint32_t vox = log[perm[s0]].voxel;
+    uint32_t owner = label[vox] - 1;
+    uint32_t pores[kListMax];
+    unsigned long long times[kListMax];
+    int of = 0;
+    int m = foreign_list(log, perm, s0, s1, owner, pores, times, &of);
+    if (m == 0) return;
+    atomicAdd(&shared_owned[owner], 1u);
+    // list = [f0, owner, f1, f2, ...]; prefix j (1-based count of foreign pores) formed at times[j-1]
+    uint64_t w[3] = {0, 0, 0};
+    auto put = [&](int slot, uint32_t pore) { w[slot / 3] |= (uint64_t)(pore + 1) << (21 * (slot % 3)); };
+    put(0, pores[0]);
+    put(1, owner);
+    uint32_t o = off[r];
+    for (int j = 1; j <= m; ++j) {
+        if (j >= 2) put(j, pores[j - 1]);
+        w0[o] = w[0];
+        w1[o] = w[1];
+        w2[o] = w[2];
+        time[o] = times[j - 1];
+        final_flag[o] = j == m;
+        voxel[o] = vox;
+        ++o;
+    }
+}
+
+__global__ void owned_counts(const uint32_t* __restrict__ label, size_t n, unsigned long long* __restrict__ counts) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && label[i]) atomicAdd(&counts[label[i] - 1], 1ull);
+}
+
+__global__ void boundary_owners(const uint32_t* __restrict__ label, int X, int Y, int Z, uint32_t* __restrict__ flag) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)X * Y * Z) return;
+    uint32_t lab = label[i];
+    if (!lab) return;
+    int x, y, z;
+    decode((uint32_t)i, X, Y, x, y, z);
+    if (x == 0 || y == 0 || z == 0 || x == X - 1 || y == Y - 1 || z == Z - 1) flag[lab - 1] = 1;
+}
+
+__global__ void labels_to_ids(uint32_t* __restrict__ label, size_t n, const uint32_t* __restrict__ pore_id) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && label[i]) label[i] = pore_id[label[i] - 1];
+}
+
+__global__ void shared_ids(uint32_t* __restrict__ label, const uint32_t* __restrict__ voxel,
+                           const uint32_t* __restrict__ final_flag, const uint32_t* __restrict__ rec_distinct,
+                           const uint32_t* __restrict__ key_id, size_t R) {
+    size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < R && final_flag[r]) label[voxel[r]] = key_id[rec_distinct[r]];
+}
+
+}  // namespace
+
+namespace {
+bool watershed_once(pma_ctx* c, unsigned long long log_cap, unsigned long long* needed);
+}
+
+void run_watershed(pma_ctx* c) {
+    // The contact log is sized from the volume; a run that overflows it is repeated once
+    // with the exact size it reported.
+    unsigned long long needed = 0;
+    if (watershed_once(c, std::max<unsigned long long>(c->n / 2, 1 << 16), &needed)) return;
+    PMA_REQUIRE(watershed_once(c, needed + needed / 8 + 1024, &needed), PMA_ERR_INTERNAL,
+                "pma_watershed: contact log overflow on the second attempt");
+}
+
+namespace {
+bool watershed_once(pma_ctx* c, unsigned long long log_cap, unsigned long long* needed) {
+    PMA_REQUIRE(c->valid[PMA_VOL_CR2], PMA_ERR_STATE, "pma_watershed: containing radii have not been computed or loaded");
+    PMA_REQUIRE(c->have_pores, PMA_ERR_STATE, "pma_watershed: no pores (run pma_pores or set pores.txt)");
+    PMA_REQUIRE(c->have_edt_stats, PMA_ERR_STATE, "pma_watershed: no maximum EDT square (intermediate statistics)");
+    const int X = c->dims.x, Y = c->dims.y, Z = c->dims.z;
+    const size_t n = c->n;
+    PMA_REQUIRE(n < 0xFFFFFFFFull, PMA_ERR_INVALID_ARGUMENT, "pma_watershed: volume too large");
+    const uint32_t maxE = c->max_e;
+    const uint32_t np = (uint32_t)c->pores.size();
+    PMA_REQUIRE(np < (1u << 21) - 1, PMA_ERR_INVALID_ARGUMENT, "pma_watershed: too many pores");
+    c->ensure_volume(PMA_VOL_PTW);
+    uint32_t* label = c->v<uint32_t>(PMA_VOL_PTW);
+    const uint32_t* cr = c->v<uint32_t>(PMA_VOL_CR2);
+    const int d3q7 = c->stencil == PMA_STENCIL_D3Q7;
+
+    // FIFO capacities: voxels with CR == d plus seeds with r2 == d.
+    const size_t L = (size_t)maxE + 1;
+    uint32_t* hist = c->buf<uint32_t>("ws_hist", L);
+    PMA_HIP(hipMemsetAsync(hist, 0, L * sizeof(uint32_t), c->stream));
+    cr_histogram<<<1024, 256, 0, c->stream>>>(cr, n, hist, (uint32_t)L);
+    PMA_LAUNCHED(c);
+    std::vector<uint32_t> h(L);
+    PMA_HIP(hipMemcpyAsync(h.data(), hist, L * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    PMA_HIP(hipStreamSynchronize(c->stream));
+    std::vector<uint32_t> seed_voxel(np), seed_r2(np), pore_id(np);
+    for (uint32_t k = 0; k < np; ++k) {
+        const PoreRow& p = c->pores[k];
+        PMA_REQUIRE(p.x >= 0 && p.x < X && p.y >= 0 && p.y < Y && p.z >= 0 && p.z < Z, PMA_ERR_INVALID_ARGUMENT,
+                    "pma_watershed: a pore seed lies outside the image");
+        PMA_REQUIRE(p.r2 <= maxE, PMA_ERR_INVALID_ARGUMENT, "pma_watershed: a seed radius exceeds the maximum EDT square");
+        seed_voxel[k] = (uint32_t)(((size_t)p.z * X + p.x) * Y + p.y);
+        seed_r2[k] = p.r2;
+        pore_id[k] = p.id;
+        h[p.r2] += 1;
+    }
+    std::vector<uint64_t> qoff_h(L + 1, 0);
+    for (size_t d = 0; d < L; ++d) qoff_h[d + 1] = qoff_h[d] + h[d];
+    const uint64_t qtotal = qoff_h[L];
+    uint64_t* qoff = c->buf<uint64_t>("ws_qoff", L + 1);
+    uint32_t* qcount = c->buf<uint32_t>("ws_qcount", L);
+    unsigned long long* qitem = c->buf<unsigned long long>("ws_qitem", std::max<uint64_t>(qtotal, 1));
+    unsigned long long* qkey = c->buf<unsigned long long>("ws_qkey", std::max<uint64_t>(qtotal, 1));
+    unsigned long long* sort_k = c->buf<unsigned long long>("ws_sort_k", std::max<uint64_t>(qtotal, 1));
+    unsigned long long* sort_v = c->buf<unsigned long long>("ws_sort_v", std::max<uint64_t>(qtotal, 1));
+    PMA_HIP(hipMemcpyAsync(qoff, qoff_h.data(), (L + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, c->stream));
+    PMA_HIP(hipMemsetAsync(qcount, 0, L * sizeof(uint32_t), c->stream));
+    PMA_HIP(hipMemsetAsync(label, 0, n * sizeof(uint32_t), c->stream));
+    unsigned long long* claim = c->buf<unsigned long long>("ws_claim", n);
+    PMA_HIP(hipMemsetAsync(claim, 0xFF, n * sizeof(unsigned long long), c->stream));
+    uint32_t* d_seed_voxel = c->buf<uint32_t>("ws_seed_voxel", std::max(np, 1u));
+    uint32_t* d_seed_r2 = c->buf<uint32_t>("ws_seed_r2", std::max(np, 1u));
+    if (np) {
+        PMA_HIP(hipMemcpyAsync(d_seed_voxel, seed_voxel.data(), np * 4, hipMemcpyHostToDevice, c->stream));
+        PMA_HIP(hipMemcpyAsync(d_seed_r2, seed_r2.data(), np * 4, hipMemcpyHostToDevice, c->stream));
+        push_seeds<<<grid_for(np, 256), 256, 0, c->stream>>>(d_seed_voxel, d_seed_r2, np, label, qoff, qcount, qitem, qkey);
+        PMA_LAUNCHED(c);
+    }
+    LogEntry* log = c->buf<LogEntry>("ws_log", log_cap);
+    unsigned long long* log_count = c->buf<unsigned long long>("ws_log_count", 1);
+    PMA_HIP(hipMemsetAsync(log_count, 0, sizeof(unsigned long long), c->stream));
+    uint32_t* same_mask = c->buf<uint32_t>("ws_same_mask", std::max<uint64_t>(qtotal, 1));
+    uint32_t* same_cnt = c->buf<uint32_t>("ws_same_cnt", std::max<uint64_t>(qtotal, 1));
+    uint32_t* same_off = c->buf<uint32_t>("ws_same_off", std::max<uint64_t>(qtotal, 1));
+
+    uint64_t rounds = 0;
+    uint32_t gr = 0;  // global round counter for claim stamps
+    for (long d = (long)maxE; d >= 1; --d) {
+        uint32_t cnt = read_scalar(c, qcount + d);
+        if (cnt == 0) continue;
+        unsigned long long* items = qitem + qoff_h[d];
+        if (cnt > 1) {
+            radix_sort_pairs_u64_u64(c, reinterpret_cast<const uint64_t*>(qkey + qoff_h[d]),
+                                     reinterpret_cast<uint64_t*>(sort_k), reinterpret_cast<const uint64_t*>(items),
+                                     reinterpret_cast<uint64_t*>(sort_v), cnt);
+            PMA_HIP(hipMemcpyAsync(items, sort_v, cnt * sizeof(uint64_t), hipMemcpyDeviceToDevice, c->stream));
+        }
+        const unsigned long long tlevel = (unsigned long long)(maxE - d + 1) << 40;
+        uint32_t a = 0, b = cnt;
+        while (a < b) {
+            PMA_REQUIRE(gr < (1u << 24) - 1, PMA_ERR_INTERNAL, "pma_watershed: too many rounds");
+            const unsigned long long hi = (unsigned long long)((1u << 24) - 1 - gr) << 40;
+            ++gr;
+            ++rounds;
+            const unsigned g = grid_for(b - a, 256);
+            ws_bid<<<g, 256, 0, c->stream>>>(items, a, b, cr, label, claim, hi, (uint32_t)d, X, Y, Z, d3q7);
+            PMA_LAUNCHED(c);
+            ws_resolve<<<g, 256, 0, c->stream>>>(items, a, b, cr, label, claim, hi, tlevel, (uint32_t)d, X, Y, Z, d3q7,
+                                                 same_mask, same_cnt, qoff, qcount, qitem, qkey, log, log_count,
+                                                 log_cap);
+            PMA_LAUNCHED(c);
+            uint64_t added = exclusive_scan_u32(c, same_cnt, same_off, b - a);
+            if (added) {
+                ws_append<<<g, 256, 0, c->stream>>>(items, a, b, same_mask, same_off, X, Y);
+                PMA_LAUNCHED(c);
+            }
+            PMA_REQUIRE(b + added <= h[d], PMA_ERR_INTERNAL, "pma_watershed: FIFO capacity exceeded");
+            a = b;
+            b = (uint32_t)(b + added);
+        }
+    }
+    const unsigned long long nlog = read_scalar(c, log_count);
+    *needed = nlog;
+    if (nlog > log_cap) return false;
+
+    // Order the log by (voxel, time): stable sort by time, then by voxel.
+    std::vector<ThroatRow> throats;
+    uint32_t* shared_owned = c->buf<uint32_t>("ws_shared_owned", std::max(np, 1u));
+    PMA_HIP(hipMemsetAsync(shared_owned, 0, std::max(np, 1u) * sizeof(uint32_t), c->stream));
+    if (nlog) {
+        unsigned long long* tk = c->buf<unsigned long long>("ws_log_t", nlog);
+        unsigned long long* tk2 = c->buf<unsigned long long>("ws_log_t2", nlog);
+        uint32_t* perm = c->buf<uint32_t>("ws_log_perm", nlog);
+        uint32_t* perm2 = c->buf<uint32_t>("ws_log_perm2", nlog);
+        uint32_t* vk = c->buf<uint32_t>("ws_log_vk", nlog);
+        uint32_t* vk2 = c->buf<uint32_t>("ws_log_vk2", nlog);
+        log_split<<<grid_for(nlog, 256), 256, 0, c->stream>>>(log, nlog, tk, perm);
+        PMA_LAUNCHED(c);
+        radix_sort_pairs_u64(c, reinterpret_cast<uint64_t*>(tk), reinterpret_cast<uint64_t*>(tk2), perm, perm2, nlog);
+        gather_voxel<<<grid_for(nlog, 256), 256, 0, c->stream>>>(log, perm2, vk, nlog);
+        PMA_LAUNCHED(c);
+        radix_sort_pairs_u32(c, vk, vk2, perm2, perm, nlog);
+        uint32_t* head = c->buf<uint32_t>("ws_head", nlog);
+        uint32_t* hscan = c->buf<uint32_t>("ws_hscan", nlog);
+        run_heads<<<grid_for(nlog, 256), 256, 0, c->stream>>>(log, perm, head, nlog);
+        PMA_LAUNCHED(c);
+        uint32_t runs = (uint32_t)exclusive_scan_u32(c, head, hscan, nlog);
+        uint32_t* start = c->buf<uint32_t>("ws_run_start", runs);
+        run_starts<<<grid_for(nlog, 256), 256, 0, c->stream>>>(head, hscan, start, nlog);
+        PMA_LAUNCHED(c);
+        uint32_t* pcnt = c->buf<uint32_t>("ws_pcnt", runs);
+        uint32_t* poff = c->buf<uint32_t>("ws_poff", runs);
+        uint32_t* of = c->buf<uint32_t>("ws_overflow", 1);
+        PMA_HIP(hipMemsetAsync(of, 0, 4, c->stream));
+        count_prefixes<<<grid_for(runs, 256), 256, 0, c->stream>>>(log, perm, start, runs, nlog, label, pcnt, of);
+        PMA_LAUNCHED(c);
+        uint64_t R = exclusive_scan_u32(c, pcnt, poff, runs);
+        PMA_REQUIRE(read_scalar(c, of) == 0, PMA_ERR_INTERNAL, "pma_watershed: a voxel touches more than 9 pores");
+        if (R) {
+            uint64_t* w0 = c->buf<uint64_t>("ws_w0", R);
+            uint64_t* w1 = c->buf<uint64_t>("ws_w1", R);
+            uint64_t* w2 = c->buf<uint64_t>("ws_w2", R);
+            uint64_t* tm = c->buf<uint64_t>("ws_tm", R);
+            uint32_t* fin = c->buf<uint32_t>("ws_fin", R);
+            uint32_t* vox = c->buf<uint32_t>("ws_vox", R);
+            uint32_t* rd = c->buf<uint32_t>("ws_rd", R);
+            write_prefixes<<<grid_for(runs, 256), 256, 0, c->stream>>>(log, perm, start, runs, nlog, label, poff, w0, w1,
+                                                                       w2, tm, fin, vox, shared_owned);
+            PMA_LAUNCHED(c);
+            ListTable lt = dedupe_lists(c, w0, w1, w2, tm, fin, R, rd);
+            std::vector<uint32_t> order(lt.distinct);
+            for (uint32_t i = 0; i < lt.distinct; ++i) order[i] = i;
+            std::sort(order.begin(), order.end(), [&](uint32_t a2, uint32_t b2) { return lt.min_time[a2] < lt.min_time[b2]; });
+            std::vector<uint32_t> key_id(lt.distinct);
+            throats.resize(lt.distinct);
+            for (uint32_t r = 0; r < lt.distinct; ++r) {
+                uint32_t dix = order[r];
+                key_id[dix] = 0xFFFFFFFFu - 1u - r;
+                ThroatRow& t = throats[lt.distinct - 1 - r];
+                t.id = key_id[dix];
+                t.volume = lt.finals[dix];
+                for (uint32_t q : list_members(&lt.words[3 * (size_t)dix])) t.pores.push_back(pore_id[q]);
+            }
+            // owners first, then the shared voxels
+            uint32_t* d_pid = c->buf<uint32_t>("ws_pore_id", std::max(np, 1u));
+            uint32_t* d_kid = c->buf<uint32_t>("ws_key_id", lt.distinct);
+            PMA_HIP(hipMemcpyAsync(d_pid, pore_id.data(), np * 4, hipMemcpyHostToDevice, c->stream));
+            PMA_HIP(hipMemcpyAsync(d_kid, key_id.data(), lt.distinct * 4ull, hipMemcpyHostToDevice, c->stream));
+            // volumes and boundary flags need the pore-index labels: compute them before relabelling
+            c->pores_ws = c->pores;
+            {
+                unsigned long long* owned = c->buf<unsigned long long>("ws_owned", std::max(np, 1u));
+                uint32_t* bflag = c->buf<uint32_t>("ws_bflag", std::max(np, 1u));
+                PMA_HIP(hipMemsetAsync(owned, 0, std::max(np, 1u) * 8ull, c->stream));
+                PMA_HIP(hipMemsetAsync(bflag, 0, std::max(np, 1u) * 4ull, c->stream));
+                owned_counts<<<grid_for(n, 256), 256, 0, c->stream>>>(label, n, owned);
+                PMA_LAUNCHED(c);
+                boundary_owners<<<grid_for(n, 256), 256, 0, c->stream>>>(label, X, Y, Z, bflag);
+                PMA_LAUNCHED(c);
+                std::vector<unsigned long long> ow(np);
+                std::vector<uint32_t> bf(np), so(np);
+                PMA_HIP(hipMemcpyAsync(ow.data(), owned, np * 8ull, hipMemcpyDeviceToHost, c->stream));
+                PMA_HIP(hipMemcpyAsync(bf.data(), bflag, np * 4ull, hipMemcpyDeviceToHost, c->stream));
+                PMA_HIP(hipMemcpyAsync(so.data(), shared_owned, np * 4ull, hipMemcpyDeviceToHost, c->stream));
+                PMA_HIP(hipStreamSynchronize(c->stream));
+                for (uint32_t k = 0; k < np; ++k) {
+                    c->pores_ws[k].volume = ow[k] - so[k];
+                    c->pores_ws[k].boundary = bf[k] != 0;
+                }
+            }
+            labels_to_ids<<<grid_for(n, 256), 256, 0, c->stream>>>(label, n, d_pid);
+            PMA_LAUNCHED(c);
+            shared_ids<<<grid_for(R, 256), 256, 0, c->stream>>>(label, vox, fin, rd, d_kid, R);
+            PMA_LAUNCHED(c);
+            PMA_HIP(hipStreamSynchronize(c->stream));
+            c->throats_ws = throats;
+            c->have_pores_ws = true;
+            c->valid[PMA_VOL_PTW] = true;
+            c->stage[PMA_STAGE_WATERSHED].work_items = rounds;
+            return true;
+        }
+    }
+    // no shared voxels
+    c->pores_ws = c->pores;
+    {
+        unsigned long long* owned = c->buf<unsigned long long>("ws_owned", std::max(np, 1u));
+        uint32_t* bflag = c->buf<uint32_t>("ws_bflag", std::max(np, 1u));
+        PMA_HIP(hipMemsetAsync(owned, 0, std::max(np, 1u) * 8ull, c->stream));
+        PMA_HIP(hipMemsetAsync(bflag, 0, std::max(np, 1u) * 4ull, c->stream));
+        owned_counts<<<grid_for(n, 256), 256, 0, c->stream>>>(label, n, owned);
+        PMA_LAUNCHED(c);
+        boundary_owners<<<grid_for(n, 256), 256, 0, c->stream>>>(label, X, Y, Z, bflag);
+        PMA_LAUNCHED(c);
+        std::vector<unsigned long long> ow(std::max(np, 1u));
+        std::vector<uint32_t> bf(std::max(np, 1u));
+        PMA_HIP(hipMemcpyAsync(ow.data(), owned, np * 8ull, hipMemcpyDeviceToHost, c->stream));
+        PMA_HIP(hipMemcpyAsync(bf.data(), bflag, np * 4ull, hipMemcpyDeviceToHost, c->stream));
+        PMA_HIP(hipStreamSynchronize(c->stream));
+        for (uint32_t k = 0; k < np; ++k) {
+            c->pores_ws[k].volume = ow[k];
+            c->pores_ws[k].boundary = bf[k] != 0;
+        }
+        uint32_t* d_pid = c->buf<uint32_t>("ws_pore_id", std::max(np, 1u));
+        if (np) PMA_HIP(hipMemcpyAsync(d_pid, pore_id.data(), np * 4, hipMemcpyHostToDevice, c->stream));
+        labels_to_ids<<<grid_for(n, 256), 256, 0, c->stream>>>(label, n, d_pid);
+        PMA_LAUNCHED(c);
+        PMA_HIP(hipStreamSynchronize(c->stream));
+    }
+    c->throats_ws.clear();
+    c->have_pores_ws = true;
+    c->valid[PMA_VOL_PTW] = true;
+    c->stage[PMA_STAGE_WATERSHED].work_items = rounds;
+    return true;
+}
+}  // namespace
+
+}  // namespace pma
