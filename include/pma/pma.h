@@ -88,7 +88,9 @@ typedef struct pma_stage_info {
     double device_ms;     /* events around the stage on the context's stream            */
     double host_ms;       /* wall time of the entry point                                */
     uint64_t launches;    /* kernel launches issued by the stage                          */
-    uint64_t work_items;  /* stage-specific work counter (see DESIGN.md of the project)  */
+    uint64_t work_items;  /* voxels (EDT), candidate centres (containing balls), DFS replay
+                             frames (pores), folded IDs (removal), flood rounds (watershed),
+                             frontier relaxations (geodesic)                                */
 } pma_stage_info;
 
 /* ---- context --------------------------------------------------------------------- */
@@ -145,6 +147,13 @@ pma_status pma_get_text(pma_ctx* ctx, pma_text which, char* dst, size_t cap, siz
  * intermediateStatistics.txt), as the reference's later stages read earlier files. */
 pma_status pma_set_text(pma_ctx* ctx, pma_text which, const char* text);
 pma_status pma_get_stage_info(pma_ctx* ctx, pma_stage stage, pma_stage_info* info);
+/* Per-kernel device time: while enabled, every launch of an instrumented kernel is bracketed
+ * by HIP events on the context's stream; totals accumulate until reset (enabled = 0 resets).
+ * For roofline figures (achieved bytes / average launch duration). */
+pma_status pma_set_kernel_timing(pma_ctx* ctx, int enabled);
+pma_status pma_get_kernel_time(pma_ctx* ctx, const char* kernel, double* total_ms, uint64_t* launches);
+/* Names of the instrumented kernels, separated by '\n' (NUL-terminated; *len = strlen + 1). */
+pma_status pma_list_kernel_times(pma_ctx* ctx, char* dst, size_t cap, size_t* len);
 
 /* ---- synthetic inputs (written into SOLID; 0 = void, 255 = solid) --------------------- */
 /* Union of n spheres with centres uniform in the box and radii uniform in [rmin, rmax]. */

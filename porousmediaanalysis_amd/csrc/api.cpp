@@ -84,6 +84,7 @@ void timed(pma_ctx* c, pma_stage s, void (*fn)(pma_ctx*)) {
     c->stage[s].device_ms = ms;
     c->stage[s].host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     c->stage[s].launches = c->launches - l0;
+    c->resolve_kernel_times();
 }
 
 std::string format_pores(const std::vector<PoreRow>& rows) {
@@ -234,6 +235,38 @@ void pma_ctx::ensure_volume(pma_volume w) { vol[w].reserve(n * volume_element_si
 
 void pma_ctx::sync() { PMA_HIP(hipStreamSynchronize(stream)); }
 
+hipEvent_t pma_ctx::kernel_mark() {
+    hipEvent_t e;
+    if (!event_pool.empty()) {
+        e = event_pool.back();
+        event_pool.pop_back();
+    } else {
+        PMA_HIP(hipEventCreate(&e));
+    }
+    PMA_HIP(hipEventRecord(e, stream));
+    return e;
+}
+
+void pma_ctx::kernel_done(const char* name, hipEvent_t start) {
+    hipEvent_t end = kernel_mark();
+    kernel_pending.push_back({name, {start, end}});
+}
+
+void pma_ctx::resolve_kernel_times() {
+    if (kernel_pending.empty()) return;
+    PMA_HIP(hipStreamSynchronize(stream));
+    for (auto& k : kernel_pending) {
+        float ms = 0;
+        PMA_HIP(hipEventElapsedTime(&ms, k.second.first, k.second.second));
+        KernelTotal& t = kernel_totals[k.first];
+        t.ms += ms;
+        t.launches++;
+        event_pool.push_back(k.second.first);
+        event_pool.push_back(k.second.second);
+    }
+    kernel_pending.clear();
+}
+
 extern "C" {
 
 const char* pma_last_error(void) { return g_last_error.c_str(); }
@@ -267,6 +300,11 @@ void pma_destroy(pma_ctx* c) {
     for (auto& b : c->vol) b.release();
     for (auto& kv : c->scratch) kv.second.release();
     if (c->pinned) (void)hipHostFree(c->pinned);
+    for (auto& k : c->kernel_pending) {
+        (void)hipEventDestroy(k.second.first);
+        (void)hipEventDestroy(k.second.second);
+    }
+    for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -428,6 +466,39 @@ pma_status pma_get_stage_info(pma_ctx* c, pma_stage s, pma_stage_info* info) {
         check_ctx(c);
         PMA_REQUIRE((int)s >= 0 && (int)s < PMA_STAGE_COUNT && info, PMA_ERR_INVALID_ARGUMENT, "bad stage or pointer");
         *info = c->stage[s];
+    });
+}
+
+pma_status pma_set_kernel_timing(pma_ctx* c, int enabled) {
+    return guard([&] {
+        check_ctx(c);
+        c->resolve_kernel_times();
+        c->kernel_timing = enabled != 0;
+        c->kernel_totals.clear();
+    });
+}
+
+pma_status pma_get_kernel_time(pma_ctx* c, const char* kernel, double* total_ms, uint64_t* launches) {
+    return guard([&] {
+        check_ctx(c);
+        PMA_REQUIRE(kernel != nullptr, PMA_ERR_INVALID_ARGUMENT, "null kernel name");
+        c->resolve_kernel_times();
+        auto it = c->kernel_totals.find(kernel);
+        if (total_ms) *total_ms = it == c->kernel_totals.end() ? 0.0 : it->second.ms;
+        if (launches) *launches = it == c->kernel_totals.end() ? 0 : it->second.launches;
+    });
+}
+
+pma_status pma_list_kernel_times(pma_ctx* c, char* dst, size_t cap, size_t* len) {
+    return guard([&] {
+        check_ctx(c);
+        PMA_REQUIRE(len != nullptr, PMA_ERR_INVALID_ARGUMENT, "null length pointer");
+        c->resolve_kernel_times();
+        std::string s;
+        for (const auto& kv : c->kernel_totals) s += kv.first + "\n";
+        *len = s.size() + 1;
+        PMA_REQUIRE(dst != nullptr && cap >= s.size() + 1, PMA_ERR_CAPACITY, "buffer too small");
+        std::memcpy(dst, s.c_str(), s.size() + 1);
     });
 }
 

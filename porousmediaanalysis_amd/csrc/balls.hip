@@ -270,24 +270,36 @@ void run_containing_balls(pma_ctx* c) {
     const uint32_t* e = c->v<uint32_t>(PMA_VOL_EDT2);
     uint8_t* mask = c->v<uint8_t>(PMA_VOL_MASK);
     dim3 vgrid(grid_for(Y, 64), grid_for(X, 4), Z);
-    prune_centres<<<vgrid, 256, 0, c->stream>>>(e, mask, far, (uint32_t)L, tile_count, X, Y, Z, NTX, NTY,
-                                                 c->stencil == PMA_STENCIL_D3Q7);
-    PMA_LAUNCHED(c);
+    {
+        PMA_TIMED(c, "prune_centres");
+        prune_centres<<<vgrid, 256, 0, c->stream>>>(e, mask, far, (uint32_t)L, tile_count, X, Y, Z, NTX, NTY,
+                                                     c->stencil == PMA_STENCIL_D3Q7);
+        PMA_LAUNCHED(c);
+    }
     uint64_t n_cand = exclusive_scan_u32(c, tile_count, tile_off, NT);
     Candidate* cand = c->buf<Candidate>("cb_candidates", std::max<uint64_t>(n_cand, 1));
-    emit_centres<<<vgrid, 256, 0, c->stream>>>(e, mask, tile_off, tile_cursor, cand, X, Y, NTX, NTY);
-    PMA_LAUNCHED(c);
+    {
+        PMA_TIMED(c, "emit_centres");
+        emit_centres<<<vgrid, 256, 0, c->stream>>>(e, mask, tile_off, tile_cursor, cand, X, Y, NTX, NTY);
+        PMA_LAUNCHED(c);
+    }
     uint32_t* err = c->buf<uint32_t>("cb_error", 1);
     PMA_HIP(hipMemsetAsync(err, 0, sizeof(uint32_t), c->stream));
-    scatter_tiles<<<(unsigned)NT, 256, 0, c->stream>>>(cand, tile_off, tile_count, e, c->v<uint32_t>(PMA_VOL_CR2),
-                                                        c->v<uint32_t>(PMA_VOL_CX), c->v<uint32_t>(PMA_VOL_CY),
-                                                        c->v<uint32_t>(PMA_VOL_CZ), mask, X, Y, Z, NTX, NTY, NTZ,
-                                                        R, err);
-    PMA_LAUNCHED(c);
+    {
+        PMA_TIMED(c, "scatter_tiles");
+        scatter_tiles<<<(unsigned)NT, 256, 0, c->stream>>>(cand, tile_off, tile_count, e, c->v<uint32_t>(PMA_VOL_CR2),
+                                                            c->v<uint32_t>(PMA_VOL_CX), c->v<uint32_t>(PMA_VOL_CY),
+                                                            c->v<uint32_t>(PMA_VOL_CZ), mask, X, Y, Z, NTX, NTY, NTZ,
+                                                            R, err);
+        PMA_LAUNCHED(c);
+    }
     unsigned long long* cnt = c->buf<unsigned long long>("cb_count", 1);
     PMA_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), c->stream));
-    count_equal_u8<<<1024, 256, 0, c->stream>>>(mask, c->n, 255, cnt);
-    PMA_LAUNCHED(c);
+    {
+        PMA_TIMED(c, "count_equal_u8");
+        count_equal_u8<<<1024, 256, 0, c->stream>>>(mask, c->n, 255, cnt);
+        PMA_LAUNCHED(c);
+    }
     PMA_REQUIRE(read_scalar(c, err) == 0, PMA_ERR_INTERNAL, "pma_containing_balls: a void voxel was left uncovered");
     c->n_max_balls = read_scalar(c, cnt);
     c->stage[PMA_STAGE_CONTAINING_BALLS].work_items = n_cand;

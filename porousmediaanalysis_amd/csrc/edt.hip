@@ -127,12 +127,25 @@ void run_edt(pma_ctx* c) {
     PMA_HIP(hipMemsetAsync(hist, 0, hist_len * sizeof(unsigned long long), c->stream));
     PMA_HIP(hipMemsetAsync(dmax, 0, sizeof(uint32_t), c->stream));
 
-    edt_pass_x<<<dim3(grid_for(Y, 256), Z), 256, 0, c->stream>>>(solid, e, X, Y);
-    PMA_LAUNCHED(c);
-    edt_pass_y<<<dim3(X, Z), 256, (size_t)Y * sizeof(uint32_t), c->stream>>>(e, tmp, X, Y);
-    PMA_LAUNCHED(c);
-    edt_pass_z<<<dim3(grid_for(Y, 256), X), 256, 0, c->stream>>>(tmp, e, X, Y, Z, hist, hist_len, dmax);
-    PMA_LAUNCHED(c);
+    {
+
+        PMA_TIMED(c, "edt_pass_x");
+
+        edt_pass_x<<<dim3(grid_for(Y, 256), Z), 256, 0, c->stream>>>(solid, e, X, Y);
+
+        PMA_LAUNCHED(c);
+
+    }
+    {
+        PMA_TIMED(c, "edt_pass_y");
+        edt_pass_y<<<dim3(X, Z), 256, (size_t)Y * sizeof(uint32_t), c->stream>>>(e, tmp, X, Y);
+        PMA_LAUNCHED(c);
+    }
+    {
+        PMA_TIMED(c, "edt_pass_z");
+        edt_pass_z<<<dim3(grid_for(Y, 256), X), 256, 0, c->stream>>>(tmp, e, X, Y, Z, hist, hist_len, dmax);
+        PMA_LAUNCHED(c);
+    }
 
     uint32_t m = read_scalar(c, dmax);
     c->max_e = m;

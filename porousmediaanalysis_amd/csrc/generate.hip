@@ -126,7 +126,7 @@ void run_generate_spheres(pma_ctx* c, int32_t n, float rmin, float rmax, uint64_
     }
     c->ensure_volume(PMA_VOL_SOLID);
     Sphere* d = c->buf<Sphere>("gen_spheres", std::max(n, 1));
-    if (n) PMA_HIP(hipMemcpyAsync(d, s.data(), n * sizeof(Sphere), hipMemcpyHostToDevice, c->stream));
+    upload(c, d, s.data(), n * sizeof(Sphere));
     spheres_kernel<<<grid_for(c->n, 256), 256, std::max(n, 1) * sizeof(Sphere), c->stream>>>(
         c->v<uint8_t>(PMA_VOL_SOLID), d, n, c->dims.x, c->dims.y, c->dims.z);
     PMA_LAUNCHED(c);
@@ -147,7 +147,7 @@ void run_generate_gaussian(pma_ctx* c, float sigma, float porosity, uint64_t see
     for (int k = -radius; k <= radius; ++k) sum += (w[k + radius] = (float)std::exp(-0.5 * k * k / ((double)sigma * sigma)));
     for (float& v : w) v = (float)(v / sum);
     float* dw = c->buf<float>("gen_weights", w.size());
-    PMA_HIP(hipMemcpyAsync(dw, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    upload(c, dw, w.data(), w.size() * sizeof(float));
     noise_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(a, n, seed);
     PMA_LAUNCHED(c);
     blur_axis<<<grid_for(n, 256), 256, 0, c->stream>>>(a, b, dw, radius, c->dims.x, c->dims.y, c->dims.z, 1);

@@ -235,17 +235,23 @@ void run_geodesic(pma_ctx* c) {
         uint32_t* nxt = fb;
         while (nf > 0) {
             PMA_HIP(hipMemsetAsync(cnt + 1, 0, sizeof(uint32_t), c->stream));
-            relax<<<grid_for(nf, 256), 256, 0, c->stream>>>(solid, dist, cur, nf, nxt, cnt + 1, stamp, round++, X, Y,
-                                                            Z, d3q7);
-            PMA_LAUNCHED(c);
+            {
+                PMA_TIMED(c, "relax");
+                relax<<<grid_for(nf, 256), 256, 0, c->stream>>>(solid, dist, cur, nf, nxt, cnt + 1, stamp, round++, X, Y,
+                                                                Z, d3q7);
+                PMA_LAUNCHED(c);
+            }
             relaxations += nf;
             nf = read_scalar(c, cnt + 1);
             std::swap(cur, nxt);
         }
         PMA_HIP(hipMemsetAsync(vcount, 0, sizeof(unsigned long long), c->stream));
         PMA_HIP(hipMemsetAsync(walls, 0, sizeof(uint32_t), c->stream));
-        finish_cavity<<<grid_for(n, 256), 256, 0, c->stream>>>(solid, dist, cav, id, X, Y, Z, vcount, walls);
-        PMA_LAUNCHED(c);
+        {
+            PMA_TIMED(c, "finish_cavity");
+            finish_cavity<<<grid_for(n, 256), 256, 0, c->stream>>>(solid, dist, cav, id, X, Y, Z, vcount, walls);
+            PMA_LAUNCHED(c);
+        }
         CavityRow row;
         row.id = id;
         row.x = start[0];

@@ -107,6 +107,19 @@ struct pma_ctx {
     uint64_t launches = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
+    // per-kernel event timing (pma_set_kernel_timing)
+    bool kernel_timing = false;
+    struct KernelTotal {
+        double ms = 0;
+        uint64_t launches = 0;
+    };
+    std::map<std::string, KernelTotal> kernel_totals;
+    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> kernel_pending;
+    std::vector<hipEvent_t> event_pool;
+    hipEvent_t kernel_mark();                      // records an event on the stream
+    void kernel_done(const char* name, hipEvent_t start);  // records the end event
+    void resolve_kernel_times();                   // after a synchronisation
+
     template <class T> T* v(pma_volume w) { return vol[w].as<T>(); }
     void* buf(const std::string& name, size_t bytes) {
         pma::DeviceBuffer& b = scratch[name];
@@ -164,6 +177,13 @@ ListTable dedupe_lists(pma_ctx* c, const uint64_t* w0, const uint64_t* w1, const
                        const uint32_t* weight, size_t R, uint32_t* rec_distinct);
 std::vector<uint32_t> list_members(const uint64_t* w);
 
+// Host -> device copy that returns only when the source may be released.
+inline void upload(pma_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (bytes == 0) return;
+    PMA_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    PMA_HIP(hipStreamSynchronize(c->stream));
+}
+
 // Copies one value from device memory through pinned staging.
 template <class T> T read_scalar(pma_ctx* c, const T* dptr) {
     T* h = static_cast<T*>(c->host_staging(sizeof(T)));
@@ -173,5 +193,19 @@ template <class T> T read_scalar(pma_ctx* c, const T* dptr) {
 }
 
 inline unsigned grid_for(size_t n, unsigned block) { return (unsigned)((n + block - 1) / block); }
+
+// Brackets one kernel launch with events when kernel timing is enabled.
+struct KernelScope {
+    pma_ctx* c;
+    const char* name;
+    hipEvent_t start = nullptr;
+    KernelScope(pma_ctx* ctx, const char* n) : c(ctx), name(n) {
+        if (c->kernel_timing) start = c->kernel_mark();
+    }
+    ~KernelScope() {
+        if (start) c->kernel_done(name, start);
+    }
+};
+#define PMA_TIMED(ctx, name) ::pma::KernelScope pma_scope_##__LINE__((ctx), (name))
 
 }  // namespace pma

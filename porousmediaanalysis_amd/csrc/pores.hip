@@ -217,8 +217,7 @@ __global__ void plateau_of_voxels(uint32_t* root, const uint32_t* __restrict__ p
     if (i >= (size_t)X * Y * Z) return;
     uint32_t r = root[i];
     if (r == kInv) return;
-    uint32_t q = pidx[r];
-    __syncthreads();  // every root read in this block happens before any write below
+    uint32_t q = pidx[r];  // only this thread reads or writes root[i]
     root[i] = q;
     atomicAdd(&plat_size[q], 1u);
     int x, y, z;
@@ -351,6 +350,7 @@ __global__ void set_union_level(const uint32_t* __restrict__ order, uint32_t lo,
         for (uint32_t t = 0; t < sl; ++t) insert(pool[so + t]);
     }
     if (of) atomicOr(overflow, 1u);
+    if (m > kListMax) atomicOr(overflow, 4u);
     unsigned long long o = atomicAdd(pool_top, (unsigned long long)m);
     if (o + m > pool_cap) {
         atomicOr(overflow, 2u);
@@ -409,7 +409,7 @@ __global__ void reach_sizes(const uint32_t* __restrict__ set_off, const uint32_t
 struct ReplayTask {
     uint32_t pore, seed, ncls, pad;
     unsigned long long stack_off;
-    unsigned long long event_local_off;  // first bit of this pore's seen bitmap in LDS (0)
+    unsigned long long stack_cap;  // frames available to this pore (its reach + 1)
 };
 
 __device__ __forceinline__ int member_index(const uint32_t* __restrict__ pool, uint32_t off, uint32_t len, uint32_t p) {
@@ -423,10 +423,13 @@ __global__ void __launch_bounds__(64) dfs_replay(const ReplayTask* __restrict__ 
                                                  const uint32_t* __restrict__ set_len, const uint32_t* __restrict__ pool,
                                                  const uint32_t* __restrict__ rbase,
                                                  const uint32_t* __restrict__ rec_distinct,
-                                                 const uint32_t* __restrict__ event_local, uint32_t* __restrict__ visited,
-                                                 uint32_t* __restrict__ stack_voxel, uint8_t* __restrict__ stack_k,
-                                                 uint32_t* __restrict__ rank, int X, int Y, int Z, int d3q7,
-                                                 uint32_t* __restrict__ error) {
+                                                 const uint32_t* __restrict__ event_local, uint32_t* visited,
+                                                 uint32_t* stack_voxel, uint8_t* stack_k, uint32_t* __restrict__ rank,
+                                                 int X, int Y, int Z, int d3q7, uint32_t* __restrict__ error) {
+    // One wave per pore.  Lane 0 owns the stack, the seen-class bitmap and the counters and
+    // broadcasts every decision; the other lanes evaluate the 26 neighbours in parallel.
+    // Frames are (voxel, next neighbour index); a recursive DFS that tries neighbours in
+    // reverse loop order visits voxels in the reference's stack-DFS order.
     __shared__ uint32_t seen[1024];  // up to 32768 classes per pore
     const ReplayTask task = tasks[blockIdx.x];
     const int lane = threadIdx.x;
@@ -439,109 +442,84 @@ __global__ void __launch_bounds__(64) dfs_replay(const ReplayTask* __restrict__ 
     __syncthreads();
     uint32_t* sv = stack_voxel + task.stack_off;
     uint8_t* sk = stack_k + task.stack_off;
-    uint32_t found = 0;
-    int top = -1;
 
-    // visit(v): mark, record the class, push a frame.  Returns false on a broken invariant.
-    auto visit = [&](uint32_t v) -> bool {
+    // Marks v visited by p and, if its earlier-pore class is new, ranks that class.
+    // Executed by lane 0 only; returns the new count of classes found.
+    auto visit = [&](uint32_t v, uint32_t found) -> uint32_t {
         const uint32_t q = plat[v];
-        const uint32_t so = set_off[q], sl = set_len[q];
-        const int j = member_index(pool, so, sl, p);
-        if (j < 0) return false;
-        if (lane == 0) {
-            atomicOr(&visited[v], 1u << j);
-            if (j >= 1) {
-                const uint32_t e = rec_distinct[rbase[q] + (uint32_t)j - 1];
-                const uint32_t l = event_local[e];
-                if (!(seen[l >> 5] & (1u << (l & 31)))) {
-                    seen[l >> 5] |= 1u << (l & 31);
-                    rank[e] = found;
-                }
-            }
-        }
-        __syncthreads();
-        // every lane recomputes 'found' from the shared bitmap owner decision
-        return true;
-    };
-    (void)visit;
-
-    // Lane 0 keeps the authoritative state; other lanes evaluate neighbours.
-    auto class_of = [&](uint32_t v, int& j) -> uint32_t {
-        const uint32_t q = plat[v];
-        j = member_index(pool, set_off[q], set_len[q], p);
-        if (j >= 1) return rec_distinct[rbase[q] + (uint32_t)j - 1];
-        return kInv;
-    };
-
-    {
-        int j;
-        uint32_t e = class_of(task.seed, j);
+        const int j = member_index(pool, set_off[q], set_len[q], p);
         if (j < 0) {
-            if (lane == 0) atomicOr(error, 1u);
-            return;
+            atomicOr(error, 1u);
+            return 0xFFFFFFFFu;
         }
-        if (lane == 0) atomicOr(&visited[task.seed], 1u << j);
-        if (e != kInv) {
-            uint32_t l = event_local[e];
-            if (lane == 0) {
+        if (j >= 32) {
+            atomicOr(error, 32u);
+            return 0xFFFFFFFFu;
+        }
+        atomicOr(&visited[v], 1u << j);
+        if (j >= 1) {
+            const uint32_t e = rec_distinct[rbase[q] + (uint32_t)j - 1];
+            const uint32_t l = event_local[e];
+            if (!(seen[l >> 5] & (1u << (l & 31)))) {
                 seen[l >> 5] |= 1u << (l & 31);
                 rank[e] = found;
+                ++found;
             }
-            ++found;
         }
-        top = 0;
-        if (lane == 0) {
-            sv[0] = task.seed;
-            sk[0] = 27;
-        }
-    }
-    __syncthreads();
-    while (top >= 0 && found < task.ncls) {
-        const uint32_t u = sv[top];
-        const int k = sk[top];
+        return found;
+    };
+
+    uint32_t found = 0;
+    if (lane == 0) found = visit(task.seed, 0);
+    found = (uint32_t)__shfl((int)found, 0);
+    if (found == 0xFFFFFFFFu) return;
+    int top = 0;
+    uint32_t u = task.seed;
+    int k = 27;
+    while (found < task.ncls) {
         int x, y, z;
         decode(u, X, Y, x, y, z);
         const uint32_t cu = cr[u];
         bool cand = false;
         uint32_t j = 0;
+        int b = -1;
         if (lane < k && lane < 27 && in_stencil(lane, d3q7) && neighbour(lane, x, y, z, X, Y, Z, j)) {
             const uint32_t cn = cr[j];
             if (cn != 0 && cn <= cu) {
                 const uint32_t q = plat[j];
-                const int b = member_index(pool, set_off[q], set_len[q], p);
-                cand = b >= 0 && !(visited[j] & (1u << b));
-                if (b < 0) atomicOr(error, 2u);
+                b = member_index(pool, set_off[q], set_len[q], p);
+                if (b < 0 || b >= 32) atomicOr(error, 2u), b = -1;
+                if (b >= 0) cand = !(__hip_atomic_load(&visited[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & (1u << b));
             }
         }
         const unsigned long long bal = __ballot(cand);
         if (!bal) {
-            --top;  // pop the frame
+            if (--top < 0) break;  // pop; the frame below is read back by lane 0
+            uint32_t fu = 0, fk = 0;
+            if (lane == 0) {
+                fu = sv[top];
+                fk = sk[top];
+            }
+            u = (uint32_t)__shfl((int)fu, 0);
+            k = __shfl((int)fk, 0);
             continue;
         }
-        const int kk = 63 - __clzll(bal);  // highest index below k: reverse loop order
-        const uint32_t n = (uint32_t)__shfl((int)j, kk);
-        int b;
-        const uint32_t e = class_of(n, b);
+        if ((unsigned long long)top + 1 >= task.stack_cap) {  // more frames than the pore's reach
+            if (lane == 0) atomicOr(error, 16u);
+            return;
+        }
+        const int kk = 63 - __clzll((long long)bal);  // highest neighbour index below k
+        const uint32_t nvox = (uint32_t)__shfl((int)j, kk);
         if (lane == 0) {
+            sv[top] = u;  // save this frame, resuming below kk
             sk[top] = (uint8_t)kk;
-            atomicOr(&visited[n], 1u << b);
-            sv[top + 1] = n;
-            sk[top + 1] = 27;
+            found = visit(nvox, found);
         }
-        if (e != kInv) {
-            const uint32_t l = event_local[e];
-            const bool is_new = !(seen[l >> 5] & (1u << (l & 31)));
-            __syncthreads();
-            if (is_new) {
-                if (lane == 0) {
-                    seen[l >> 5] |= 1u << (l & 31);
-                    rank[e] = found;
-                }
-                ++found;
-            }
-        }
+        found = (uint32_t)__shfl((int)found, 0);
+        if (found == 0xFFFFFFFFu) return;
         ++top;
-        __syncthreads();
+        u = nvox;
+        k = 27;
     }
 }
 
@@ -568,6 +546,21 @@ __global__ void single_volumes(const uint32_t* __restrict__ set_off, const uint3
                                unsigned long long* __restrict__ vol) {
     size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q < P && set_len[q] == 1) atomicAdd(&vol[pool[set_off[q]]], (unsigned long long)plat_size[q]);
+}
+
+__global__ void seed_plateaus(const uint32_t* __restrict__ plat, const uint32_t* __restrict__ seed_voxel, uint32_t n,
+                              size_t nvox, uint32_t nplat, uint32_t* __restrict__ out, uint32_t* __restrict__ plat_pore,
+                              uint32_t* __restrict__ error) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t v = seed_voxel[k];
+    const uint32_t q = v < nvox ? plat[v] : kInv;
+    if (q >= nplat) {
+        atomicOr(error, 8u);
+        return;
+    }
+    out[k] = q;
+    plat_pore[q] = k;
 }
 
 __global__ void fill_u32(uint32_t* p, uint32_t v, size_t n) {
@@ -609,14 +602,20 @@ void run_pores(pma_ctx* c) {
     uint32_t* seedpos = c->buf<uint32_t>("pt_seedpos", n);
     cc_init<<<nb, 256, 0, c->stream>>>(cr, n, root);
     PMA_LAUNCHED(c);
-    cc_union<<<nb, 256, 0, c->stream>>>(cr, root, X, Y, Z, d3q7);
-    PMA_LAUNCHED(c);
+    {
+        PMA_TIMED(c, "cc_union");
+        cc_union<<<nb, 256, 0, c->stream>>>(cr, root, X, Y, Z, d3q7);
+        PMA_LAUNCHED(c);
+    }
     cc_flatten<<<nb, 256, 0, c->stream>>>(root, n);
     PMA_LAUNCHED(c);
     PMA_HIP(hipMemsetAsync(not_max, 0, n, c->stream));
     PMA_HIP(hipMemsetAsync(seedpos, 0xFF, n * sizeof(uint32_t), c->stream));
-    cc_not_max<<<nb, 256, 0, c->stream>>>(cr, root, not_max, X, Y, Z, d3q7);
-    PMA_LAUNCHED(c);
+    {
+        PMA_TIMED(c, "cc_not_max");
+        cc_not_max<<<nb, 256, 0, c->stream>>>(cr, root, not_max, X, Y, Z, d3q7);
+        PMA_LAUNCHED(c);
+    }
 
     // 3. seeds: the first sorted maximum ball of every regional-maximum plateau.
     uint32_t npores = 0;
@@ -654,19 +653,7 @@ void run_pores(pma_ctx* c) {
     PMA_HIP(hipMemsetAsync(plat_border, 0, P1 * 4, c->stream));
     fill_u32<<<grid_for(P1, 256), 256, 0, c->stream>>>(plat_pore, kInv, P1);
     PMA_LAUNCHED(c);
-    // pore of each seed plateau (host knows the seed voxels)
     PMA_HIP(hipStreamSynchronize(c->stream));
-    std::vector<uint32_t> seed_root(npores), seed_plat(npores);
-    {
-        // read root and plateau index of every seed voxel
-        std::vector<uint32_t> tmp(npores);
-        uint32_t* d_sv = c->buf<uint32_t>("pt_seed_voxels", std::max(npores, 1u));
-        for (uint32_t k = 0; k < npores; ++k) tmp[k] = (uint32_t)seeds_h[k];
-        for (uint32_t k = 0; k < npores; ++k) {
-            seed_root[k] = 0;
-        }
-        (void)d_sv;
-    }
     plateau_of_voxels<<<nb, 256, 0, c->stream>>>(root, pidx, X, Y, Z, plat_size, plat_border);
     PMA_LAUNCHED(c);
     uint32_t* plat = root;  // now the plateau index of every void voxel
@@ -675,23 +662,14 @@ void run_pores(pma_ctx* c) {
         std::vector<uint32_t> sv(npores);
         for (uint32_t k = 0; k < npores; ++k) sv[k] = (uint32_t)seeds_h[k];
         uint32_t* d_sv = c->buf<uint32_t>("pt_seed_voxels", npores);
-        PMA_HIP(hipMemcpyAsync(d_sv, sv.data(), npores * 4ull, hipMemcpyHostToDevice, c->stream));
-        std::vector<uint32_t> sp(npores);
+        upload(c, d_sv, sv.data(), npores * 4ull);
         uint32_t* d_sp = c->buf<uint32_t>("pt_seed_plat", npores);
-        // gather plat of seeds on the device, then scatter pores
-        struct G {
-            static __global__ void gather(const uint32_t* plat, const uint32_t* sv, uint32_t n, uint32_t* out,
-                                          uint32_t* plat_pore) {
-                uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-                if (k < n) {
-                    out[k] = plat[sv[k]];
-                    plat_pore[out[k]] = k;
-                }
-            }
-        };
-        G::gather<<<grid_for(npores, 256), 256, 0, c->stream>>>(plat, d_sv, npores, d_sp, plat_pore);
+        uint32_t* err = c->buf<uint32_t>("pt_overflow", 1);
+        PMA_HIP(hipMemsetAsync(err, 0, 4, c->stream));
+        seed_plateaus<<<grid_for(npores, 256), 256, 0, c->stream>>>(plat, d_sv, npores, n, (uint32_t)P, d_sp,
+                                                                    plat_pore, err);
         PMA_LAUNCHED(c);
-        (void)sp;
+        PMA_REQUIRE(read_scalar(c, err) == 0, PMA_ERR_INTERNAL, "pma_pores: a seed lies outside the plateaus");
     }
 
     // 5. plateau DAG edges (distinct), grouped by target.
@@ -704,8 +682,11 @@ void run_pores(pma_ctx* c) {
         table = c->buf<unsigned long long>("pt_edge_table", cap);
         PMA_HIP(hipMemsetAsync(table, 0xFF, cap * 8, c->stream));
         PMA_HIP(hipMemsetAsync(of, 0, 4, c->stream));
-        plateau_edges<<<nb, 256, 0, c->stream>>>(cr, plat, X, Y, Z, d3q7, table, cap - 1, of);
-        PMA_LAUNCHED(c);
+        {
+            PMA_TIMED(c, "plateau_edges");
+            plateau_edges<<<nb, 256, 0, c->stream>>>(cr, plat, X, Y, Z, d3q7, table, cap - 1, of);
+            PMA_LAUNCHED(c);
+        }
         if (read_scalar(c, of) == 0) break;
         PMA_REQUIRE(attempt < 4, PMA_ERR_INTERNAL, "pma_pores: plateau edge table overflow");
         cap <<= 2;
@@ -768,13 +749,16 @@ void run_pores(pma_ctx* c) {
         PMA_HIP(hipMemsetAsync(of, 0, 4, c->stream));
         for (size_t l = 0; l + 1 < lstart_h.size(); ++l) {
             uint32_t lo = lstart_h[l], hi = lstart_h[l + 1];
-            set_union_level<<<grid_for(hi - lo, 64), 64, 0, c->stream>>>(order, lo, hi, in_start, in_count, edges,
-                                                                         plat_pore, set_off, set_len, pool, pool_top,
-                                                                         pool_cap, of);
-            PMA_LAUNCHED(c);
+            {
+                PMA_TIMED(c, "set_union_level");
+                set_union_level<<<grid_for(hi - lo, 64), 64, 0, c->stream>>>(order, lo, hi, in_start, in_count, edges,
+                                                                             plat_pore, set_off, set_len, pool, pool_top,
+                                                                             pool_cap, of);
+                PMA_LAUNCHED(c);
+            }
         }
         uint32_t o = read_scalar(c, of);
-        PMA_REQUIRE(!(o & 1u), PMA_ERR_INTERNAL, "pma_pores: a plateau is reached by more than 64 pores");
+        PMA_REQUIRE(!(o & 5u), PMA_ERR_INTERNAL, "pma_pores: a plateau is reached by more than 9 pores");
         if (!(o & 2u)) break;
         PMA_REQUIRE(attempt < 3, PMA_ERR_INTERNAL, "pma_pores: pore set pool overflow");
         pool_cap = read_scalar(c, pool_top) + 1024;
@@ -843,6 +827,7 @@ void run_pores(pma_ctx* c) {
             t.seed = (uint32_t)seeds_h[k];
             t.ncls = ncls[k];
             t.stack_off = stack_total;
+            t.stack_cap = reach_h[k] + 1;
             stack_total += reach_h[k] + 1;
             tasks.push_back(t);
         }
@@ -853,17 +838,20 @@ void run_pores(pma_ctx* c) {
             uint32_t* visited = c->buf<uint32_t>("pt_visited", n);
             uint32_t* stack_voxel = c->buf<uint32_t>("pt_stack_voxel", stack_total);
             uint8_t* stack_k = c->buf<uint8_t>("pt_stack_k", stack_total);
-            PMA_HIP(hipMemcpyAsync(d_tasks, tasks.data(), tasks.size() * sizeof(ReplayTask), hipMemcpyHostToDevice,
-                                   c->stream));
-            PMA_HIP(hipMemcpyAsync(d_local, ev_local.data(), D * 4ull, hipMemcpyHostToDevice, c->stream));
+            upload(c, d_tasks, tasks.data(), tasks.size() * sizeof(ReplayTask));
+            upload(c, d_local, ev_local.data(), D * 4ull);
             PMA_HIP(hipMemsetAsync(d_rank, 0, D * 4ull, c->stream));
             PMA_HIP(hipMemsetAsync(visited, 0, n * 4, c->stream));
             PMA_HIP(hipMemsetAsync(of, 0, 4, c->stream));
-            dfs_replay<<<(unsigned)tasks.size(), 64, 0, c->stream>>>(d_tasks, cr, plat, set_off, set_len, pool, rbase,
-                                                                    rd, d_local, visited, stack_voxel, stack_k, d_rank,
-                                                                    X, Y, Z, d3q7, of);
-            PMA_LAUNCHED(c);
-            PMA_REQUIRE(read_scalar(c, of) == 0, PMA_ERR_INTERNAL, "pma_pores: DFS replay invariant broken");
+            {
+                PMA_TIMED(c, "dfs_replay");
+                dfs_replay<<<(unsigned)tasks.size(), 64, 0, c->stream>>>(d_tasks, cr, plat, set_off, set_len, pool, rbase,
+                                                                        rd, d_local, visited, stack_voxel, stack_k, d_rank,
+                                                                        X, Y, Z, d3q7, of);
+                PMA_LAUNCHED(c);
+            }
+            uint32_t rerr = read_scalar(c, of);
+            PMA_REQUIRE(rerr == 0, PMA_ERR_INTERNAL, "pma_pores: DFS replay invariant broken (code " + std::to_string(rerr) + ")");
             PMA_HIP(hipMemcpyAsync(rank_h.data(), d_rank, D * 4ull, hipMemcpyDeviceToHost, c->stream));
             PMA_HIP(hipStreamSynchronize(c->stream));
         }
@@ -881,9 +869,12 @@ void run_pores(pma_ctx* c) {
 
     // 9. labels.
     uint32_t* d_kid = c->buf<uint32_t>("pt_key_id", std::max(D, 1u));
-    if (D) PMA_HIP(hipMemcpyAsync(d_kid, key_id.data(), D * 4ull, hipMemcpyHostToDevice, c->stream));
-    write_labels<<<nb, 256, 0, c->stream>>>(plat, set_off, set_len, pool, rbase, rd, d_kid, n, pt);
-    PMA_LAUNCHED(c);
+    upload(c, d_kid, key_id.data(), D * 4ull);
+    {
+        PMA_TIMED(c, "write_labels");
+        write_labels<<<nb, 256, 0, c->stream>>>(plat, set_off, set_len, pool, rbase, rd, d_kid, n, pt);
+        PMA_LAUNCHED(c);
+    }
     unsigned long long* pvol = c->buf<unsigned long long>("pt_pvol", std::max(npores, 1u));
     PMA_HIP(hipMemsetAsync(pvol, 0, std::max(npores, 1u) * 8ull, c->stream));
     if (P) {

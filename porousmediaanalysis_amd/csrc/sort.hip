@@ -260,7 +260,7 @@ void sort_like_libstdcxx_desc(pma_ctx* c, uint64_t* A, size_t M) {
     PMA_HIP(hipMemsetAsync(nleaves, 0, sizeof(uint32_t), c->stream));
     if (M <= (size_t)kThreshold) {
         uint2 one = make_uint2(0, (uint32_t)M);
-        PMA_HIP(hipMemcpyAsync(leaves, &one, sizeof one, hipMemcpyHostToDevice, c->stream));
+        upload(c, leaves, &one, sizeof one);
         insertion_sort_leaves<<<1, 64, 0, c->stream>>>(A, leaves, 1);
         PMA_LAUNCHED(c);
         PMA_HIP(hipStreamSynchronize(c->stream));
@@ -285,7 +285,7 @@ void sort_like_libstdcxx_desc(pma_ctx* c, uint64_t* A, size_t M) {
     uint32_t* rankL = c->buf<uint32_t>("sort_rankL", M);
 
     Seg root{0, (uint32_t)M, 2 * lg, 0};
-    PMA_HIP(hipMemcpyAsync(segs, &root, sizeof root, hipMemcpyHostToDevice, c->stream));
+    upload(c, segs, &root, sizeof root);
     fill_u32<<<grid_for(M, 256), 256, 0, c->stream>>>(seg_of, 0u, M);
     PMA_LAUNCHED(c);
     uint32_t nseg = 1;

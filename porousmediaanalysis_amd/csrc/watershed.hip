@@ -104,7 +104,8 @@ __global__ void __launch_bounds__(256) ws_resolve(const unsigned long long* __re
                                                   const uint64_t* __restrict__ qoff, uint32_t* __restrict__ qcount,
                                                   unsigned long long* __restrict__ qitem,
                                                   unsigned long long* __restrict__ qkey, LogEntry* __restrict__ log,
-                                                  unsigned long long* __restrict__ log_count, unsigned long long log_cap) {
+                                                  unsigned long long* __restrict__ log_count, unsigned long long log_cap,
+                                                  uint32_t* __restrict__ error) {
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t pos = a + t;
     if (pos >= b) return;
@@ -133,8 +134,12 @@ __global__ void __launch_bounds__(256) ws_resolve(const unsigned long long* __re
                     mask |= 1u << k;
                 } else {
                     uint32_t slot = atomicAdd(&qcount[c], 1u);
-                    qitem[qoff[c] + slot] = (unsigned long long)j | ((unsigned long long)p << 32);
-                    qkey[qoff[c] + slot] = tlevel | ((unsigned long long)pos << 5) | (unsigned long long)k;
+                    if (qoff[c] + slot < qoff[c + 1]) {
+                        qitem[qoff[c] + slot] = (unsigned long long)j | ((unsigned long long)p << 32);
+                        qkey[qoff[c] + slot] = tlevel | ((unsigned long long)pos << 5) | (unsigned long long)k;
+                    } else {
+                        atomicOr(error, 1u);
+                    }
                 }
                 continue;
             }
@@ -359,7 +364,7 @@ bool watershed_once(pma_ctx* c, unsigned long long log_cap, unsigned long long* 
     unsigned long long* qkey = c->buf<unsigned long long>("ws_qkey", std::max<uint64_t>(qtotal, 1));
     unsigned long long* sort_k = c->buf<unsigned long long>("ws_sort_k", std::max<uint64_t>(qtotal, 1));
     unsigned long long* sort_v = c->buf<unsigned long long>("ws_sort_v", std::max<uint64_t>(qtotal, 1));
-    PMA_HIP(hipMemcpyAsync(qoff, qoff_h.data(), (L + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, c->stream));
+    upload(c, qoff, qoff_h.data(), (L + 1) * sizeof(uint64_t));
     PMA_HIP(hipMemsetAsync(qcount, 0, L * sizeof(uint32_t), c->stream));
     PMA_HIP(hipMemsetAsync(label, 0, n * sizeof(uint32_t), c->stream));
     unsigned long long* claim = c->buf<unsigned long long>("ws_claim", n);
@@ -367,14 +372,16 @@ bool watershed_once(pma_ctx* c, unsigned long long log_cap, unsigned long long* 
     uint32_t* d_seed_voxel = c->buf<uint32_t>("ws_seed_voxel", std::max(np, 1u));
     uint32_t* d_seed_r2 = c->buf<uint32_t>("ws_seed_r2", std::max(np, 1u));
     if (np) {
-        PMA_HIP(hipMemcpyAsync(d_seed_voxel, seed_voxel.data(), np * 4, hipMemcpyHostToDevice, c->stream));
-        PMA_HIP(hipMemcpyAsync(d_seed_r2, seed_r2.data(), np * 4, hipMemcpyHostToDevice, c->stream));
+        upload(c, d_seed_voxel, seed_voxel.data(), np * 4ull);
+        upload(c, d_seed_r2, seed_r2.data(), np * 4ull);
         push_seeds<<<grid_for(np, 256), 256, 0, c->stream>>>(d_seed_voxel, d_seed_r2, np, label, qoff, qcount, qitem, qkey);
         PMA_LAUNCHED(c);
     }
     LogEntry* log = c->buf<LogEntry>("ws_log", log_cap);
     unsigned long long* log_count = c->buf<unsigned long long>("ws_log_count", 1);
     PMA_HIP(hipMemsetAsync(log_count, 0, sizeof(unsigned long long), c->stream));
+    uint32_t* ws_err = c->buf<uint32_t>("ws_error", 1);
+    PMA_HIP(hipMemsetAsync(ws_err, 0, 4, c->stream));
     uint32_t* same_mask = c->buf<uint32_t>("ws_same_mask", std::max<uint64_t>(qtotal, 1));
     uint32_t* same_cnt = c->buf<uint32_t>("ws_same_cnt", std::max<uint64_t>(qtotal, 1));
     uint32_t* same_off = c->buf<uint32_t>("ws_same_off", std::max<uint64_t>(qtotal, 1));
@@ -399,18 +406,25 @@ bool watershed_once(pma_ctx* c, unsigned long long log_cap, unsigned long long* 
             ++gr;
             ++rounds;
             const unsigned g = grid_for(b - a, 256);
-            ws_bid<<<g, 256, 0, c->stream>>>(items, a, b, cr, label, claim, hi, (uint32_t)d, X, Y, Z, d3q7);
-            PMA_LAUNCHED(c);
-            ws_resolve<<<g, 256, 0, c->stream>>>(items, a, b, cr, label, claim, hi, tlevel, (uint32_t)d, X, Y, Z, d3q7,
-                                                 same_mask, same_cnt, qoff, qcount, qitem, qkey, log, log_count,
-                                                 log_cap);
-            PMA_LAUNCHED(c);
+            {
+                PMA_TIMED(c, "ws_bid");
+                ws_bid<<<g, 256, 0, c->stream>>>(items, a, b, cr, label, claim, hi, (uint32_t)d, X, Y, Z, d3q7);
+                PMA_LAUNCHED(c);
+            }
+            {
+                PMA_TIMED(c, "ws_resolve");
+                ws_resolve<<<g, 256, 0, c->stream>>>(items, a, b, cr, label, claim, hi, tlevel, (uint32_t)d, X, Y, Z, d3q7,
+                                                     same_mask, same_cnt, qoff, qcount, qitem, qkey, log, log_count,
+                                                     log_cap, ws_err);
+                PMA_LAUNCHED(c);
+            }
             uint64_t added = exclusive_scan_u32(c, same_cnt, same_off, b - a);
+            PMA_REQUIRE(b + added <= h[d] && read_scalar(c, ws_err) == 0, PMA_ERR_INTERNAL,
+                        "pma_watershed: FIFO capacity exceeded");
             if (added) {
                 ws_append<<<g, 256, 0, c->stream>>>(items, a, b, same_mask, same_off, X, Y);
                 PMA_LAUNCHED(c);
             }
-            PMA_REQUIRE(b + added <= h[d], PMA_ERR_INTERNAL, "pma_watershed: FIFO capacity exceeded");
             a = b;
             b = (uint32_t)(b + added);
         }
@@ -480,8 +494,8 @@ bool watershed_once(pma_ctx* c, unsigned long long log_cap, unsigned long long* 
             // owners first, then the shared voxels
             uint32_t* d_pid = c->buf<uint32_t>("ws_pore_id", std::max(np, 1u));
             uint32_t* d_kid = c->buf<uint32_t>("ws_key_id", lt.distinct);
-            PMA_HIP(hipMemcpyAsync(d_pid, pore_id.data(), np * 4, hipMemcpyHostToDevice, c->stream));
-            PMA_HIP(hipMemcpyAsync(d_kid, key_id.data(), lt.distinct * 4ull, hipMemcpyHostToDevice, c->stream));
+            upload(c, d_pid, pore_id.data(), np * 4ull);
+            upload(c, d_kid, key_id.data(), lt.distinct * 4ull);
             // volumes and boundary flags need the pore-index labels: compute them before relabelling
             c->pores_ws = c->pores;
             {
@@ -537,7 +551,7 @@ bool watershed_once(pma_ctx* c, unsigned long long log_cap, unsigned long long* 
             c->pores_ws[k].boundary = bf[k] != 0;
         }
         uint32_t* d_pid = c->buf<uint32_t>("ws_pore_id", std::max(np, 1u));
-        if (np) PMA_HIP(hipMemcpyAsync(d_pid, pore_id.data(), np * 4, hipMemcpyHostToDevice, c->stream));
+        upload(c, d_pid, pore_id.data(), np * 4ull);
         labels_to_ids<<<grid_for(n, 256), 256, 0, c->stream>>>(label, n, d_pid);
         PMA_LAUNCHED(c);
         PMA_HIP(hipStreamSynchronize(c->stream));
