@@ -1,110 +1,165 @@
-// Exact deduplication of ordered pore lists (throat keys).
+// Exact deduplication of ordered pore lists (throat keys) of any length.
 //
 // The reference keys shared pixel types by the exact ordered list of pore IDs
 // (std::map<std::vector<PoreThroatIdType>, ...>, BasePoreAndThroatComputer.cpp:112-133), and
-// numbers a key by the moment its list is first formed.  Here every formation is one record
-// (list, time, weight); records are sorted by list content with three stable radix
-// passes (no hashing, so distinct lists are never merged), equal runs give one distinct
-// list with its earliest formation time and the summed weight (voxels whose final list it is).
+// every list it ever forms is a prefix of the final list of some voxel (a list only grows by
+// push_back, :95-99).  So the distinct lists are the nodes of a prefix trie over the final
+// lists: the list of length L + 1 is (its length-L prefix, its last element).  The trie is
+// built one length at a time: at length L + 1 every sequence longer than L contributes the
+// key (node of its length-L prefix << 32 | its element L); a radix sort and a run split give
+// the distinct nodes of that length.  No hashing is involved, so distinct lists are never
+// merged.
 #include "pma_internal.h"
+
+#include <algorithm>
 
 namespace pma {
 namespace {
 
-__global__ void iota_u32(uint32_t* p, size_t n) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = (uint32_t)i;
-}
+constexpr uint32_t kNodeBit = kTrieNode;
 
-__global__ void gather_u64(const uint64_t* __restrict__ src, const uint32_t* __restrict__ perm,
-                           uint64_t* __restrict__ dst, size_t n) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = src[perm[i]];
-}
-
-__global__ void mark_heads(const uint64_t* __restrict__ w0, const uint64_t* __restrict__ w1,
-                           const uint64_t* __restrict__ w2, const uint32_t* __restrict__ perm,
-                           uint32_t* __restrict__ head, size_t n) {
+__global__ void length_keys(const uint32_t* __restrict__ len, size_t S, uint32_t* __restrict__ key,
+                            uint32_t* __restrict__ val, unsigned int* __restrict__ max_len) {
     size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    uint32_t a = perm[s];
-    uint32_t h = 1;
-    if (s > 0) {
-        uint32_t b = perm[s - 1];
-        h = (w0[a] != w0[b]) || (w1[a] != w1[b]) || (w2[a] != w2[b]);
-    }
-    head[s] = h;
+    if (s >= S) return;
+    key[s] = 0xFFFFFFFFu - len[s];  // ascending key = descending length
+    val[s] = (uint32_t)s;
+    atomicMax(max_len, len[s]);
 }
 
-__global__ void reduce_runs(const uint64_t* __restrict__ w0, const uint64_t* __restrict__ w1,
-                            const uint64_t* __restrict__ w2, const uint64_t* __restrict__ time,
-                            const uint32_t* __restrict__ weight, const uint32_t* __restrict__ perm,
-                            const uint32_t* __restrict__ head, const uint32_t* __restrict__ head_scan,
-                            uint64_t* __restrict__ dwords, unsigned long long* __restrict__ dtime,
-                            unsigned long long* __restrict__ dfinal, uint32_t* __restrict__ rec_distinct, size_t n) {
+__global__ void length_histogram(const uint32_t* __restrict__ len, size_t S, uint32_t max_len,
+                                 unsigned int* __restrict__ hist) {
     size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    uint32_t r = perm[s];
-    uint32_t d = head_scan[s] + head[s] - 1;
-    rec_distinct[r] = d;
-    if (head[s]) {
-        dwords[3 * (size_t)d] = w0[r];
-        dwords[3 * (size_t)d + 1] = w1[r];
-        dwords[3 * (size_t)d + 2] = w2[r];
-    }
-    atomicMin(&dtime[d], (unsigned long long)time[r]);
-    if (weight[r]) atomicAdd(&dfinal[d], (unsigned long long)weight[r]);
+    if (s < S && len[s] <= max_len) atomicAdd(&hist[len[s]], 1u);
+}
+
+__global__ void trie_start(const uint32_t* __restrict__ act, size_t A, const uint32_t* __restrict__ elems,
+                           const uint64_t* __restrict__ off, uint32_t* __restrict__ cur) {
+    size_t a = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a < A) cur[a] = elems[off[act[a]]];
+}
+
+__global__ void trie_keys(const uint32_t* __restrict__ act, size_t A, const uint32_t* __restrict__ elems,
+                          const uint64_t* __restrict__ off, const uint32_t* __restrict__ cur, uint32_t L,
+                          uint64_t* __restrict__ key, uint32_t* __restrict__ val) {
+    size_t a = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= A) return;
+    key[a] = ((uint64_t)cur[a] << 32) | elems[off[act[a]] + L];
+    val[a] = (uint32_t)a;
+}
+
+__global__ void trie_heads(const uint64_t* __restrict__ key, size_t A, uint32_t* __restrict__ head) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < A) head[i] = (i == 0 || key[i] != key[i - 1]) ? 1u : 0u;
+}
+
+__global__ void trie_assign(const uint64_t* __restrict__ key, const uint32_t* __restrict__ val,
+                            const uint32_t* __restrict__ head, const uint32_t* __restrict__ hscan, size_t A,
+                            uint32_t dbase, uint32_t L, const uint32_t* __restrict__ act,
+                            const uint32_t* __restrict__ len, const uint64_t* __restrict__ rbase,
+                            const uint64_t* __restrict__ rec_time, const uint32_t* __restrict__ weight,
+                            uint32_t* __restrict__ cur, uint32_t* __restrict__ rec_node,
+                            uint64_t* __restrict__ node_key, unsigned long long* __restrict__ node_time,
+                            unsigned long long* __restrict__ node_final) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A) return;
+    const uint32_t a = val[i];
+    const uint32_t d = dbase + hscan[i] + head[i] - 1;
+    const uint32_t s = act[a];
+    const uint64_t r = rbase[s] + (L - 1);  // record of the prefix of length L + 1
+    cur[a] = kNodeBit | d;
+    rec_node[r] = d;
+    if (head[i]) node_key[d] = key[i];
+    if (rec_time) atomicMin(&node_time[d], (unsigned long long)rec_time[r]);
+    if (L + 1 == len[s] && weight) atomicAdd(&node_final[d], (unsigned long long)weight[s]);
 }
 
 }  // namespace
 
-std::vector<uint32_t> list_members(const uint64_t* w) {
-    std::vector<uint32_t> out;
-    for (int k = 0; k < kListMax; ++k) {
-        uint64_t v = (w[k / 3] >> (21 * (k % 3))) & 0x1FFFFFull;
-        if (v == 0) break;
-        out.push_back((uint32_t)(v - 1));
+std::vector<uint32_t> PrefixTrie::members(uint32_t d) const {
+    std::vector<uint32_t> rev;
+    uint64_t k = key[d];
+    while (true) {
+        rev.push_back((uint32_t)k);
+        const uint32_t parent = (uint32_t)(k >> 32);
+        if (!(parent & kNodeBit)) {
+            rev.push_back(parent);
+            break;
+        }
+        k = key[parent & ~kNodeBit];
     }
-    return out;
+    std::reverse(rev.begin(), rev.end());
+    return rev;
 }
 
-ListTable dedupe_lists(pma_ctx* c, const uint64_t* w0, const uint64_t* w1, const uint64_t* w2, const uint64_t* time,
-                       const uint32_t* weight, size_t R, uint32_t* rec_distinct) {
-    ListTable t;
-    if (R == 0) return t;
-    uint32_t* perm = c->buf<uint32_t>("list_perm", R);
-    uint32_t* perm2 = c->buf<uint32_t>("list_perm2", R);
-    uint64_t* keys = c->buf<uint64_t>("list_keys", R);
-    uint64_t* keys2 = c->buf<uint64_t>("list_keys2", R);
-    iota_u32<<<grid_for(R, 256), 256, 0, c->stream>>>(perm, R);
+PrefixTrie build_prefix_trie(pma_ctx* c, const uint32_t* elems, const uint64_t* seq_off, const uint32_t* seq_len,
+                             const uint64_t* rbase, const uint64_t* rec_time, const uint32_t* seq_weight, size_t S,
+                             uint64_t R, uint32_t* rec_node) {
+    PrefixTrie t;
+    if (S == 0 || R == 0) return t;
+    PMA_REQUIRE(R < kNodeBit, PMA_ERR_CAPACITY, "more than 2^31 throat-key formations");
+    // sequences by descending length: the ones longer than L are a prefix of this order
+    uint32_t* lkey = c->buf<uint32_t>("trie_lkey", S);
+    uint32_t* lkey2 = c->buf<uint32_t>("trie_lkey2", S);
+    uint32_t* lval = c->buf<uint32_t>("trie_lval", S);
+    uint32_t* act = c->buf<uint32_t>("trie_act", S);
+    unsigned int* dmax = c->buf<unsigned int>("trie_max_len", 1);
+    PMA_HIP(hipMemsetAsync(dmax, 0, 4, c->stream));
+    length_keys<<<grid_for(S, 256), 256, 0, c->stream>>>(seq_len, S, lkey, lval, dmax);
     PMA_LAUNCHED(c);
-    const uint64_t* words[3] = {w0, w1, w2};
-    for (int k = 2; k >= 0; --k) {  // least significant word first: stable passes give lexicographic order
-        gather_u64<<<grid_for(R, 256), 256, 0, c->stream>>>(words[k], perm, keys, R);
+    radix_sort_pairs_u32(c, lkey, lkey2, lval, act, S);
+    const uint32_t max_len = read_scalar(c, dmax);
+    unsigned int* dhist = c->buf<unsigned int>("trie_hist", (size_t)max_len + 1);
+    PMA_HIP(hipMemsetAsync(dhist, 0, ((size_t)max_len + 1) * 4, c->stream));
+    length_histogram<<<grid_for(S, 256), 256, 0, c->stream>>>(seq_len, S, max_len, dhist);
+    PMA_LAUNCHED(c);
+    std::vector<uint32_t> hist((size_t)max_len + 1);
+    PMA_HIP(hipMemcpyAsync(hist.data(), dhist, hist.size() * 4, hipMemcpyDeviceToHost, c->stream));
+    PMA_HIP(hipStreamSynchronize(c->stream));
+    // longer[L] = number of sequences with length > L
+    std::vector<uint64_t> longer((size_t)max_len + 1, 0);
+    for (size_t L = max_len; L-- > 0;) longer[L] = longer[L + 1] + hist[L + 1];
+    if (max_len < 2) return t;
+
+    const size_t A1 = longer[1];
+    uint32_t* cur = c->buf<uint32_t>("trie_cur", A1);
+    uint64_t* key = c->buf<uint64_t>("trie_key", A1);
+    uint64_t* key2 = c->buf<uint64_t>("trie_key2", A1);
+    uint32_t* val = c->buf<uint32_t>("trie_val", A1);
+    uint32_t* val2 = c->buf<uint32_t>("trie_val2", A1);
+    uint32_t* head = c->buf<uint32_t>("trie_head", A1);
+    uint32_t* hscan = c->buf<uint32_t>("trie_hscan", A1);
+    uint64_t* node_key = c->buf<uint64_t>("trie_node_key", R);
+    unsigned long long* node_time = c->buf<unsigned long long>("trie_node_time", R);
+    unsigned long long* node_final = c->buf<unsigned long long>("trie_node_final", R);
+    PMA_HIP(hipMemsetAsync(node_time, 0xFF, R * 8, c->stream));
+    PMA_HIP(hipMemsetAsync(node_final, 0, R * 8, c->stream));
+    trie_start<<<grid_for(A1, 256), 256, 0, c->stream>>>(act, A1, elems, seq_off, cur);
+    PMA_LAUNCHED(c);
+    uint64_t D = 0;
+    for (uint32_t L = 1; L < max_len; ++L) {
+        const size_t A = longer[L];
+        if (A == 0) break;
+        trie_keys<<<grid_for(A, 256), 256, 0, c->stream>>>(act, A, elems, seq_off, cur, L, key, val);
         PMA_LAUNCHED(c);
-        radix_sort_pairs_u64(c, keys, keys2, perm, perm2, R, 0, 63);
-        std::swap(perm, perm2);
+        radix_sort_pairs_u64(c, key, key2, val, val2, A);
+        trie_heads<<<grid_for(A, 256), 256, 0, c->stream>>>(key2, A, head);
+        PMA_LAUNCHED(c);
+        const uint64_t DL = exclusive_scan_u32(c, head, hscan, A);
+        PMA_REQUIRE(D + DL <= R, PMA_ERR_INTERNAL, "prefix trie: more nodes than formations");
+        trie_assign<<<grid_for(A, 256), 256, 0, c->stream>>>(key2, val2, head, hscan, A, (uint32_t)D, L, act,
+                                                             seq_len, rbase, rec_time, seq_weight, cur, rec_node,
+                                                             node_key, node_time, node_final);
+        PMA_LAUNCHED(c);
+        D += DL;
     }
-    uint32_t* head = c->buf<uint32_t>("list_head", R);
-    uint32_t* hscan = c->buf<uint32_t>("list_hscan", R);
-    mark_heads<<<grid_for(R, 256), 256, 0, c->stream>>>(w0, w1, w2, perm, head, R);
-    PMA_LAUNCHED(c);
-    uint64_t D = exclusive_scan_u32(c, head, hscan, R);
-    uint64_t* dwords = c->buf<uint64_t>("list_dwords", 3 * D);
-    unsigned long long* dtime = c->buf<unsigned long long>("list_dtime", D);
-    unsigned long long* dfinal = c->buf<unsigned long long>("list_dfinal", D);
-    PMA_HIP(hipMemsetAsync(dtime, 0xFF, D * sizeof(uint64_t), c->stream));
-    PMA_HIP(hipMemsetAsync(dfinal, 0, D * sizeof(uint64_t), c->stream));
-    reduce_runs<<<grid_for(R, 256), 256, 0, c->stream>>>(w0, w1, w2, time, weight, perm, head, hscan, dwords, dtime,
-                                                         dfinal, rec_distinct, R);
-    PMA_LAUNCHED(c);
     t.distinct = (uint32_t)D;
-    t.words.resize(3 * D);
+    t.key.resize(D);
     t.min_time.resize(D);
     t.finals.resize(D);
-    PMA_HIP(hipMemcpyAsync(t.words.data(), dwords, 3 * D * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
-    PMA_HIP(hipMemcpyAsync(t.min_time.data(), dtime, D * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
-    PMA_HIP(hipMemcpyAsync(t.finals.data(), dfinal, D * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+    PMA_HIP(hipMemcpyAsync(t.key.data(), node_key, D * 8, hipMemcpyDeviceToHost, c->stream));
+    PMA_HIP(hipMemcpyAsync(t.min_time.data(), node_time, D * 8, hipMemcpyDeviceToHost, c->stream));
+    PMA_HIP(hipMemcpyAsync(t.finals.data(), node_final, D * 8, hipMemcpyDeviceToHost, c->stream));
     PMA_HIP(hipStreamSynchronize(c->stream));
     return t;
 }

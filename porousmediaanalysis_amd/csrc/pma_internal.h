@@ -151,7 +151,7 @@ void run_generate_gaussian(pma_ctx* c, float sigma, float porosity, uint64_t see
 // Device-wide helpers (scan.hip).
 // Exclusive scan of n u32 values into out (out may alias in); returns the total.
 uint64_t exclusive_scan_u32(pma_ctx* c, const uint32_t* in, uint32_t* out, size_t n);
-void exclusive_scan_u64(pma_ctx* c, const uint64_t* in, uint64_t* out, size_t n);
+uint64_t exclusive_scan_u64(pma_ctx* c, const uint64_t* in, uint64_t* out, size_t n);
 // Stable LSD radix sorts over bits [begin_bit, end_bit) of the keys (rsort.hip).
 void radix_sort_pairs_u64(pma_ctx* c, const uint64_t* keys_in, uint64_t* keys_out, const uint32_t* vals_in,
                           uint32_t* vals_out, size_t n, int begin_bit = 0, int end_bit = 64);
@@ -162,20 +162,24 @@ void radix_sort_pairs_u32(pma_ctx* c, const uint32_t* keys_in, uint32_t* keys_ou
 // Exact emulation of libstdc++ std::sort with comp(a, b) = (a >> 32) > (b >> 32) (sort.hip).
 void sort_like_libstdcxx_desc(pma_ctx* c, uint64_t* records, size_t n);
 
-// Ordered pore lists (lists.hip).  A list holds up to kListMax pore indices, stored as
-// index + 1 in 21-bit slots, three slots per word (0 = end of list).
-constexpr int kListMax = 9;
-struct ListTable {
+// Ordered pore lists of any length (lists.hip).  Sequence s holds the elements
+// elems[seq_off[s] .. seq_off[s] + seq_len[s]) (each < 2^31); its prefixes of length 2 .. len
+// are its formation records rbase[s] + (length - 2).  Distinct lists are the nodes of a prefix
+// trie: key[d] = (parent << 32) | last element, where parent is the first element itself
+// for a list of length 2 and kTrieNode | node otherwise; nodes are numbered by length, then
+// by key.
+constexpr uint32_t kTrieNode = 0x80000000u;
+struct PrefixTrie {
     uint32_t distinct = 0;           // number of distinct lists
-    std::vector<uint64_t> words;     // 3 words per distinct list
-    std::vector<uint64_t> min_time;  // earliest formation time of each distinct list
-    std::vector<uint64_t> finals;    // summed record weights per distinct list
+    std::vector<uint64_t> key;       // per node: parent << 32 | last element
+    std::vector<uint64_t> min_time;  // per node: earliest rec_time of its records (~0 without times)
+    std::vector<uint64_t> finals;    // per node: summed seq_weight of sequences whose full list it is
+    std::vector<uint32_t> members(uint32_t d) const;  // the list, in sequence order
 };
-// Dedupes R records (w0, w1, w2, time, weight); rec_distinct[r] receives the distinct
-// index of record r (device).  Host receives the distinct table.
-ListTable dedupe_lists(pma_ctx* c, const uint64_t* w0, const uint64_t* w1, const uint64_t* w2, const uint64_t* time,
-                       const uint32_t* weight, size_t R, uint32_t* rec_distinct);
-std::vector<uint32_t> list_members(const uint64_t* w);
+// rec_time and seq_weight may be null.  rec_node[r] receives the node of record r (device).
+PrefixTrie build_prefix_trie(pma_ctx* c, const uint32_t* elems, const uint64_t* seq_off, const uint32_t* seq_len,
+                             const uint64_t* rbase, const uint64_t* rec_time, const uint32_t* seq_weight, size_t S,
+                             uint64_t R, uint32_t* rec_node);
 
 // Host -> device copy that returns only when the source may be released.
 inline void upload(pma_ctx* c, void* dst, const void* src, size_t bytes) {

@@ -35,7 +35,6 @@ namespace pma {
 namespace {
 
 constexpr uint32_t kInv = 0xFFFFFFFFu;
-constexpr int kSetCap = 64;  // largest pore set a plateau may carry
 
 __device__ __forceinline__ void decode(uint32_t i, int X, int Y, int& x, int& y, int& z) {
     y = (int)(i % (uint32_t)Y);
@@ -210,16 +209,18 @@ __global__ void plateau_index(const uint32_t* __restrict__ flag, const uint32_t*
     }
 }
 
-// root -> plateau index for every void voxel (in place), plus plateau size and border flag.
+// root -> plateau index for every void voxel (in place), plus plateau size, border flag and
+// the voxel's rank inside its plateau.
 __global__ void plateau_of_voxels(uint32_t* root, const uint32_t* __restrict__ pidx, int X, int Y, int Z,
-                                  uint32_t* __restrict__ plat_size, uint32_t* __restrict__ plat_border) {
+                                  uint32_t* __restrict__ plat_size, uint32_t* __restrict__ plat_border,
+                                  uint32_t* __restrict__ vrank) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)X * Y * Z) return;
     uint32_t r = root[i];
     if (r == kInv) return;
     uint32_t q = pidx[r];  // only this thread reads or writes root[i]
     root[i] = q;
-    atomicAdd(&plat_size[q], 1u);
+    vrank[i] = atomicAdd(&plat_size[q], 1u);  // rank of the voxel inside its plateau (any order)
     int x, y, z;
     decode((uint32_t)i, X, Y, x, y, z);
     if (x == 0 || y == 0 || z == 0 || x == X - 1 || y == Y - 1 || z == Z - 1) plat_border[q] = 1;
@@ -318,84 +319,71 @@ __global__ void level_starts(const uint32_t* __restrict__ head, const uint32_t* 
     if (s < P && head[s]) start[off[s]] = (uint32_t)s;
 }
 
+// k-way merge of the sorted pore sets of a plateau's higher neighbours and its own pore.
+// cur/end hold one cursor per in-edge; out (if not null) receives the union in ascending order.
+__device__ uint32_t merge_sets(uint32_t own, uint32_t e0, uint32_t ne, const unsigned long long* __restrict__ edges,
+                               const unsigned long long* __restrict__ set_off, const uint32_t* __restrict__ set_len,
+                               const uint32_t* __restrict__ pool, unsigned long long* __restrict__ cur,
+                               unsigned long long* __restrict__ end, uint32_t* __restrict__ out) {
+    for (uint32_t e = e0; e < e0 + ne; ++e) {
+        const uint32_t src = (uint32_t)edges[e];
+        cur[e] = set_off[src];
+        end[e] = set_off[src] + set_len[src];
+    }
+    uint32_t m = 0;
+    uint32_t own_left = own;
+    while (true) {
+        uint32_t best = own_left;
+        for (uint32_t e = e0; e < e0 + ne; ++e)
+            if (cur[e] < end[e]) best = min(best, pool[cur[e]]);
+        if (best == kInv) break;
+        if (out) out[m] = best;
+        ++m;
+        if (own_left == best) own_left = kInv;
+        for (uint32_t e = e0; e < e0 + ne; ++e)
+            if (cur[e] < end[e] && pool[cur[e]] == best) ++cur[e];
+    }
+    return m;
+}
+
+// Pore sets of the plateaus of one CR level (all their sources are on higher levels).
 __global__ void set_union_level(const uint32_t* __restrict__ order, uint32_t lo, uint32_t hi,
                                 const uint32_t* __restrict__ in_start, const uint32_t* __restrict__ in_count,
                                 const unsigned long long* __restrict__ edges, const uint32_t* __restrict__ plat_pore,
-                                uint32_t* __restrict__ set_off, uint32_t* __restrict__ set_len, uint32_t* __restrict__ pool,
-                                unsigned long long* __restrict__ pool_top, unsigned long long pool_cap,
-                                uint32_t* __restrict__ overflow) {
-    uint32_t s = lo + blockIdx.x * blockDim.x + threadIdx.x;
+                                unsigned long long* __restrict__ set_off, uint32_t* __restrict__ set_len,
+                                uint32_t* __restrict__ pool, unsigned long long* __restrict__ pool_top,
+                                unsigned long long pool_cap, unsigned long long* __restrict__ cur,
+                                unsigned long long* __restrict__ end, uint32_t* __restrict__ overflow) {
+    const uint32_t s = lo + blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= hi) return;
     const uint32_t q = order[s];
-    uint32_t loc[kSetCap];
-    int m = 0;
-    bool of = false;
-    auto insert = [&](uint32_t p) {
-        int pos = 0;
-        while (pos < m && loc[pos] < p) ++pos;
-        if (pos < m && loc[pos] == p) return;
-        if (m == kSetCap) {
-            of = true;
-            return;
-        }
-        for (int t = m; t > pos; --t) loc[t] = loc[t - 1];
-        loc[pos] = p;
-        ++m;
-    };
-    if (plat_pore[q] != kInv) insert(plat_pore[q]);
+    const uint32_t own = plat_pore[q];
     const uint32_t e0 = in_start[q], ne = in_count[q];
-    for (uint32_t e = e0; e < e0 + ne; ++e) {
-        const uint32_t src = (uint32_t)edges[e];
-        const uint32_t so = set_off[src], sl = set_len[src];
-        for (uint32_t t = 0; t < sl; ++t) insert(pool[so + t]);
-    }
-    if (of) atomicOr(overflow, 1u);
-    if (m > kListMax) atomicOr(overflow, 4u);
-    unsigned long long o = atomicAdd(pool_top, (unsigned long long)m);
+    const uint32_t m = merge_sets(own, e0, ne, edges, set_off, set_len, pool, cur, end, nullptr);
+    const unsigned long long o = atomicAdd(pool_top, (unsigned long long)m);
     if (o + m > pool_cap) {
         atomicOr(overflow, 2u);
         set_off[q] = 0;
         set_len[q] = 0;
         return;
     }
-    for (int t = 0; t < m; ++t) pool[o + t] = loc[t];
-    set_off[q] = (uint32_t)o;
-    set_len[q] = (uint32_t)m;
+    merge_sets(own, e0, ne, edges, set_off, set_len, pool, cur, end, pool + o);
+    set_off[q] = o;
+    set_len[q] = m;
 }
 
 // ---- key records -------------------------------------------------------------------------
-__global__ void prefix_counts(const uint32_t* __restrict__ set_len, size_t P, uint32_t* __restrict__ cnt) {
+// A plateau whose set is s0 < s1 < ... < s(m-1) is first visited by pore s_j (j >= 1) while
+// its earlier-pore class is {s0 .. s(j-1)}: that visit forms the list of {s0 .. s_j}.
+__global__ void prefix_counts(const uint32_t* __restrict__ set_len, size_t P, unsigned long long* __restrict__ cnt) {
     size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q < P) cnt[q] = set_len[q] >= 2 ? set_len[q] - 1 : 0;
 }
 
-__global__ void prefix_records(const uint32_t* __restrict__ set_off, const uint32_t* __restrict__ set_len,
-                               const uint32_t* __restrict__ pool, const uint32_t* __restrict__ plat_size, size_t P,
-                               const uint32_t* __restrict__ rbase, uint64_t* __restrict__ w0, uint64_t* __restrict__ w1,
-                               uint64_t* __restrict__ w2, uint64_t* __restrict__ time, uint32_t* __restrict__ weight) {
-    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= P) return;
-    const uint32_t m = set_len[q];
-    if (m < 2) return;
-    const uint32_t* s = pool + set_off[q];
-    uint64_t w[3] = {0, 0, 0};
-    w[0] = (uint64_t)(s[0] + 1);
-    uint32_t r = rbase[q];
-    for (uint32_t j = 1; j < m && j < (uint32_t)kListMax; ++j) {
-        w[j / 3] |= (uint64_t)(s[j] + 1) << (21 * (j % 3));
-        w0[r] = w[0];
-        w1[r] = w[1];
-        w2[r] = w[2];
-        time[r] = 0;
-        weight[r] = j == m - 1 ? plat_size[q] : 0;
-        ++r;
-    }
-}
-
-__global__ void reach_sizes(const uint32_t* __restrict__ set_off, const uint32_t* __restrict__ set_len,
+__global__ void reach_sizes(const unsigned long long* __restrict__ set_off, const uint32_t* __restrict__ set_len,
                             const uint32_t* __restrict__ pool, const uint32_t* __restrict__ plat_size,
                             const uint32_t* __restrict__ plat_border, size_t P, unsigned long long* __restrict__ reach,
-                            uint32_t* __restrict__ boundary) {
+                            uint32_t* __restrict__ boundary, unsigned long long* __restrict__ plat_bits) {
     size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= P) return;
     const uint32_t* s = pool + set_off[q];
@@ -403,45 +391,60 @@ __global__ void reach_sizes(const uint32_t* __restrict__ set_off, const uint32_t
         atomicAdd(&reach[s[t]], (unsigned long long)plat_size[q]);
         if (plat_border[q]) boundary[s[t]] = 1;
     }
+    plat_bits[q] = (unsigned long long)plat_size[q] * set_len[q];
 }
 
 // ---- DFS replay: one wave per pore ------------------------------------------------------------
 struct ReplayTask {
-    uint32_t pore, seed, ncls, pad;
+    uint32_t pore, seed, ncls, slot;
     unsigned long long stack_off;
-    unsigned long long stack_cap;  // frames available to this pore (its reach + 1)
+    unsigned long long stack_cap;  // frames available to this task
+    unsigned long long seen_off;   // first word of this pore's class bitmap
 };
 
-__device__ __forceinline__ int member_index(const uint32_t* __restrict__ pool, uint32_t off, uint32_t len, uint32_t p) {
-    for (uint32_t t = 0; t < len; ++t)
-        if (pool[off + t] == p) return (int)t;
-    return -1;
+// Position of pore p in the ascending set pool[off .. off + len), or -1.
+__device__ __forceinline__ int member_index(const uint32_t* __restrict__ pool, unsigned long long off, uint32_t len,
+                                            uint32_t p) {
+    uint32_t lo = 0, hi = len;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (pool[off + mid] < p) lo = mid + 1;
+        else hi = mid;
+    }
+    return (lo < len && pool[off + lo] == p) ? (int)lo : -1;
+}
+
+// Bit of (voxel v, member j of its plateau's set) in the visited bitmap.
+__device__ __forceinline__ unsigned long long visit_bit(const unsigned long long* __restrict__ bit_base,
+                                                        const uint32_t* __restrict__ vrank,
+                                                        const uint32_t* __restrict__ set_len, uint32_t q, uint32_t v,
+                                                        uint32_t j) {
+    return bit_base[q] + (unsigned long long)vrank[v] * set_len[q] + j;
 }
 
 __global__ void __launch_bounds__(64) dfs_replay(const ReplayTask* __restrict__ tasks, const uint32_t* __restrict__ cr,
-                                                 const uint32_t* __restrict__ plat, const uint32_t* __restrict__ set_off,
+                                                 const uint32_t* __restrict__ plat,
+                                                 const unsigned long long* __restrict__ set_off,
                                                  const uint32_t* __restrict__ set_len, const uint32_t* __restrict__ pool,
-                                                 const uint32_t* __restrict__ rbase,
-                                                 const uint32_t* __restrict__ rec_distinct,
-                                                 const uint32_t* __restrict__ event_local, uint32_t* visited,
-                                                 uint32_t* stack_voxel, uint8_t* stack_k, uint32_t* __restrict__ rank,
-                                                 int X, int Y, int Z, int d3q7, uint32_t* __restrict__ error) {
+                                                 const unsigned long long* __restrict__ rbase,
+                                                 const uint32_t* __restrict__ rec_node,
+                                                 const uint32_t* __restrict__ event_local,
+                                                 const unsigned long long* __restrict__ bit_base,
+                                                 const uint32_t* __restrict__ vrank, unsigned long long* visited,
+                                                 uint32_t* __restrict__ seen_words, uint32_t* __restrict__ stack_voxel,
+                                                 uint8_t* __restrict__ stack_k, uint32_t* __restrict__ rank,
+                                                 uint32_t* __restrict__ task_status, int X, int Y, int Z, int d3q7,
+                                                 uint32_t* __restrict__ error) {
     // One wave per pore.  Lane 0 owns the stack, the seen-class bitmap and the counters and
     // broadcasts every decision; the other lanes evaluate the 26 neighbours in parallel.
     // Frames are (voxel, next neighbour index); a recursive DFS that tries neighbours in
     // reverse loop order visits voxels in the reference's stack-DFS order.
-    __shared__ uint32_t seen[1024];  // up to 32768 classes per pore
     const ReplayTask task = tasks[blockIdx.x];
     const int lane = threadIdx.x;
     const uint32_t p = task.pore;
-    if (task.ncls > 32768) {
-        if (lane == 0) atomicOr(error, 4u);
-        return;
-    }
-    for (int i = lane; i < 1024; i += 64) seen[i] = 0;
-    __syncthreads();
     uint32_t* sv = stack_voxel + task.stack_off;
     uint8_t* sk = stack_k + task.stack_off;
+    uint32_t* seen = seen_words + task.seen_off;
 
     // Marks v visited by p and, if its earlier-pore class is new, ranks that class.
     // Executed by lane 0 only; returns the new count of classes found.
@@ -452,14 +455,15 @@ __global__ void __launch_bounds__(64) dfs_replay(const ReplayTask* __restrict__ 
             atomicOr(error, 1u);
             return 0xFFFFFFFFu;
         }
-        if (j >= 32) {
-            atomicOr(error, 32u);
-            return 0xFFFFFFFFu;
-        }
-        atomicOr(&visited[v], 1u << j);
+        const unsigned long long bit = visit_bit(bit_base, vrank, set_len, q, v, (uint32_t)j);
+        atomicOr(&visited[bit >> 6], 1ull << (bit & 63));
         if (j >= 1) {
-            const uint32_t e = rec_distinct[rbase[q] + (uint32_t)j - 1];
+            const uint32_t e = rec_node[rbase[q] + (uint32_t)j - 1];
             const uint32_t l = event_local[e];
+            if (l >= task.ncls) {
+                atomicOr(error, 4u);
+                return 0xFFFFFFFFu;
+            }
             if (!(seen[l >> 5] & (1u << (l & 31)))) {
                 seen[l >> 5] |= 1u << (l & 31);
                 rank[e] = found;
@@ -473,7 +477,7 @@ __global__ void __launch_bounds__(64) dfs_replay(const ReplayTask* __restrict__ 
     if (lane == 0) found = visit(task.seed, 0);
     found = (uint32_t)__shfl((int)found, 0);
     if (found == 0xFFFFFFFFu) return;
-    int top = 0;
+    unsigned long long top = 0;
     uint32_t u = task.seed;
     int k = 27;
     while (found < task.ncls) {
@@ -482,19 +486,25 @@ __global__ void __launch_bounds__(64) dfs_replay(const ReplayTask* __restrict__ 
         const uint32_t cu = cr[u];
         bool cand = false;
         uint32_t j = 0;
-        int b = -1;
         if (lane < k && lane < 27 && in_stencil(lane, d3q7) && neighbour(lane, x, y, z, X, Y, Z, j)) {
             const uint32_t cn = cr[j];
             if (cn != 0 && cn <= cu) {
                 const uint32_t q = plat[j];
-                b = member_index(pool, set_off[q], set_len[q], p);
-                if (b < 0 || b >= 32) atomicOr(error, 2u), b = -1;
-                if (b >= 0) cand = !(__hip_atomic_load(&visited[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & (1u << b));
+                const int b = member_index(pool, set_off[q], set_len[q], p);
+                if (b < 0) {
+                    atomicOr(error, 2u);
+                } else {
+                    const unsigned long long bit = visit_bit(bit_base, vrank, set_len, q, j, (uint32_t)b);
+                    const unsigned long long w =
+                        __hip_atomic_load(&visited[bit >> 6], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    cand = !(w & (1ull << (bit & 63)));
+                }
             }
         }
         const unsigned long long bal = __ballot(cand);
         if (!bal) {
-            if (--top < 0) break;  // pop; the frame below is read back by lane 0
+            if (top == 0) break;  // stack empty: the pore's reach is exhausted
+            --top;                // pop; the frame below is read back by lane 0
             uint32_t fu = 0, fk = 0;
             if (lane == 0) {
                 fu = sv[top];
@@ -504,8 +514,8 @@ __global__ void __launch_bounds__(64) dfs_replay(const ReplayTask* __restrict__ 
             k = __shfl((int)fk, 0);
             continue;
         }
-        if ((unsigned long long)top + 1 >= task.stack_cap) {  // more frames than the pore's reach
-            if (lane == 0) atomicOr(error, 16u);
+        if (top + 1 >= task.stack_cap) {  // no room for this frame: the host repeats the pore
+            if (lane == 0) task_status[task.slot] = 1u;
             return;
         }
         const int kk = 63 - __clzll((long long)bal);  // highest neighbour index below k
@@ -521,12 +531,13 @@ __global__ void __launch_bounds__(64) dfs_replay(const ReplayTask* __restrict__ 
         u = nvox;
         k = 27;
     }
+    if (lane == 0 && found < task.ncls) atomicOr(error, 8u);  // reach exhausted before every class was seen
 }
 
 // ---- labels ------------------------------------------------------------------------------------
-__global__ void write_labels(const uint32_t* __restrict__ plat, const uint32_t* __restrict__ set_off,
+__global__ void write_labels(const uint32_t* __restrict__ plat, const unsigned long long* __restrict__ set_off,
                              const uint32_t* __restrict__ set_len, const uint32_t* __restrict__ pool,
-                             const uint32_t* __restrict__ rbase, const uint32_t* __restrict__ rec_distinct,
+                             const unsigned long long* __restrict__ rbase, const uint32_t* __restrict__ rec_node,
                              const uint32_t* __restrict__ key_id, size_t n, uint32_t* __restrict__ pt) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -538,10 +549,10 @@ __global__ void write_labels(const uint32_t* __restrict__ plat, const uint32_t* 
     uint32_t m = set_len[q];
     if (m == 0) pt[i] = 0;
     else if (m == 1) pt[i] = pore_id_of(pool[set_off[q]]);
-    else pt[i] = key_id[rec_distinct[rbase[q] + m - 2]];
+    else pt[i] = key_id[rec_node[rbase[q] + m - 2]];
 }
 
-__global__ void single_volumes(const uint32_t* __restrict__ set_off, const uint32_t* __restrict__ set_len,
+__global__ void single_volumes(const unsigned long long* __restrict__ set_off, const uint32_t* __restrict__ set_len,
                                const uint32_t* __restrict__ pool, const uint32_t* __restrict__ plat_size, size_t P,
                                unsigned long long* __restrict__ vol) {
     size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -654,7 +665,8 @@ void run_pores(pma_ctx* c) {
     fill_u32<<<grid_for(P1, 256), 256, 0, c->stream>>>(plat_pore, kInv, P1);
     PMA_LAUNCHED(c);
     PMA_HIP(hipStreamSynchronize(c->stream));
-    plateau_of_voxels<<<nb, 256, 0, c->stream>>>(root, pidx, X, Y, Z, plat_size, plat_border);
+    uint32_t* vrank = c->buf<uint32_t>("pt_vrank", n);
+    plateau_of_voxels<<<nb, 256, 0, c->stream>>>(root, pidx, X, Y, Z, plat_size, plat_border, vrank);
     PMA_LAUNCHED(c);
     uint32_t* plat = root;  // now the plateau index of every void voxel
     if (npores) {
@@ -719,8 +731,11 @@ void run_pores(pma_ctx* c) {
     unsigned long long* lkeys2 = c->buf<unsigned long long>("pt_lkeys2", P1);
     uint32_t* lvals = c->buf<uint32_t>("pt_lvals", P1);
     uint32_t* order = c->buf<uint32_t>("pt_order", P1);
-    uint32_t* set_off = c->buf<uint32_t>("pt_set_off", P1);
+    unsigned long long* set_off = c->buf<unsigned long long>("pt_set_off", P1);
     uint32_t* set_len = c->buf<uint32_t>("pt_set_len", P1);
+    unsigned long long* cur = c->buf<unsigned long long>("pt_merge_cur", E1);
+    unsigned long long* cend = c->buf<unsigned long long>("pt_merge_end", E1);
+    PMA_HIP(hipMemsetAsync(set_off, 0, P1 * 8, c->stream));
     PMA_HIP(hipMemsetAsync(set_len, 0, P1 * 4, c->stream));
     std::vector<uint32_t> lstart_h;
     if (P) {
@@ -753,59 +768,54 @@ void run_pores(pma_ctx* c) {
                 PMA_TIMED(c, "set_union_level");
                 set_union_level<<<grid_for(hi - lo, 64), 64, 0, c->stream>>>(order, lo, hi, in_start, in_count, edges,
                                                                              plat_pore, set_off, set_len, pool, pool_top,
-                                                                             pool_cap, of);
+                                                                             pool_cap, cur, cend, of);
                 PMA_LAUNCHED(c);
             }
         }
-        uint32_t o = read_scalar(c, of);
-        PMA_REQUIRE(!(o & 5u), PMA_ERR_INTERNAL, "pma_pores: a plateau is reached by more than 9 pores");
+        const uint32_t o = read_scalar(c, of);
         if (!(o & 2u)) break;
         PMA_REQUIRE(attempt < 3, PMA_ERR_INTERNAL, "pma_pores: pore set pool overflow");
         pool_cap = read_scalar(c, pool_top) + 1024;
     }
+    const unsigned long long pool_used = read_scalar(c, pool_top);
 
-    // 7. prefix classes (key creations), their distinct lists and final volumes.
-    uint32_t* pcnt = c->buf<uint32_t>("pt_pcnt", P1);
-    uint32_t* rbase = c->buf<uint32_t>("pt_rbase", P1);
+    // 7. key formations (prefix records) and their distinct lists.
+    unsigned long long* pcnt = c->buf<unsigned long long>("pt_pcnt", P1);
+    unsigned long long* rbase = c->buf<unsigned long long>("pt_rbase", P1);
     uint64_t R = 0;
     if (P) {
         prefix_counts<<<grid_for(P, 256), 256, 0, c->stream>>>(set_len, P, pcnt);
         PMA_LAUNCHED(c);
-        R = exclusive_scan_u32(c, pcnt, rbase, P);
+        R = exclusive_scan_u64(c, reinterpret_cast<const uint64_t*>(pcnt), reinterpret_cast<uint64_t*>(rbase), P);
     }
-    const size_t R1 = std::max<uint64_t>(R, 1);
-    uint64_t* w0 = c->buf<uint64_t>("pt_w0", R1);
-    uint64_t* w1 = c->buf<uint64_t>("pt_w1", R1);
-    uint64_t* w2 = c->buf<uint64_t>("pt_w2", R1);
-    uint64_t* tm = c->buf<uint64_t>("pt_tm", R1);
-    uint32_t* wt = c->buf<uint32_t>("pt_wt", R1);
-    uint32_t* rd = c->buf<uint32_t>("pt_rd", R1);
-    ListTable lt;
-    if (R) {
-        prefix_records<<<grid_for(P, 256), 256, 0, c->stream>>>(set_off, set_len, pool, plat_size, P, rbase, w0, w1, w2,
-                                                                tm, wt);
-        PMA_LAUNCHED(c);
-        lt = dedupe_lists(c, w0, w1, w2, tm, wt, R, rd);
-    }
-    // events per pore (the largest pore of each distinct prefix list creates it)
-    const uint32_t D = lt.distinct;
+    uint32_t* rd = c->buf<uint32_t>("pt_rd", std::max<uint64_t>(R, 1));
+    PrefixTrie trie;
+    if (R)
+        trie = build_prefix_trie(c, pool, reinterpret_cast<const uint64_t*>(set_off), set_len,
+                                 reinterpret_cast<const uint64_t*>(rbase), nullptr, plat_size, P, R, rd);
+    // the pore that forms a list is its largest member, the last one in ascending order
+    const uint32_t D = trie.distinct;
     std::vector<uint32_t> ev_owner(D), ev_local(D), ncls(npores, 0);
     for (uint32_t d = 0; d < D; ++d) {
-        std::vector<uint32_t> mem = list_members(&lt.words[3 * (size_t)d]);
-        PMA_REQUIRE(mem.size() >= 2 && mem.size() <= (size_t)kListMax, PMA_ERR_INTERNAL,
-                    "pma_pores: a throat touches more than 9 pores");
-        ev_owner[d] = mem.back();
+        ev_owner[d] = (uint32_t)trie.key[d];
+        PMA_REQUIRE(ev_owner[d] < npores, PMA_ERR_INTERNAL, "pma_pores: list member out of range");
         ev_local[d] = ncls[ev_owner[d]]++;
     }
-    // reach sizes and boundary flags
+
+    // reach sizes, boundary flags and the visited-bitmap layout (one bit per voxel and member)
     unsigned long long* reach = c->buf<unsigned long long>("pt_reach", std::max(npores, 1u));
     uint32_t* bflag = c->buf<uint32_t>("pt_bflag", std::max(npores, 1u));
+    unsigned long long* plat_bits = c->buf<unsigned long long>("pt_plat_bits", P1);
+    unsigned long long* bit_base = c->buf<unsigned long long>("pt_bit_base", P1);
     PMA_HIP(hipMemsetAsync(reach, 0, std::max(npores, 1u) * 8ull, c->stream));
     PMA_HIP(hipMemsetAsync(bflag, 0, std::max(npores, 1u) * 4ull, c->stream));
+    uint64_t total_bits = 0;
     if (P) {
         reach_sizes<<<grid_for(P, 256), 256, 0, c->stream>>>(set_off, set_len, pool, plat_size, plat_border, P, reach,
-                                                             bflag);
+                                                             bflag, plat_bits);
         PMA_LAUNCHED(c);
+        total_bits = exclusive_scan_u64(c, reinterpret_cast<const uint64_t*>(plat_bits),
+                                        reinterpret_cast<uint64_t*>(bit_base), P);
     }
     std::vector<unsigned long long> reach_h(npores);
     std::vector<uint32_t> bflag_h(npores);
@@ -815,47 +825,90 @@ void run_pores(pma_ctx* c) {
         PMA_HIP(hipStreamSynchronize(c->stream));
     }
 
-    // 8. DFS replays for pores with two or more classes: rank of each class inside its pore.
+    // 8. DFS replays for pores that form two or more lists: rank of each list inside its pore.
     std::vector<uint32_t> rank_h(D, 0);
+    uint64_t replay_frames = 0;
     {
-        std::vector<ReplayTask> tasks;
-        unsigned long long stack_total = 0;
+        std::vector<ReplayTask> all;
+        unsigned long long seen_total = 0;
         for (uint32_t k = 0; k < npores; ++k) {
             if (ncls[k] < 2) continue;
             ReplayTask t{};
             t.pore = k;
             t.seed = (uint32_t)seeds_h[k];
             t.ncls = ncls[k];
-            t.stack_off = stack_total;
-            t.stack_cap = reach_h[k] + 1;
-            stack_total += reach_h[k] + 1;
-            tasks.push_back(t);
+            t.seen_off = seen_total;
+            seen_total += (ncls[k] + 31) / 32;
+            all.push_back(t);
         }
-        if (!tasks.empty()) {
-            ReplayTask* d_tasks = c->buf<ReplayTask>("pt_tasks", tasks.size());
+        if (!all.empty()) {
+            const size_t T = all.size();
             uint32_t* d_local = c->buf<uint32_t>("pt_ev_local", std::max(D, 1u));
             uint32_t* d_rank = c->buf<uint32_t>("pt_rank", std::max(D, 1u));
-            uint32_t* visited = c->buf<uint32_t>("pt_visited", n);
-            uint32_t* stack_voxel = c->buf<uint32_t>("pt_stack_voxel", stack_total);
-            uint8_t* stack_k = c->buf<uint8_t>("pt_stack_k", stack_total);
-            upload(c, d_tasks, tasks.data(), tasks.size() * sizeof(ReplayTask));
+            uint32_t* seen = c->buf<uint32_t>("pt_seen", seen_total);
+            const size_t vwords = total_bits / 64 + 1;
+            unsigned long long* visited = c->buf<unsigned long long>("pt_visited", vwords);
+            uint32_t* status = c->buf<uint32_t>("pt_task_status", T);
+            ReplayTask* d_tasks = c->buf<ReplayTask>("pt_tasks", T);
             upload(c, d_local, ev_local.data(), D * 4ull);
             PMA_HIP(hipMemsetAsync(d_rank, 0, D * 4ull, c->stream));
-            PMA_HIP(hipMemsetAsync(visited, 0, n * 4, c->stream));
-            PMA_HIP(hipMemsetAsync(of, 0, 4, c->stream));
-            {
-                PMA_TIMED(c, "dfs_replay");
-                dfs_replay<<<(unsigned)tasks.size(), 64, 0, c->stream>>>(d_tasks, cr, plat, set_off, set_len, pool, rbase,
-                                                                        rd, d_local, visited, stack_voxel, stack_k, d_rank,
-                                                                        X, Y, Z, d3q7, of);
-                PMA_LAUNCHED(c);
+            // Stack frames: a first pass gives every task an equal share of a fixed budget;
+            // tasks that run out are repeated with room for their whole reach.
+            const unsigned long long budget = std::max<unsigned long long>(1ull << 26, n);
+            const unsigned long long share = std::max<unsigned long long>(1024, budget / T);
+            std::vector<ReplayTask> pending = all;
+            for (int pass = 0; !pending.empty(); ++pass) {
+                PMA_REQUIRE(pass < 2, PMA_ERR_INTERNAL, "pma_pores: DFS replay stack overflow with full capacity");
+                PMA_HIP(hipMemsetAsync(visited, 0, vwords * 8, c->stream));
+                PMA_HIP(hipMemsetAsync(seen, 0, seen_total * 4, c->stream));
+                std::vector<ReplayTask> failed;
+                size_t first = 0;
+                while (first < pending.size()) {
+                    // one batch: as many tasks as fit the budget (at least one)
+                    unsigned long long frames = 0;
+                    size_t last = first;
+                    std::vector<ReplayTask> batch;
+                    while (last < pending.size()) {
+                        ReplayTask t = pending[last];
+                        const unsigned long long need =
+                            pass == 0 ? std::min<unsigned long long>(reach_h[t.pore] + 1, share) : reach_h[t.pore] + 1;
+                        if (!batch.empty() && frames + need > budget && pass > 0) break;
+                        t.stack_off = frames;
+                        t.stack_cap = need;
+                        t.slot = (uint32_t)batch.size();
+                        frames += need;
+                        batch.push_back(t);
+                        ++last;
+                    }
+                    uint32_t* stack_voxel = c->buf<uint32_t>("pt_stack_voxel", frames);
+                    uint8_t* stack_k = c->buf<uint8_t>("pt_stack_k", frames);
+                    upload(c, d_tasks, batch.data(), batch.size() * sizeof(ReplayTask));
+                    PMA_HIP(hipMemsetAsync(status, 0, batch.size() * 4, c->stream));
+                    PMA_HIP(hipMemsetAsync(of, 0, 4, c->stream));
+                    {
+                        PMA_TIMED(c, "dfs_replay");
+                        dfs_replay<<<(unsigned)batch.size(), 64, 0, c->stream>>>(
+                            d_tasks, cr, plat, set_off, set_len, pool, rbase, rd, d_local, bit_base, vrank, visited,
+                            seen, stack_voxel, stack_k, d_rank, status, X, Y, Z, d3q7, of);
+                        PMA_LAUNCHED(c);
+                    }
+                    const uint32_t rerr = read_scalar(c, of);
+                    PMA_REQUIRE(rerr == 0, PMA_ERR_INTERNAL,
+                                "pma_pores: DFS replay invariant broken (code " + std::to_string(rerr) + ")");
+                    std::vector<uint32_t> st(batch.size());
+                    PMA_HIP(hipMemcpyAsync(st.data(), status, batch.size() * 4ull, hipMemcpyDeviceToHost, c->stream));
+                    PMA_HIP(hipStreamSynchronize(c->stream));
+                    for (size_t i = 0; i < batch.size(); ++i)
+                        if (st[i]) failed.push_back(batch[i]);
+                    replay_frames += frames;
+                    first = last;
+                }
+                pending.swap(failed);
             }
-            uint32_t rerr = read_scalar(c, of);
-            PMA_REQUIRE(rerr == 0, PMA_ERR_INTERNAL, "pma_pores: DFS replay invariant broken (code " + std::to_string(rerr) + ")");
             PMA_HIP(hipMemcpyAsync(rank_h.data(), d_rank, D * 4ull, hipMemcpyDeviceToHost, c->stream));
             PMA_HIP(hipStreamSynchronize(c->stream));
         }
-        c->stage[PMA_STAGE_PORES].work_items = stack_total;
+        c->stage[PMA_STAGE_PORES].work_items = replay_frames;
     }
     // creation order: by pore, then by first visit inside the pore's DFS
     std::vector<uint32_t> ord(D);
@@ -884,6 +937,7 @@ void run_pores(pma_ctx* c) {
     std::vector<unsigned long long> pvol_h(npores);
     if (npores) PMA_HIP(hipMemcpyAsync(pvol_h.data(), pvol, npores * 8ull, hipMemcpyDeviceToHost, c->stream));
     PMA_HIP(hipStreamSynchronize(c->stream));
+    (void)pool_used;
 
     // 10. tables: pores in ascending ID, keys with volume in ascending ID, lists in the
     // reference's order [p2, p1, p3, ...].
@@ -902,11 +956,11 @@ void run_pores(pma_ctx* c) {
     c->throats.clear();
     for (uint32_t r = D; r-- > 0;) {
         uint32_t d = ord[r];
-        if (lt.finals[d] == 0) continue;
-        std::vector<uint32_t> mem = list_members(&lt.words[3 * (size_t)d]);
+        if (trie.finals[d] == 0) continue;
+        std::vector<uint32_t> mem = trie.members(d);
         ThroatRow t;
         t.id = key_id[d];
-        t.volume = lt.finals[d];
+        t.volume = trie.finals[d];
         t.pores.push_back(127u + 255u * (mem[1] / 129u) + (mem[1] % 129u));
         t.pores.push_back(127u + 255u * (mem[0] / 129u) + (mem[0] % 129u));
         for (size_t q = 2; q < mem.size(); ++q) t.pores.push_back(127u + 255u * (mem[q] / 129u) + (mem[q] % 129u));

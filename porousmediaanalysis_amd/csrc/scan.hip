@@ -84,9 +84,12 @@ uint64_t exclusive_scan_u32(pma_ctx* c, const uint32_t* in, uint32_t* out, size_
     return (uint64_t)last_out + last_in;
 }
 
-void exclusive_scan_u64(pma_ctx* c, const uint64_t* in, uint64_t* out, size_t n) {
+uint64_t exclusive_scan_u64(pma_ctx* c, const uint64_t* in, uint64_t* out, size_t n) {
+    if (n == 0) return 0;
+    uint64_t last_in = read_scalar(c, in + n - 1);
     scan_impl<unsigned long long>(c, reinterpret_cast<const unsigned long long*>(in),
                                   reinterpret_cast<unsigned long long*>(out), n, 0);
+    return read_scalar(c, out + n - 1) + last_in;
 }
 
 }  // namespace pma

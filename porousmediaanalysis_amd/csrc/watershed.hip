@@ -206,7 +206,11 @@ __global__ void run_starts(const uint32_t* __restrict__ head, const uint32_t* __
     if (s < n && head[s]) start[hscan[s]] = (uint32_t)s;
 }
 
-// Distinct foreign pores of one voxel in order of first contact.
+// Distinct foreign pores of one voxel in order of first contact.  A voxel is contacted only
+// by the popped nodes among its neighbours, each popped once, so it has at most 26 foreign
+// pores.
+constexpr int kMaxContacts = 26;
+
 __device__ int foreign_list(const LogEntry* __restrict__ log, const uint32_t* __restrict__ perm, uint32_t s0,
                             uint32_t s1, uint32_t owner, uint32_t* pores, unsigned long long* times, int* overflow) {
     int m = 0;
@@ -216,7 +220,7 @@ __device__ int foreign_list(const LogEntry* __restrict__ log, const uint32_t* __
         bool seen = false;
         for (int q = 0; q < m; ++q) seen |= pores[q] == e.pore;
         if (seen) continue;
-        if (m == kListMax - 1) {
+        if (m == kMaxContacts) {
             *overflow = 1;
             break;
         }
@@ -227,54 +231,55 @@ __device__ int foreign_list(const LogEntry* __restrict__ log, const uint32_t* __
     return m;
 }
 
-__global__ void count_prefixes(const LogEntry* __restrict__ log, const uint32_t* __restrict__ perm,
+// Per voxel with contacts: the length of its list [f0, owner, f1, ...] and its formations.
+__global__ void count_contacts(const LogEntry* __restrict__ log, const uint32_t* __restrict__ perm,
                                const uint32_t* __restrict__ start, uint32_t runs, size_t n,
-                               const uint32_t* __restrict__ label, uint32_t* __restrict__ cnt,
-                               uint32_t* __restrict__ overflow) {
+                               const uint32_t* __restrict__ label, uint32_t* __restrict__ seq_len,
+                               unsigned long long* __restrict__ nrec, uint32_t* __restrict__ overflow) {
     uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= runs) return;
     uint32_t s0 = start[r], s1 = r + 1 < runs ? start[r + 1] : (uint32_t)n;
     uint32_t owner = label[log[perm[s0]].voxel] - 1;
-    uint32_t pores[kListMax];
-    unsigned long long times[kListMax];
+    uint32_t pores[kMaxContacts];
+    unsigned long long times[kMaxContacts];
     int of = 0;
-    cnt[r] = (uint32_t)foreign_list(log, perm, s0, s1, owner, pores, times, &of);
+    const int m = foreign_list(log, perm, s0, s1, owner, pores, times, &of);
+    seq_len[r] = m ? (uint32_t)m + 1 : 0;
+    nrec[r] = (unsigned long long)m;
     if (of) atomicOr(overflow, 1u);
 }
 
-__global__ void write_prefixes(const LogEntry* __restrict__ log, const uint32_t* __restrict__ perm,
-                               const uint32_t* __restrict__ start, uint32_t runs, size_t n,
-                               const uint32_t* __restrict__ label, const uint32_t* __restrict__ off,
-                               uint64_t* __restrict__ w0, uint64_t* __restrict__ w1, uint64_t* __restrict__ w2,
-                               uint64_t* __restrict__ time, uint32_t* __restrict__ final_flag,
-                               uint32_t* __restrict__ voxel, uint32_t* __restrict__ shared_owned) {
+// Writes the list [f0, owner, f1, f2, ...] of every voxel with contacts; the prefix with j
+// foreign pores (length j + 1) is formed at the first contact of f(j-1).
+__global__ void write_sequences(const LogEntry* __restrict__ log, const uint32_t* __restrict__ perm,
+                                const uint32_t* __restrict__ start, uint32_t runs, size_t n,
+                                const uint32_t* __restrict__ label, const unsigned long long* __restrict__ seq_off,
+                                const unsigned long long* __restrict__ rbase, uint32_t* __restrict__ elems,
+                                unsigned long long* __restrict__ rec_time, uint32_t* __restrict__ weight,
+                                uint32_t* __restrict__ voxel, uint32_t* __restrict__ shared_owned) {
     uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= runs) return;
     uint32_t s0 = start[r], s1 = r + 1 < runs ? start[r + 1] : (uint32_t)n;
     uint32_t vox = log[perm[s0]].voxel;
     uint32_t owner = label[vox] - 1;
-    uint32_t pores[kListMax];
-    unsigned long long times[kListMax];
+    uint32_t pores[kMaxContacts];
+    unsigned long long times[kMaxContacts];
     int of = 0;
     int m = foreign_list(log, perm, s0, s1, owner, pores, times, &of);
+    voxel[r] = vox;
+    weight[r] = m ? 1u : 0u;
     if (m == 0) return;
     atomicAdd(&shared_owned[owner], 1u);
-    // list = [f0, owner, f1, f2, ...]; prefix j (1-based count of foreign pores) formed at times[j-1]
-    uint64_t w[3] = {0, 0, 0};
-    auto put = [&](int slot, uint32_t pore) { w[slot / 3] |= (uint64_t)(pore + 1) << (21 * (slot % 3)); };
-    put(0, pores[0]);
-    put(1, owner);
-    uint32_t o = off[r];
-    for (int j = 1; j <= m; ++j) {
-        if (j >= 2) put(j, pores[j - 1]);
-        w0[o] = w[0];
-        w1[o] = w[1];
-        w2[o] = w[2];
-        time[o] = times[j - 1];
-        final_flag[o] = j == m;
-        voxel[o] = vox;
-        ++o;
-    }
+    uint32_t* e = elems + seq_off[r];
+    e[0] = pores[0];
+    e[1] = owner;
+    for (int j = 2; j <= m; ++j) e[j] = pores[j - 1];
+    for (int j = 1; j <= m; ++j) rec_time[rbase[r] + j - 1] = times[j - 1];
+}
+
+__global__ void widen_u32(const uint32_t* __restrict__ in, uint64_t* __restrict__ out, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[i];
 }
 
 __global__ void owned_counts(const uint32_t* __restrict__ label, size_t n, unsigned long long* __restrict__ counts) {
@@ -298,10 +303,10 @@ __global__ void labels_to_ids(uint32_t* __restrict__ label, size_t n, const uint
 }
 
 __global__ void shared_ids(uint32_t* __restrict__ label, const uint32_t* __restrict__ voxel,
-                           const uint32_t* __restrict__ final_flag, const uint32_t* __restrict__ rec_distinct,
-                           const uint32_t* __restrict__ key_id, size_t R) {
+                           const uint32_t* __restrict__ seq_len, const unsigned long long* __restrict__ rbase,
+                           const uint32_t* __restrict__ rec_node, const uint32_t* __restrict__ key_id, size_t runs) {
     size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < R && final_flag[r]) label[voxel[r]] = key_id[rec_distinct[r]];
+    if (r < runs && seq_len[r] >= 2) label[voxel[r]] = key_id[rec_node[rbase[r] + seq_len[r] - 2]];
 }
 
 }  // namespace
@@ -458,26 +463,33 @@ bool watershed_once(pma_ctx* c, unsigned long long log_cap, unsigned long long* 
         uint32_t* start = c->buf<uint32_t>("ws_run_start", runs);
         run_starts<<<grid_for(nlog, 256), 256, 0, c->stream>>>(head, hscan, start, nlog);
         PMA_LAUNCHED(c);
-        uint32_t* pcnt = c->buf<uint32_t>("ws_pcnt", runs);
-        uint32_t* poff = c->buf<uint32_t>("ws_poff", runs);
+        uint32_t* slen = c->buf<uint32_t>("ws_seq_len", runs);
+        unsigned long long* nrec = c->buf<unsigned long long>("ws_nrec", runs);
+        unsigned long long* soff = c->buf<unsigned long long>("ws_seq_off", runs);
+        unsigned long long* rbase = c->buf<unsigned long long>("ws_rbase", runs);
         uint32_t* of = c->buf<uint32_t>("ws_overflow", 1);
         PMA_HIP(hipMemsetAsync(of, 0, 4, c->stream));
-        count_prefixes<<<grid_for(runs, 256), 256, 0, c->stream>>>(log, perm, start, runs, nlog, label, pcnt, of);
+        count_contacts<<<grid_for(runs, 256), 256, 0, c->stream>>>(log, perm, start, runs, nlog, label, slen, nrec, of);
         PMA_LAUNCHED(c);
-        uint64_t R = exclusive_scan_u32(c, pcnt, poff, runs);
-        PMA_REQUIRE(read_scalar(c, of) == 0, PMA_ERR_INTERNAL, "pma_watershed: a voxel touches more than 9 pores");
+        PMA_REQUIRE(read_scalar(c, of) == 0, PMA_ERR_INTERNAL, "pma_watershed: a voxel has more than 26 contacts");
+        uint64_t* slen64 = c->buf<uint64_t>("ws_seq_len64", runs);
+        widen_u32<<<grid_for(runs, 256), 256, 0, c->stream>>>(slen, slen64, runs);
+        PMA_LAUNCHED(c);
+        const uint64_t nelem = exclusive_scan_u64(c, slen64, reinterpret_cast<uint64_t*>(soff), runs);
+        const uint64_t R = exclusive_scan_u64(c, reinterpret_cast<const uint64_t*>(nrec),
+                                              reinterpret_cast<uint64_t*>(rbase), runs);
         if (R) {
-            uint64_t* w0 = c->buf<uint64_t>("ws_w0", R);
-            uint64_t* w1 = c->buf<uint64_t>("ws_w1", R);
-            uint64_t* w2 = c->buf<uint64_t>("ws_w2", R);
-            uint64_t* tm = c->buf<uint64_t>("ws_tm", R);
-            uint32_t* fin = c->buf<uint32_t>("ws_fin", R);
-            uint32_t* vox = c->buf<uint32_t>("ws_vox", R);
+            uint32_t* elems = c->buf<uint32_t>("ws_elems", nelem);
+            unsigned long long* tm = c->buf<unsigned long long>("ws_tm", R);
+            uint32_t* wt = c->buf<uint32_t>("ws_weight", runs);
+            uint32_t* vox = c->buf<uint32_t>("ws_vox", runs);
             uint32_t* rd = c->buf<uint32_t>("ws_rd", R);
-            write_prefixes<<<grid_for(runs, 256), 256, 0, c->stream>>>(log, perm, start, runs, nlog, label, poff, w0, w1,
-                                                                       w2, tm, fin, vox, shared_owned);
+            write_sequences<<<grid_for(runs, 256), 256, 0, c->stream>>>(log, perm, start, runs, nlog, label, soff, rbase,
+                                                                        elems, tm, wt, vox, shared_owned);
             PMA_LAUNCHED(c);
-            ListTable lt = dedupe_lists(c, w0, w1, w2, tm, fin, R, rd);
+            PrefixTrie lt = build_prefix_trie(c, elems, reinterpret_cast<const uint64_t*>(soff), slen,
+                                              reinterpret_cast<const uint64_t*>(rbase),
+                                              reinterpret_cast<const uint64_t*>(tm), wt, runs, R, rd);
             std::vector<uint32_t> order(lt.distinct);
             for (uint32_t i = 0; i < lt.distinct; ++i) order[i] = i;
             std::sort(order.begin(), order.end(), [&](uint32_t a2, uint32_t b2) { return lt.min_time[a2] < lt.min_time[b2]; });
@@ -489,7 +501,7 @@ bool watershed_once(pma_ctx* c, unsigned long long log_cap, unsigned long long* 
                 ThroatRow& t = throats[lt.distinct - 1 - r];
                 t.id = key_id[dix];
                 t.volume = lt.finals[dix];
-                for (uint32_t q : list_members(&lt.words[3 * (size_t)dix])) t.pores.push_back(pore_id[q]);
+                for (uint32_t q : lt.members(dix)) t.pores.push_back(pore_id[q]);
             }
             // owners first, then the shared voxels
             uint32_t* d_pid = c->buf<uint32_t>("ws_pore_id", std::max(np, 1u));
@@ -520,7 +532,7 @@ bool watershed_once(pma_ctx* c, unsigned long long log_cap, unsigned long long* 
             }
             labels_to_ids<<<grid_for(n, 256), 256, 0, c->stream>>>(label, n, d_pid);
             PMA_LAUNCHED(c);
-            shared_ids<<<grid_for(R, 256), 256, 0, c->stream>>>(label, vox, fin, rd, d_kid, R);
+            shared_ids<<<grid_for(runs, 256), 256, 0, c->stream>>>(label, vox, slen, rbase, rd, d_kid, runs);
             PMA_LAUNCHED(c);
             PMA_HIP(hipStreamSynchronize(c->stream));
             c->throats_ws = throats;
