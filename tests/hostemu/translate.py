@@ -1,0 +1,45 @@
+"""Translates one of the library's .hip sources into host C++ over shim.h (test tooling).
+
+Every `kernel<<<grid, block, shmem, stream>>>(args);` becomes a launch_cpu call that runs the
+kernel body for each thread of the grid; kernels that use cross-lane operations are launched
+as wave kernels (one fiber per GPU thread).
+"""
+import re
+import sys
+
+WAVE_KERNELS = {"count_centres", "write_centres", "dfs_replay", "cr_histogram", "scan_reduce<T>", "scan_down<T>"}
+
+
+def split_args(s):
+    s = s.replace("->", "\x01")
+    out, depth, cur = [], 0, ""
+    for ch in s:
+        if ch in "(<[":
+            depth += 1
+        if ch in ")>]":
+            depth -= 1
+        if ch == "," and depth == 0:
+            out.append(cur)
+            cur = ""
+        else:
+            cur += ch
+    out.append(cur)
+    return [o.replace("\x01", "->") for o in out]
+
+
+def launch(m):
+    name, cfg, args = m.group(1), split_args(m.group(2)), m.group(3)
+    wave = "true" if name in WAVE_KERNELS else "false"
+    return f"launch_cpu({wave}, Dim3({cfg[0].strip()}), Dim3({cfg[1].strip()}), [&] {{ {name}({args}); }});"
+
+
+def translate(text):
+    text = text.replace('#include "pma_internal.h"', '#include "shim.h"')
+    text = re.sub(r"#include <rocprim/.*?>", "", text)
+    return re.sub(r"([\w<>]+?)<<<(.*?)>>>\((.*?)\);", launch, text, flags=re.S)
+
+
+if __name__ == "__main__":
+    src, dst = sys.argv[1], sys.argv[2]
+    with open(src) as f, open(dst, "w") as g:
+        g.write(f"// translated from {src}\n" + translate(f.read()))

@@ -1,0 +1,166 @@
+"""Host emulation of the pore, empty-pore-removal and watershed stages against the oracle.
+
+The library's own .hip sources (scan, libstdc++-sort emulation, prefix trie, pores, removal,
+watershed) are translated to host C++ (tests/hostemu/translate.py), built with
+AddressSanitizer and UndefinedBehaviorSanitizer and run on the oracle's inputs: every kernel
+executes its real body, thread by thread (wave kernels as fibers), with exact-size buffers, so
+an out-of-bounds access fails the test.  Outputs must equal the oracle bit for bit.  rocPRIM
+sorts are replaced by std::stable_sort, the one device helper not emulated here.
+"""
+import hashlib
+import os
+import shutil
+import subprocess
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+import pytest
+
+import oracle
+from tests import example_fixture as ex
+from tests import volumes
+from tests.volume_utils import canonical_relabel, sha256_array
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(ROOT, "porousmediaanalysis_amd", "csrc")
+EMU = os.path.join(ROOT, "tests", "hostemu")
+SOURCES = ["scan", "sort", "lists", "pores", "remove", "watershed"]
+SANITIZED = ["-std=c++20", "-O1", "-g", "-fsanitize=address,undefined", "-fno-omit-frame-pointer",
+             "-fno-sanitize-recover=undefined", f"-I{EMU}"]
+FAST = ["-std=c++20", "-O2", f"-I{EMU}"]
+ENV = {"PATH": "/usr/bin:/bin", "PMA_UNSAFE_GPU_PORES": "1",
+       "ASAN_OPTIONS": "detect_leaks=0:detect_stack_use_after_return=0",
+       "UBSAN_OPTIONS": "print_stacktrace=1"}
+
+
+def _digest():
+    h = hashlib.sha256()
+    for p in [os.path.join(CSRC, f + ".hip") for f in SOURCES] + \
+             [os.path.join(EMU, f) for f in ["shim.h", "shim.cpp", "main.cpp", "translate.py"]]:
+        with open(p, "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def _build(tmp_path_factory, flags, tag):
+    if shutil.which("g++") is None:
+        pytest.skip("g++ is not available")
+    out = tmp_path_factory.getbasetemp().parent / f"pma_hostemu_{tag}_{_digest()}"
+    exe = out / "hostemu"
+    if exe.exists():
+        return str(exe)
+    out.mkdir(parents=True, exist_ok=True)
+    import sys
+    sys.path.insert(0, EMU)
+    from translate import translate
+    jobs = []
+    for f in SOURCES:
+        with open(os.path.join(CSRC, f + ".hip")) as src:
+            (out / f"{f}.cpp").write_text(translate(src.read()))
+        jobs.append(["g++", *flags, "-c", str(out / f"{f}.cpp"), "-o", str(out / f"{f}.o")])
+    jobs.append(["g++", *flags, "-c", os.path.join(EMU, "main.cpp"), "-o", str(out / "main.o")])
+    jobs.append(["g++", "-c", os.path.join(EMU, "shim.cpp"), "-o", str(out / "shim.o")])
+    with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as pool:
+        for r in pool.map(lambda cmd: subprocess.run(cmd, capture_output=True, text=True), jobs):
+            assert r.returncode == 0, r.stderr[-4000:]
+    objs = [str(out / f"{f}.o") for f in SOURCES + ["main", "shim"]]
+    link = [f for f in flags if f.startswith("-fsanitize=")]
+    r = subprocess.run(["g++", *link, "-o", str(exe) + ".tmp", *objs, "-lpthread"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-4000:]
+    os.replace(str(exe) + ".tmp", exe)
+    return str(exe)
+
+
+@pytest.fixture(scope="session")
+def hostemu(tmp_path_factory):
+    """Sanitized build: every access is bounds-checked."""
+    return _build(tmp_path_factory, SANITIZED, "asan")
+
+
+@pytest.fixture(scope="session")
+def hostemu_fast(tmp_path_factory):
+    """Optimised build without sanitizers, for the larger fixture volume."""
+    return _build(tmp_path_factory, FAST, "fast")
+
+
+def _run(exe, stage, payload, tmp_path, shape):
+    inp, vol, txt = tmp_path / "in.bin", tmp_path / "out.bin", tmp_path / "out.txt"
+    inp.write_bytes(payload)
+    r = subprocess.run([exe, stage, str(inp), str(vol), str(txt)], capture_output=True, text=True, env=ENV,
+                       timeout=600)
+    assert r.returncode == 0, f"{stage}: rc {r.returncode}\n{r.stdout[-2000:]}\n{r.stderr[-6000:]}"
+    pores, throats = txt.read_text().split("@@\n")
+    return np.fromfile(vol, np.uint32).reshape(shape), pores, throats
+
+
+def emu_pores(exe, tmp_path, e, cr, mask, d3q7):
+    Z, X, Y = e.shape
+    payload = np.array([X, Y, Z, int(d3q7)], np.int32).tobytes() + e.astype(np.uint32).tobytes() + \
+        cr.astype(np.uint32).tobytes() + mask.astype(np.uint8).tobytes()
+    return _run(exe, "pores", payload, tmp_path, e.shape)
+
+
+def emu_remove(exe, tmp_path, pt, ptxt, ktxt):
+    Z, X, Y = pt.shape
+    payload = np.array([X, Y, Z], np.int32).tobytes() + pt.astype(np.uint32).tobytes() + \
+        (ptxt.replace("\r", "") + "@@\n" + ktxt.replace("\r", "")).encode()
+    return _run(exe, "remove", payload, tmp_path, pt.shape)
+
+
+def emu_watershed(exe, tmp_path, cr, max_e, ptxt, d3q7):
+    Z, X, Y = cr.shape
+    rows = [list(map(int, line.split())) for line in ptxt.replace("\r", "").split("\n")[1:] if line.strip()]
+    payload = np.array([X, Y, Z, int(d3q7), max_e], np.int32).tobytes() + cr.astype(np.uint32).tobytes() + \
+        np.array([len(rows)], np.int32).tobytes() + np.array(rows, np.int32).reshape(-1, 7).tobytes()
+    return _run(exe, "watershed", payload, tmp_path, cr.shape)
+
+
+def front(solid):
+    e, max_e, _ = oracle.edt(solid, threads=4)
+    cr, _, _, _, mask, _ = oracle.containing_balls(e, threads=4)
+    return e, max_e, cr, mask
+
+
+def assert_same(g, o, what):
+    assert g[1] == o[1], f"{what}: pore table differs"
+    assert g[2] == o[2], f"{what}: throat table differs"
+    if not np.array_equal(g[0], o[0]):
+        bad = np.argwhere(g[0] != o[0])
+        raise AssertionError(f"{what}: {len(bad)} labels differ, first {tuple(bad[0])}")
+
+
+CASES = dict(volumes.edge_cases())
+CASES["spheres_48"] = volumes.spheres((48, 48, 48), 40, 3, 7, 1)
+CASES["gauss_s2"] = volumes.gaussian((29, 41, 67), 2.0, 0.65, 2)
+CASES["gauss_s3_low"] = volumes.gaussian((40, 40, 40), 3.0, 0.5, 12)
+CASES["noise"] = volumes.random_noise((17, 19, 23), 0.15, 4)
+CASES["noise_many_pores"] = volumes.random_noise((24, 26, 28), 0.45, 6)
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+@pytest.mark.parametrize("d3q7", [False, True])
+def test_pores_removal_watershed(hostemu, tmp_path, name, d3q7):
+    e, max_e, cr, mask = front(CASES[name])
+    o = oracle.pores(mask, e, cr, d3q7)
+    assert_same(emu_pores(hostemu, tmp_path, e, cr, mask, d3q7), o, "pores")
+    o2 = oracle.remove_empty_pores(*o)
+    assert_same(emu_remove(hostemu, tmp_path, *o), o2, "empty-pore removal")
+    ow = oracle.watershed(cr, max_e, o2[1], d3q7)
+    assert_same(emu_watershed(hostemu, tmp_path, cr, max_e, o2[1], d3q7), ow, "watershed")
+
+
+def test_example_partitions(hostemu_fast, tmp_path):
+    """Fed the Docs/Example mask, the pore and watershed partitions equal the fixture's."""
+    hostemu = hostemu_fast
+    solid = ex.solid()
+    e, max_e, cr, _ = front(solid)
+    o = oracle.pores(ex.mask(), e, cr)
+    g = emu_pores(hostemu, tmp_path, e, cr, ex.mask(), False)
+    assert_same(g, o, "pores")
+    o2 = oracle.remove_empty_pores(*o)
+    g2 = emu_remove(hostemu, tmp_path, *o)
+    assert_same(g2, o2, "empty-pore removal")
+    assert sha256_array(canonical_relabel(g2[0])) == ex.digests()["pt_partition"]
+    gw = emu_watershed(hostemu, tmp_path, cr, max_e, o2[1], False)
+    assert_same(gw, oracle.watershed(cr, max_e, o2[1], False), "watershed")
+    assert sha256_array(canonical_relabel(gw[0])) == ex.digests()["ptw_partition"]
