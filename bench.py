@@ -149,7 +149,8 @@ def main():
         if per_launch is not None:
             achieved = per_launch / (avg_ms * 1e-3) / 1e9
             roofline = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                        "traffic": measured_traffic(name, S),
                         "bytes_per_launch": round(per_launch), "avg_launch_ms": round(avg_ms, 4)}
         else:
             roofline = {"bound": "hbm", "kernel": name, "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -187,6 +188,16 @@ def main():
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(result))
+
+
+def measured_traffic(kernel, size):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc (profiles/traffic.json)."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if size != 1024 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        entry = json.load(f).get(kernel)
+    return round(entry["traffic_bytes"]) if entry else None
 
 
 def kernel_times(pma, ctx):
