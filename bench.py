@@ -31,14 +31,15 @@ sys.path.insert(0, ROOT)
 METRIC = "voxels/s, 1024³ EDT→watershed→geodesic; batched & surrogate vs plain at MAE"
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic (compulsory) bytes per voxel of one launch of each instrumented kernel that
-# sweeps the whole volume: each input read once, each output written once.
+# Algorithmic (compulsory) bytes per voxel of each instrumented kernel that sweeps the whole
+# volume once per step (in one launch, or in z-slabs over several): each input read once,
+# each output written once.
 KERNEL_BYTES_PER_VOXEL = {
     "edt_pass_x": 1 + 4,        # u8 solid -> u32 1-D distance square
     "edt_pass_y": 4 + 4,        # u32 -> u32
     "edt_pass_z": 4 + 4,        # u32 -> u32 EDT square (+ histogram)
     "prune_centres": 4 + 1,     # u32 EDT -> u8 mask / candidate flag
-    "scatter_tiles": 4 + 16,    # u32 EDT -> u32 CR, CX, CY, CZ
+    "cover_tiles": 4 + 16,      # u32 EDT -> u32 CR, CX, CY, CZ (over all the launches of a step)
     "count_equal_u8": 1,
     "cc_union": 4 + 4,          # u32 CR -> u32 plateau parent
     "cc_not_max": 4 + 4,
@@ -47,9 +48,9 @@ KERNEL_BYTES_PER_VOXEL = {
     "finish_cavity": 1 + 4 + 4,  # u8 solid, f32 distance -> u32 cavity id
     "geo_init": 1 + 4 + 4,       # u8 solid -> f32 distance, u32 cavity id
 }
-# Per frontier item of one relax launch: its voxel index and distance read, its new-frontier
-# entry written (neighbour reads are shared between items and are not counted).
-RELAX_BYTES_PER_ITEM = 4 + 4 + 4
+# Per block visit of one relax_blocks launch: the 8^3 distances read and written once (the
+# one-voxel halo is shared with the neighbouring blocks and not counted).
+RELAX_BYTES_PER_BLOCK = 512 * (4 + 4)
 STAGES = ("edt", "containing_balls", "geodesic")
 
 
@@ -143,9 +144,9 @@ def main():
         name, (tot_ms, launches) = dom
         avg_ms = tot_ms / max(launches, 1)
         bpv = KERNEL_BYTES_PER_VOXEL.get(name)
-        per_launch = bpv * nvox if bpv is not None else None
-        if name == "relax" and launches:
-            per_launch = RELAX_BYTES_PER_ITEM * relax_items / launches
+        per_launch = bpv * nvox * args.steps / launches if bpv is not None and launches else None
+        if name == "relax_blocks" and launches:
+            per_launch = RELAX_BYTES_PER_BLOCK * relax_items / launches
         if per_launch is not None:
             achieved = per_launch / (avg_ms * 1e-3) / 1e9
             roofline = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

@@ -2,30 +2,35 @@
 //
 // Reference semantics with one thread: for every void centre c with r2 = e(c), every void
 // voxel v with |v - c|^2 <= r2 gets CR(v) = max r2; among equal maxima the first centre in
-// x, y, z scan order wins (strict '<' at :170).  Both rules are one 64-bit maximum of
-//     key(c) = r2 << 32 | ~scan(c),   scan(c) = (x * Y + y) * Z + z,
-// so the result does not depend on the order in which balls are applied.
+// x, y, z scan order wins (strict '<' at :170).  So the ball that a voxel takes is the first
+// ball covering it in the order (r2 descending, scan(c) ascending), scan(c) = (x * Y + y) * Z + z.
 //
 // Work plan:
 //   1. far tables: far_k(a) = max over lattice p with |p|^2 <= a of |p - d_k|^2 for the face,
 //      edge and corner offsets d_k.  A centre c is skipped when a stencil neighbour q has
-//      e(q) > e(c) and far_k(e(c)) <= e(q): then B(c) is a subset of B(q) and every voxel of
-//      B(c) has CR >= e(q) > e(c), so c can never be a maximiser (exact pruning).
-//   2. surviving centres are bucketed by the voxel tile that contains them.
-//   3. one workgroup per tile keeps the tile's 64-bit keys in LDS, applies every candidate
-//      ball that touches the tile with ds_max_u64 (one wave per ball, lanes along y, rows
-//      packed several per instruction for small balls) and writes CR, CX, CY, CZ once.
-//   4. every void voxel marks its centre 255 in the mask (SetMaximumBallsMaskInActiveArea).
+//      e(q) > e(c) and far_k(e(c)) <= e(q): then B(c) is a subset of B(q), every voxel of B(c)
+//      has a ball of larger r2, and c can never be taken (exact pruning).
+//   2. the surviving centres are sorted by (r2 descending, scan ascending): their rank is the
+//      order in which the reference would let them win.
+//   3. every centre lists the 8 x 8 x 64 voxel tiles its ball reaches; the (tile, rank) pairs are
+//      sorted stably by tile, so each tile sees its balls in rank order.
+//   4. one wave per tile: lane l owns the row (z, x) = (l / 8, l % 8) of the tile and a 64-bit mask
+//      of the row's voxels that are solid or already covered.  For each ball in rank order a lane
+//      takes the row's chord [y - h, y + h] of the ball; its uncovered voxels take this ball.
+//      Every voxel is written once, the first time it is covered, so no atomics are needed.
+//   5. every void voxel marks its centre 255 in the mask (SetMaximumBallsMaskInActiveArea).
 #include "pma_internal.h"
 
 namespace pma {
 namespace {
 
-constexpr int TZ = 8, TX = 16, TY = 64;
-constexpr int kTileVoxels = TZ * TX * TY;
-constexpr uint8_t kCandidate = 126;  // temporary mask value of a surviving centre
+constexpr int TZ = 8, TX = 8, TY = 64;  // tile: 64 rows (z, x) of 64 voxels along y
+#ifndef PMA_CB_SLAB_PAIRS
+#define PMA_CB_SLAB_PAIRS (1ull << 30)  // most (tile, ball) pairs sorted at once
+#endif
+constexpr uint8_t kCandidate = 126;    // temporary mask value of a surviving centre
 
-struct Candidate {
+struct alignas(16) Ball {
     int x, y, z;
     uint32_t r2;
 };
@@ -81,15 +86,16 @@ __global__ void __launch_bounds__(1024) far_prefix_max(uint32_t* __restrict__ sh
     }
 }
 
-__device__ __forceinline__ int tile_of(int x, int y, int z, int NTX, int NTY) {
-    return ((z / TZ) * NTX + (x / TX)) * NTY + (y / TY);
+__device__ __forceinline__ size_t flat_block() {
+    return ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
 }
 
-// Block = 4 rows of 64 voxels along y (one wave per row, inside one tile).
+// Block = 4 rows of 64 voxels along y.  Marks solid 0, void 127 and surviving centres
+// kCandidate, and writes the block's count of surviving centres.
 __global__ void __launch_bounds__(256) prune_centres(const uint32_t* __restrict__ e, uint8_t* __restrict__ mask,
                                                      const uint32_t* __restrict__ far, uint32_t L,
-                                                     uint32_t* __restrict__ tile_count, int X, int Y, int Z,
-                                                     int NTX, int NTY, int d3q7) {
+                                                     uint32_t* __restrict__ block_count, int X, int Y, int Z) {
+    __shared__ uint32_t part[4];
     int y = blockIdx.x * 64 + (threadIdx.x & 63);
     int x = blockIdx.y * 4 + (threadIdx.x >> 6);
     int z = blockIdx.z;
@@ -101,7 +107,6 @@ __global__ void __launch_bounds__(256) prune_centres(const uint32_t* __restrict_
             mask[i] = 0;
         } else {
             keep = true;
-            const uint32_t* fc = far;
             for (int dz = -1; dz <= 1 && keep; ++dz)
                 for (int dx = -1; dx <= 1 && keep; ++dx)
                     for (int dy = -1; dy <= 1; ++dy) {
@@ -110,22 +115,25 @@ __global__ void __launch_bounds__(256) prune_centres(const uint32_t* __restrict_
                         int nx = x + dx, ny = y + dy, nz = z + dz;
                         if (nx < 0 || nx >= X || ny < 0 || ny >= Y || nz < 0 || nz >= Z) continue;
                         uint32_t q = e[((size_t)nz * X + nx) * Y + ny];
-                        if (q > c && fc[(size_t)(k - 1) * L + c] <= q) { keep = false; break; }
+                        if (q > c && far[(size_t)(k - 1) * L + c] <= q) { keep = false; break; }
                     }
             mask[i] = keep ? kCandidate : 127;
         }
     }
     uint64_t bal = __ballot(keep);
-    if ((threadIdx.x & 63) == 0 && bal && x < X)
-        atomicAdd(&tile_count[tile_of(x, blockIdx.x * 64, z, NTX, NTY)], (uint32_t)__popcll(bal));
-    (void)d3q7;
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    if (threadIdx.x == 0) block_count[flat_block()] = part[0] + part[1] + part[2] + part[3];
 }
 
+// Sort key of every surviving centre: (maxE - r2) << 32 | scan, ascending = the win order.
+// Written at the block's offset (exclusive scan of the block counts) plus its rank in the block.
 __global__ void __launch_bounds__(256) emit_centres(const uint32_t* __restrict__ e, uint8_t* __restrict__ mask,
-                                                    const uint32_t* __restrict__ tile_off,
-                                                    uint32_t* __restrict__ tile_cursor, Candidate* __restrict__ cand,
-                                                    int X, int Y, int NTX, int NTY) {
-    int lane = threadIdx.x & 63;
+                                                    const unsigned long long* __restrict__ block_off,
+                                                    unsigned long long* __restrict__ keys, unsigned long long cap,
+                                                    uint32_t max_e, int X, int Y, int Z) {
+    __shared__ uint32_t part[4];
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int y = blockIdx.x * 64 + lane;
     int x = blockIdx.y * 4 + (threadIdx.x >> 6);
     int z = blockIdx.z;
@@ -136,99 +144,172 @@ __global__ void __launch_bounds__(256) emit_centres(const uint32_t* __restrict__
         keep = mask[i] == kCandidate;
     }
     uint64_t bal = __ballot(keep);
-    if (!bal) return;
-    int t = x < X ? tile_of(x, blockIdx.x * 64, z, NTX, NTY) : 0;
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(&tile_cursor[t], (uint32_t)__popcll(bal));
-    base = __shfl(base, 0);
+    if (lane == 0) part[wave] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    unsigned long long base = block_off[flat_block()];
+    for (int w = 0; w < wave; ++w) base += part[w];
     if (keep) {
-        uint32_t rank = (uint32_t)__popcll(bal & ((1ull << lane) - 1));
-        cand[tile_off[t] + base + rank] = Candidate{x, y, z, e[i]};
+        unsigned long long slot = base + (unsigned long long)__popcll(bal & ((1ull << lane) - 1));
+        const uint32_t scan = (uint32_t)(((uint64_t)x * Y + y) * (uint64_t)Z + z);
+        if (slot < cap) keys[slot] = ((unsigned long long)(max_e - e[i]) << 32) | scan;
         mask[i] = 127;
     }
 }
 
-__global__ void __launch_bounds__(256) scatter_tiles(const Candidate* __restrict__ cand,
-                                                     const uint32_t* __restrict__ tile_off,
-                                                     const uint32_t* __restrict__ tile_count,
-                                                     const uint32_t* __restrict__ e, uint32_t* __restrict__ cr,
-                                                     uint32_t* __restrict__ cx, uint32_t* __restrict__ cy,
-                                                     uint32_t* __restrict__ cz, uint8_t* __restrict__ mask,
-                                                     int X, int Y, int Z, int NTX, int NTY, int NTZ, int Rmax,
-                                                     uint32_t* __restrict__ error_flag) {
-    __shared__ unsigned long long key[kTileVoxels];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int t = blockIdx.x;
-    const int ty = t % NTY, tx = (t / NTY) % NTX, tz = t / (NTY * NTX);
-    const int z0 = tz * TZ, x0 = tx * TX, y0 = ty * TY;
-    const int z1 = min(z0 + TZ, Z) - 1, x1 = min(x0 + TX, X) - 1, y1 = min(y0 + TY, Y) - 1;
-    for (int k = threadIdx.x; k < kTileVoxels; k += blockDim.x) key[k] = 0ull;
-    __syncthreads();
+__global__ void widen_counts(const uint32_t* __restrict__ in, size_t n, unsigned long long* __restrict__ out) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[i];
+}
 
-    const int rz = (Rmax + TZ - 1) / TZ, rx = (Rmax + TX - 1) / TX, ry = (Rmax + TY - 1) / TY;
-    for (int ntz = max(tz - rz, 0); ntz <= min(tz + rz, NTZ - 1); ++ntz)
-        for (int ntx = max(tx - rx, 0); ntx <= min(tx + rx, NTX - 1); ++ntx)
-            for (int nty = max(ty - ry, 0); nty <= min(ty + ry, NTY - 1); ++nty) {
-                const int nt = (ntz * NTX + ntx) * NTY + nty;
-                const uint32_t start = tile_off[nt], cnt = tile_count[nt];
-                for (uint32_t k0 = wave * 64; k0 < cnt; k0 += blockDim.x) {
-                    Candidate cd{0, 0, 0, 0};
-                    bool touch = false;
-                    if (k0 + lane < cnt) {
-                        cd = cand[start + k0 + lane];
-                        int dz = max(max(z0 - cd.z, cd.z - z1), 0);
-                        int dx = max(max(x0 - cd.x, cd.x - x1), 0);
-                        int dy = max(max(y0 - cd.y, cd.y - y1), 0);
-                        touch = (uint32_t)(dz * dz + dx * dx + dy * dy) <= cd.r2;
-                    }
-                    uint64_t todo = __ballot(touch);
-                    while (todo) {
-                        int src = __ffsll((long long)todo) - 1;
-                        todo &= todo - 1;
-                        const int bx = __shfl(cd.x, src), by = __shfl(cd.y, src), bz = __shfl(cd.z, src);
-                        const uint32_t r2 = (uint32_t)__shfl((int)cd.r2, src);
-                        const int R = (int)isqrt_u32(r2);
-                        const uint32_t scan = (uint32_t)(((uint64_t)bx * Y + by) * (uint64_t)Z + bz);
-                        const unsigned long long K = ((unsigned long long)r2 << 32) | (uint32_t)~scan;
-                        const int zlo = max(bz - R, z0), zhi = min(bz + R, z1);
-                        const int xlo = max(bx - R, x0), xhi = min(bx + R, x1);
-                        const int nxr = xhi - xlo + 1;
-                        const int rows = (zhi - zlo + 1) * nxr;
-                        int width = min(2 * R + 1, TY);
-                        int P = 1;
-                        while (P < width) P <<= 1;
-                        const int G = 64 / P, g = lane / P, off = lane % P;
-                        for (int r = g; r < rows; r += G) {
-                            const int zz = zlo + r / nxr, xx = xlo + r % nxr;
-                            const int rem = (int)r2 - (zz - bz) * (zz - bz) - (xx - bx) * (xx - bx);
-                            if (rem < 0) continue;
-                            const int h = (int)isqrt_u32((uint32_t)rem);
-                            const int ylo = max(by - h, y0), yhi = min(by + h, y1);
-                            const int yy = ylo + off;
-                            if (yy <= yhi)
-                                atomicMax(&key[((zz - z0) * TX + (xx - x0)) * TY + (yy - y0)], K);
-                        }
-                    }
+__global__ void decode_centres(const unsigned long long* __restrict__ keys, size_t n, uint32_t max_e, int Y, int Z,
+                               Ball* __restrict__ balls) {
+    size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const unsigned long long k = keys[r];
+    const uint32_t scan = (uint32_t)k;
+    Ball b;
+    b.z = (int)(scan % (uint32_t)Z);
+    const uint32_t xy = scan / (uint32_t)Z;
+    b.y = (int)(xy % (uint32_t)Y);
+    b.x = (int)(xy / (uint32_t)Y);
+    b.r2 = max_e - (uint32_t)(k >> 32);
+    balls[r] = b;
+}
+
+// Tiles of the slab [tz0, tz1) that ball b reaches (box distance^2 <= r2): counted (out == null)
+// or listed as (tile - tile_base, rank) pairs from `at`.
+__device__ uint32_t ball_tiles(const Ball& b, int X, int Y, int Z, int NTX, int NTY, int tz0, int tz1,
+                               uint32_t tile_base, uint32_t rank, uint32_t* __restrict__ pk,
+                               uint32_t* __restrict__ pv, unsigned long long at) {
+    const int R = (int)isqrt_u32(b.r2);
+    const int za = max(b.z - R, 0) / TZ, zb = min(b.z + R, Z - 1) / TZ;
+    const int xa = max(b.x - R, 0) / TX, xb = min(b.x + R, X - 1) / TX;
+    const int ya = max(b.y - R, 0) / TY, yb = min(b.y + R, Y - 1) / TY;
+    uint32_t n = 0;
+    for (int tz = max(za, tz0); tz <= min(zb, tz1 - 1); ++tz) {
+        const int z0 = tz * TZ, z1 = min(z0 + TZ, Z) - 1;
+        const int dz = max(max(z0 - b.z, b.z - z1), 0);
+        for (int tx = xa; tx <= xb; ++tx) {
+            const int x0 = tx * TX, x1 = min(x0 + TX, X) - 1;
+            const int dx = max(max(x0 - b.x, b.x - x1), 0);
+            for (int ty = ya; ty <= yb; ++ty) {
+                const int y0 = ty * TY, y1 = min(y0 + TY, Y) - 1;
+                const int dy = max(max(y0 - b.y, b.y - y1), 0);
+                if ((uint32_t)(dz * dz + dx * dx + dy * dy) > b.r2) continue;
+                if (pk) {
+                    pk[at + n] = (uint32_t)((tz * NTX + tx) * NTY + ty) - tile_base;
+                    pv[at + n] = rank;
                 }
+                ++n;
             }
-    __syncthreads();
-    const uint64_t YZ = (uint64_t)Y * Z;
-    for (int k = threadIdx.x; k < kTileVoxels; k += blockDim.x) {
-        const int zz = z0 + k / (TX * TY), xx = x0 + (k / TY) % TX, yy = y0 + k % TY;
-        if (zz > z1 || xx > x1 || yy > y1) continue;
-        const size_t i = ((size_t)zz * X + xx) * Y + yy;
-        const unsigned long long K = key[k];
+        }
+    }
+    return n;
+}
+
+__global__ void count_pairs(const Ball* __restrict__ balls, size_t n, int X, int Y, int Z, int NTX, int NTY, int tz0,
+                            int tz1, unsigned long long* __restrict__ cnt) {
+    size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    cnt[r] = ball_tiles(balls[r], X, Y, Z, NTX, NTY, tz0, tz1, 0, 0, nullptr, nullptr, 0);
+}
+
+__global__ void emit_pairs(const Ball* __restrict__ balls, size_t n, const unsigned long long* __restrict__ off,
+                           int X, int Y, int Z, int NTX, int NTY, int tz0, int tz1, uint32_t tile_base,
+                           uint32_t* __restrict__ pk, uint32_t* __restrict__ pv) {
+    size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    ball_tiles(balls[r], X, Y, Z, NTX, NTY, tz0, tz1, tile_base, (uint32_t)r, pk, pv, off[r]);
+}
+
+// [start, end) of every tile's run in the tile-sorted pair list.
+__global__ void tile_ranges(const uint32_t* __restrict__ pk, size_t P, uint32_t* __restrict__ start,
+                            uint32_t* __restrict__ end) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    const uint32_t t = pk[i];
+    if (i == 0 || pk[i - 1] != t) start[t] = (uint32_t)i;
+    if (i + 1 == P || pk[i + 1] != t) end[t] = (uint32_t)(i + 1);
+}
+
+// One wave per tile (see the work plan, step 4).
+__global__ void __launch_bounds__(64) cover_tiles(const Ball* __restrict__ balls, const uint32_t* __restrict__ rank_of,
+                                                  const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
+                                                  const uint32_t* __restrict__ e, uint32_t* __restrict__ cr,
+                                                  uint32_t* __restrict__ cx, uint32_t* __restrict__ cy,
+                                                  uint32_t* __restrict__ cz, uint8_t* __restrict__ mask, int X, int Y,
+                                                  int Z, int NTX, int NTY, uint32_t tile_base,
+                                                  uint32_t* __restrict__ error_flag) {
+    __shared__ uint32_t win[TZ * TX * TY];
+    const int lane = threadIdx.x;
+    const uint32_t local = blockIdx.x;
+    const uint32_t t = tile_base + local;
+    const int ty = (int)(t % (uint32_t)NTY), tx = (int)((t / (uint32_t)NTY) % (uint32_t)NTX);
+    const int tz = (int)(t / ((uint32_t)NTY * (uint32_t)NTX));
+    const int z = tz * TZ + lane / TX, x = tx * TX + lane % TX, y0 = ty * TY;
+
+    // coverage of this lane's row: solid voxels and voxels outside the image count as covered
+    uint64_t cov = ~0ull;
+    for (int r = 0; r < 64; ++r) {
+        const int zr = tz * TZ + r / TX, xr = tx * TX + r % TX, yr = y0 + lane;
+        const bool open = zr < Z && xr < X && yr < Y && e[((size_t)zr * X + xr) * Y + yr] != 0;
+        const uint64_t b = __ballot(open);
+        if (lane == r) cov = ~b;
+    }
+
+    const uint32_t s = start[local], en = end[local];
+    for (uint32_t base = s; base < en; base += 64) {
+        if (__ballot(cov != ~0ull) == 0) break;  // every void voxel of the tile has its ball
+        uint32_t rk = 0;
+        Ball mb{0, 0, 0, 0};
+        if (base + lane < en) {
+            rk = rank_of[base + lane];
+            mb = balls[rk];
+        }
+        const int cnt = (int)min(64u, en - base);
+        for (int j = 0; j < cnt; ++j) {
+            const int bx = __shfl(mb.x, j), by = __shfl(mb.y, j), bz = __shfl(mb.z, j);
+            const int r2 = __shfl((int)mb.r2, j);
+            const uint32_t rank = (uint32_t)__shfl((int)rk, j);
+            const int dz = z - bz, dx = x - bx;
+            const int rem = r2 - dz * dz - dx * dx;
+            if (rem < 0) continue;
+            const int h = (int)isqrt_u32((uint32_t)rem);
+            const int lo = max(by - h - y0, 0), hi = min(by + h - y0, 63);
+            if (lo > hi) continue;
+            uint64_t nw = ((~0ull) >> (63 - (hi - lo))) << lo;
+            nw &= ~cov;
+            if (!nw) continue;
+            cov |= nw;
+            do {
+                const int bit = __ffsll((long long)nw) - 1;
+                win[lane * TY + bit] = rank;
+                nw &= nw - 1;
+            } while (nw);
+        }
+    }
+
+    // outputs, one row of 64 voxels per step (lanes along y)
+    for (int r = 0; r < 64; ++r) {
+        const uint64_t row_cov = __shfl(cov, r);
+        const int zr = tz * TZ + r / TX, xr = tx * TX + r % TX, yr = y0 + lane;
+        if (zr >= Z || xr >= X || yr >= Y) continue;
+        const size_t i = ((size_t)zr * X + xr) * Y + yr;
         if (e[i] == 0) {
             cr[i] = 0u;
             cx[i] = cy[i] = cz[i] = 0xFFFFFFFFu;
             continue;
         }
-        if (K == 0ull) { atomicOr(error_flag, 1u); continue; }
-        const uint64_t scan = (uint32_t)~(uint32_t)K;
-        const uint32_t ox = (uint32_t)(scan / YZ), oy = (uint32_t)((scan / Z) % Y), oz = (uint32_t)(scan % Z);
-        cr[i] = (uint32_t)(K >> 32);
-        cx[i] = ox; cy[i] = oy; cz[i] = oz;
-        mask[((size_t)oz * X + ox) * Y + oy] = 255;
+        if (!((row_cov >> lane) & 1ull)) {
+            atomicOr(error_flag, 1u);
+            continue;
+        }
+        const Ball b = balls[win[r * TY + lane]];
+        cr[i] = b.r2;
+        cx[i] = (uint32_t)b.x;
+        cy[i] = (uint32_t)b.y;
+        cz[i] = (uint32_t)b.z;
+        mask[((size_t)b.z * X + b.x) * Y + b.y] = 255;
     }
 }
 
@@ -260,39 +341,102 @@ void run_containing_balls(pma_ctx* c) {
     far_prefix_max<<<3, 1024, 0, c->stream>>>(far, L);
     PMA_LAUNCHED(c);
 
-    const int NTZ = (Z + TZ - 1) / TZ, NTX = (X + TX - 1) / TX, NTY = (Y + TY - 1) / TY;
-    const size_t NT = (size_t)NTZ * NTX * NTY;
-    uint32_t* tile_count = c->buf<uint32_t>("cb_tile_count", NT);
-    uint32_t* tile_off = c->buf<uint32_t>("cb_tile_off", NT);
-    uint32_t* tile_cursor = c->buf<uint32_t>("cb_tile_cursor", NT);
-    PMA_HIP(hipMemsetAsync(tile_count, 0, NT * sizeof(uint32_t), c->stream));
-    PMA_HIP(hipMemsetAsync(tile_cursor, 0, NT * sizeof(uint32_t), c->stream));
     const uint32_t* e = c->v<uint32_t>(PMA_VOL_EDT2);
     uint8_t* mask = c->v<uint8_t>(PMA_VOL_MASK);
-    dim3 vgrid(grid_for(Y, 64), grid_for(X, 4), Z);
+    const dim3 vgrid(grid_for(Y, 64), grid_for(X, 4), Z);
+
+    // 1-2. surviving centres, sorted into win order.
+    const size_t nblocks = (size_t)vgrid.x * vgrid.y * vgrid.z;
+    uint32_t* bcount = c->buf<uint32_t>("cb_block_count", nblocks);
+    unsigned long long* bcount64 = c->buf<unsigned long long>("cb_block_count64", nblocks);
+    unsigned long long* boff = c->buf<unsigned long long>("cb_block_off", nblocks);
     {
         PMA_TIMED(c, "prune_centres");
-        prune_centres<<<vgrid, 256, 0, c->stream>>>(e, mask, far, (uint32_t)L, tile_count, X, Y, Z, NTX, NTY,
-                                                     c->stencil == PMA_STENCIL_D3Q7);
+        prune_centres<<<vgrid, 256, 0, c->stream>>>(e, mask, far, (uint32_t)L, bcount, X, Y, Z);
         PMA_LAUNCHED(c);
     }
-    uint64_t n_cand = exclusive_scan_u32(c, tile_count, tile_off, NT);
-    Candidate* cand = c->buf<Candidate>("cb_candidates", std::max<uint64_t>(n_cand, 1));
+    widen_counts<<<grid_for(nblocks, 256), 256, 0, c->stream>>>(bcount, nblocks, bcount64);
+    PMA_LAUNCHED(c);
+    const unsigned long long n_cand = exclusive_scan_u64(c, reinterpret_cast<const uint64_t*>(bcount64),
+                                                         reinterpret_cast<uint64_t*>(boff), nblocks);
+    const size_t N1 = std::max<unsigned long long>(n_cand, 1);
+    unsigned long long* keys = c->buf<unsigned long long>("cb_keys", N1);
+    unsigned long long* keys2 = c->buf<unsigned long long>("cb_keys2", N1);
     {
         PMA_TIMED(c, "emit_centres");
-        emit_centres<<<vgrid, 256, 0, c->stream>>>(e, mask, tile_off, tile_cursor, cand, X, Y, NTX, NTY);
+        emit_centres<<<vgrid, 256, 0, c->stream>>>(e, mask, boff, keys, n_cand, max_e, X, Y, Z);
         PMA_LAUNCHED(c);
     }
+    int hi_bits = 0;
+    while (hi_bits < 32 && (max_e >> hi_bits) != 0) ++hi_bits;
+    radix_sort_keys_u64(c, reinterpret_cast<const uint64_t*>(keys), reinterpret_cast<uint64_t*>(keys2), n_cand, 0,
+                        32 + hi_bits);
+    Ball* balls = c->buf<Ball>("cb_balls", N1);
+    if (n_cand) {
+        decode_centres<<<grid_for(n_cand, 256), 256, 0, c->stream>>>(keys2, n_cand, max_e, Y, Z, balls);
+        PMA_LAUNCHED(c);
+    }
+
+    // 3-4. (tile, rank) pairs sorted by tile, then one wave per tile; z-slabs of tiles keep the
+    // pair list below kSlabPairs.
+    const int NTZ = (Z + TZ - 1) / TZ, NTX = (X + TX - 1) / TX, NTY = (Y + TY - 1) / TY;
+    const uint32_t layer_tiles = (uint32_t)NTX * (uint32_t)NTY;
+    constexpr unsigned long long kSlabPairs = PMA_CB_SLAB_PAIRS;
+    unsigned long long* pcount = c->buf<unsigned long long>("cb_pair_count", N1);
+    unsigned long long* poff = c->buf<unsigned long long>("cb_pair_off", N1);
     uint32_t* err = c->buf<uint32_t>("cb_error", 1);
     PMA_HIP(hipMemsetAsync(err, 0, sizeof(uint32_t), c->stream));
-    {
-        PMA_TIMED(c, "scatter_tiles");
-        scatter_tiles<<<(unsigned)NT, 256, 0, c->stream>>>(cand, tile_off, tile_count, e, c->v<uint32_t>(PMA_VOL_CR2),
-                                                            c->v<uint32_t>(PMA_VOL_CX), c->v<uint32_t>(PMA_VOL_CY),
-                                                            c->v<uint32_t>(PMA_VOL_CZ), mask, X, Y, Z, NTX, NTY, NTZ,
-                                                            R, err);
-        PMA_LAUNCHED(c);
+    uint64_t total_pairs = 0;
+    int slabs = 1;
+    for (int tz0 = 0; tz0 < NTZ;) {
+        int tz1 = std::min(NTZ, tz0 + (NTZ + slabs - 1) / slabs);
+        uint64_t P = 0;
+        while (true) {
+            if (n_cand) {
+                count_pairs<<<grid_for(n_cand, 256), 256, 0, c->stream>>>(balls, n_cand, X, Y, Z, NTX, NTY, tz0, tz1,
+                                                                          pcount);
+                PMA_LAUNCHED(c);
+                P = exclusive_scan_u64(c, reinterpret_cast<const uint64_t*>(pcount), reinterpret_cast<uint64_t*>(poff),
+                                       n_cand);
+            }
+            if (P <= kSlabPairs || tz1 - tz0 == 1) break;
+            tz1 = tz0 + (tz1 - tz0 + 1) / 2;  // halve the slab until its pairs fit
+        }
+        PMA_REQUIRE(P <= 0xFFFFFFFFull, PMA_ERR_CAPACITY, "pma_containing_balls: one tile layer reaches too many balls");
+        const uint32_t tile_base = (uint32_t)tz0 * layer_tiles;
+        const uint32_t slab_tiles = (uint32_t)(tz1 - tz0) * layer_tiles;
+        const size_t P1 = std::max<uint64_t>(P, 1);
+        uint32_t* pk = c->buf<uint32_t>("cb_pair_tile", P1);
+        uint32_t* pv = c->buf<uint32_t>("cb_pair_rank", P1);
+        uint32_t* pk2 = c->buf<uint32_t>("cb_pair_tile2", P1);
+        uint32_t* pv2 = c->buf<uint32_t>("cb_pair_rank2", P1);
+        uint32_t* tstart = c->buf<uint32_t>("cb_tile_start", slab_tiles);
+        uint32_t* tend = c->buf<uint32_t>("cb_tile_end", slab_tiles);
+        PMA_HIP(hipMemsetAsync(tstart, 0, slab_tiles * sizeof(uint32_t), c->stream));
+        PMA_HIP(hipMemsetAsync(tend, 0, slab_tiles * sizeof(uint32_t), c->stream));
+        if (P) {
+            emit_pairs<<<grid_for(n_cand, 256), 256, 0, c->stream>>>(balls, n_cand, poff, X, Y, Z, NTX, NTY, tz0, tz1,
+                                                                     tile_base, pk, pv);
+            PMA_LAUNCHED(c);
+            int tile_bits = 1;
+            while (tile_bits < 32 && (slab_tiles >> tile_bits) != 0) ++tile_bits;
+            radix_sort_pairs_u32(c, pk, pk2, pv, pv2, P, 0, tile_bits);
+            tile_ranges<<<grid_for(P, 256), 256, 0, c->stream>>>(pk2, P, tstart, tend);
+            PMA_LAUNCHED(c);
+        }
+        {
+            PMA_TIMED(c, "cover_tiles");
+            cover_tiles<<<slab_tiles, 64, 0, c->stream>>>(balls, pv2, tstart, tend, e, c->v<uint32_t>(PMA_VOL_CR2),
+                                                         c->v<uint32_t>(PMA_VOL_CX), c->v<uint32_t>(PMA_VOL_CY),
+                                                         c->v<uint32_t>(PMA_VOL_CZ), mask, X, Y, Z, NTX, NTY, tile_base,
+                                                         err);
+            PMA_LAUNCHED(c);
+        }
+        total_pairs += P;
+        tz0 = tz1;
+        slabs = std::max(slabs, (int)((total_pairs + kSlabPairs - 1) / kSlabPairs));
     }
+
     unsigned long long* cnt = c->buf<unsigned long long>("cb_count", 1);
     PMA_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), c->stream));
     {
@@ -302,7 +446,7 @@ void run_containing_balls(pma_ctx* c) {
     }
     PMA_REQUIRE(read_scalar(c, err) == 0, PMA_ERR_INTERNAL, "pma_containing_balls: a void voxel was left uncovered");
     c->n_max_balls = read_scalar(c, cnt);
-    c->stage[PMA_STAGE_CONTAINING_BALLS].work_items = n_cand;
+    c->stage[PMA_STAGE_CONTAINING_BALLS].work_items = total_pairs;
     for (pma_volume w : {PMA_VOL_CR2, PMA_VOL_CX, PMA_VOL_CY, PMA_VOL_CZ, PMA_VOL_MASK}) c->valid[w] = true;
 }
 

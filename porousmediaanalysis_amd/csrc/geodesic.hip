@@ -4,9 +4,13 @@
 // fl(d(u) + w) with w = sqrtf(1|2|3) by |offset|_1 (1.0f with D3Q7).  fl(a + w) is isotone
 // in a and never below a, so the result is the least fixed point
 //     d(v) = min(d(v), min_u fl(d(u) + w)),  i.e. the minimum over paths of the left-fold
-// float32 sum, whatever the settling order.  It is computed here by frontier relaxation:
-// an atomicMin on the bit pattern of non-negative floats, re-queueing improved voxels
-// until no distance changes.  Every add is one correctly rounded fp32 add (no contraction).
+// float32 sum, whatever the settling order.  It is computed here by block-synchronous label
+// correcting: the volume is cut into 8 x 8 x 8 blocks; an active block stages itself and a
+// one-voxel halo in LDS (solid = -1, as in the distance volume), relaxes its own voxels by
+// pulling from their neighbours until nothing changes, writes back what improved, and
+// activates the neighbour blocks across any face, edge or corner where a border voxel
+// improved.  Rounds repeat until no block is active.  Every add is one correctly rounded
+// fp32 add (no contraction).
 //
 // Start voxels follow SelectStartingPosition / FindVoidPixelOnBoxBoundaries (:396-511):
 // the centre min((int)(0.5 * size + 0.5), size - 1) if void and unreached, else the first
@@ -57,79 +61,150 @@ __device__ unsigned long long shell_key(const int v[3], const int c[3], const in
     return best;
 }
 
-__global__ void find_start(const uint8_t* __restrict__ solid, const float* __restrict__ dist, int X, int Y, int Z,
-                           int cx, int cy, int cz, int lo_h, int hi_h, unsigned long long* __restrict__ out) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Grid-stride; one atomicMin per block.
+__global__ void __launch_bounds__(256) find_start(const uint8_t* __restrict__ solid, const float* __restrict__ dist,
+                                                  int X, int Y, int Z, int cx, int cy, int cz, int lo_h, int hi_h,
+                                                  unsigned long long* __restrict__ out) {
+    __shared__ unsigned long long part[4];
     unsigned long long best = ~0ull;
-    if (i < (size_t)X * Y * Z && solid[i] == 0 && __float_as_uint(dist[i]) == kFltMaxBits) {
+    const size_t n = (size_t)X * Y * Z;
+    const int c[3] = {cx, cy, cz}, S[3] = {X, Y, Z};
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        if (solid[i] != 0 || __float_as_uint(dist[i]) != kFltMaxBits) continue;
         int v[3];
         v[1] = (int)(i % Y);
         size_t r = i / Y;
         v[0] = (int)(r % X);
         v[2] = (int)(r / X);
-        const int c[3] = {cx, cy, cz}, S[3] = {X, Y, Z};
-        best = shell_key(v, c, S, lo_h, hi_h);
+        best = min(best, shell_key(v, c, S, lo_h, hi_h));
     }
     for (int off = 32; off > 0; off >>= 1) best = min(best, (unsigned long long)__shfl_xor(best, off));
-    if ((threadIdx.x & 63) == 0 && best != ~0ull) atomicMin(out, best);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        best = min(min(part[0], part[1]), min(part[2], part[3]));
+        if (best != ~0ull) atomicMin(out, best);
+    }
 }
 
-__global__ void set_source(float* dist, size_t i, uint32_t* frontier, uint32_t* count) {
-    dist[i] = 0.0f;
-    frontier[0] = (uint32_t)i;
-    *count = 1;
-}
 
-__global__ void __launch_bounds__(256) relax(const uint8_t* __restrict__ solid, float* __restrict__ dist,
-                                             const uint32_t* __restrict__ frontier, uint32_t nf,
-                                             uint32_t* __restrict__ next, uint32_t* __restrict__ next_count,
-                                             uint32_t* __restrict__ stamp, uint32_t round, int X, int Y, int Z,
-                                             int d3q7) {
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nf) return;
-    const uint32_t i = frontier[t];
-    const int y = (int)(i % (uint32_t)Y);
-    const uint32_t r = i / (uint32_t)Y;
-    const int x = (int)(r % (uint32_t)X), z = (int)(r / (uint32_t)X);
-    const float d = dist[i];
+constexpr int kB = 8;           // block edge
+constexpr int kH = kB + 2;      // with halo
+constexpr int kHalo = kH * kH * kH;
+
+// One workgroup (256 threads, two voxels each) per active block.  The voxel `src` (or none,
+// ~0) starts at distance 0 and counts as improved, so the blocks around it wake up too.
+__global__ void __launch_bounds__(256) relax_blocks(float* __restrict__ dist, const uint32_t* __restrict__ active,
+                                                    uint32_t* __restrict__ next, uint32_t* __restrict__ next_count,
+                                                    uint32_t* __restrict__ block_tag, uint32_t tag, int X, int Y, int Z,
+                                                    int NBX, int NBY, int NBZ, int d3q7, unsigned long long src) {
+    __shared__ float sd[kHalo];
+    __shared__ int s_changed;
+    __shared__ uint32_t s_touch;  // bit (dz+1)*9 + (dx+1)*3 + (dy+1): neighbour blocks to activate
+    __shared__ uint32_t s_push[27];
+    __shared__ uint32_t s_npush, s_base;
+    const uint32_t b = active[blockIdx.x];
+    const int by = (int)(b % (uint32_t)NBY), bx = (int)((b / (uint32_t)NBY) % (uint32_t)NBX);
+    const int bz = (int)(b / ((uint32_t)NBY * (uint32_t)NBX));
+    const int z0 = bz * kB - 1, x0 = bx * kB - 1, y0 = by * kB - 1;
+    for (int k = threadIdx.x; k < kHalo; k += blockDim.x) {
+        const int lz = k / (kH * kH), lx = (k / kH) % kH, ly = k % kH;
+        const int z = z0 + lz, x = x0 + lx, y = y0 + ly;
+        sd[k] = (z >= 0 && z < Z && x >= 0 && x < X && y >= 0 && y < Y) ? dist[((size_t)z * X + x) * Y + y] : -1.0f;
+    }
+    if (threadIdx.x == 0) s_touch = 0;
+    __syncthreads();
+
+    int own[2];
+    float orig[2];
+    for (int o = 0; o < 2; ++o) {
+        const int v = threadIdx.x + o * 256;
+        own[o] = ((v / 64 + 1) * kH + (v / 8) % 8 + 1) * kH + v % 8 + 1;
+        orig[o] = sd[own[o]];
+        const int lz = v / 64, lx = (v / 8) % 8, ly = v % 8;
+        const int z = z0 + 1 + lz, x = x0 + 1 + lx, y = y0 + 1 + ly;
+        if (z < Z && x < X && y < Y && ((size_t)z * X + x) * Y + y == src) sd[own[o]] = 0.0f;
+    }
     const float w1 = 1.0f, w2 = sqrtf(2.0f), w3 = sqrtf(3.0f);
-    for (int dz = -1; dz <= 1; ++dz) {
-        const int nz = z + dz;
-        if (nz < 0 || nz >= Z) continue;
-        for (int dx = -1; dx <= 1; ++dx) {
-            const int nx = x + dx;
-            if (nx < 0 || nx >= X) continue;
-            for (int dy = -1; dy <= 1; ++dy) {
+    while (true) {
+        if (threadIdx.x == 0) s_changed = 0;
+        __syncthreads();
+        int changed = 0;
+        for (int o = 0; o < 2; ++o) {
+            const int c = own[o];
+            const float dv = sd[c];
+            if (dv < 0.0f) continue;  // solid or outside the image
+            float best = dv;
+            for (int k = 0; k < 27; ++k) {
+                const int dz = k / 9 - 1, dx = (k / 3) % 3 - 1, dy = k % 3 - 1;
                 const int m = abs(dz) + abs(dx) + abs(dy);
                 if (m == 0 || (d3q7 && m > 1)) continue;
-                const int ny = y + dy;
-                if (ny < 0 || ny >= Y) continue;
-                const uint32_t j = ((uint32_t)nz * (uint32_t)X + (uint32_t)nx) * (uint32_t)Y + (uint32_t)ny;
-                if (solid[j]) continue;
+                const float du = sd[c + (dz * kH + dx) * kH + dy];
+                if (du < 0.0f || __float_as_uint(du) >= kFltMaxBits) continue;
                 const float w = d3q7 ? 1.0f : (m == 1 ? w1 : (m == 2 ? w2 : w3));
-                const float nd = __fadd_rn(d, w);
-                const uint32_t nb = __float_as_uint(nd);
-                if (nb < __float_as_uint(dist[j])) {
-                    const uint32_t old = atomicMin(reinterpret_cast<unsigned int*>(dist) + j, nb);
-                    if (nb < old && atomicExch(&stamp[j], round) != round) next[atomicAdd(next_count, 1u)] = j;
-                }
+                best = fminf(best, __fadd_rn(du, w));
+            }
+            if (best < dv) {
+                sd[c] = best;
+                changed = 1;
             }
         }
+        if (changed) s_changed = 1;
+        __syncthreads();
+        const int any = s_changed;
+        __syncthreads();
+        if (!any) break;
     }
+
+    // write back what improved; border voxels that improved wake the blocks across
+    uint32_t touch = 0;
+    for (int o = 0; o < 2; ++o) {
+        const int v = threadIdx.x + o * 256;
+        const float dv = sd[own[o]];
+        if (!(dv < orig[o])) continue;
+        const int lz = v / 64, lx = (v / 8) % 8, ly = v % 8;
+        const int z = z0 + 1 + lz, x = x0 + 1 + lx, y = y0 + 1 + ly;
+        dist[((size_t)z * X + x) * Y + y] = dv;
+        const int zlo = lz == 0 ? -1 : 0, zhi = lz == kB - 1 ? 1 : 0;
+        const int xlo = lx == 0 ? -1 : 0, xhi = lx == kB - 1 ? 1 : 0;
+        const int ylo = ly == 0 ? -1 : 0, yhi = ly == kB - 1 ? 1 : 0;
+        for (int dz = zlo; dz <= zhi; ++dz)
+            for (int dx = xlo; dx <= xhi; ++dx)
+                for (int dy = ylo; dy <= yhi; ++dy)
+                    if (dz | dx | dy) touch |= 1u << ((dz + 1) * 9 + (dx + 1) * 3 + (dy + 1));
+    }
+    if (touch) atomicOr(&s_touch, touch);
+    if (threadIdx.x == 0) s_npush = 0;
+    __syncthreads();
+    if (threadIdx.x < 27 && ((s_touch >> threadIdx.x) & 1u)) {
+        const int dz = (int)threadIdx.x / 9 - 1, dx = ((int)threadIdx.x / 3) % 3 - 1, dy = (int)threadIdx.x % 3 - 1;
+        const int nz = bz + dz, nx = bx + dx, ny = by + dy;
+        if (nz >= 0 && nz < NBZ && nx >= 0 && nx < NBX && ny >= 0 && ny < NBY) {
+            const uint32_t nb = ((uint32_t)nz * (uint32_t)NBX + (uint32_t)nx) * (uint32_t)NBY + (uint32_t)ny;
+            if (atomicExch(&block_tag[nb], tag) != tag) s_push[atomicAdd(&s_npush, 1u)] = nb;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && s_npush) s_base = atomicAdd(next_count, s_npush);
+    __syncthreads();
+    if (threadIdx.x < s_npush) next[s_base + threadIdx.x] = s_push[threadIdx.x];
 }
 
 // Voxels reached in this cavity (void, cavity id unset, distance set) take the id; walls
 // touched are OR-ed as bits (axis * 2 + side).
-__global__ void finish_cavity(const uint8_t* __restrict__ solid, const float* __restrict__ dist,
-                              uint32_t* __restrict__ cav, uint32_t id, int X, int Y, int Z,
-                              unsigned long long* __restrict__ count, uint32_t* __restrict__ walls) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) finish_cavity(const uint8_t* __restrict__ solid, const float* __restrict__ dist,
+                                                     uint32_t* __restrict__ cav, uint32_t id, int X, int Y, int Z,
+                                                     unsigned long long* __restrict__ count,
+                                                     uint32_t* __restrict__ walls) {
+    __shared__ unsigned long long part_n[4];
+    __shared__ uint32_t part_w[4];
     unsigned long long mine = 0;
     uint32_t wb = 0;
-    if (i < (size_t)X * Y * Z && solid[i] == 0 && cav[i] == 0xFFFFFFFFu &&
-        __float_as_uint(dist[i]) != kFltMaxBits) {
+    const size_t n = (size_t)X * Y * Z;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        if (solid[i] != 0 || cav[i] != 0xFFFFFFFFu || __float_as_uint(dist[i]) == kFltMaxBits) continue;
         cav[i] = id;
-        mine = 1;
+        ++mine;
         int y = (int)(i % Y);
         size_t r = i / Y;
         int x = (int)(r % X), z = (int)(r / X);
@@ -145,6 +220,13 @@ __global__ void finish_cavity(const uint8_t* __restrict__ solid, const float* __
         wb |= (uint32_t)__shfl_xor((int)wb, off);
     }
     if ((threadIdx.x & 63) == 0) {
+        part_n[threadIdx.x >> 6] = mine;
+        part_w[threadIdx.x >> 6] = wb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        mine = part_n[0] + part_n[1] + part_n[2] + part_n[3];
+        wb = part_w[0] | part_w[1] | part_w[2] | part_w[3];
         if (mine) atomicAdd(count, mine);
         if (wb) atomicOr(walls, wb);
     }
@@ -176,14 +258,16 @@ void run_geodesic(pma_ctx* c) {
     const size_t n = c->n;
     geo_init<<<grid_for(n, 256), 256, 0, c->stream>>>(solid, dist, cav, n);
     PMA_LAUNCHED(c);
-    uint32_t* fa = c->buf<uint32_t>("geo_frontier_a", n);
-    uint32_t* fb = c->buf<uint32_t>("geo_frontier_b", n);
-    uint32_t* stamp = c->buf<uint32_t>("geo_stamp", n);
-    uint32_t* cnt = c->buf<uint32_t>("geo_count", 2);
+    const int NBZ = (Z + kB - 1) / kB, NBX = (X + kB - 1) / kB, NBY = (Y + kB - 1) / kB;
+    const size_t NB = (size_t)NBZ * NBX * NBY;
+    uint32_t* fa = c->buf<uint32_t>("geo_active_a", NB);
+    uint32_t* fb = c->buf<uint32_t>("geo_active_b", NB);
+    uint32_t* block_tag = c->buf<uint32_t>("geo_block_tag", NB);
+    uint32_t* cnt = c->buf<uint32_t>("geo_count", 1);
     unsigned long long* key = c->buf<unsigned long long>("geo_key", 1);
     unsigned long long* vcount = c->buf<unsigned long long>("geo_vcount", 1);
     uint32_t* walls = c->buf<uint32_t>("geo_walls", 1);
-    PMA_HIP(hipMemsetAsync(stamp, 0xFF, n * sizeof(uint32_t), c->stream));
+    PMA_HIP(hipMemsetAsync(block_tag, 0xFF, NB * sizeof(uint32_t), c->stream));
 
     const int S[3] = {X, Y, Z};
     int ctr[3];
@@ -209,7 +293,8 @@ void run_geodesic(pma_ctx* c) {
             found = true;
         } else if (half < max_half) {
             PMA_HIP(hipMemsetAsync(key, 0xFF, sizeof(unsigned long long), c->stream));
-            find_start<<<grid_for(n, 256), 256, 0, c->stream>>>(solid, dist, X, Y, Z, ctr[0], ctr[1], ctr[2], half,
+            find_start<<<std::min(grid_for(n, 256), 8192u), 256, 0, c->stream>>>(solid, dist, X, Y, Z, ctr[0], ctr[1],
+                                                                                 ctr[2], half,
                                                                   max_half, key);
             PMA_LAUNCHED(c);
             unsigned long long k = read_scalar(c, key);
@@ -228,28 +313,32 @@ void run_geodesic(pma_ctx* c) {
         if (!found) break;
         id += 127;
         const size_t si = ((size_t)start[2] * X + start[0]) * Y + start[1];
-        set_source<<<1, 1, 0, c->stream>>>(dist, si, fa, cnt);
-        PMA_LAUNCHED(c);
+        const uint32_t b0 = ((uint32_t)(start[2] / kB) * (uint32_t)NBX + (uint32_t)(start[0] / kB)) * (uint32_t)NBY +
+                            (uint32_t)(start[1] / kB);
+        upload(c, fa, &b0, sizeof b0);
         uint32_t nf = 1;
         uint32_t* cur = fa;
         uint32_t* nxt = fb;
+        bool first = true;
         while (nf > 0) {
-            PMA_HIP(hipMemsetAsync(cnt + 1, 0, sizeof(uint32_t), c->stream));
+            PMA_HIP(hipMemsetAsync(cnt, 0, sizeof(uint32_t), c->stream));
             {
-                PMA_TIMED(c, "relax");
-                relax<<<grid_for(nf, 256), 256, 0, c->stream>>>(solid, dist, cur, nf, nxt, cnt + 1, stamp, round++, X, Y,
-                                                                Z, d3q7);
+                PMA_TIMED(c, "relax_blocks");
+                relax_blocks<<<nf, 256, 0, c->stream>>>(dist, cur, nxt, cnt, block_tag, round++, X, Y, Z, NBX, NBY, NBZ,
+                                                        d3q7, first ? (unsigned long long)si : ~0ull);
                 PMA_LAUNCHED(c);
             }
             relaxations += nf;
-            nf = read_scalar(c, cnt + 1);
+            first = false;
+            nf = read_scalar(c, cnt);
             std::swap(cur, nxt);
         }
         PMA_HIP(hipMemsetAsync(vcount, 0, sizeof(unsigned long long), c->stream));
         PMA_HIP(hipMemsetAsync(walls, 0, sizeof(uint32_t), c->stream));
         {
             PMA_TIMED(c, "finish_cavity");
-            finish_cavity<<<grid_for(n, 256), 256, 0, c->stream>>>(solid, dist, cav, id, X, Y, Z, vcount, walls);
+            finish_cavity<<<std::min(grid_for(n, 256), 8192u), 256, 0, c->stream>>>(solid, dist, cav, id, X, Y, Z,
+                                                                                     vcount, walls);
             PMA_LAUNCHED(c);
         }
         CavityRow row;

@@ -159,6 +159,8 @@ void radix_sort_pairs_u64_u64(pma_ctx* c, const uint64_t* keys_in, uint64_t* key
                               uint64_t* vals_out, size_t n, int begin_bit = 0, int end_bit = 64);
 void radix_sort_pairs_u32(pma_ctx* c, const uint32_t* keys_in, uint32_t* keys_out, const uint32_t* vals_in,
                           uint32_t* vals_out, size_t n, int begin_bit = 0, int end_bit = 32);
+void radix_sort_keys_u64(pma_ctx* c, const uint64_t* keys_in, uint64_t* keys_out, size_t n, int begin_bit = 0,
+                         int end_bit = 64);
 // Exact emulation of libstdc++ std::sort with comp(a, b) = (a >> 32) > (b >> 32) (sort.hip).
 void sort_like_libstdcxx_desc(pma_ctx* c, uint64_t* records, size_t n);
 

@@ -3,6 +3,10 @@
 //   hostemu pores     <in> <out_volume> <out_text>   in: X Y Z d3q7 | EDT2 u32 | CR2 u32 | mask u8
 //   hostemu watershed <in> <out_volume> <out_text>   in: X Y Z d3q7 maxE | CR2 u32 | np | np x 7 i32
 //   hostemu remove    <in> <out_volume> <out_text>   in: X Y Z | PT u32 | text (pores, "@@", throats)
+//   hostemu balls     <in> <out_volumes> <out_text>  in: X Y Z maxE | EDT2 u32;
+//                     out: CR2, CX, CY, CZ u32 | mask u8; text: maximum-ball count
+//   hostemu geodesic  <in> <out_volumes> <out_text>  in: X Y Z d3q7 | solid u8;
+//                     out: distance f32 | cavity u32; text: cavityDescriptions.txt
 #include <fstream>
 #include <sstream>
 
@@ -12,6 +16,8 @@ namespace pma {
 void run_pores(pma_ctx* c);
 void run_watershed(pma_ctx* c);
 void run_remove_empty_pores(pma_ctx* c);
+void run_containing_balls(pma_ctx* c);
+void run_geodesic(pma_ctx* c);
 }  // namespace pma
 
 namespace {
@@ -107,6 +113,48 @@ int main(int argc, char** argv) {
                 pma::run_watershed(&c);
                 write_outputs(c, PMA_VOL_PTW, c.pores_ws, c.throats_ws, argv[3], argv[4]);
             }
+        } else if (stage == "balls") {
+            int32_t h[4];
+            read_into(f, h, 4);
+            c.dims = {h[0], h[1], h[2]};
+            c.n = (size_t)h[0] * h[1] * h[2];
+            c.max_e = (uint32_t)h[3];
+            c.have_edt_stats = true;
+            c.ensure_volume(PMA_VOL_EDT2);
+            read_into(f, c.v<uint32_t>(PMA_VOL_EDT2), c.n);
+            c.valid[PMA_VOL_EDT2] = true;
+            pma::run_containing_balls(&c);
+            FILE* o = std::fopen(argv[3], "wb");
+            for (pma_volume w : {PMA_VOL_CR2, PMA_VOL_CX, PMA_VOL_CY, PMA_VOL_CZ}) std::fwrite(c.v<uint32_t>(w), 4, c.n, o);
+            std::fwrite(c.v<uint8_t>(PMA_VOL_MASK), 1, c.n, o);
+            std::fclose(o);
+            o = std::fopen(argv[4], "wb");
+            std::fprintf(o, "%llu\n", (unsigned long long)c.n_max_balls);
+            std::fclose(o);
+        } else if (stage == "geodesic") {
+            int32_t h[4];
+            read_into(f, h, 4);
+            c.dims = {h[0], h[1], h[2]};
+            c.n = (size_t)h[0] * h[1] * h[2];
+            c.stencil = h[3] ? PMA_STENCIL_D3Q7 : PMA_STENCIL_D3Q27;
+            c.ensure_volume(PMA_VOL_SOLID);
+            read_into(f, c.v<uint8_t>(PMA_VOL_SOLID), c.n);
+            c.valid[PMA_VOL_SOLID] = true;
+            pma::run_geodesic(&c);
+            FILE* o = std::fopen(argv[3], "wb");
+            std::fwrite(c.v<float>(PMA_VOL_SPD), 4, c.n, o);
+            std::fwrite(c.v<uint32_t>(PMA_VOL_CAVITY), 4, c.n, o);
+            std::fclose(o);
+            std::string s = "CavityId SeedX SeedY SeedZ DistanceToCenter VoidPixelsCount ReachedWallsCount IsReachingAllWalls\n";
+            char b[256];
+            for (const auto& r : c.cavities) {
+                std::snprintf(b, sizeof b, "%u %d %d %d %f %zu %d %d\n", r.id, r.x, r.y, r.z, r.distance_to_center,
+                              (size_t)r.void_count, r.reached_walls, r.reaches_all ? 1 : 0);
+                s += b;
+            }
+            o = std::fopen(argv[4], "wb");
+            std::fwrite(s.data(), 1, s.size(), o);
+            std::fclose(o);
         } else if (stage == "remove") {
             int32_t h[3];
             read_into(f, h, 3);

@@ -7,6 +7,8 @@
 // replaced by std::stable_sort; scans and the libstdc++-sort emulation run their real kernels.
 #pragma once
 #include <algorithm>
+#include <cfloat>
+#include <cmath>
 #include <atomic>
 #include <barrier>
 #include <cstdint>
@@ -18,6 +20,7 @@
 #include <stdexcept>
 #include <string>
 #include <thread>
+#include <tuple>
 #include <vector>
 
 #define __global__
@@ -107,6 +110,10 @@ template <class T> T __shfl(T v, int src) {
     if (!g_wave) throw std::runtime_error("__shfl outside a wave launch");
     return wave_read(v, (int)(threadIdx.x & ~63u) + (src & 63));
 }
+template <class T> T __shfl_xor(T v, int mask) {
+    if (!g_wave) throw std::runtime_error("__shfl_xor outside a wave launch");
+    return wave_read(v, (int)(threadIdx.x & ~63u) + (((int)threadIdx.x & 63) ^ (mask & 63)));
+}
 template <class T> T __shfl_up(T v, unsigned d) {
     if (!g_wave) throw std::runtime_error("__shfl_up outside a wave launch");
     const int lane = threadIdx.x & 63;
@@ -124,11 +131,17 @@ inline unsigned long long __ballot(int pred) {
     shim_yield();
     return r;
 }
+inline unsigned __float_as_uint(float f) { unsigned u; std::memcpy(&u, &f, 4); return u; }
+inline float __uint_as_float(unsigned u) { float f; std::memcpy(&f, &u, 4); return f; }
+inline float __fadd_rn(float a, float b) { volatile float s = a + b; return s; }
 inline int __popc(unsigned x) { return __builtin_popcount(x); }
+inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
+inline int __ffsll(long long x) { return __builtin_ffsll(x); }
 inline int __ffs(unsigned x) { return __builtin_ffs((int)x); }
 inline int __clzll(long long x) { return x ? __builtin_clzll((unsigned long long)x) : 64; }
 
 template <class T, class U> T atomicAdd(T* p, U v) { return __atomic_fetch_add(p, (T)v, __ATOMIC_SEQ_CST); }
+template <class T, class U> T atomicExch(T* p, U v) { return __atomic_exchange_n(p, (T)v, __ATOMIC_SEQ_CST); }
 template <class T, class U> T atomicOr(T* p, U v) { return __atomic_fetch_or(p, (T)v, __ATOMIC_SEQ_CST); }
 template <class T, class U> T atomicCAS(T* p, U cmp, T val) {
     T e = (T)cmp;
@@ -153,6 +166,10 @@ template <class T> T __hip_atomic_load(T* p, int, int) { return __atomic_load_n(
 inline uint32_t min(uint32_t a, uint32_t b) { return a < b ? a : b; }
 inline uint32_t max(uint32_t a, uint32_t b) { return a > b ? a : b; }
 inline int min(int a, int b) { return a < b ? a : b; }
+inline size_t min(size_t a, size_t b) { return a < b ? a : b; }
+inline unsigned long long min(unsigned long long a, unsigned long long b) { return a < b ? a : b; }
+inline unsigned long long max(unsigned long long a, unsigned long long b) { return a > b ? a : b; }
+inline size_t max(size_t a, size_t b) { return a > b ? a : b; }
 inline int max(int a, int b) { return a > b ? a : b; }
 
 inline std::vector<char*>& fiber_stacks() {
@@ -255,6 +272,14 @@ struct Error : std::runtime_error {
 };
 struct PoreRow { uint32_t id = 0; int32_t x = 0, y = 0, z = 0; uint32_t r2 = 0; bool boundary = false; uint64_t volume = 0; };
 struct ThroatRow { uint32_t id = 0; uint64_t volume = 0; std::vector<uint32_t> pores; };
+struct CavityRow {
+    uint32_t id = 0;
+    int32_t x = 0, y = 0, z = 0;
+    double distance_to_center = 0;
+    uint64_t void_count = 0;
+    int32_t reached_walls = 0;
+    bool reaches_all = false;
+};
 inline unsigned grid_for(size_t n, unsigned block) { return (unsigned)((n + block - 1) / block); }
 }  // namespace pma
 #define PMA_REQUIRE(cond, status, msg) do { if (!(cond)) throw ::pma::Error((status), (msg)); } while (0)
@@ -276,9 +301,12 @@ struct pma_ctx {
     bool valid[PMA_VOL_COUNT] = {};
     std::map<std::string, std::pair<void*, size_t>> bufs;
     uint32_t max_e = 0;
+    uint64_t n_max_balls = 0;
     bool have_edt_stats = false, have_pores = false, have_pores_ws = false;
     std::vector<pma::PoreRow> pores, pores_ws;
     std::vector<pma::ThroatRow> throats, throats_ws;
+    std::vector<pma::CavityRow> cavities;
+    bool have_cavities = false;
     StageInfo stage[PMA_STAGE_COUNT];
     template <class T> T* v(int w) { return (T*)vol[w].data(); }
     void ensure_volume(int w) {
@@ -321,6 +349,11 @@ template <class K, class V> void rs(const K* ki, K* ko, const V* vi, V* vo, size
 inline void radix_sort_pairs_u64(pma_ctx*, const uint64_t* a, uint64_t* b, const uint32_t* c, uint32_t* d, size_t n, int b0 = 0, int b1 = 64) { rs(a, b, c, d, n, b0, b1); }
 inline void radix_sort_pairs_u64_u64(pma_ctx*, const uint64_t* a, uint64_t* b, const uint64_t* c, uint64_t* d, size_t n, int b0 = 0, int b1 = 64) { rs(a, b, c, d, n, b0, b1); }
 inline void radix_sort_pairs_u32(pma_ctx*, const uint32_t* a, uint32_t* b, const uint32_t* c, uint32_t* d, size_t n, int b0 = 0, int b1 = 32) { rs(a, b, c, d, n, b0, b1); }
+inline void radix_sort_keys_u64(pma_ctx*, const uint64_t* a, uint64_t* b, size_t n, int b0 = 0, int b1 = 64) {
+    std::vector<uint8_t> none(n);
+    std::vector<uint8_t> none2(n);
+    rs(a, b, none.data(), none2.data(), n, b0, b1);
+}
 void sort_like_libstdcxx_desc(pma_ctx* c, uint64_t* r, size_t n);
 
 constexpr uint32_t kTrieNode = 0x80000000u;

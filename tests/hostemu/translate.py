@@ -7,7 +7,9 @@ as wave kernels (one fiber per GPU thread).
 import re
 import sys
 
-WAVE_KERNELS = {"count_centres", "write_centres", "dfs_replay", "cr_histogram", "scan_reduce<T>", "scan_down<T>"}
+WAVE_KERNELS = {"count_centres", "write_centres", "dfs_replay", "cr_histogram", "scan_reduce<T>", "scan_down<T>",
+                "prune_centres", "emit_centres", "cover_tiles", "count_equal_u8", "far_prefix_max",
+                "relax_blocks", "find_start", "finish_cavity"}
 
 
 def split_args(s):
@@ -28,9 +30,12 @@ def split_args(s):
 
 
 def launch(m):
+    # The arguments are evaluated once, before the launch, as a real launch does.
     name, cfg, args = m.group(1), split_args(m.group(2)), m.group(3)
     wave = "true" if name in WAVE_KERNELS else "false"
-    return f"launch_cpu({wave}, Dim3({cfg[0].strip()}), Dim3({cfg[1].strip()}), [&] {{ {name}({args}); }});"
+    return (f"{{ auto hostemu_args = std::make_tuple({args}); "
+            f"launch_cpu({wave}, Dim3({cfg[0].strip()}), Dim3({cfg[1].strip()}), [&] {{ "
+            f"std::apply([&](auto... a) {{ {name}(a...); }}, hostemu_args); }}); }}")
 
 
 def translate(text):

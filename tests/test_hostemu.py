@@ -1,7 +1,7 @@
-"""Host emulation of the pore, empty-pore-removal and watershed stages against the oracle.
+"""Host emulation of the containing-ball, pore, removal, watershed and geodesic stages.
 
-The library's own .hip sources (scan, libstdc++-sort emulation, prefix trie, pores, removal,
-watershed) are translated to host C++ (tests/hostemu/translate.py), built with
+The library's own .hip sources (scan, libstdc++-sort emulation, prefix trie, containing balls,
+pores, removal, watershed, geodesic) are translated to host C++ (tests/hostemu/translate.py), built with
 AddressSanitizer and UndefinedBehaviorSanitizer and run on the oracle's inputs: every kernel
 executes its real body, thread by thread (wave kernels as fibers), with exact-size buffers, so
 an out-of-bounds access fails the test.  Outputs must equal the oracle bit for bit.  rocPRIM
@@ -24,7 +24,7 @@ from tests.volume_utils import canonical_relabel, sha256_array
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(ROOT, "porousmediaanalysis_amd", "csrc")
 EMU = os.path.join(ROOT, "tests", "hostemu")
-SOURCES = ["scan", "sort", "lists", "pores", "remove", "watershed"]
+SOURCES = ["scan", "sort", "lists", "balls", "pores", "remove", "watershed", "geodesic"]
 SANITIZED = ["-std=c++20", "-O1", "-g", "-fsanitize=address,undefined", "-fno-omit-frame-pointer",
              "-fno-sanitize-recover=undefined", f"-I{EMU}"]
 FAST = ["-std=c++20", "-O2", f"-I{EMU}"]
@@ -78,6 +78,12 @@ def hostemu(tmp_path_factory):
 
 
 @pytest.fixture(scope="session")
+def hostemu_slabs(tmp_path_factory):
+    """Sanitized build whose containing-ball pair list is cut into slabs of at most 4000 pairs."""
+    return _build(tmp_path_factory, SANITIZED + ["-DPMA_CB_SLAB_PAIRS=4000ull"], "slabs")
+
+
+@pytest.fixture(scope="session")
 def hostemu_fast(tmp_path_factory):
     """Optimised build without sanitizers, for the larger fixture volume."""
     return _build(tmp_path_factory, FAST, "fast")
@@ -115,6 +121,22 @@ def emu_watershed(exe, tmp_path, cr, max_e, ptxt, d3q7):
     return _run(exe, "watershed", payload, tmp_path, cr.shape)
 
 
+def emu_balls(exe, tmp_path, e, max_e):
+    Z, X, Y = e.shape
+    inp, vol, txt = tmp_path / "in.bin", tmp_path / "out.bin", tmp_path / "out.txt"
+    inp.write_bytes(np.array([X, Y, Z, max_e], np.int32).tobytes() + e.astype(np.uint32).tobytes())
+    r = subprocess.run([exe, "balls", str(inp), str(vol), str(txt)], capture_output=True, text=True, env=ENV,
+                       timeout=600)
+    assert r.returncode == 0, f"balls: rc {r.returncode}\n{r.stdout[-2000:]}\n{r.stderr[-6000:]}"
+    n = e.size
+    raw = np.fromfile(vol, np.uint8)
+    u32 = raw[:16 * n].view(np.uint32)
+    out = [u32[k * n:(k + 1) * n].reshape(e.shape) for k in range(4)]
+    out.append(raw[16 * n:].reshape(e.shape))
+    out.append(int(txt.read_text()))
+    return out
+
+
 def front(solid):
     e, max_e, _ = oracle.edt(solid, threads=4)
     cr, _, _, _, mask, _ = oracle.containing_balls(e, threads=4)
@@ -135,6 +157,59 @@ CASES["gauss_s2"] = volumes.gaussian((29, 41, 67), 2.0, 0.65, 2)
 CASES["gauss_s3_low"] = volumes.gaussian((40, 40, 40), 3.0, 0.5, 12)
 CASES["noise"] = volumes.random_noise((17, 19, 23), 0.15, 4)
 CASES["noise_many_pores"] = volumes.random_noise((24, 26, 28), 0.45, 6)
+
+
+BALL_CASES = dict(CASES)
+BALL_CASES["gauss_s4"] = volumes.gaussian((48, 56, 64), 4.0, 0.7, 3)
+BALL_CASES["gauss_ragged"] = volumes.gaussian((19, 70, 130), 3.0, 0.75, 3)
+
+
+@pytest.mark.parametrize("name", sorted(BALL_CASES))
+def test_containing_balls(hostemu, tmp_path, name):
+    e, max_e, _ = oracle.edt(BALL_CASES[name], threads=4)
+    g = emu_balls(hostemu, tmp_path, e, max_e)
+    o = oracle.containing_balls(e, threads=4)
+    for k, what in enumerate(["CR2", "CX", "CY", "CZ", "mask"]):
+        if not np.array_equal(g[k], o[k]):
+            bad = np.argwhere(g[k] != o[k])
+            raise AssertionError(f"{what}: {len(bad)} voxels differ, first {tuple(bad[0])}: "
+                                 f"{g[k][tuple(bad[0])]} vs {o[k][tuple(bad[0])]}")
+    assert g[5] == o[5], "maximum-ball count differs"
+
+
+GEO_CASES = dict(volumes.edge_cases())
+GEO_CASES["spheres_40"] = volumes.spheres((40, 44, 36), 60, 3, 6, 5)
+GEO_CASES["gauss"] = volumes.gaussian((33, 47, 52), 2.0, 0.55, 6)
+GEO_CASES["noise"] = volumes.random_noise((20, 21, 22), 0.62, 7)
+
+
+@pytest.mark.parametrize("name", sorted(GEO_CASES))
+@pytest.mark.parametrize("d3q7", [False, True])
+def test_geodesic(hostemu, tmp_path, name, d3q7):
+    solid = GEO_CASES[name]
+    Z, X, Y = solid.shape
+    inp, vol, txt = tmp_path / "in.bin", tmp_path / "out.bin", tmp_path / "out.txt"
+    inp.write_bytes(np.array([X, Y, Z, int(d3q7)], np.int32).tobytes() + solid.astype(np.uint8).tobytes())
+    r = subprocess.run([hostemu, "geodesic", str(inp), str(vol), str(txt)], capture_output=True, text=True, env=ENV,
+                       timeout=600)
+    assert r.returncode == 0, f"geodesic: rc {r.returncode}\n{r.stdout[-2000:]}\n{r.stderr[-6000:]}"
+    raw = np.fromfile(vol, np.uint32)
+    dist, cav = raw[:solid.size].reshape(solid.shape), raw[solid.size:].reshape(solid.shape)
+    od, oc, ot = oracle.geodesic(solid, d3q7)
+    assert np.array_equal(dist, od.view(np.uint32)), "distances differ"
+    assert np.array_equal(cav, oc), "cavity ids differ"
+    assert txt.read_text() == ot.replace("\r", ""), "cavityDescriptions.txt differs"
+
+
+def test_containing_balls_in_slabs(hostemu_slabs, tmp_path):
+    """The same result when the (tile, ball) pairs are processed in many z-slabs."""
+    solid = BALL_CASES["gauss_s4"]
+    e, max_e, _ = oracle.edt(solid, threads=4)
+    g = emu_balls(hostemu_slabs, tmp_path, e, max_e)
+    o = oracle.containing_balls(e, threads=4)
+    for k in range(5):
+        assert np.array_equal(g[k], o[k])
+    assert g[5] == o[5]
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
