@@ -28,6 +28,9 @@ constexpr int TZ = 8, TX = 8, TY = 64;  // tile: 64 rows (z, x) of 64 voxels alo
 #ifndef PMA_CB_SLAB_PAIRS
 #define PMA_CB_SLAB_PAIRS (1ull << 30)  // most (tile, ball) pairs sorted at once
 #endif
+#ifndef PMA_CB_WIN16_MAX
+#define PMA_CB_WIN16_MAX 0xFFFFu  // longest tile run whose winners fit 16-bit slots
+#endif
 constexpr uint8_t kCandidate = 126;    // temporary mask value of a surviving centre
 
 struct alignas(16) Ball {
@@ -232,7 +235,27 @@ __global__ void tile_ranges(const uint32_t* __restrict__ pk, size_t P, uint32_t*
     if (i + 1 == P || pk[i + 1] != t) end[t] = (uint32_t)(i + 1);
 }
 
-// One wave per tile (see the work plan, step 4).
+// floor(sqrt(v)) from one correctly rounded float square root and a one-step correction.
+__device__ __forceinline__ uint32_t isqrt_fast(uint32_t v) {
+    uint32_t s = (uint32_t)sqrtf((float)v);
+    if ((uint64_t)s * s > v) --s;
+    else if ((uint64_t)(s + 1) * (s + 1) <= v) ++s;
+    return s;
+}
+
+// Largest pair run of any tile of the slab (decides the width of the per-voxel winner slot).
+__global__ void __launch_bounds__(256) max_run(const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
+                                               size_t n, unsigned int* __restrict__ out) {
+    uint32_t m = 0;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+        m = max(m, end[t] - start[t]);
+    for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, off));
+    if ((threadIdx.x & 63) == 0 && m) atomicMax(out, m);
+}
+
+// One wave per tile (see the work plan, step 4).  The winner of a voxel is kept in LDS as the
+// position of its ball in the tile's run (16 bits when every run is shorter than 65536).
+template <class W>
 __global__ void __launch_bounds__(64) cover_tiles(const Ball* __restrict__ balls, const uint32_t* __restrict__ rank_of,
                                                   const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
                                                   const uint32_t* __restrict__ e, uint32_t* __restrict__ cr,
@@ -240,7 +263,7 @@ __global__ void __launch_bounds__(64) cover_tiles(const Ball* __restrict__ balls
                                                   uint32_t* __restrict__ cz, uint8_t* __restrict__ mask, int X, int Y,
                                                   int Z, int NTX, int NTY, uint32_t tile_base,
                                                   uint32_t* __restrict__ error_flag) {
-    __shared__ uint32_t win[TZ * TX * TY];
+    __shared__ W win[TZ * TX * TY];
     const int lane = threadIdx.x;
     const uint32_t local = blockIdx.x;
     const uint32_t t = tile_base + local;
@@ -260,30 +283,28 @@ __global__ void __launch_bounds__(64) cover_tiles(const Ball* __restrict__ balls
     const uint32_t s = start[local], en = end[local];
     for (uint32_t base = s; base < en; base += 64) {
         if (__ballot(cov != ~0ull) == 0) break;  // every void voxel of the tile has its ball
-        uint32_t rk = 0;
         Ball mb{0, 0, 0, 0};
-        if (base + lane < en) {
-            rk = rank_of[base + lane];
-            mb = balls[rk];
-        }
+        if (base + lane < en) mb = balls[rank_of[base + lane]];
         const int cnt = (int)min(64u, en - base);
         for (int j = 0; j < cnt; ++j) {
-            const int bx = __shfl(mb.x, j), by = __shfl(mb.y, j), bz = __shfl(mb.z, j);
-            const int r2 = __shfl((int)mb.r2, j);
-            const uint32_t rank = (uint32_t)__shfl((int)rk, j);
+            // the ball's fields are uniform: read them into scalar registers
+            const int bx = __builtin_amdgcn_readlane(mb.x, j), by = __builtin_amdgcn_readlane(mb.y, j);
+            const int bz = __builtin_amdgcn_readlane(mb.z, j);
+            const int r2 = __builtin_amdgcn_readlane((int)mb.r2, j);
             const int dz = z - bz, dx = x - bx;
             const int rem = r2 - dz * dz - dx * dx;
             if (rem < 0) continue;
-            const int h = (int)isqrt_u32((uint32_t)rem);
+            const int h = (int)isqrt_fast((uint32_t)rem);
             const int lo = max(by - h - y0, 0), hi = min(by + h - y0, 63);
             if (lo > hi) continue;
             uint64_t nw = ((~0ull) >> (63 - (hi - lo))) << lo;
             nw &= ~cov;
             if (!nw) continue;
             cov |= nw;
+            const W slot = (W)(base - s + (uint32_t)j);
             do {
                 const int bit = __ffsll((long long)nw) - 1;
-                win[lane * TY + bit] = rank;
+                win[lane * TY + bit] = slot;
                 nw &= nw - 1;
             } while (nw);
         }
@@ -304,7 +325,7 @@ __global__ void __launch_bounds__(64) cover_tiles(const Ball* __restrict__ balls
             atomicOr(error_flag, 1u);
             continue;
         }
-        const Ball b = balls[win[r * TY + lane]];
+        const Ball b = balls[rank_of[s + (uint32_t)win[r * TY + lane]]];
         cr[i] = b.r2;
         cx[i] = (uint32_t)b.x;
         cy[i] = (uint32_t)b.y;
@@ -424,12 +445,26 @@ void run_containing_balls(pma_ctx* c) {
             tile_ranges<<<grid_for(P, 256), 256, 0, c->stream>>>(pk2, P, tstart, tend);
             PMA_LAUNCHED(c);
         }
+        unsigned int longest = 0;
+        if (P) {
+            unsigned int* dmax = c->buf<unsigned int>("cb_max_run", 1);
+            PMA_HIP(hipMemsetAsync(dmax, 0, 4, c->stream));
+            max_run<<<std::min(grid_for(slab_tiles, 256), 1024u), 256, 0, c->stream>>>(tstart, tend, slab_tiles, dmax);
+            PMA_LAUNCHED(c);
+            longest = read_scalar(c, dmax);
+        }
         {
             PMA_TIMED(c, "cover_tiles");
-            cover_tiles<<<slab_tiles, 64, 0, c->stream>>>(balls, pv2, tstart, tend, e, c->v<uint32_t>(PMA_VOL_CR2),
-                                                         c->v<uint32_t>(PMA_VOL_CX), c->v<uint32_t>(PMA_VOL_CY),
-                                                         c->v<uint32_t>(PMA_VOL_CZ), mask, X, Y, Z, NTX, NTY, tile_base,
-                                                         err);
+            uint32_t* cr = c->v<uint32_t>(PMA_VOL_CR2);
+            uint32_t* cx = c->v<uint32_t>(PMA_VOL_CX);
+            uint32_t* cy = c->v<uint32_t>(PMA_VOL_CY);
+            uint32_t* cz = c->v<uint32_t>(PMA_VOL_CZ);
+            if (longest <= PMA_CB_WIN16_MAX)
+                cover_tiles<uint16_t><<<slab_tiles, 64, 0, c->stream>>>(balls, pv2, tstart, tend, e, cr, cx, cy, cz, mask,
+                                                                       X, Y, Z, NTX, NTY, tile_base, err);
+            else
+                cover_tiles<uint32_t><<<slab_tiles, 64, 0, c->stream>>>(balls, pv2, tstart, tend, e, cr, cx, cy, cz, mask,
+                                                                       X, Y, Z, NTX, NTY, tile_base, err);
             PMA_LAUNCHED(c);
         }
         total_pairs += P;

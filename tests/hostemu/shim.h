@@ -110,6 +110,10 @@ template <class T> T __shfl(T v, int src) {
     if (!g_wave) throw std::runtime_error("__shfl outside a wave launch");
     return wave_read(v, (int)(threadIdx.x & ~63u) + (src & 63));
 }
+inline int __builtin_amdgcn_readlane(int v, int lane) {
+    if (!g_wave) throw std::runtime_error("readlane outside a wave launch");
+    return wave_read(v, (int)(threadIdx.x & ~63u) + (lane & 63));
+}
 template <class T> T __shfl_xor(T v, int mask) {
     if (!g_wave) throw std::runtime_error("__shfl_xor outside a wave launch");
     return wave_read(v, (int)(threadIdx.x & ~63u) + (((int)threadIdx.x & 63) ^ (mask & 63)));
@@ -329,6 +333,7 @@ struct pma_ctx {
         return b.first;
     }
     template <class T> T* buf(const std::string& name, size_t count) { return (T*)buf(name, count * sizeof(T)); }
+    void* host_staging(size_t bytes) { return buf("host_staging", bytes); }
 };
 
 namespace pma {

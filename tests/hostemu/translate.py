@@ -9,7 +9,7 @@ import sys
 
 WAVE_KERNELS = {"count_centres", "write_centres", "dfs_replay", "cr_histogram", "scan_reduce<T>", "scan_down<T>",
                 "prune_centres", "emit_centres", "cover_tiles", "count_equal_u8", "far_prefix_max",
-                "relax_blocks", "find_start", "finish_cavity"}
+                "relax_blocks", "find_start", "finish_cavity", "max_run", "partition_active"}
 
 
 def split_args(s):
@@ -32,7 +32,7 @@ def split_args(s):
 def launch(m):
     # The arguments are evaluated once, before the launch, as a real launch does.
     name, cfg, args = m.group(1), split_args(m.group(2)), m.group(3)
-    wave = "true" if name in WAVE_KERNELS else "false"
+    wave = "true" if name in WAVE_KERNELS or name.split("<")[0] in WAVE_KERNELS else "false"
     return (f"{{ auto hostemu_args = std::make_tuple({args}); "
             f"launch_cpu({wave}, Dim3({cfg[0].strip()}), Dim3({cfg[1].strip()}), [&] {{ "
             f"std::apply([&](auto... a) {{ {name}(a...); }}, hostemu_args); }}); }}")

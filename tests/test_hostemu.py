@@ -79,8 +79,9 @@ def hostemu(tmp_path_factory):
 
 @pytest.fixture(scope="session")
 def hostemu_slabs(tmp_path_factory):
-    """Sanitized build whose containing-ball pair list is cut into slabs of at most 4000 pairs."""
-    return _build(tmp_path_factory, SANITIZED + ["-DPMA_CB_SLAB_PAIRS=4000ull"], "slabs")
+    """Sanitized build whose containing-ball pair list is cut into slabs of at most 4000 pairs and
+    whose tiles keep 32-bit winner slots."""
+    return _build(tmp_path_factory, SANITIZED + ["-DPMA_CB_SLAB_PAIRS=4000ull", "-DPMA_CB_WIN16_MAX=0u"], "slabs")
 
 
 @pytest.fixture(scope="session")
