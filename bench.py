@@ -122,15 +122,16 @@ def main():
             progress(f"warmup step {w + 1}/{args.warmup}: {time.perf_counter() - t0:.3f} s")
         pma.lib().pma_set_kernel_timing(ctx._h, 1)
         stage_ms = {name: 0.0 for name in STAGES}
-        relax_items = 0
+        stage_items = {name: 0 for name in STAGES}
         t0 = time.perf_counter()
         for s in range(args.steps):
             ts = time.perf_counter()
             step()
             for k, name in enumerate(pma.STAGE_NAMES):
                 if name in stage_ms:
-                    stage_ms[name] += ctx.stage_info(k)["device_ms"]
-            relax_items += ctx.stage_info(pma.STAGE_NAMES.index("geodesic"))["work_items"]
+                    info = ctx.stage_info(k)
+                    stage_ms[name] += info["device_ms"]
+                    stage_items[name] += info["work_items"]
             progress(f"step {s + 1}/{args.steps}: {time.perf_counter() - ts:.3f} s")
         elapsed = time.perf_counter() - t0
         kernels = kernel_times(pma, ctx)
@@ -146,7 +147,7 @@ def main():
         bpv = KERNEL_BYTES_PER_VOXEL.get(name)
         per_launch = bpv * nvox * args.steps / launches if bpv is not None and launches else None
         if name == "relax_blocks" and launches:
-            per_launch = RELAX_BYTES_PER_BLOCK * relax_items / launches
+            per_launch = RELAX_BYTES_PER_BLOCK * stage_items["geodesic"] / launches
         if per_launch is not None:
             achieved = per_launch / (avg_ms * 1e-3) / 1e9
             roofline = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -180,6 +181,8 @@ def main():
                    "size": S, "sigma": args.sigma, "porosity": args.porosity,
                    "seed": args.seed, "max_edt2": max_e, "maximum_balls": n_max},
         "stages_ms": {k: round(v / args.steps, 3) for k, v in stage_ms.items()},
+        # EDT: voxels; containing balls: (ball, tile) pairs; geodesic: block relaxations
+        "stage_work_items": {k: v // args.steps for k, v in stage_items.items()},
         "kernels_ms": {k: {"total_ms_per_step": round(v[0] / args.steps, 3), "launches_per_step": v[1] / args.steps}
                        for k, v in sorted(kernels.items(), key=lambda kv: -kv[1][0])},
         "pipeline_algorithmic_GBs": round(pipeline_gbs, 1),

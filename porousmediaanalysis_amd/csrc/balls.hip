@@ -12,13 +12,16 @@
 //      has a ball of larger r2, and c can never be taken (exact pruning).
 //   2. the surviving centres are sorted by (r2 descending, scan ascending): their rank is the
 //      order in which the reference would let them win.
-//   3. every centre lists the 8 x 8 x 64 voxel tiles its ball reaches; the (tile, rank) pairs are
-//      sorted stably by tile, so each tile sees its balls in rank order.
-//   4. one wave per tile: lane l owns the row (z, x) = (l / 8, l % 8) of the tile and a 64-bit mask
+//   3. every centre lists the pair cells (4 x 2 tiles of 8 x 8 x 64 voxels along z, x) its ball
+//      reaches; the (cell, rank) pairs are sorted stably by cell, so each cell sees its balls in
+//      rank order.
+//   4. one wave per tile, walking its cell's balls and skipping those that cannot reach the tile's
+//      still-uncovered voxels: lane l owns the row (z, x) = (l / 8, l % 8) of the tile and a 64-bit mask
 //      of the row's voxels that are solid or already covered.  For each ball in rank order a lane
 //      takes the row's chord [y - h, y + h] of the ball; its uncovered voxels take this ball.
 //      Every voxel is written once, the first time it is covered, so no atomics are needed.
-//   5. every void voxel marks its centre 255 in the mask (SetMaximumBallsMaskInActiveArea).
+//   5. every ball that wins a voxel marks its centre 255 in the mask
+//      (SetMaximumBallsMaskInActiveArea).
 #include "pma_internal.h"
 
 namespace pma {
@@ -28,10 +31,17 @@ constexpr int TZ = 8, TX = 8, TY = 64;  // tile: 64 rows (z, x) of 64 voxels alo
 #ifndef PMA_CB_SLAB_PAIRS
 #define PMA_CB_SLAB_PAIRS (1ull << 30)  // most (tile, ball) pairs sorted at once
 #endif
+#ifndef PMA_CB_GZ
+#define PMA_CB_GZ 4  // tiles per pair cell along z (4 x 2 measured best of 1..4 x 1..4 at 1024^3)
+#endif
+#ifndef PMA_CB_GX
+#define PMA_CB_GX 2  // tiles per pair cell along x
+#endif
 #ifndef PMA_CB_WIN16_MAX
 #define PMA_CB_WIN16_MAX 0xFFFFu  // longest tile run whose winners fit 16-bit slots
 #endif
 constexpr uint8_t kCandidate = 126;    // temporary mask value of a surviving centre
+constexpr uint32_t kXcds = 8;          // accelerator complex dies; workgroups go round-robin over them
 
 struct alignas(16) Ball {
     int x, y, z;
@@ -93,40 +103,68 @@ __device__ __forceinline__ size_t flat_block() {
     return ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
 }
 
-// Block = 4 rows of 64 voxels along y.  Marks solid 0, void 127 and surviving centres
-// kCandidate, and writes the block's count of surviving centres.
+// Block = 4 rows of 64 voxels along y, walking kPruneZ planes along z with the planes z - 1,
+// z, z + 1 (and a one-voxel halo in x and y; outside the image reads 0) in an LDS ring.  Marks
+// solid 0, void 127 and surviving centres kCandidate, and writes every plane's count of
+// surviving centres at the index emit_centres' block (same 4 x 64 rows, one plane) has.
+// c is dropped when a neighbour q of offset class k has q > c and q >= far_k(c), i.e. when the
+// class maximum of q reaches max(c + 1, far_k(c)).
+constexpr int kPruneZ = 16;
+constexpr int kPruneW = 64 + 2, kPruneH = 4 + 2, kPrunePlane = kPruneW * kPruneH;
+
 __global__ void __launch_bounds__(256) prune_centres(const uint32_t* __restrict__ e, uint8_t* __restrict__ mask,
                                                      const uint32_t* __restrict__ far, uint32_t L,
                                                      uint32_t* __restrict__ block_count, int X, int Y, int Z) {
+    __shared__ uint32_t ring[3][kPrunePlane];
     __shared__ uint32_t part[4];
-    int y = blockIdx.x * 64 + (threadIdx.x & 63);
-    int x = blockIdx.y * 4 + (threadIdx.x >> 6);
-    int z = blockIdx.z;
-    bool keep = false;
-    if (x < X && y < Y) {
-        size_t i = ((size_t)z * X + x) * Y + y;
-        uint32_t c = e[i];
-        if (c == 0) {
-            mask[i] = 0;
-        } else {
-            keep = true;
-            for (int dz = -1; dz <= 1 && keep; ++dz)
-                for (int dx = -1; dx <= 1 && keep; ++dx)
-                    for (int dy = -1; dy <= 1; ++dy) {
-                        int k = abs(dz) + abs(dx) + abs(dy);
-                        if (k == 0) continue;
-                        int nx = x + dx, ny = y + dy, nz = z + dz;
-                        if (nx < 0 || nx >= X || ny < 0 || ny >= Y || nz < 0 || nz >= Z) continue;
-                        uint32_t q = e[((size_t)nz * X + nx) * Y + ny];
-                        if (q > c && far[(size_t)(k - 1) * L + c] <= q) { keep = false; break; }
-                    }
-            mask[i] = keep ? kCandidate : 127;
+    const int lane = threadIdx.x & 63, row = threadIdx.x >> 6;
+    const int y = blockIdx.x * 64 + lane, x = blockIdx.y * 4 + row;
+    const int za = blockIdx.z * kPruneZ, zb = min(za + kPruneZ, Z);
+    auto load_plane = [&](int z) {
+        uint32_t* dst = ring[(z + 3) % 3];
+        for (int k = threadIdx.x; k < kPrunePlane; k += 256) {
+            const int xx = blockIdx.y * 4 - 1 + k / kPruneW, yy = blockIdx.x * 64 - 1 + k % kPruneW;
+            dst[k] = (z >= 0 && z < Z && xx >= 0 && xx < X && yy >= 0 && yy < Y) ? e[((size_t)z * X + xx) * Y + yy]
+                                                                                  : 0u;
         }
+    };
+    load_plane(za - 1);
+    load_plane(za);
+    for (int z = za; z < zb; ++z) {
+        load_plane(z + 1);
+        __syncthreads();
+        bool keep = false;
+        if (x < X && y < Y) {
+            const size_t i = ((size_t)z * X + x) * Y + y;
+            const int o = (row + 1) * kPruneW + lane + 1;
+            const uint32_t c = ring[(z + 3) % 3][o];
+            if (c == 0) {
+                mask[i] = 0;
+            } else {
+                uint32_t m1 = 0, m2 = 0, m3 = 0;
+                for (int dz = -1; dz <= 1; ++dz) {
+                    const uint32_t* pl = ring[(z + dz + 3) % 3];
+                    for (int dx = -1; dx <= 1; ++dx)
+                        for (int dy = -1; dy <= 1; ++dy) {
+                            const int k = abs(dz) + abs(dx) + abs(dy);
+                            const uint32_t q = pl[o + dx * kPruneW + dy];
+                            if (k == 1) m1 = max(m1, q);
+                            else if (k == 2) m2 = max(m2, q);
+                            else if (k == 3) m3 = max(m3, q);
+                        }
+                }
+                keep = !(m1 >= max(c + 1, far[c]) || m2 >= max(c + 1, far[L + c]) ||
+                         m3 >= max(c + 1, far[2 * (size_t)L + c]));
+                mask[i] = keep ? kCandidate : 127;
+            }
+        }
+        const uint64_t bal = __ballot(keep);
+        if (lane == 0) part[row] = (uint32_t)__popcll(bal);
+        __syncthreads();
+        if (threadIdx.x == 0)
+            block_count[((size_t)z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = part[0] + part[1] + part[2] +
+                                                                                        part[3];
     }
-    uint64_t bal = __ballot(keep);
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = (uint32_t)__popcll(bal);
-    __syncthreads();
-    if (threadIdx.x == 0) block_count[flat_block()] = part[0] + part[1] + part[2] + part[3];
 }
 
 // Sort key of every surviving centre: (maxE - r2) << 32 | scan, ascending = the win order.
@@ -179,28 +217,36 @@ __global__ void decode_centres(const unsigned long long* __restrict__ keys, size
     balls[r] = b;
 }
 
-// Tiles of the slab [tz0, tz1) that ball b reaches (box distance^2 <= r2): counted (out == null)
-// or listed as (tile - tile_base, rank) pairs from `at`.
-__device__ uint32_t ball_tiles(const Ball& b, int X, int Y, int Z, int NTX, int NTY, int tz0, int tz1,
-                               uint32_t tile_base, uint32_t rank, uint32_t* __restrict__ pk,
-                               uint32_t* __restrict__ pv, unsigned long long at) {
+// Pair cells: gz x gx tiles in (z, x) share one pair list, so a ball is listed once per cell
+// instead of once per tile; each tile's wave still skips the balls that miss its own voxels.
+struct Cells {
+    int sz, sx;    // cell edge in voxels along z and x (the y edge is TY)
+    int gz, gx;    // tiles per cell along z and x
+    int NCX, NCY;  // cells along x and y
+};
+
+// Cells of the slab [cz0, cz1) that ball b reaches (box distance^2 <= r2): counted (pk == null)
+// or listed as (cell - cell_base, rank) pairs from `at`.
+__device__ uint32_t ball_tiles(const Ball& b, int X, int Y, int Z, Cells g, int cz0, int cz1, uint32_t cell_base,
+                               uint32_t rank, uint32_t* __restrict__ pk, uint32_t* __restrict__ pv,
+                               unsigned long long at) {
     const int R = (int)isqrt_u32(b.r2);
-    const int za = max(b.z - R, 0) / TZ, zb = min(b.z + R, Z - 1) / TZ;
-    const int xa = max(b.x - R, 0) / TX, xb = min(b.x + R, X - 1) / TX;
+    const int za = max(b.z - R, 0) / g.sz, zb = min(b.z + R, Z - 1) / g.sz;
+    const int xa = max(b.x - R, 0) / g.sx, xb = min(b.x + R, X - 1) / g.sx;
     const int ya = max(b.y - R, 0) / TY, yb = min(b.y + R, Y - 1) / TY;
     uint32_t n = 0;
-    for (int tz = max(za, tz0); tz <= min(zb, tz1 - 1); ++tz) {
-        const int z0 = tz * TZ, z1 = min(z0 + TZ, Z) - 1;
+    for (int tz = max(za, cz0); tz <= min(zb, cz1 - 1); ++tz) {
+        const int z0 = tz * g.sz, z1 = min(z0 + g.sz, Z) - 1;
         const int dz = max(max(z0 - b.z, b.z - z1), 0);
         for (int tx = xa; tx <= xb; ++tx) {
-            const int x0 = tx * TX, x1 = min(x0 + TX, X) - 1;
+            const int x0 = tx * g.sx, x1 = min(x0 + g.sx, X) - 1;
             const int dx = max(max(x0 - b.x, b.x - x1), 0);
             for (int ty = ya; ty <= yb; ++ty) {
                 const int y0 = ty * TY, y1 = min(y0 + TY, Y) - 1;
                 const int dy = max(max(y0 - b.y, b.y - y1), 0);
                 if ((uint32_t)(dz * dz + dx * dx + dy * dy) > b.r2) continue;
                 if (pk) {
-                    pk[at + n] = (uint32_t)((tz * NTX + tx) * NTY + ty) - tile_base;
+                    pk[at + n] = (uint32_t)((tz * g.NCX + tx) * g.NCY + ty) - cell_base;
                     pv[at + n] = rank;
                 }
                 ++n;
@@ -210,22 +256,22 @@ __device__ uint32_t ball_tiles(const Ball& b, int X, int Y, int Z, int NTX, int 
     return n;
 }
 
-__global__ void count_pairs(const Ball* __restrict__ balls, size_t n, int X, int Y, int Z, int NTX, int NTY, int tz0,
-                            int tz1, unsigned long long* __restrict__ cnt) {
+__global__ void count_pairs(const Ball* __restrict__ balls, size_t n, int X, int Y, int Z, Cells g, int cz0, int cz1,
+                            unsigned long long* __restrict__ cnt) {
     size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
-    cnt[r] = ball_tiles(balls[r], X, Y, Z, NTX, NTY, tz0, tz1, 0, 0, nullptr, nullptr, 0);
+    cnt[r] = ball_tiles(balls[r], X, Y, Z, g, cz0, cz1, 0, 0, nullptr, nullptr, 0);
 }
 
 __global__ void emit_pairs(const Ball* __restrict__ balls, size_t n, const unsigned long long* __restrict__ off,
-                           int X, int Y, int Z, int NTX, int NTY, int tz0, int tz1, uint32_t tile_base,
+                           int X, int Y, int Z, Cells g, int cz0, int cz1, uint32_t cell_base,
                            uint32_t* __restrict__ pk, uint32_t* __restrict__ pv) {
     size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
-    ball_tiles(balls[r], X, Y, Z, NTX, NTY, tz0, tz1, tile_base, (uint32_t)r, pk, pv, off[r]);
+    ball_tiles(balls[r], X, Y, Z, g, cz0, cz1, cell_base, (uint32_t)r, pk, pv, off[r]);
 }
 
-// [start, end) of every tile's run in the tile-sorted pair list.
+// [start, end) of every cell's run in the cell-sorted pair list.
 __global__ void tile_ranges(const uint32_t* __restrict__ pk, size_t P, uint32_t* __restrict__ start,
                             uint32_t* __restrict__ end) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -243,7 +289,7 @@ __device__ __forceinline__ uint32_t isqrt_fast(uint32_t v) {
     return s;
 }
 
-// Largest pair run of any tile of the slab (decides the width of the per-voxel winner slot).
+// Largest pair run of any cell of the slab (decides the width of the per-voxel winner slot).
 __global__ void __launch_bounds__(256) max_run(const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
                                                size_t n, unsigned int* __restrict__ out) {
     uint32_t m = 0;
@@ -253,22 +299,28 @@ __global__ void __launch_bounds__(256) max_run(const uint32_t* __restrict__ star
     if ((threadIdx.x & 63) == 0 && m) atomicMax(out, m);
 }
 
-// One wave per tile (see the work plan, step 4).  The winner of a voxel is kept in LDS as the
-// position of its ball in the tile's run (16 bits when every run is shorter than 65536).
+// One wave per tile (work plan, step 4), walking the pair run of the tile's cell.  The winner of
+// a voxel is kept in LDS as the position of its ball in that run (16 bits when every run is
+// shorter than 65536).  Workgroups are dispatched round-robin over the 8 XCDs, so the block
+// index is mapped such that the tiles of one cell run on the same XCD and share its L2.
 template <class W>
 __global__ void __launch_bounds__(64) cover_tiles(const Ball* __restrict__ balls, const uint32_t* __restrict__ rank_of,
                                                   const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
                                                   const uint32_t* __restrict__ e, uint32_t* __restrict__ cr,
                                                   uint32_t* __restrict__ cx, uint32_t* __restrict__ cy,
                                                   uint32_t* __restrict__ cz, uint8_t* __restrict__ mask, int X, int Y,
-                                                  int Z, int NTX, int NTY, uint32_t tile_base,
+                                                  int Z, Cells g, uint32_t cell_base, uint32_t slab_cells,
                                                   uint32_t* __restrict__ error_flag) {
     __shared__ W win[TZ * TX * TY];
     const int lane = threadIdx.x;
-    const uint32_t local = blockIdx.x;
-    const uint32_t t = tile_base + local;
-    const int ty = (int)(t % (uint32_t)NTY), tx = (int)((t / (uint32_t)NTY) % (uint32_t)NTX);
-    const int tz = (int)(t / ((uint32_t)NTY * (uint32_t)NTX));
+    const uint32_t per_cell = (uint32_t)(g.gz * g.gx);
+    const uint32_t k = blockIdx.x / kXcds, sub = k % per_cell;
+    const uint32_t local = (k / per_cell) * kXcds + blockIdx.x % kXcds;  // cell within the slab
+    if (local >= slab_cells) return;
+    const uint32_t cell = cell_base + local;
+    const int ty = (int)(cell % (uint32_t)g.NCY), ccx = (int)((cell / (uint32_t)g.NCY) % (uint32_t)g.NCX);
+    const int ccz = (int)(cell / ((uint32_t)g.NCY * (uint32_t)g.NCX));
+    const int tz = ccz * g.gz + (int)(sub / (uint32_t)g.gx), tx = ccx * g.gx + (int)(sub % (uint32_t)g.gx);
     const int z = tz * TZ + lane / TX, x = tx * TX + lane % TX, y0 = ty * TY;
 
     // coverage of this lane's row: solid voxels and voxels outside the image count as covered
@@ -282,32 +334,56 @@ __global__ void __launch_bounds__(64) cover_tiles(const Ball* __restrict__ balls
 
     const uint32_t s = start[local], en = end[local];
     for (uint32_t base = s; base < en; base += 64) {
-        if (__ballot(cov != ~0ull) == 0) break;  // every void voxel of the tile has its ball
+        const bool open_row = cov != ~0ull;
+        const uint64_t rows = __ballot(open_row);
+        if (rows == 0) break;  // every void voxel of the tile has its ball
+        // bounding box of the voxels still uncovered: a ball that cannot reach it covers nothing new
+        uint64_t ys = open_row ? ~cov : 0ull;
+        for (int off = 32; off > 0; off >>= 1) ys |= __shfl_xor(ys, off);
+        uint32_t xs = 0;
+        for (int r = 0; r < TZ; ++r) xs |= (uint32_t)(rows >> (r * TX)) & 0xFFu;
+        const int zlo = tz * TZ + __builtin_ctzll(rows) / TX, zhi = tz * TZ + (63 - __builtin_clzll(rows)) / TX;
+        const int xlo = tx * TX + __builtin_ctz(xs), xhi = tx * TX + 31 - __builtin_clz(xs);
+        const int ylo = y0 + __builtin_ctzll(ys), yhi = y0 + 63 - __builtin_clzll(ys);
+
         Ball mb{0, 0, 0, 0};
-        if (base + lane < en) mb = balls[rank_of[base + lane]];
-        const int cnt = (int)min(64u, en - base);
-        for (int j = 0; j < cnt; ++j) {
+        bool reach = false;
+        if (base + lane < en) {
+            mb = balls[rank_of[base + lane]];
+            const int ez = max(max(zlo - mb.z, mb.z - zhi), 0), ex = max(max(xlo - mb.x, mb.x - xhi), 0);
+            const int ey = max(max(ylo - mb.y, mb.y - yhi), 0);
+            reach = (uint32_t)(ez * ez + ex * ex + ey * ey) <= mb.r2;
+        }
+        uint64_t todo = __ballot(reach), won = 0;
+        while (todo) {
+            const int j = __builtin_ctzll(todo);
+            todo &= todo - 1;
             // the ball's fields are uniform: read them into scalar registers
             const int bx = __builtin_amdgcn_readlane(mb.x, j), by = __builtin_amdgcn_readlane(mb.y, j);
             const int bz = __builtin_amdgcn_readlane(mb.z, j);
             const int r2 = __builtin_amdgcn_readlane((int)mb.r2, j);
             const int dz = z - bz, dx = x - bx;
             const int rem = r2 - dz * dz - dx * dx;
-            if (rem < 0) continue;
-            const int h = (int)isqrt_fast((uint32_t)rem);
-            const int lo = max(by - h - y0, 0), hi = min(by + h - y0, 63);
-            if (lo > hi) continue;
-            uint64_t nw = ((~0ull) >> (63 - (hi - lo))) << lo;
-            nw &= ~cov;
-            if (!nw) continue;
-            cov |= nw;
-            const W slot = (W)(base - s + (uint32_t)j);
-            do {
-                const int bit = __ffsll((long long)nw) - 1;
-                win[lane * TY + bit] = slot;
-                nw &= nw - 1;
-            } while (nw);
+            uint64_t nw = 0;
+            if (rem >= 0) {
+                const int h = (int)isqrt_fast((uint32_t)rem);
+                const int lo = max(by - h - y0, 0), hi = min(by + h - y0, 63);
+                if (lo <= hi) nw = (((~0ull) >> (63 - (hi - lo))) << lo) & ~cov;
+            }
+            if (__ballot(nw != 0) == 0) continue;
+            won |= 1ull << j;
+            if (nw) {
+                cov |= nw;
+                const W slot = (W)(base - s + (uint32_t)j);
+                do {
+                    const int bit = __ffsll((long long)nw) - 1;
+                    win[lane * TY + bit] = slot;
+                    nw &= nw - 1;
+                } while (nw);
+            }
         }
+        // a ball that won a voxel of this tile is a maximal ball
+        if ((won >> lane) & 1ull) mask[((size_t)mb.z * X + mb.x) * Y + mb.y] = 255;
     }
 
     // outputs, one row of 64 voxels per step (lanes along y)
@@ -330,7 +406,6 @@ __global__ void __launch_bounds__(64) cover_tiles(const Ball* __restrict__ balls
         cx[i] = (uint32_t)b.x;
         cy[i] = (uint32_t)b.y;
         cz[i] = (uint32_t)b.z;
-        mask[((size_t)b.z * X + b.x) * Y + b.y] = 255;
     }
 }
 
@@ -373,7 +448,8 @@ void run_containing_balls(pma_ctx* c) {
     unsigned long long* boff = c->buf<unsigned long long>("cb_block_off", nblocks);
     {
         PMA_TIMED(c, "prune_centres");
-        prune_centres<<<vgrid, 256, 0, c->stream>>>(e, mask, far, (uint32_t)L, bcount, X, Y, Z);
+        const dim3 pgrid(vgrid.x, vgrid.y, (unsigned)((Z + kPruneZ - 1) / kPruneZ));
+        prune_centres<<<pgrid, 256, 0, c->stream>>>(e, mask, far, (uint32_t)L, bcount, X, Y, Z);
         PMA_LAUNCHED(c);
     }
     widen_counts<<<grid_for(nblocks, 256), 256, 0, c->stream>>>(bcount, nblocks, bcount64);
@@ -398,10 +474,18 @@ void run_containing_balls(pma_ctx* c) {
         PMA_LAUNCHED(c);
     }
 
-    // 3-4. (tile, rank) pairs sorted by tile, then one wave per tile; z-slabs of tiles keep the
+    // 3-4. (cell, rank) pairs sorted by cell, then one wave per tile; z-slabs of cells keep the
     // pair list below kSlabPairs.
-    const int NTZ = (Z + TZ - 1) / TZ, NTX = (X + TX - 1) / TX, NTY = (Y + TY - 1) / TY;
-    const uint32_t layer_tiles = (uint32_t)NTX * (uint32_t)NTY;
+    Cells g;
+    g.gz = PMA_CB_GZ;
+    g.gx = PMA_CB_GX;
+    g.sz = TZ * g.gz;
+    g.sx = TX * g.gx;
+    const int NCZ = (Z + g.sz - 1) / g.sz;
+    g.NCX = (X + g.sx - 1) / g.sx;
+    g.NCY = (Y + TY - 1) / TY;
+    const uint32_t per_cell = (uint32_t)(g.gz * g.gx);
+    const uint32_t layer_cells = (uint32_t)g.NCX * (uint32_t)g.NCY;
     constexpr unsigned long long kSlabPairs = PMA_CB_SLAB_PAIRS;
     unsigned long long* pcount = c->buf<unsigned long long>("cb_pair_count", N1);
     unsigned long long* poff = c->buf<unsigned long long>("cb_pair_off", N1);
@@ -409,13 +493,12 @@ void run_containing_balls(pma_ctx* c) {
     PMA_HIP(hipMemsetAsync(err, 0, sizeof(uint32_t), c->stream));
     uint64_t total_pairs = 0;
     int slabs = 1;
-    for (int tz0 = 0; tz0 < NTZ;) {
-        int tz1 = std::min(NTZ, tz0 + (NTZ + slabs - 1) / slabs);
+    for (int tz0 = 0; tz0 < NCZ;) {
+        int tz1 = std::min(NCZ, tz0 + (NCZ + slabs - 1) / slabs);
         uint64_t P = 0;
         while (true) {
             if (n_cand) {
-                count_pairs<<<grid_for(n_cand, 256), 256, 0, c->stream>>>(balls, n_cand, X, Y, Z, NTX, NTY, tz0, tz1,
-                                                                          pcount);
+                count_pairs<<<grid_for(n_cand, 256), 256, 0, c->stream>>>(balls, n_cand, X, Y, Z, g, tz0, tz1, pcount);
                 PMA_LAUNCHED(c);
                 P = exclusive_scan_u64(c, reinterpret_cast<const uint64_t*>(pcount), reinterpret_cast<uint64_t*>(poff),
                                        n_cand);
@@ -424,24 +507,24 @@ void run_containing_balls(pma_ctx* c) {
             tz1 = tz0 + (tz1 - tz0 + 1) / 2;  // halve the slab until its pairs fit
         }
         PMA_REQUIRE(P <= 0xFFFFFFFFull, PMA_ERR_CAPACITY, "pma_containing_balls: one tile layer reaches too many balls");
-        const uint32_t tile_base = (uint32_t)tz0 * layer_tiles;
-        const uint32_t slab_tiles = (uint32_t)(tz1 - tz0) * layer_tiles;
+        const uint32_t cell_base = (uint32_t)tz0 * layer_cells;
+        const uint32_t slab_cells = (uint32_t)(tz1 - tz0) * layer_cells;
         const size_t P1 = std::max<uint64_t>(P, 1);
         uint32_t* pk = c->buf<uint32_t>("cb_pair_tile", P1);
         uint32_t* pv = c->buf<uint32_t>("cb_pair_rank", P1);
         uint32_t* pk2 = c->buf<uint32_t>("cb_pair_tile2", P1);
         uint32_t* pv2 = c->buf<uint32_t>("cb_pair_rank2", P1);
-        uint32_t* tstart = c->buf<uint32_t>("cb_tile_start", slab_tiles);
-        uint32_t* tend = c->buf<uint32_t>("cb_tile_end", slab_tiles);
-        PMA_HIP(hipMemsetAsync(tstart, 0, slab_tiles * sizeof(uint32_t), c->stream));
-        PMA_HIP(hipMemsetAsync(tend, 0, slab_tiles * sizeof(uint32_t), c->stream));
+        uint32_t* tstart = c->buf<uint32_t>("cb_cell_start", slab_cells);
+        uint32_t* tend = c->buf<uint32_t>("cb_cell_end", slab_cells);
+        PMA_HIP(hipMemsetAsync(tstart, 0, slab_cells * sizeof(uint32_t), c->stream));
+        PMA_HIP(hipMemsetAsync(tend, 0, slab_cells * sizeof(uint32_t), c->stream));
         if (P) {
-            emit_pairs<<<grid_for(n_cand, 256), 256, 0, c->stream>>>(balls, n_cand, poff, X, Y, Z, NTX, NTY, tz0, tz1,
-                                                                     tile_base, pk, pv);
+            emit_pairs<<<grid_for(n_cand, 256), 256, 0, c->stream>>>(balls, n_cand, poff, X, Y, Z, g, tz0, tz1,
+                                                                     cell_base, pk, pv);
             PMA_LAUNCHED(c);
-            int tile_bits = 1;
-            while (tile_bits < 32 && (slab_tiles >> tile_bits) != 0) ++tile_bits;
-            radix_sort_pairs_u32(c, pk, pk2, pv, pv2, P, 0, tile_bits);
+            int cell_bits = 1;
+            while (cell_bits < 32 && (slab_cells >> cell_bits) != 0) ++cell_bits;
+            radix_sort_pairs_u32(c, pk, pk2, pv, pv2, P, 0, cell_bits);
             tile_ranges<<<grid_for(P, 256), 256, 0, c->stream>>>(pk2, P, tstart, tend);
             PMA_LAUNCHED(c);
         }
@@ -449,7 +532,7 @@ void run_containing_balls(pma_ctx* c) {
         if (P) {
             unsigned int* dmax = c->buf<unsigned int>("cb_max_run", 1);
             PMA_HIP(hipMemsetAsync(dmax, 0, 4, c->stream));
-            max_run<<<std::min(grid_for(slab_tiles, 256), 1024u), 256, 0, c->stream>>>(tstart, tend, slab_tiles, dmax);
+            max_run<<<std::min(grid_for(slab_cells, 256), 1024u), 256, 0, c->stream>>>(tstart, tend, slab_cells, dmax);
             PMA_LAUNCHED(c);
             longest = read_scalar(c, dmax);
         }
@@ -459,12 +542,14 @@ void run_containing_balls(pma_ctx* c) {
             uint32_t* cx = c->v<uint32_t>(PMA_VOL_CX);
             uint32_t* cy = c->v<uint32_t>(PMA_VOL_CY);
             uint32_t* cz = c->v<uint32_t>(PMA_VOL_CZ);
+            // one workgroup per tile, cell-major within each XCD's share (see cover_tiles)
+            const uint32_t blocks = (slab_cells + kXcds - 1) / kXcds * kXcds * per_cell;
             if (longest <= PMA_CB_WIN16_MAX)
-                cover_tiles<uint16_t><<<slab_tiles, 64, 0, c->stream>>>(balls, pv2, tstart, tend, e, cr, cx, cy, cz, mask,
-                                                                       X, Y, Z, NTX, NTY, tile_base, err);
+                cover_tiles<uint16_t><<<blocks, 64, 0, c->stream>>>(balls, pv2, tstart, tend, e, cr, cx, cy, cz, mask,
+                                                                   X, Y, Z, g, cell_base, slab_cells, err);
             else
-                cover_tiles<uint32_t><<<slab_tiles, 64, 0, c->stream>>>(balls, pv2, tstart, tend, e, cr, cx, cy, cz, mask,
-                                                                       X, Y, Z, NTX, NTY, tile_base, err);
+                cover_tiles<uint32_t><<<blocks, 64, 0, c->stream>>>(balls, pv2, tstart, tend, e, cr, cx, cy, cz, mask,
+                                                                   X, Y, Z, g, cell_base, slab_cells, err);
             PMA_LAUNCHED(c);
         }
         total_pairs += P;

@@ -6,9 +6,10 @@
 //   pass X: one lane per (y, z) column, forward + backward scan of the 1-D distance,
 //   pass Y: one workgroup per (x, z) row staged in LDS, exact windowed minimum of
 //           g(y') + (y-y')^2 over |y-y'| <= floor(sqrt(g(y))) (the reference's window),
-//   pass Z: one lane per (x, y) column, lanes along y so every load is coalesced, the same
-//           windowed minimum along z, fused with the maximum and the histogram of squares
-//           whose ball does not cross the image boundary (:310-327).
+//   pass Z: 64 (x, y) columns per workgroup, lanes along y so every load is coalesced, staged
+//           in LDS per chunk of z with a halo; the same windowed minimum along z, fused with the
+//           maximum and the histogram of squares whose ball does not cross the image boundary
+//           (:310-327).
 #include "pma_internal.h"
 
 namespace pma {
@@ -60,31 +61,45 @@ __global__ void __launch_bounds__(256) edt_pass_y(const uint32_t* __restrict__ g
     }
 }
 
+// Pass Z on chunks of kChunkZ planes: a block stages 64 columns (lanes along y) of its chunk and
+// kHaloZ planes either side in LDS, and each of its 4 waves takes every 4th plane.  A voxel
+// whose window reaches beyond the staged planes reads the rest from global memory.
+constexpr int kChunkZ = 128, kHaloZ = 32, kStageZ = kChunkZ + 2 * kHaloZ;
+
 __global__ void __launch_bounds__(256) edt_pass_z(const uint32_t* __restrict__ h,
                                                   uint32_t* __restrict__ e, int X, int Y, int Z,
                                                   unsigned long long* __restrict__ hist,
                                                   uint32_t hist_len, uint32_t* __restrict__ max_out) {
+    __shared__ uint32_t col[kStageZ][64];
     __shared__ uint32_t lhist[kLocalBins];
     __shared__ uint32_t wave_max[4];
+    const int lane = threadIdx.x & 63, sub = threadIdx.x >> 6;
+    const int y = blockIdx.x * 64 + lane, x = blockIdx.y;
+    const int z0 = blockIdx.z * kChunkZ, z1 = min(z0 + kChunkZ, Z);
+    const int s0 = max(z0 - kHaloZ, 0), s1 = min(z1 + kHaloZ, Z);  // staged planes [s0, s1)
+    const size_t stride = (size_t)X * Y;
+    const size_t base = (size_t)x * Y + y;
     for (int i = threadIdx.x; i < kLocalBins; i += blockDim.x) lhist[i] = 0;
+    if (y < Y)
+        for (int z = s0 + sub; z < s1; z += 4) col[z - s0][lane] = h[base + (size_t)z * stride];
     __syncthreads();
-    int y = blockIdx.x * blockDim.x + threadIdx.x;
-    int x = blockIdx.y;
     uint32_t my_max = 0;
     if (y < Y) {
-        size_t stride = (size_t)X * Y;
-        size_t base = (size_t)x * Y + y;
-        int mxy = min(min(x, y), min(X - 1 - x, Y - 1 - y));
-        for (int z = 0; z < Z; ++z) {
-            uint32_t best = h[base + (size_t)z * stride];
+        const int mxy = min(min(x, y), min(X - 1 - x, Y - 1 - y));
+        for (int z = z0 + sub; z < z1; z += 4) {
+            uint32_t best = col[z - s0][lane];
             if (best != 0) {
-                int w = (int)isqrt_u32(best) + 1;
-                int lo = min(w, z), hi = min(w, Z - z);
-                for (int dz = -lo; dz < hi; ++dz)
-                    best = min(best, h[base + (size_t)(z + dz) * stride] + (uint32_t)(dz * dz));
+                const int w = (int)isqrt_u32(best) + 1;
+                const int lo = min(w, z), hi = min(w, Z - z);
+                if (z - lo >= s0 && z + hi <= s1) {
+                    for (int dz = -lo; dz < hi; ++dz) best = min(best, col[z + dz - s0][lane] + (uint32_t)(dz * dz));
+                } else {
+                    for (int dz = -lo; dz < hi; ++dz)
+                        best = min(best, h[base + (size_t)(z + dz) * stride] + (uint32_t)(dz * dz));
+                }
                 my_max = max(my_max, best);
                 // sqrt(best) > m  <=>  best > m*m for integer m >= 0
-                int m = min(mxy, min(z, Z - 1 - z));
+                const int m = min(mxy, min(z, Z - 1 - z));
                 if ((uint64_t)best <= (uint64_t)m * (uint64_t)m) {
                     if (best < kLocalBins) atomicAdd(&lhist[best], 1u);
                     else if (best < hist_len) atomicAdd(&hist[best], 1ull);
@@ -94,14 +109,10 @@ __global__ void __launch_bounds__(256) edt_pass_z(const uint32_t* __restrict__ h
         }
     }
     for (int off = 32; off > 0; off >>= 1) my_max = max(my_max, (uint32_t)__shfl_xor((int)my_max, off));
-    if ((threadIdx.x & 63) == 0) wave_max[threadIdx.x >> 6] = my_max;
+    if (lane == 0) wave_max[sub] = my_max;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t m = 0;
-        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) m = max(m, wave_max[i]);
-        atomicMax(max_out, m);
-    }
-    int lim = min((uint32_t)kLocalBins, hist_len);
+    if (threadIdx.x == 0) atomicMax(max_out, max(max(wave_max[0], wave_max[1]), max(wave_max[2], wave_max[3])));
+    const int lim = min((uint32_t)kLocalBins, hist_len);
     for (int i = threadIdx.x; i < lim; i += blockDim.x)
         if (lhist[i]) atomicAdd(&hist[i], (unsigned long long)lhist[i]);
 }
@@ -143,7 +154,8 @@ void run_edt(pma_ctx* c) {
     }
     {
         PMA_TIMED(c, "edt_pass_z");
-        edt_pass_z<<<dim3(grid_for(Y, 256), X), 256, 0, c->stream>>>(tmp, e, X, Y, Z, hist, hist_len, dmax);
+        edt_pass_z<<<dim3(grid_for(Y, 64), X, grid_for(Z, kChunkZ)), 256, 0, c->stream>>>(tmp, e, X, Y, Z, hist,
+                                                                                        hist_len, dmax);
         PMA_LAUNCHED(c);
     }
 

@@ -9,8 +9,11 @@
 // one-voxel halo in LDS (solid = -1, as in the distance volume), relaxes its own voxels by
 // pulling from their neighbours until nothing changes, writes back what improved, and
 // activates the neighbour blocks across any face, edge or corner where a border voxel
-// improved.  Rounds repeat until no block is active.  Every add is one correctly rounded
-// fp32 add (no contraction).
+// improved.  An active block's key is the smallest improved border distance that woke it; a
+// round processes the active blocks whose key is within kDelta of the smallest key and
+// carries the others, so blocks are visited roughly in distance order (fewer re-visits).
+// Rounds repeat until no block is active.  Every add is one correctly rounded fp32 add (no
+// contraction).
 //
 // Start voxels follow SelectStartingPosition / FindVoidPixelOnBoxBoundaries (:396-511):
 // the centre min((int)(0.5 * size + 0.5), size - 1) if void and unreached, else the first
@@ -89,105 +92,174 @@ __global__ void __launch_bounds__(256) find_start(const uint8_t* __restrict__ so
 
 
 constexpr int kB = 8;           // block edge
+constexpr float kDelta = 16.0f;  // bucket width of a round (distance units; 4..64 measured within 10%)
+constexpr uint32_t kRelaxGrid = 4096;  // workgroups of a relax launch (16 per CU)
 constexpr int kH = kB + 2;      // with halo
 constexpr int kHalo = kH * kH * kH;
 
-// One workgroup (256 threads, two voxels each) per active block.  The voxel `src` (or none,
-// ~0) starts at distance 0 and counts as improved, so the blocks around it wake up too.
-__global__ void __launch_bounds__(256) relax_blocks(float* __restrict__ dist, const uint32_t* __restrict__ active,
-                                                    uint32_t* __restrict__ next, uint32_t* __restrict__ next_count,
-                                                    uint32_t* __restrict__ block_tag, uint32_t tag, int X, int Y, int Z,
-                                                    int NBX, int NBY, int NBZ, int d3q7, unsigned long long src) {
+// Round state (device): [0] next list length, [1] smallest key of the next list (float
+// bits), [2] length of this round's processing list.
+__global__ void fill_key(uint32_t* __restrict__ p, uint32_t v, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void round_init(uint32_t* __restrict__ st) {
+    st[0] = 0;
+    st[1] = 0x7F800000u;  // +inf
+    st[2] = 0;
+}
+
+// Splits the active list: keys within the threshold go to `proc` (their key is cleared, so that
+// activations during the round start it afresh), the others are carried into the next list
+// (tagged so that a new activation does not list them twice).
+__global__ void __launch_bounds__(256) partition_active(const uint32_t* __restrict__ active, uint32_t n,
+                                                        uint32_t* __restrict__ block_key, uint32_t threshold,
+                                                        uint32_t* __restrict__ proc, uint32_t* __restrict__ next,
+                                                        uint32_t* __restrict__ block_tag, uint32_t tag,
+                                                        uint32_t* __restrict__ st) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    uint32_t b = 0, key = 0;
+    bool now = false, later = false;
+    if (t < n) {
+        b = active[t];
+        key = block_key[b];
+        now = key <= threshold;
+        later = !now;
+    }
+    if (now) block_key[b] = kFltMaxBits;
+    if (later) {
+        (void)atomicExch(&block_tag[b], tag);
+        atomicMin(&st[1], key);
+    }
+    const unsigned long long bn = __ballot(now), bl = __ballot(later);
+    uint32_t base_n = 0, base_l = 0;
+    if (lane == 0) {
+        if (bn) base_n = atomicAdd(&st[2], (uint32_t)__popcll(bn));
+        if (bl) base_l = atomicAdd(&st[0], (uint32_t)__popcll(bl));
+    }
+    base_n = (uint32_t)__shfl((int)base_n, 0);
+    base_l = (uint32_t)__shfl((int)base_l, 0);
+    const unsigned long long below = (1ull << lane) - 1;
+    if (now) proc[base_n + (uint32_t)__popcll(bn & below)] = b;
+    if (later) next[base_l + (uint32_t)__popcll(bl & below)] = b;
+}
+
+// A workgroup (256 threads, two voxels each) relaxes one block of this round's processing list
+// (length st[2]) at a time, striding over the list by the grid size.  The voxel `src` (or
+// none, ~0) starts at distance 0 and counts as improved, so the blocks around it wake up too.
+__global__ void __launch_bounds__(256) relax_blocks(float* __restrict__ dist, const uint32_t* __restrict__ proc,
+                                                    uint32_t* __restrict__ next, uint32_t* __restrict__ block_tag,
+                                                    uint32_t tag, uint32_t* __restrict__ block_key,
+                                                    uint32_t* __restrict__ st, int X, int Y, int Z, int NBX, int NBY,
+                                                    int NBZ, int d3q7, unsigned long long src) {
     __shared__ float sd[kHalo];
     __shared__ int s_changed;
     __shared__ uint32_t s_touch;  // bit (dz+1)*9 + (dx+1)*3 + (dy+1): neighbour blocks to activate
+    __shared__ uint32_t s_bmin;   // smallest improved border distance (float bits)
     __shared__ uint32_t s_push[27];
     __shared__ uint32_t s_npush, s_base;
-    const uint32_t b = active[blockIdx.x];
-    const int by = (int)(b % (uint32_t)NBY), bx = (int)((b / (uint32_t)NBY) % (uint32_t)NBX);
-    const int bz = (int)(b / ((uint32_t)NBY * (uint32_t)NBX));
-    const int z0 = bz * kB - 1, x0 = bx * kB - 1, y0 = by * kB - 1;
-    for (int k = threadIdx.x; k < kHalo; k += blockDim.x) {
-        const int lz = k / (kH * kH), lx = (k / kH) % kH, ly = k % kH;
-        const int z = z0 + lz, x = x0 + lx, y = y0 + ly;
-        sd[k] = (z >= 0 && z < Z && x >= 0 && x < X && y >= 0 && y < Y) ? dist[((size_t)z * X + x) * Y + y] : -1.0f;
-    }
-    if (threadIdx.x == 0) s_touch = 0;
-    __syncthreads();
-
-    int own[2];
-    float orig[2];
-    for (int o = 0; o < 2; ++o) {
-        const int v = threadIdx.x + o * 256;
-        own[o] = ((v / 64 + 1) * kH + (v / 8) % 8 + 1) * kH + v % 8 + 1;
-        orig[o] = sd[own[o]];
-        const int lz = v / 64, lx = (v / 8) % 8, ly = v % 8;
-        const int z = z0 + 1 + lz, x = x0 + 1 + lx, y = y0 + 1 + ly;
-        if (z < Z && x < X && y < Y && ((size_t)z * X + x) * Y + y == src) sd[own[o]] = 0.0f;
-    }
-    const float w1 = 1.0f, w2 = sqrtf(2.0f), w3 = sqrtf(3.0f);
-    while (true) {
-        if (threadIdx.x == 0) s_changed = 0;
+    const uint32_t count = st[2];
+    for (uint32_t item = blockIdx.x; item < count; item += gridDim.x) {
+        __syncthreads();  // the previous item's shared state has been consumed
+        const uint32_t b = proc[item];
+        const int by = (int)(b % (uint32_t)NBY), bx = (int)((b / (uint32_t)NBY) % (uint32_t)NBX);
+        const int bz = (int)(b / ((uint32_t)NBY * (uint32_t)NBX));
+        const int z0 = bz * kB - 1, x0 = bx * kB - 1, y0 = by * kB - 1;
+        for (int k = threadIdx.x; k < kHalo; k += blockDim.x) {
+            const int lz = k / (kH * kH), lx = (k / kH) % kH, ly = k % kH;
+            const int z = z0 + lz, x = x0 + lx, y = y0 + ly;
+            sd[k] = (z >= 0 && z < Z && x >= 0 && x < X && y >= 0 && y < Y) ? dist[((size_t)z * X + x) * Y + y] : -1.0f;
+        }
+        if (threadIdx.x == 0) {
+            s_touch = 0;
+            s_bmin = kFltMaxBits;
+        }
         __syncthreads();
-        int changed = 0;
+
+        int own[2];
+        float orig[2];
         for (int o = 0; o < 2; ++o) {
-            const int c = own[o];
-            const float dv = sd[c];
-            if (dv < 0.0f) continue;  // solid or outside the image
-            float best = dv;
-            for (int k = 0; k < 27; ++k) {
-                const int dz = k / 9 - 1, dx = (k / 3) % 3 - 1, dy = k % 3 - 1;
-                const int m = abs(dz) + abs(dx) + abs(dy);
-                if (m == 0 || (d3q7 && m > 1)) continue;
-                const float du = sd[c + (dz * kH + dx) * kH + dy];
-                if (du < 0.0f || __float_as_uint(du) >= kFltMaxBits) continue;
-                const float w = d3q7 ? 1.0f : (m == 1 ? w1 : (m == 2 ? w2 : w3));
-                best = fminf(best, __fadd_rn(du, w));
-            }
-            if (best < dv) {
-                sd[c] = best;
-                changed = 1;
-            }
+            const int v = threadIdx.x + o * 256;
+            own[o] = ((v / 64 + 1) * kH + (v / 8) % 8 + 1) * kH + v % 8 + 1;
+            orig[o] = sd[own[o]];
+            const int lz = v / 64, lx = (v / 8) % 8, ly = v % 8;
+            const int z = z0 + 1 + lz, x = x0 + 1 + lx, y = y0 + 1 + ly;
+            if (z < Z && x < X && y < Y && ((size_t)z * X + x) * Y + y == src) sd[own[o]] = 0.0f;
         }
-        if (changed) s_changed = 1;
-        __syncthreads();
-        const int any = s_changed;
-        __syncthreads();
-        if (!any) break;
-    }
+        const float w1 = 1.0f, w2 = sqrtf(2.0f), w3 = sqrtf(3.0f);
+        while (true) {
+            if (threadIdx.x == 0) s_changed = 0;
+            __syncthreads();
+            int changed = 0;
+            for (int o = 0; o < 2; ++o) {
+                const int c = own[o];
+                const float dv = sd[c];
+                if (dv < 0.0f) continue;  // solid or outside the image
+                float best = dv;
+                for (int k = 0; k < 27; ++k) {
+                    const int dz = k / 9 - 1, dx = (k / 3) % 3 - 1, dy = k % 3 - 1;
+                    const int m = abs(dz) + abs(dx) + abs(dy);
+                    if (m == 0 || (d3q7 && m > 1)) continue;
+                    const float du = sd[c + (dz * kH + dx) * kH + dy];
+                    if (du < 0.0f || __float_as_uint(du) >= kFltMaxBits) continue;
+                    const float w = d3q7 ? 1.0f : (m == 1 ? w1 : (m == 2 ? w2 : w3));
+                    best = fminf(best, __fadd_rn(du, w));
+                }
+                if (best < dv) {
+                    sd[c] = best;
+                    changed = 1;
+                }
+            }
+            if (changed) s_changed = 1;
+            __syncthreads();
+            const int any = s_changed;
+            __syncthreads();
+            if (!any) break;
+        }
 
-    // write back what improved; border voxels that improved wake the blocks across
-    uint32_t touch = 0;
-    for (int o = 0; o < 2; ++o) {
-        const int v = threadIdx.x + o * 256;
-        const float dv = sd[own[o]];
-        if (!(dv < orig[o])) continue;
-        const int lz = v / 64, lx = (v / 8) % 8, ly = v % 8;
-        const int z = z0 + 1 + lz, x = x0 + 1 + lx, y = y0 + 1 + ly;
-        dist[((size_t)z * X + x) * Y + y] = dv;
-        const int zlo = lz == 0 ? -1 : 0, zhi = lz == kB - 1 ? 1 : 0;
-        const int xlo = lx == 0 ? -1 : 0, xhi = lx == kB - 1 ? 1 : 0;
-        const int ylo = ly == 0 ? -1 : 0, yhi = ly == kB - 1 ? 1 : 0;
-        for (int dz = zlo; dz <= zhi; ++dz)
-            for (int dx = xlo; dx <= xhi; ++dx)
-                for (int dy = ylo; dy <= yhi; ++dy)
-                    if (dz | dx | dy) touch |= 1u << ((dz + 1) * 9 + (dx + 1) * 3 + (dy + 1));
-    }
-    if (touch) atomicOr(&s_touch, touch);
-    if (threadIdx.x == 0) s_npush = 0;
-    __syncthreads();
-    if (threadIdx.x < 27 && ((s_touch >> threadIdx.x) & 1u)) {
-        const int dz = (int)threadIdx.x / 9 - 1, dx = ((int)threadIdx.x / 3) % 3 - 1, dy = (int)threadIdx.x % 3 - 1;
-        const int nz = bz + dz, nx = bx + dx, ny = by + dy;
-        if (nz >= 0 && nz < NBZ && nx >= 0 && nx < NBX && ny >= 0 && ny < NBY) {
-            const uint32_t nb = ((uint32_t)nz * (uint32_t)NBX + (uint32_t)nx) * (uint32_t)NBY + (uint32_t)ny;
-            if (atomicExch(&block_tag[nb], tag) != tag) s_push[atomicAdd(&s_npush, 1u)] = nb;
+        // write back what improved; border voxels that improved wake the blocks across
+        uint32_t touch = 0, bmin = kFltMaxBits;
+        for (int o = 0; o < 2; ++o) {
+            const int v = threadIdx.x + o * 256;
+            const float dv = sd[own[o]];
+            if (!(dv < orig[o])) continue;
+            const int lz = v / 64, lx = (v / 8) % 8, ly = v % 8;
+            const int z = z0 + 1 + lz, x = x0 + 1 + lx, y = y0 + 1 + ly;
+            dist[((size_t)z * X + x) * Y + y] = dv;
+            const int zlo = lz == 0 ? -1 : 0, zhi = lz == kB - 1 ? 1 : 0;
+            const int xlo = lx == 0 ? -1 : 0, xhi = lx == kB - 1 ? 1 : 0;
+            const int ylo = ly == 0 ? -1 : 0, yhi = ly == kB - 1 ? 1 : 0;
+            for (int dz = zlo; dz <= zhi; ++dz)
+                for (int dx = xlo; dx <= xhi; ++dx)
+                    for (int dy = ylo; dy <= yhi; ++dy)
+                        if (dz | dx | dy) touch |= 1u << ((dz + 1) * 9 + (dx + 1) * 3 + (dy + 1));
+            if (zlo | zhi | xlo | xhi | ylo | yhi) bmin = min(bmin, __float_as_uint(dv));
         }
+        if (touch) {
+            atomicOr(&s_touch, touch);
+            atomicMin(&s_bmin, bmin);
+        }
+        if (threadIdx.x == 0) s_npush = 0;
+        __syncthreads();
+        if (threadIdx.x < 27 && ((s_touch >> threadIdx.x) & 1u)) {
+            const int dz = (int)threadIdx.x / 9 - 1, dx = ((int)threadIdx.x / 3) % 3 - 1, dy = (int)threadIdx.x % 3 - 1;
+            const int nz = bz + dz, nx = bx + dx, ny = by + dy;
+            if (nz >= 0 && nz < NBZ && nx >= 0 && nx < NBX && ny >= 0 && ny < NBY) {
+                const uint32_t nb = ((uint32_t)nz * (uint32_t)NBX + (uint32_t)nx) * (uint32_t)NBY + (uint32_t)ny;
+                atomicMin(&block_key[nb], s_bmin);
+                if (atomicExch(&block_tag[nb], tag) != tag) s_push[atomicAdd(&s_npush, 1u)] = nb;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if (s_npush) s_base = atomicAdd(&st[0], s_npush);
+            if (s_touch) atomicMin(&st[1], s_bmin);
+        }
+        __syncthreads();
+        if (threadIdx.x < s_npush) next[s_base + threadIdx.x] = s_push[threadIdx.x];
     }
-    __syncthreads();
-    if (threadIdx.x == 0 && s_npush) s_base = atomicAdd(next_count, s_npush);
-    __syncthreads();
-    if (threadIdx.x < s_npush) next[s_base + threadIdx.x] = s_push[threadIdx.x];
 }
 
 // Voxels reached in this cavity (void, cavity id unset, distance set) take the id; walls
@@ -262,12 +334,16 @@ void run_geodesic(pma_ctx* c) {
     const size_t NB = (size_t)NBZ * NBX * NBY;
     uint32_t* fa = c->buf<uint32_t>("geo_active_a", NB);
     uint32_t* fb = c->buf<uint32_t>("geo_active_b", NB);
+    uint32_t* fp = c->buf<uint32_t>("geo_active_proc", NB);
     uint32_t* block_tag = c->buf<uint32_t>("geo_block_tag", NB);
-    uint32_t* cnt = c->buf<uint32_t>("geo_count", 1);
+    uint32_t* block_key = c->buf<uint32_t>("geo_block_key", NB);
+    uint32_t* st = c->buf<uint32_t>("geo_round_state", 4);
     unsigned long long* key = c->buf<unsigned long long>("geo_key", 1);
     unsigned long long* vcount = c->buf<unsigned long long>("geo_vcount", 1);
     uint32_t* walls = c->buf<uint32_t>("geo_walls", 1);
     PMA_HIP(hipMemsetAsync(block_tag, 0xFF, NB * sizeof(uint32_t), c->stream));
+    fill_key<<<grid_for(NB, 256), 256, 0, c->stream>>>(block_key, kFltMaxBits, NB);
+    PMA_LAUNCHED(c);
 
     const int S[3] = {X, Y, Z};
     int ctr[3];
@@ -316,21 +392,34 @@ void run_geodesic(pma_ctx* c) {
         const uint32_t b0 = ((uint32_t)(start[2] / kB) * (uint32_t)NBX + (uint32_t)(start[0] / kB)) * (uint32_t)NBY +
                             (uint32_t)(start[1] / kB);
         upload(c, fa, &b0, sizeof b0);
-        uint32_t nf = 1;
+        const uint32_t zero = 0;
+        upload(c, block_key + b0, &zero, sizeof zero);
+        uint32_t nf = 1, kmin = 0;
         uint32_t* cur = fa;
         uint32_t* nxt = fb;
         bool first = true;
         while (nf > 0) {
-            PMA_HIP(hipMemsetAsync(cnt, 0, sizeof(uint32_t), c->stream));
+            const float thr = __builtin_bit_cast(float, kmin) + kDelta;
+            const uint32_t threshold = std::isfinite(thr) ? __builtin_bit_cast(uint32_t, thr) : 0xFFFFFFFFu;
+            round_init<<<1, 1, 0, c->stream>>>(st);
+            PMA_LAUNCHED(c);
+            partition_active<<<grid_for(nf, 256), 256, 0, c->stream>>>(cur, nf, block_key, threshold, fp, nxt,
+                                                                       block_tag, round, st);
+            PMA_LAUNCHED(c);
             {
                 PMA_TIMED(c, "relax_blocks");
-                relax_blocks<<<nf, 256, 0, c->stream>>>(dist, cur, nxt, cnt, block_tag, round++, X, Y, Z, NBX, NBY, NBZ,
-                                                        d3q7, first ? (unsigned long long)si : ~0ull);
+                relax_blocks<<<std::min(nf, kRelaxGrid), 256, 0, c->stream>>>(dist, fp, nxt, block_tag, round, block_key, st, X, Y, Z, NBX,
+                                                        NBY, NBZ, d3q7, first ? (unsigned long long)si : ~0ull);
                 PMA_LAUNCHED(c);
             }
-            relaxations += nf;
+            ++round;
             first = false;
-            nf = read_scalar(c, cnt);
+            uint32_t* h = static_cast<uint32_t*>(c->host_staging(3 * sizeof(uint32_t)));
+            PMA_HIP(hipMemcpyAsync(h, st, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+            PMA_HIP(hipStreamSynchronize(c->stream));
+            nf = h[0];
+            kmin = h[1];
+            relaxations += h[2];
             std::swap(cur, nxt);
         }
         PMA_HIP(hipMemsetAsync(vcount, 0, sizeof(unsigned long long), c->stream));

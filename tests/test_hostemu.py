@@ -81,7 +81,8 @@ def hostemu(tmp_path_factory):
 def hostemu_slabs(tmp_path_factory):
     """Sanitized build whose containing-ball pair list is cut into slabs of at most 4000 pairs and
     whose tiles keep 32-bit winner slots."""
-    return _build(tmp_path_factory, SANITIZED + ["-DPMA_CB_SLAB_PAIRS=4000ull", "-DPMA_CB_WIN16_MAX=0u"], "slabs")
+    return _build(tmp_path_factory, SANITIZED + ["-DPMA_CB_SLAB_PAIRS=4000ull", "-DPMA_CB_WIN16_MAX=0u",
+                                                 "-DPMA_CB_GZ=1", "-DPMA_CB_GX=3"], "slabs")
 
 
 @pytest.fixture(scope="session")
@@ -203,7 +204,8 @@ def test_geodesic(hostemu, tmp_path, name, d3q7):
 
 
 def test_containing_balls_in_slabs(hostemu_slabs, tmp_path):
-    """The same result when the (tile, ball) pairs are processed in many z-slabs."""
+    """The same result when the (cell, ball) pairs are processed in many z-slabs, with 32-bit winner
+    slots and pair cells of 1 x 3 tiles."""
     solid = BALL_CASES["gauss_s4"]
     e, max_e, _ = oracle.edt(solid, threads=4)
     g = emu_balls(hostemu_slabs, tmp_path, e, max_e)
