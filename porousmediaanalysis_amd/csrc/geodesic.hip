@@ -149,13 +149,16 @@ __global__ void __launch_bounds__(256) partition_active(const uint32_t* __restri
 // A workgroup (256 threads, two voxels each) relaxes one block of this round's processing list
 // (length st[2]) at a time, striding over the list by the grid size.  The voxel `src` (or
 // none, ~0) starts at distance 0 and counts as improved, so the blocks around it wake up too.
+// In LDS, solid voxels and voxels outside the image hold +inf, so that every neighbour is one
+// add and one min: inf + w stays inf, and an unreached FLT_MAX + w rounds back to FLT_MAX,
+// which never lowers a void voxel's value.
+template <bool D3Q7>
 __global__ void __launch_bounds__(256) relax_blocks(float* __restrict__ dist, const uint32_t* __restrict__ proc,
                                                     uint32_t* __restrict__ next, uint32_t* __restrict__ block_tag,
                                                     uint32_t tag, uint32_t* __restrict__ block_key,
                                                     uint32_t* __restrict__ st, int X, int Y, int Z, int NBX, int NBY,
-                                                    int NBZ, int d3q7, unsigned long long src) {
+                                                    int NBZ, unsigned long long src) {
     __shared__ float sd[kHalo];
-    __shared__ int s_changed;
     __shared__ uint32_t s_touch;  // bit (dz+1)*9 + (dx+1)*3 + (dy+1): neighbour blocks to activate
     __shared__ uint32_t s_bmin;   // smallest improved border distance (float bits)
     __shared__ uint32_t s_push[27];
@@ -170,7 +173,10 @@ __global__ void __launch_bounds__(256) relax_blocks(float* __restrict__ dist, co
         for (int k = threadIdx.x; k < kHalo; k += blockDim.x) {
             const int lz = k / (kH * kH), lx = (k / kH) % kH, ly = k % kH;
             const int z = z0 + lz, x = x0 + lx, y = y0 + ly;
-            sd[k] = (z >= 0 && z < Z && x >= 0 && x < X && y >= 0 && y < Y) ? dist[((size_t)z * X + x) * Y + y] : -1.0f;
+            const bool in = z >= 0 && z < Z && x >= 0 && x < X && y >= 0 && y < Y;
+            const size_t gi = ((size_t)z * X + x) * Y + y;
+            const float v = in ? dist[gi] : -1.0f;
+            sd[k] = (in && gi == src) ? 0.0f : (v < 0.0f ? INFINITY : v);
         }
         if (threadIdx.x == 0) {
             s_touch = 0;
@@ -178,45 +184,45 @@ __global__ void __launch_bounds__(256) relax_blocks(float* __restrict__ dist, co
         }
         __syncthreads();
 
+        // No store may sit between the barrier above and the sweep loop: the source voxel was
+        // set to 0 while staging, and its original value (unreached) is restored here by a
+        // select.  A conditional store at this point can be sunk by the compiler into a path
+        // that runs the barrier loop twice in one wave.
         int own[2];
         float orig[2];
+        bool open[2];
         for (int o = 0; o < 2; ++o) {
             const int v = threadIdx.x + o * 256;
             own[o] = ((v / 64 + 1) * kH + (v / 8) % 8 + 1) * kH + v % 8 + 1;
-            orig[o] = sd[own[o]];
             const int lz = v / 64, lx = (v / 8) % 8, ly = v % 8;
             const int z = z0 + 1 + lz, x = x0 + 1 + lx, y = y0 + 1 + ly;
-            if (z < Z && x < X && y < Y && ((size_t)z * X + x) * Y + y == src) sd[own[o]] = 0.0f;
+            const bool is_src = z < Z && x < X && y < Y && ((size_t)z * X + x) * Y + y == src;
+            const float cur = sd[own[o]];
+            orig[o] = is_src ? FLT_MAX : cur;
+            open[o] = cur != INFINITY;  // void and inside the image
         }
         const float w1 = 1.0f, w2 = sqrtf(2.0f), w3 = sqrtf(3.0f);
-        while (true) {
-            if (threadIdx.x == 0) s_changed = 0;
-            __syncthreads();
-            int changed = 0;
+        int changed = 1;
+        while (__syncthreads_or(changed)) {
+            changed = 0;
             for (int o = 0; o < 2; ++o) {
+                if (!open[o]) continue;
                 const int c = own[o];
                 const float dv = sd[c];
-                if (dv < 0.0f) continue;  // solid or outside the image
                 float best = dv;
+#pragma unroll
                 for (int k = 0; k < 27; ++k) {
                     const int dz = k / 9 - 1, dx = (k / 3) % 3 - 1, dy = k % 3 - 1;
                     const int m = abs(dz) + abs(dx) + abs(dy);
-                    if (m == 0 || (d3q7 && m > 1)) continue;
-                    const float du = sd[c + (dz * kH + dx) * kH + dy];
-                    if (du < 0.0f || __float_as_uint(du) >= kFltMaxBits) continue;
-                    const float w = d3q7 ? 1.0f : (m == 1 ? w1 : (m == 2 ? w2 : w3));
-                    best = fminf(best, __fadd_rn(du, w));
+                    if (m == 0 || (D3Q7 && m > 1)) continue;
+                    const float w = D3Q7 ? 1.0f : (m == 1 ? w1 : (m == 2 ? w2 : w3));
+                    best = fminf(best, __fadd_rn(sd[c + (dz * kH + dx) * kH + dy], w));
                 }
                 if (best < dv) {
                     sd[c] = best;
                     changed = 1;
                 }
             }
-            if (changed) s_changed = 1;
-            __syncthreads();
-            const int any = s_changed;
-            __syncthreads();
-            if (!any) break;
         }
 
         // write back what improved; border voxels that improved wake the blocks across
@@ -408,8 +414,14 @@ void run_geodesic(pma_ctx* c) {
             PMA_LAUNCHED(c);
             {
                 PMA_TIMED(c, "relax_blocks");
-                relax_blocks<<<std::min(nf, kRelaxGrid), 256, 0, c->stream>>>(dist, fp, nxt, block_tag, round, block_key, st, X, Y, Z, NBX,
-                                                        NBY, NBZ, d3q7, first ? (unsigned long long)si : ~0ull);
+                const unsigned long long src = first ? (unsigned long long)si : ~0ull;
+                const uint32_t grid = std::min(nf, kRelaxGrid);
+                if (d3q7)
+                    relax_blocks<true><<<grid, 256, 0, c->stream>>>(dist, fp, nxt, block_tag, round, block_key, st, X,
+                                                                    Y, Z, NBX, NBY, NBZ, src);
+                else
+                    relax_blocks<false><<<grid, 256, 0, c->stream>>>(dist, fp, nxt, block_tag, round, block_key, st, X,
+                                                                     Y, Z, NBX, NBY, NBZ, src);
                 PMA_LAUNCHED(c);
             }
             ++round;

@@ -135,6 +135,15 @@ inline unsigned long long __ballot(int pred) {
     shim_yield();
     return r;
 }
+inline int __syncthreads_or(int pred) {
+    if (!g_wave) throw std::runtime_error("__syncthreads_or outside a wave launch");
+    g_wave->slot[threadIdx.x] = pred ? 1 : 0;
+    shim_yield();
+    int r = 0;
+    for (uint64_t v : g_wave->slot) r |= v != 0;
+    shim_yield();
+    return r;
+}
 inline unsigned __float_as_uint(float f) { unsigned u; std::memcpy(&u, &f, 4); return u; }
 inline float __uint_as_float(unsigned u) { float f; std::memcpy(&f, &u, 4); return f; }
 inline float __fadd_rn(float a, float b) { volatile float s = a + b; return s; }
