@@ -37,6 +37,9 @@ constexpr int TZ = 8, TX = 8, TY = 64;  // tile: 64 rows (z, x) of 64 voxels alo
 #ifndef PMA_CB_GX
 #define PMA_CB_GX 2  // tiles per pair cell along x
 #endif
+#ifndef PMA_CB_SMALL_R2
+#define PMA_CB_SMALL_R2 (1u << 22)  // r2 below this use isqrt_small (its exactness bound)
+#endif
 #ifndef PMA_CB_WIN16_MAX
 #define PMA_CB_WIN16_MAX 0xFFFFu  // longest tile run whose winners fit 16-bit slots
 #endif
@@ -271,11 +274,14 @@ __global__ void emit_pairs(const Ball* __restrict__ balls, size_t n, const unsig
     ball_tiles(balls[r], X, Y, Z, g, cz0, cz1, cell_base, (uint32_t)r, pk, pv, off[r]);
 }
 
-// [start, end) of every cell's run in the cell-sorted pair list.
-__global__ void tile_ranges(const uint32_t* __restrict__ pk, size_t P, uint32_t* __restrict__ start,
-                            uint32_t* __restrict__ end) {
+// [start, end) of every cell's run in the cell-sorted pair list, and the balls themselves in
+// that order (so that a tile reads its cell's balls contiguously).
+__global__ void tile_ranges(const uint32_t* __restrict__ pk, const uint32_t* __restrict__ rank, size_t P,
+                            const Ball* __restrict__ balls, Ball* __restrict__ cell_balls,
+                            uint32_t* __restrict__ start, uint32_t* __restrict__ end) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P) return;
+    cell_balls[i] = balls[rank[i]];
     const uint32_t t = pk[i];
     if (i == 0 || pk[i - 1] != t) start[t] = (uint32_t)i;
     if (i + 1 == P || pk[i + 1] != t) end[t] = (uint32_t)(i + 1);
@@ -286,6 +292,15 @@ __device__ __forceinline__ uint32_t isqrt_fast(uint32_t v) {
     uint32_t s = (uint32_t)sqrtf((float)v);
     if ((uint64_t)s * s > v) --s;
     else if ((uint64_t)(s + 1) * (s + 1) <= v) ++s;
+    return s;
+}
+
+// floor(sqrt(v)) for 0 <= v < 2^22: the hardware square root is within 1 ulp, so the truncated
+// root is off by at most one, and one correction each way (in 32-bit integers) makes it exact.
+__device__ __forceinline__ int isqrt_small(int v) {
+    int s = (int)__builtin_amdgcn_sqrtf((float)v);
+    s -= s * s > v;
+    s += (s + 1) * (s + 1) <= v;
     return s;
 }
 
@@ -303,8 +318,8 @@ __global__ void __launch_bounds__(256) max_run(const uint32_t* __restrict__ star
 // a voxel is kept in LDS as the position of its ball in that run (16 bits when every run is
 // shorter than 65536).  Workgroups are dispatched round-robin over the 8 XCDs, so the block
 // index is mapped such that the tiles of one cell run on the same XCD and share its L2.
-template <class W>
-__global__ void __launch_bounds__(64) cover_tiles(const Ball* __restrict__ balls, const uint32_t* __restrict__ rank_of,
+template <class W, bool SmallR2>
+__global__ void __launch_bounds__(64) cover_tiles(const Ball* __restrict__ cell_balls,
                                                   const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
                                                   const uint32_t* __restrict__ e, uint32_t* __restrict__ cr,
                                                   uint32_t* __restrict__ cx, uint32_t* __restrict__ cy,
@@ -325,31 +340,46 @@ __global__ void __launch_bounds__(64) cover_tiles(const Ball* __restrict__ balls
 
     // coverage of this lane's row: solid voxels and voxels outside the image count as covered
     uint64_t cov = ~0ull;
-    for (int r = 0; r < 64; ++r) {
-        const int zr = tz * TZ + r / TX, xr = tx * TX + r % TX, yr = y0 + lane;
-        const bool open = zr < Z && xr < X && yr < Y && e[((size_t)zr * X + xr) * Y + yr] != 0;
-        const uint64_t b = __ballot(open);
-        if (lane == r) cov = ~b;
+    if (z < Z && x < X) {
+        const uint32_t* row = e + ((size_t)z * X + x) * Y + y0;
+        if ((Y & 3) == 0 && y0 + TY <= Y) {  // 16-byte aligned full row
+            const uint4* row4 = reinterpret_cast<const uint4*>(row);
+            uint4 v[TY / 4];
+#pragma unroll
+            for (int q = 0; q < TY / 4; ++q) v[q] = row4[q];
+            uint64_t open = 0;
+#pragma unroll
+            for (int q = 0; q < TY / 4; ++q)
+                open |= (uint64_t)((v[q].x != 0) | (v[q].y != 0) << 1 | (v[q].z != 0) << 2 | (v[q].w != 0) << 3)
+                        << (4 * q);
+            cov = ~open;
+        } else {
+            for (int j = 0; j < TY && y0 + j < Y; ++j)
+                if (row[j] != 0) cov &= ~(1ull << j);
+        }
     }
 
     const uint32_t s = start[local], en = end[local];
+    Ball next{0, 0, 0, 0};
+    if (s + lane < en) next = cell_balls[s + lane];
     for (uint32_t base = s; base < en; base += 64) {
         const bool open_row = cov != ~0ull;
         const uint64_t rows = __ballot(open_row);
         if (rows == 0) break;  // every void voxel of the tile has its ball
         // bounding box of the voxels still uncovered: a ball that cannot reach it covers nothing new
-        uint64_t ys = open_row ? ~cov : 0ull;
-        for (int off = 32; off > 0; off >>= 1) ys |= __shfl_xor(ys, off);
+        const uint64_t unc = open_row ? ~cov : 0ull;
+        const uint64_t ys = (uint64_t)__ockl_wfred_or_u32((uint32_t)unc) |
+                            (uint64_t)__ockl_wfred_or_u32((uint32_t)(unc >> 32)) << 32;
         uint32_t xs = 0;
         for (int r = 0; r < TZ; ++r) xs |= (uint32_t)(rows >> (r * TX)) & 0xFFu;
         const int zlo = tz * TZ + __builtin_ctzll(rows) / TX, zhi = tz * TZ + (63 - __builtin_clzll(rows)) / TX;
         const int xlo = tx * TX + __builtin_ctz(xs), xhi = tx * TX + 31 - __builtin_clz(xs);
         const int ylo = y0 + __builtin_ctzll(ys), yhi = y0 + 63 - __builtin_clzll(ys);
 
-        Ball mb{0, 0, 0, 0};
+        const Ball mb = next;  // this batch; the next one is loaded while it is processed
+        if (base + 64 + lane < en) next = cell_balls[base + 64 + lane];
         bool reach = false;
         if (base + lane < en) {
-            mb = balls[rank_of[base + lane]];
             const int ez = max(max(zlo - mb.z, mb.z - zhi), 0), ex = max(max(xlo - mb.x, mb.x - xhi), 0);
             const int ey = max(max(ylo - mb.y, mb.y - yhi), 0);
             reach = (uint32_t)(ez * ez + ex * ex + ey * ey) <= mb.r2;
@@ -364,12 +394,10 @@ __global__ void __launch_bounds__(64) cover_tiles(const Ball* __restrict__ balls
             const int r2 = __builtin_amdgcn_readlane((int)mb.r2, j);
             const int dz = z - bz, dx = x - bx;
             const int rem = r2 - dz * dz - dx * dx;
-            uint64_t nw = 0;
-            if (rem >= 0) {
-                const int h = (int)isqrt_fast((uint32_t)rem);
-                const int lo = max(by - h - y0, 0), hi = min(by + h - y0, 63);
-                if (lo <= hi) nw = (((~0ull) >> (63 - (hi - lo))) << lo) & ~cov;
-            }
+            // chord [by - h, by + h] of the ball on this lane's row; h = -1 when the row misses it
+            const int h = rem < 0 ? -1 : (SmallR2 ? isqrt_small(rem) : (int)isqrt_fast((uint32_t)rem));
+            const int lo = max(by - h - y0, 0), hi = min(by + h - y0, 63);
+            uint64_t nw = lo <= hi ? (((~0ull) >> (63 - (hi - lo))) << lo) & ~cov : 0ull;
             if (__ballot(nw != 0) == 0) continue;
             won |= 1ull << j;
             if (nw) {
@@ -401,7 +429,7 @@ __global__ void __launch_bounds__(64) cover_tiles(const Ball* __restrict__ balls
             atomicOr(error_flag, 1u);
             continue;
         }
-        const Ball b = balls[rank_of[s + (uint32_t)win[r * TY + lane]]];
+        const Ball b = cell_balls[s + (uint32_t)win[r * TY + lane]];
         cr[i] = b.r2;
         cx[i] = (uint32_t)b.x;
         cy[i] = (uint32_t)b.y;
@@ -514,6 +542,7 @@ void run_containing_balls(pma_ctx* c) {
         uint32_t* pv = c->buf<uint32_t>("cb_pair_rank", P1);
         uint32_t* pk2 = c->buf<uint32_t>("cb_pair_tile2", P1);
         uint32_t* pv2 = c->buf<uint32_t>("cb_pair_rank2", P1);
+        Ball* cell_balls = c->buf<Ball>("cb_cell_balls", P1);
         uint32_t* tstart = c->buf<uint32_t>("cb_cell_start", slab_cells);
         uint32_t* tend = c->buf<uint32_t>("cb_cell_end", slab_cells);
         PMA_HIP(hipMemsetAsync(tstart, 0, slab_cells * sizeof(uint32_t), c->stream));
@@ -525,7 +554,7 @@ void run_containing_balls(pma_ctx* c) {
             int cell_bits = 1;
             while (cell_bits < 32 && (slab_cells >> cell_bits) != 0) ++cell_bits;
             radix_sort_pairs_u32(c, pk, pk2, pv, pv2, P, 0, cell_bits);
-            tile_ranges<<<grid_for(P, 256), 256, 0, c->stream>>>(pk2, P, tstart, tend);
+            tile_ranges<<<grid_for(P, 256), 256, 0, c->stream>>>(pk2, pv2, P, balls, cell_balls, tstart, tend);
             PMA_LAUNCHED(c);
         }
         unsigned int longest = 0;
@@ -544,12 +573,20 @@ void run_containing_balls(pma_ctx* c) {
             uint32_t* cz = c->v<uint32_t>(PMA_VOL_CZ);
             // one workgroup per tile, cell-major within each XCD's share (see cover_tiles)
             const uint32_t blocks = (slab_cells + kXcds - 1) / kXcds * kXcds * per_cell;
-            if (longest <= PMA_CB_WIN16_MAX)
-                cover_tiles<uint16_t><<<blocks, 64, 0, c->stream>>>(balls, pv2, tstart, tend, e, cr, cx, cy, cz, mask,
-                                                                   X, Y, Z, g, cell_base, slab_cells, err);
+            const bool small = max_e < PMA_CB_SMALL_R2;  // chord widths from the short integer square root
+            const bool w16 = longest <= PMA_CB_WIN16_MAX;
+            if (w16 && small)
+                cover_tiles<uint16_t, true><<<blocks, 64, 0, c->stream>>>(cell_balls, tstart, tend, e, cr, cx, cy, cz,
+                                                                          mask, X, Y, Z, g, cell_base, slab_cells, err);
+            else if (w16)
+                cover_tiles<uint16_t, false><<<blocks, 64, 0, c->stream>>>(cell_balls, tstart, tend, e, cr, cx, cy, cz,
+                                                                           mask, X, Y, Z, g, cell_base, slab_cells, err);
+            else if (small)
+                cover_tiles<uint32_t, true><<<blocks, 64, 0, c->stream>>>(cell_balls, tstart, tend, e, cr, cx, cy, cz,
+                                                                          mask, X, Y, Z, g, cell_base, slab_cells, err);
             else
-                cover_tiles<uint32_t><<<blocks, 64, 0, c->stream>>>(balls, pv2, tstart, tend, e, cr, cx, cy, cz, mask,
-                                                                   X, Y, Z, g, cell_base, slab_cells, err);
+                cover_tiles<uint32_t, false><<<blocks, 64, 0, c->stream>>>(cell_balls, tstart, tend, e, cr, cx, cy, cz,
+                                                                           mask, X, Y, Z, g, cell_base, slab_cells, err);
             PMA_LAUNCHED(c);
         }
         total_pairs += P;

@@ -38,6 +38,7 @@ typedef Dim3 dim3;
 inline thread_local Dim3 threadIdx, blockIdx;
 inline Dim3 blockDim, gridDim;
 struct uint2 { uint32_t x, y; };
+struct alignas(16) uint4 { uint32_t x, y, z, w; };
 inline uint2 make_uint2(uint32_t a, uint32_t b) { return uint2{a, b}; }
 #define hipSetDevice(d) 0
 
@@ -134,6 +135,26 @@ inline unsigned long long __ballot(int pred) {
         if (g_wave->slot[base + l]) r |= 1ull << l;
     shim_yield();
     return r;
+}
+inline unsigned __ockl_wfred_or_u32(unsigned v) {
+    if (!g_wave) throw std::runtime_error("__ockl_wfred_or_u32 outside a wave launch");
+    g_wave->slot[threadIdx.x] = v;
+    shim_yield();
+    unsigned r = 0;
+    const unsigned base = threadIdx.x & ~63u;
+    for (unsigned l = 0; l < 64 && base + l < g_wave->slot.size(); ++l) r |= (unsigned)g_wave->slot[base + l];
+    shim_yield();
+    return r;
+}
+// The hardware square root is only guaranteed within 1 ulp: return the correctly rounded root
+// moved by -1, 0 or +1 ulp (pseudo-randomly), so that callers' corrections are exercised.
+inline float __builtin_amdgcn_sqrtf(float x) {
+    static thread_local uint32_t state = 12345u;
+    state = state * 1664525u + 1013904223u;
+    const float r = std::sqrt(x);
+    if (r == 0.0f) return r;
+    const int k = (int)(state >> 30) % 3;
+    return k == 0 ? r : (k == 1 ? std::nextafter(r, 0.0f) : std::nextafter(r, INFINITY));
 }
 inline int __syncthreads_or(int pred) {
     if (!g_wave) throw std::runtime_error("__syncthreads_or outside a wave launch");

@@ -41,7 +41,7 @@ def launch(m):
 def translate(text):
     text = text.replace('#include "pma_internal.h"', '#include "shim.h"')
     text = re.sub(r"#include <rocprim/.*?>", "", text)
-    return re.sub(r"([\w<>]+?)<<<(.*?)>>>\((.*?)\);", launch, text, flags=re.S)
+    return re.sub(r"(\w+(?:<[\w, ]*>)?)<<<(.*?)>>>\((.*?)\);", launch, text, flags=re.S)
 
 
 if __name__ == "__main__":

@@ -82,7 +82,7 @@ def hostemu_slabs(tmp_path_factory):
     """Sanitized build whose containing-ball pair list is cut into slabs of at most 4000 pairs and
     whose tiles keep 32-bit winner slots."""
     return _build(tmp_path_factory, SANITIZED + ["-DPMA_CB_SLAB_PAIRS=4000ull", "-DPMA_CB_WIN16_MAX=0u",
-                                                 "-DPMA_CB_GZ=1", "-DPMA_CB_GX=3"], "slabs")
+                                                 "-DPMA_CB_GZ=1", "-DPMA_CB_GX=3", "-DPMA_CB_SMALL_R2=0u"], "slabs")
 
 
 @pytest.fixture(scope="session")
@@ -205,7 +205,7 @@ def test_geodesic(hostemu, tmp_path, name, d3q7):
 
 def test_containing_balls_in_slabs(hostemu_slabs, tmp_path):
     """The same result when the (cell, ball) pairs are processed in many z-slabs, with 32-bit winner
-    slots and pair cells of 1 x 3 tiles."""
+    slots, pair cells of 1 x 3 tiles and the general integer square root."""
     solid = BALL_CASES["gauss_s4"]
     e, max_e, _ = oracle.edt(solid, threads=4)
     g = emu_balls(hostemu_slabs, tmp_path, e, max_e)
