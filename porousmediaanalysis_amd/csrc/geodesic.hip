@@ -93,12 +93,20 @@ __global__ void __launch_bounds__(256) find_start(const uint8_t* __restrict__ so
 
 constexpr int kB = 8;           // block edge
 constexpr float kDelta = 16.0f;  // bucket width of a round (distance units; 4..64 measured within 10%)
-constexpr uint32_t kRelaxGrid = 4096;  // workgroups of a relax launch (16 per CU)
+#ifndef PMA_GEO_RELAX_GRID
+#define PMA_GEO_RELAX_GRID 8192u  // one-wave workgroups of a relax launch (32 per CU)
+#endif
+#ifndef PMA_GEO_PUSH_BUF
+#define PMA_GEO_PUSH_BUF 256u  // per-wave buffer of activated blocks (more than 27)
+#endif
+constexpr uint32_t kRelaxGrid = PMA_GEO_RELAX_GRID;
+constexpr uint32_t kPushBuf = PMA_GEO_PUSH_BUF;
+static_assert(kPushBuf > 27, "a block activates up to 26 neighbours at once");
 constexpr int kH = kB + 2;      // with halo
 constexpr int kHalo = kH * kH * kH;
 
 // Round state (device): [0] next list length, [1] smallest key of the next list (float
-// bits), [2] length of this round's processing list.
+// bits, from next_min_key), [2] length of this round's processing list.
 __global__ void fill_key(uint32_t* __restrict__ p, uint32_t v, size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -129,10 +137,7 @@ __global__ void __launch_bounds__(256) partition_active(const uint32_t* __restri
         later = !now;
     }
     if (now) block_key[b] = kFltMaxBits;
-    if (later) {
-        (void)atomicExch(&block_tag[b], tag);
-        atomicMin(&st[1], key);
-    }
+    if (later) (void)atomicExch(&block_tag[b], tag);
     const unsigned long long bn = __ballot(now), bl = __ballot(later);
     uint32_t base_n = 0, base_l = 0;
     if (lane == 0) {
@@ -146,31 +151,95 @@ __global__ void __launch_bounds__(256) partition_active(const uint32_t* __restri
     if (later) next[base_l + (uint32_t)__popcll(bl & below)] = b;
 }
 
-// A workgroup (256 threads, two voxels each) relaxes one block of this round's processing list
-// (length st[2]) at a time, striding over the list by the grid size.  The voxel `src` (or
-// none, ~0) starts at distance 0 and counts as improved, so the blocks around it wake up too.
-// In LDS, solid voxels and voxels outside the image hold +inf, so that every neighbour is one
-// add and one min: inf + w stays inf, and an unreached FLT_MAX + w rounds back to FLT_MAX,
-// which never lowers a void voxel's value.
+// Smallest key over the next list (its length st[0] is read on the device); one atomic per
+// workgroup.
+__global__ void __launch_bounds__(256) next_min_key(const uint32_t* __restrict__ next,
+                                                    const uint32_t* __restrict__ block_key, uint32_t* __restrict__ st) {
+    __shared__ uint32_t part[4];
+    const uint32_t n = st[0];
+    uint32_t m = 0x7F800000u;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        m = min(m, block_key[next[i]]);
+    m = __ockl_wfred_min_u32(m);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        m = min(min(part[0], part[1]), min(part[2], part[3]));
+        if (m != 0x7F800000u) atomicMin(&st[1], m);
+    }
+}
+
+// One wave relaxes one block of this round's processing list (length st[2]) at a time, striding
+// over the list by the grid size.  Lane l owns the z-column (x, y) = (l / 8, l % 8) of the block
+// and keeps its 8 values in registers; a sweep walks every column along z (up and down in turn),
+// so a value crosses the whole column in one sweep and one column per sweep sideways.  Sweeps
+// repeat until one changes nothing.  The block and a one-voxel halo sit in LDS with +inf for
+// solid voxels and voxels outside the image: inf + w stays inf, and an unreached FLT_MAX + w
+// rounds back to FLT_MAX, which never lowers a void voxel, so every neighbour is one add and one
+// min.  The voxel `src` (or none, ~0) is set to 0 while staging and counts as improved, so the
+// blocks around it wake up too.
+template <bool D3Q7, bool UP>
+__device__ __forceinline__ bool sweep_columns(float* sd, int col, float (&v)[kB], uint32_t open) {
+    const float w1 = 1.0f, w2 = sqrtf(2.0f), w3 = sqrtf(3.0f);
+    bool changed = false;
+#pragma unroll
+    for (int s = 0; s < kB; ++s) {
+        const int z = UP ? s : kB - 1 - s;
+        if (!((open >> z) & 1u)) continue;
+        const int c = (z + 1) * kH * kH + col;
+        float best = v[z];
+        best = fminf(best, __fadd_rn(z > 0 ? v[z > 0 ? z - 1 : 0] : sd[c - kH * kH], w1));
+        best = fminf(best, __fadd_rn(z < kB - 1 ? v[z < kB - 1 ? z + 1 : 0] : sd[c + kH * kH], w1));
+#pragma unroll
+        for (int k = 0; k < 27; ++k) {
+            const int dz = k / 9 - 1, dx = (k / 3) % 3 - 1, dy = k % 3 - 1;
+            if (dx == 0 && dy == 0) continue;  // own column, above
+            const int m = abs(dz) + abs(dx) + abs(dy);
+            if (D3Q7 && m > 1) continue;
+            const float w = D3Q7 ? 1.0f : (m == 1 ? w1 : (m == 2 ? w2 : w3));
+            best = fminf(best, __fadd_rn(sd[c + (dz * kH + dx) * kH + dy], w));
+        }
+        if (best < v[z]) {
+            v[z] = best;
+            sd[c] = best;
+            changed = true;
+        }
+    }
+    return changed;
+}
+
 template <bool D3Q7>
-__global__ void __launch_bounds__(256) relax_blocks(float* __restrict__ dist, const uint32_t* __restrict__ proc,
-                                                    uint32_t* __restrict__ next, uint32_t* __restrict__ block_tag,
-                                                    uint32_t tag, uint32_t* __restrict__ block_key,
-                                                    uint32_t* __restrict__ st, int X, int Y, int Z, int NBX, int NBY,
-                                                    int NBZ, unsigned long long src) {
+__global__ void __launch_bounds__(64) relax_blocks(float* __restrict__ dist, const uint32_t* __restrict__ proc,
+                                                   uint32_t* __restrict__ next, uint32_t* __restrict__ block_tag,
+                                                   uint32_t tag, uint32_t* __restrict__ block_key,
+                                                   uint32_t* __restrict__ st, int X, int Y, int Z, int NBX, int NBY,
+                                                   int NBZ, unsigned long long src) {
     __shared__ float sd[kHalo];
-    __shared__ uint32_t s_touch;  // bit (dz+1)*9 + (dx+1)*3 + (dy+1): neighbour blocks to activate
-    __shared__ uint32_t s_bmin;   // smallest improved border distance (float bits)
-    __shared__ uint32_t s_push[27];
-    __shared__ uint32_t s_npush, s_base;
+    __shared__ uint32_t pushbuf[kPushBuf];  // activated blocks, reserved in `next` a batch at a time
+    const int lane = threadIdx.x;
+    const int cx = lane >> 3, cy = lane & 7;
+    const int col = (cx + 1) * kH + cy + 1;  // the lane's column within a z-layer of sd
     const uint32_t count = st[2];
+    uint32_t npush = 0;                      // entries in pushbuf (uniform)
+    auto flush = [&]() {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(&st[0], npush);
+        base = (uint32_t)__shfl((int)base, 0);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t k = lane; k < npush; k += 64) next[base + k] = pushbuf[k];
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        npush = 0;
+    };
     for (uint32_t item = blockIdx.x; item < count; item += gridDim.x) {
-        __syncthreads();  // the previous item's shared state has been consumed
         const uint32_t b = proc[item];
         const int by = (int)(b % (uint32_t)NBY), bx = (int)((b / (uint32_t)NBY) % (uint32_t)NBX);
         const int bz = (int)(b / ((uint32_t)NBY * (uint32_t)NBX));
         const int z0 = bz * kB - 1, x0 = bx * kB - 1, y0 = by * kB - 1;
-        for (int k = threadIdx.x; k < kHalo; k += blockDim.x) {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // the previous block's LDS use is over
+        __builtin_amdgcn_wave_barrier();
+        for (int k = lane; k < kHalo; k += 64) {
             const int lz = k / (kH * kH), lx = (k / kH) % kH, ly = k % kH;
             const int z = z0 + lz, x = x0 + lx, y = y0 + ly;
             const bool in = z >= 0 && z < Z && x >= 0 && x < X && y >= 0 && y < Y;
@@ -178,94 +247,64 @@ __global__ void __launch_bounds__(256) relax_blocks(float* __restrict__ dist, co
             const float v = in ? dist[gi] : -1.0f;
             sd[k] = (in && gi == src) ? 0.0f : (v < 0.0f ? INFINITY : v);
         }
-        if (threadIdx.x == 0) {
-            s_touch = 0;
-            s_bmin = kFltMaxBits;
-        }
-        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
 
-        // No store may sit between the barrier above and the sweep loop: the source voxel was
-        // set to 0 while staging, and its original value (unreached) is restored here by a
-        // select.  A conditional store at this point can be sunk by the compiler into a path
-        // that runs the barrier loop twice in one wave.
-        int own[2];
-        float orig[2];
-        bool open[2];
-        for (int o = 0; o < 2; ++o) {
-            const int v = threadIdx.x + o * 256;
-            own[o] = ((v / 64 + 1) * kH + (v / 8) % 8 + 1) * kH + v % 8 + 1;
-            const int lz = v / 64, lx = (v / 8) % 8, ly = v % 8;
-            const int z = z0 + 1 + lz, x = x0 + 1 + lx, y = y0 + 1 + ly;
-            const bool is_src = z < Z && x < X && y < Y && ((size_t)z * X + x) * Y + y == src;
-            const float cur = sd[own[o]];
-            orig[o] = is_src ? FLT_MAX : cur;
-            open[o] = cur != INFINITY;  // void and inside the image
-        }
-        const float w1 = 1.0f, w2 = sqrtf(2.0f), w3 = sqrtf(3.0f);
-        int changed = 1;
-        while (__syncthreads_or(changed)) {
-            changed = 0;
-            for (int o = 0; o < 2; ++o) {
-                if (!open[o]) continue;
-                const int c = own[o];
-                const float dv = sd[c];
-                float best = dv;
+        const int gx = x0 + 1 + cx, gy = y0 + 1 + cy;
+        float v[kB], orig[kB];
+        uint32_t open = 0;  // bit z: the voxel is void and inside the image
 #pragma unroll
-                for (int k = 0; k < 27; ++k) {
-                    const int dz = k / 9 - 1, dx = (k / 3) % 3 - 1, dy = k % 3 - 1;
-                    const int m = abs(dz) + abs(dx) + abs(dy);
-                    if (m == 0 || (D3Q7 && m > 1)) continue;
-                    const float w = D3Q7 ? 1.0f : (m == 1 ? w1 : (m == 2 ? w2 : w3));
-                    best = fminf(best, __fadd_rn(sd[c + (dz * kH + dx) * kH + dy], w));
-                }
-                if (best < dv) {
-                    sd[c] = best;
-                    changed = 1;
-                }
-            }
+        for (int z = 0; z < kB; ++z) {
+            const float cur = sd[(z + 1) * kH * kH + col];
+            const int gz = z0 + 1 + z;
+            const bool is_src = gz < Z && gx < X && gy < Y && ((size_t)gz * X + gx) * Y + gy == src;
+            v[z] = cur;
+            orig[z] = is_src ? FLT_MAX : cur;
+            open |= (cur != INFINITY ? 1u : 0u) << z;
+        }
+        for (bool up = true;; up = !up) {
+            const bool changed = up ? sweep_columns<D3Q7, true>(sd, col, v, open)
+                                    : sweep_columns<D3Q7, false>(sd, col, v, open);
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // this sweep's writes precede the next reads
+            __builtin_amdgcn_wave_barrier();
+            if (__ballot(changed) == 0) break;
         }
 
         // write back what improved; border voxels that improved wake the blocks across
         uint32_t touch = 0, bmin = kFltMaxBits;
-        for (int o = 0; o < 2; ++o) {
-            const int v = threadIdx.x + o * 256;
-            const float dv = sd[own[o]];
-            if (!(dv < orig[o])) continue;
-            const int lz = v / 64, lx = (v / 8) % 8, ly = v % 8;
-            const int z = z0 + 1 + lz, x = x0 + 1 + lx, y = y0 + 1 + ly;
-            dist[((size_t)z * X + x) * Y + y] = dv;
-            const int zlo = lz == 0 ? -1 : 0, zhi = lz == kB - 1 ? 1 : 0;
-            const int xlo = lx == 0 ? -1 : 0, xhi = lx == kB - 1 ? 1 : 0;
-            const int ylo = ly == 0 ? -1 : 0, yhi = ly == kB - 1 ? 1 : 0;
+        const int xlo = cx == 0 ? -1 : 0, xhi = cx == kB - 1 ? 1 : 0;
+        const int ylo = cy == 0 ? -1 : 0, yhi = cy == kB - 1 ? 1 : 0;
+#pragma unroll
+        for (int z = 0; z < kB; ++z) {
+            if (!(v[z] < orig[z])) continue;
+            dist[((size_t)(z0 + 1 + z) * X + gx) * Y + gy] = v[z];
+            const int zlo = z == 0 ? -1 : 0, zhi = z == kB - 1 ? 1 : 0;
+            if (!(zlo | zhi | xlo | xhi | ylo | yhi)) continue;
             for (int dz = zlo; dz <= zhi; ++dz)
                 for (int dx = xlo; dx <= xhi; ++dx)
                     for (int dy = ylo; dy <= yhi; ++dy)
                         if (dz | dx | dy) touch |= 1u << ((dz + 1) * 9 + (dx + 1) * 3 + (dy + 1));
-            if (zlo | zhi | xlo | xhi | ylo | yhi) bmin = min(bmin, __float_as_uint(dv));
+            bmin = min(bmin, __float_as_uint(v[z]));
         }
-        if (touch) {
-            atomicOr(&s_touch, touch);
-            atomicMin(&s_bmin, bmin);
-        }
-        if (threadIdx.x == 0) s_npush = 0;
-        __syncthreads();
-        if (threadIdx.x < 27 && ((s_touch >> threadIdx.x) & 1u)) {
-            const int dz = (int)threadIdx.x / 9 - 1, dx = ((int)threadIdx.x / 3) % 3 - 1, dy = (int)threadIdx.x % 3 - 1;
+        const uint32_t wtouch = __ockl_wfred_or_u32(touch);
+        const uint32_t wbmin = __ockl_wfred_min_u32(bmin);
+        bool pushed = false;
+        uint32_t nb = 0;
+        if (lane < 27 && ((wtouch >> lane) & 1u)) {
+            const int dz = lane / 9 - 1, dx = (lane / 3) % 3 - 1, dy = lane % 3 - 1;
             const int nz = bz + dz, nx = bx + dx, ny = by + dy;
             if (nz >= 0 && nz < NBZ && nx >= 0 && nx < NBX && ny >= 0 && ny < NBY) {
-                const uint32_t nb = ((uint32_t)nz * (uint32_t)NBX + (uint32_t)nx) * (uint32_t)NBY + (uint32_t)ny;
-                atomicMin(&block_key[nb], s_bmin);
-                if (atomicExch(&block_tag[nb], tag) != tag) s_push[atomicAdd(&s_npush, 1u)] = nb;
+                nb = ((uint32_t)nz * (uint32_t)NBX + (uint32_t)nx) * (uint32_t)NBY + (uint32_t)ny;
+                atomicMin(&block_key[nb], wbmin);
+                pushed = atomicExch(&block_tag[nb], tag) != tag;
             }
         }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            if (s_npush) s_base = atomicAdd(&st[0], s_npush);
-            if (s_touch) atomicMin(&st[1], s_bmin);
-        }
-        __syncthreads();
-        if (threadIdx.x < s_npush) next[s_base + threadIdx.x] = s_push[threadIdx.x];
+        const uint64_t pm = __ballot(pushed);
+        if (pushed) pushbuf[npush + (uint32_t)__popcll(pm & ((1ull << lane) - 1))] = nb;
+        npush += (uint32_t)__popcll(pm);
+        if (npush > kPushBuf - 27) flush();
     }
+    if (npush) flush();
 }
 
 // Voxels reached in this cavity (void, cavity id unset, distance set) take the id; walls
@@ -417,13 +456,15 @@ void run_geodesic(pma_ctx* c) {
                 const unsigned long long src = first ? (unsigned long long)si : ~0ull;
                 const uint32_t grid = std::min(nf, kRelaxGrid);
                 if (d3q7)
-                    relax_blocks<true><<<grid, 256, 0, c->stream>>>(dist, fp, nxt, block_tag, round, block_key, st, X,
+                    relax_blocks<true><<<grid, 64, 0, c->stream>>>(dist, fp, nxt, block_tag, round, block_key, st, X,
                                                                     Y, Z, NBX, NBY, NBZ, src);
                 else
-                    relax_blocks<false><<<grid, 256, 0, c->stream>>>(dist, fp, nxt, block_tag, round, block_key, st, X,
+                    relax_blocks<false><<<grid, 64, 0, c->stream>>>(dist, fp, nxt, block_tag, round, block_key, st, X,
                                                                      Y, Z, NBX, NBY, NBZ, src);
                 PMA_LAUNCHED(c);
             }
+            next_min_key<<<256, 256, 0, c->stream>>>(nxt, block_key, st);
+            PMA_LAUNCHED(c);
             ++round;
             first = false;
             uint32_t* h = static_cast<uint32_t*>(c->host_staging(3 * sizeof(uint32_t)));

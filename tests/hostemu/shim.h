@@ -146,6 +146,21 @@ inline unsigned __ockl_wfred_or_u32(unsigned v) {
     shim_yield();
     return r;
 }
+inline unsigned __ockl_wfred_min_u32(unsigned v) {
+    if (!g_wave) throw std::runtime_error("__ockl_wfred_min_u32 outside a wave launch");
+    g_wave->slot[threadIdx.x] = v;
+    shim_yield();
+    unsigned r = ~0u;
+    const unsigned base = threadIdx.x & ~63u;
+    for (unsigned l = 0; l < 64 && base + l < g_wave->slot.size(); ++l) r = std::min(r, (unsigned)g_wave->slot[base + l]);
+    shim_yield();
+    return r;
+}
+// Lanes of a wave interleave at a wave barrier (on the GPU they run in lockstep).
+inline void __builtin_amdgcn_wave_barrier() {
+    if (g_wave) shim_yield();
+}
+#define __builtin_amdgcn_fence(order, scope) ((void)0)
 // The hardware square root is only guaranteed within 1 ulp: return the correctly rounded root
 // moved by -1, 0 or +1 ulp (pseudo-randomly), so that callers' corrections are exercised.
 inline float __builtin_amdgcn_sqrtf(float x) {

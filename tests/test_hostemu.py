@@ -79,10 +79,11 @@ def hostemu(tmp_path_factory):
 
 @pytest.fixture(scope="session")
 def hostemu_slabs(tmp_path_factory):
-    """Sanitized build whose containing-ball pair list is cut into slabs of at most 4000 pairs and
-    whose tiles keep 32-bit winner slots."""
+    """Sanitized build with small work limits: containing-ball pair lists cut into slabs of at most
+    4000 pairs, 32-bit winner slots, and geodesic relaxation on 2 waves with 32-entry push buffers."""
     return _build(tmp_path_factory, SANITIZED + ["-DPMA_CB_SLAB_PAIRS=4000ull", "-DPMA_CB_WIN16_MAX=0u",
-                                                 "-DPMA_CB_GZ=1", "-DPMA_CB_GX=3", "-DPMA_CB_SMALL_R2=0u"], "slabs")
+                                                 "-DPMA_CB_GZ=1", "-DPMA_CB_GX=3", "-DPMA_CB_SMALL_R2=0u",
+                                                 "-DPMA_GEO_RELAX_GRID=2u", "-DPMA_GEO_PUSH_BUF=32u"], "slabs")
 
 
 @pytest.fixture(scope="session")
@@ -185,14 +186,11 @@ GEO_CASES["gauss"] = volumes.gaussian((33, 47, 52), 2.0, 0.55, 6)
 GEO_CASES["noise"] = volumes.random_noise((20, 21, 22), 0.62, 7)
 
 
-@pytest.mark.parametrize("name", sorted(GEO_CASES))
-@pytest.mark.parametrize("d3q7", [False, True])
-def test_geodesic(hostemu, tmp_path, name, d3q7):
-    solid = GEO_CASES[name]
+def check_geodesic(binary, tmp_path, solid, d3q7):
     Z, X, Y = solid.shape
     inp, vol, txt = tmp_path / "in.bin", tmp_path / "out.bin", tmp_path / "out.txt"
     inp.write_bytes(np.array([X, Y, Z, int(d3q7)], np.int32).tobytes() + solid.astype(np.uint8).tobytes())
-    r = subprocess.run([hostemu, "geodesic", str(inp), str(vol), str(txt)], capture_output=True, text=True, env=ENV,
+    r = subprocess.run([binary, "geodesic", str(inp), str(vol), str(txt)], capture_output=True, text=True, env=ENV,
                        timeout=600)
     assert r.returncode == 0, f"geodesic: rc {r.returncode}\n{r.stdout[-2000:]}\n{r.stderr[-6000:]}"
     raw = np.fromfile(vol, np.uint32)
@@ -201,6 +199,19 @@ def test_geodesic(hostemu, tmp_path, name, d3q7):
     assert np.array_equal(dist, od.view(np.uint32)), "distances differ"
     assert np.array_equal(cav, oc), "cavity ids differ"
     assert txt.read_text() == ot.replace("\r", ""), "cavityDescriptions.txt differs"
+
+
+@pytest.mark.parametrize("name", sorted(GEO_CASES))
+@pytest.mark.parametrize("d3q7", [False, True])
+def test_geodesic(hostemu, tmp_path, name, d3q7):
+    check_geodesic(hostemu, tmp_path, GEO_CASES[name], d3q7)
+
+
+@pytest.mark.parametrize("name", ["gauss", "spheres_40"])
+def test_geodesic_small_grid(hostemu_slabs, tmp_path, name):
+    """The same distances when each relaxing wave takes many blocks in turn and flushes its
+    buffer of activated blocks many times (2 waves per launch, 32-entry buffer)."""
+    check_geodesic(hostemu_slabs, tmp_path, GEO_CASES[name], False)
 
 
 def test_containing_balls_in_slabs(hostemu_slabs, tmp_path):
