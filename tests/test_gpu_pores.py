@@ -18,7 +18,7 @@ pytestmark = [
     pytest.mark.gpu,
     pytest.mark.skip(reason="pore stage switched off on the GPU: test_pores_match_oracle[False-gauss_s3_low] "
                             "made an illegal memory access on MI355X (cause not found on the CPU; the same kernels "
-                            "are parity-checked in tests/test_hostemu_pores.py)"),
+                            "are parity-checked in tests/test_hostemu.py)"),
 ]
 
 
