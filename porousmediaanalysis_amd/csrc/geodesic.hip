@@ -178,27 +178,35 @@ __global__ void __launch_bounds__(256) next_min_key(const uint32_t* __restrict__
 // rounds back to FLT_MAX, which never lowers a void voxel, so every neighbour is one add and one
 // min.  The voxel `src` (or none, ~0) is set to 0 while staging and counts as improved, so the
 // blocks around it wake up too.
+// Rounding is monotone, so fl(min(a, b) + w) = min(fl(a + w), fl(b + w)): the neighbours of one
+// weight are reduced first and the weight is added once per class (faces 1, edges sqrt 2,
+// corners sqrt 3), which gives the same value as adding it to each neighbour.
 template <bool D3Q7, bool UP>
 __device__ __forceinline__ bool sweep_columns(float* sd, int col, float (&v)[kB], uint32_t open) {
-    const float w1 = 1.0f, w2 = sqrtf(2.0f), w3 = sqrtf(3.0f);
+    const float w2 = sqrtf(2.0f), w3 = sqrtf(3.0f);
     bool changed = false;
 #pragma unroll
     for (int s = 0; s < kB; ++s) {
         const int z = UP ? s : kB - 1 - s;
         if (!((open >> z) & 1u)) continue;
         const int c = (z + 1) * kH * kH + col;
-        float best = v[z];
-        best = fminf(best, __fadd_rn(z > 0 ? v[z > 0 ? z - 1 : 0] : sd[c - kH * kH], w1));
-        best = fminf(best, __fadd_rn(z < kB - 1 ? v[z < kB - 1 ? z + 1 : 0] : sd[c + kH * kH], w1));
+        // own column: registers inside the block, LDS at its ends
+        float face = fminf(z > 0 ? v[z > 0 ? z - 1 : 0] : sd[c - kH * kH],
+                           z < kB - 1 ? v[z < kB - 1 ? z + 1 : 0] : sd[c + kH * kH]);
+        float edge = INFINITY, corner = INFINITY;
 #pragma unroll
         for (int k = 0; k < 27; ++k) {
             const int dz = k / 9 - 1, dx = (k / 3) % 3 - 1, dy = k % 3 - 1;
             if (dx == 0 && dy == 0) continue;  // own column, above
             const int m = abs(dz) + abs(dx) + abs(dy);
             if (D3Q7 && m > 1) continue;
-            const float w = D3Q7 ? 1.0f : (m == 1 ? w1 : (m == 2 ? w2 : w3));
-            best = fminf(best, __fadd_rn(sd[c + (dz * kH + dx) * kH + dy], w));
+            const float u = sd[c + (dz * kH + dx) * kH + dy];
+            if (m == 1) face = fminf(face, u);
+            else if (m == 2) edge = fminf(edge, u);
+            else corner = fminf(corner, u);
         }
+        float best = fminf(v[z], __fadd_rn(face, 1.0f));
+        if (!D3Q7) best = fminf(best, fminf(__fadd_rn(edge, w2), __fadd_rn(corner, w3)));
         if (best < v[z]) {
             v[z] = best;
             sd[c] = best;
