@@ -299,10 +299,18 @@ __device__ __forceinline__ uint32_t isqrt_fast(uint32_t v) {
 // root is off by at most one, and one correction each way (in 32-bit integers) makes it exact.
 __device__ __forceinline__ int isqrt_small(int v) {
     int s = (int)__builtin_amdgcn_sqrtf((float)v);
-    s -= s * s > v;
-    s += (s + 1) * (s + 1) <= v;
+    s -= __mul24(s, s) > v;
+    s += __mul24(s + 1, s + 1) <= v;
     return s;
 }
+
+// How cover_tiles takes the chord half-width floor(sqrt(rem)).
+enum ChordRoot : int {
+    kRootGeneral = 0,  // isqrt_fast (any r2)
+    kRootSmall = 1,    // isqrt_small (r2 < 2^22)
+    kRootTable = 2,    // a table of floor(sqrt(v)) in LDS (r2 < kRootTableSize)
+};
+constexpr int kRootTableSize = 2048;
 
 // Largest pair run of any cell of the slab (decides the width of the per-voxel winner slot).
 __global__ void __launch_bounds__(256) max_run(const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
@@ -318,7 +326,7 @@ __global__ void __launch_bounds__(256) max_run(const uint32_t* __restrict__ star
 // a voxel is kept in LDS as the position of its ball in that run (16 bits when every run is
 // shorter than 65536).  Workgroups are dispatched round-robin over the 8 XCDs, so the block
 // index is mapped such that the tiles of one cell run on the same XCD and share its L2.
-template <class W, bool SmallR2>
+template <class W, int Root>
 __global__ void __launch_bounds__(64) cover_tiles(const Ball* __restrict__ cell_balls,
                                                   const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
                                                   const uint32_t* __restrict__ e, uint32_t* __restrict__ cr,
@@ -327,6 +335,7 @@ __global__ void __launch_bounds__(64) cover_tiles(const Ball* __restrict__ cell_
                                                   int Z, Cells g, uint32_t cell_base, uint32_t slab_cells,
                                                   uint32_t* __restrict__ error_flag) {
     __shared__ W win[TZ * TX * TY];
+    __shared__ uint8_t root_tab[Root == kRootTable ? kRootTableSize : 1];
     const int lane = threadIdx.x;
     const uint32_t per_cell = (uint32_t)(g.gz * g.gx);
     const uint32_t k = blockIdx.x / kXcds, sub = k % per_cell;
@@ -337,6 +346,8 @@ __global__ void __launch_bounds__(64) cover_tiles(const Ball* __restrict__ cell_
     const int ccz = (int)(cell / ((uint32_t)g.NCY * (uint32_t)g.NCX));
     const int tz = ccz * g.gz + (int)(sub / (uint32_t)g.gx), tx = ccx * g.gx + (int)(sub % (uint32_t)g.gx);
     const int z = tz * TZ + lane / TX, x = tx * TX + lane % TX, y0 = ty * TY;
+    if (Root == kRootTable)
+        for (int v = lane; v < kRootTableSize; v += 64) root_tab[v] = (uint8_t)isqrt_small(v);
 
     // coverage of this lane's row: solid voxels and voxels outside the image count as covered
     uint64_t cov = ~0ull;
@@ -393,9 +404,15 @@ __global__ void __launch_bounds__(64) cover_tiles(const Ball* __restrict__ cell_
             const int bz = __builtin_amdgcn_readlane(mb.z, j);
             const int r2 = __builtin_amdgcn_readlane((int)mb.r2, j);
             const int dz = z - bz, dx = x - bx;
-            const int rem = r2 - dz * dz - dx * dx;
+            const int rem = r2 - (__mul24(dz, dz) + __mul24(dx, dx));  // |dz|, |dx| < 2^16
             // chord [by - h, by + h] of the ball on this lane's row; h = -1 when the row misses it
-            const int h = rem < 0 ? -1 : (SmallR2 ? isqrt_small(rem) : (int)isqrt_fast((uint32_t)rem));
+            int h;
+            if (Root == kRootTable) {
+                const int t = root_tab[max(rem, 0)];  // rem <= r2 < kRootTableSize
+                h = rem < 0 ? -1 : t;
+            } else {
+                h = rem < 0 ? -1 : (Root == kRootSmall ? isqrt_small(rem) : (int)isqrt_fast((uint32_t)rem));
+            }
             const int lo = max(by - h - y0, 0), hi = min(by + h - y0, 63);
             uint64_t nw = lo <= hi ? (((~0ull) >> (63 - (hi - lo))) << lo) & ~cov : 0ull;
             if (__ballot(nw != 0) == 0) continue;
@@ -573,20 +590,30 @@ void run_containing_balls(pma_ctx* c) {
             uint32_t* cz = c->v<uint32_t>(PMA_VOL_CZ);
             // one workgroup per tile, cell-major within each XCD's share (see cover_tiles)
             const uint32_t blocks = (slab_cells + kXcds - 1) / kXcds * kXcds * per_cell;
-            const bool small = max_e < PMA_CB_SMALL_R2;  // chord widths from the short integer square root
+            // chord widths: a table below kRootTableSize, the short integer root below its
+            // exactness bound, else the general one
+            const int root = max_e < (uint32_t)std::min<uint64_t>(kRootTableSize, PMA_CB_SMALL_R2) ? kRootTable
+                             : max_e < PMA_CB_SMALL_R2                                            ? kRootSmall
+                                                                                                   : kRootGeneral;
             const bool w16 = longest <= PMA_CB_WIN16_MAX;
-            if (w16 && small)
-                cover_tiles<uint16_t, true><<<blocks, 64, 0, c->stream>>>(cell_balls, tstart, tend, e, cr, cx, cy, cz,
-                                                                          mask, X, Y, Z, g, cell_base, slab_cells, err);
+            if (w16 && root == kRootTable)
+                cover_tiles<uint16_t, kRootTable><<<blocks, 64, 0, c->stream>>>(
+                    cell_balls, tstart, tend, e, cr, cx, cy, cz, mask, X, Y, Z, g, cell_base, slab_cells, err);
+            else if (w16 && root == kRootSmall)
+                cover_tiles<uint16_t, kRootSmall><<<blocks, 64, 0, c->stream>>>(
+                    cell_balls, tstart, tend, e, cr, cx, cy, cz, mask, X, Y, Z, g, cell_base, slab_cells, err);
             else if (w16)
-                cover_tiles<uint16_t, false><<<blocks, 64, 0, c->stream>>>(cell_balls, tstart, tend, e, cr, cx, cy, cz,
-                                                                           mask, X, Y, Z, g, cell_base, slab_cells, err);
-            else if (small)
-                cover_tiles<uint32_t, true><<<blocks, 64, 0, c->stream>>>(cell_balls, tstart, tend, e, cr, cx, cy, cz,
-                                                                          mask, X, Y, Z, g, cell_base, slab_cells, err);
+                cover_tiles<uint16_t, kRootGeneral><<<blocks, 64, 0, c->stream>>>(
+                    cell_balls, tstart, tend, e, cr, cx, cy, cz, mask, X, Y, Z, g, cell_base, slab_cells, err);
+            else if (root == kRootTable)
+                cover_tiles<uint32_t, kRootTable><<<blocks, 64, 0, c->stream>>>(
+                    cell_balls, tstart, tend, e, cr, cx, cy, cz, mask, X, Y, Z, g, cell_base, slab_cells, err);
+            else if (root == kRootSmall)
+                cover_tiles<uint32_t, kRootSmall><<<blocks, 64, 0, c->stream>>>(
+                    cell_balls, tstart, tend, e, cr, cx, cy, cz, mask, X, Y, Z, g, cell_base, slab_cells, err);
             else
-                cover_tiles<uint32_t, false><<<blocks, 64, 0, c->stream>>>(cell_balls, tstart, tend, e, cr, cx, cy, cz,
-                                                                           mask, X, Y, Z, g, cell_base, slab_cells, err);
+                cover_tiles<uint32_t, kRootGeneral><<<blocks, 64, 0, c->stream>>>(
+                    cell_balls, tstart, tend, e, cr, cx, cy, cz, mask, X, Y, Z, g, cell_base, slab_cells, err);
             PMA_LAUNCHED(c);
         }
         total_pairs += P;

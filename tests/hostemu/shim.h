@@ -180,6 +180,11 @@ inline int __syncthreads_or(int pred) {
     shim_yield();
     return r;
 }
+// Signed 24-bit multiply: operands are the sign-extended low 24 bits, the result the low 32 bits.
+inline int __mul24(int a, int b) {
+    const int64_t x = (int32_t)((uint32_t)a << 8) >> 8, y = (int32_t)((uint32_t)b << 8) >> 8;
+    return (int)(uint32_t)(uint64_t)(x * y);
+}
 inline unsigned __float_as_uint(float f) { unsigned u; std::memcpy(&u, &f, 4); return u; }
 inline float __uint_as_float(unsigned u) { float f; std::memcpy(&f, &u, 4); return f; }
 inline float __fadd_rn(float a, float b) { volatile float s = a + b; return s; }
