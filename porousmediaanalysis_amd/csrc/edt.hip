@@ -5,7 +5,8 @@
 // X-1 implies (:164, :180).  The same value is produced here by three separable passes:
 //   pass X: one lane per (y, z) column, forward + backward scan of the 1-D distance,
 //   pass Y: one workgroup per (x, z) row staged in LDS, exact windowed minimum of
-//           g(y') + (y-y')^2 over |y-y'| <= floor(sqrt(g(y))) (the reference's window),
+//           g(y') + (y-y')^2 over |y-y'| <= floor(sqrt(g(y))) (the reference's window), found
+//           by scanning outward from y until dy^2 >= the best value so far,
 //   pass Z: 64 (x, y) columns per workgroup, lanes along y so every load is coalesced, staged
 //           in LDS per chunk of z with a halo; the same windowed minimum along z, fused with the
 //           maximum and the histogram of squares whose ball does not cross the image boundary
@@ -51,11 +52,18 @@ __global__ void __launch_bounds__(256) edt_pass_y(const uint32_t* __restrict__ g
     for (int y = threadIdx.x; y < Y; y += blockDim.x) line[y] = g[base + y];
     __syncthreads();
     for (int y = threadIdx.x; y < Y; y += blockDim.x) {
+        // outward from y: once dy^2 >= best no farther y' can do better, so the minimum over the
+        // reference's window (|dy| <= floor(sqrt(g(y)))) is reached without walking all of it.
+        // The test runs every 4 steps (steps past it add dy^2 >= best and change nothing).
         uint32_t best = line[y];
-        if (best != 0) {
-            int w = (int)isqrt_u32(best) + 1;
-            int lo = min(w, y), hi = min(w, Y - y);
-            for (int dy = -lo; dy < hi; ++dy) best = min(best, line[y + dy] + (uint32_t)(dy * dy));
+        for (uint32_t dy = 1; dy * dy < best; dy += 4) {  // sums fit in 32 bits (X <= 16383)
+#pragma unroll
+            for (uint32_t u = 0; u < 4; ++u) {
+                const uint32_t d = dy + u, dd = d * d;
+                const uint32_t a = (int)d <= y ? line[y - (int)d] + dd : best;
+                const uint32_t b = y + (int)d < Y ? line[y + (int)d] + dd : best;
+                best = min(best, min(a, b));
+            }
         }
         h[base + y] = best;
     }
@@ -63,7 +71,7 @@ __global__ void __launch_bounds__(256) edt_pass_y(const uint32_t* __restrict__ g
 
 // Pass Z on chunks of kChunkZ planes: a block stages 64 columns (lanes along y) of its chunk and
 // kHaloZ planes either side in LDS, and each of its 4 waves takes every 4th plane.  A voxel
-// whose window reaches beyond the staged planes reads the rest from global memory.
+// whose search reaches beyond the staged planes reads the rest from global memory.
 constexpr int kChunkZ = 128, kHaloZ = 32, kStageZ = kChunkZ + 2 * kHaloZ;
 
 __global__ void __launch_bounds__(256) edt_pass_z(const uint32_t* __restrict__ h,
@@ -89,6 +97,8 @@ __global__ void __launch_bounds__(256) edt_pass_z(const uint32_t* __restrict__ h
         for (int z = z0 + sub; z < z1; z += 4) {
             uint32_t best = col[z - s0][lane];
             if (best != 0) {
+                // the reference's window |dz| <= floor(sqrt(h(z))); after passes X and Y the values are
+                // short 2-D distances, so the fixed window is short and its loads pipeline
                 const int w = (int)isqrt_u32(best) + 1;
                 const int lo = min(w, z), hi = min(w, Z - z);
                 if (z - lo >= s0 && z + hi <= s1) {
@@ -122,7 +132,8 @@ __global__ void __launch_bounds__(256) edt_pass_z(const uint32_t* __restrict__ h
 void run_edt(pma_ctx* c) {
     PMA_REQUIRE(c->valid[PMA_VOL_SOLID], PMA_ERR_STATE, "pma_edt: the solid volume has not been loaded");
     const int X = c->dims.x, Y = c->dims.y, Z = c->dims.z;
-    PMA_REQUIRE(Z <= 65535 && X <= 65535, PMA_ERR_INVALID_ARGUMENT, "pma_edt: X and Z must be <= 65535");
+    PMA_REQUIRE(Z <= 65535 && X <= 16383, PMA_ERR_INVALID_ARGUMENT,
+                "pma_edt: X must be <= 16383 (squared distances and their sums in 32 bits) and Z <= 65535");
     PMA_REQUIRE((size_t)Y * 4 <= 64 * 1024, PMA_ERR_INVALID_ARGUMENT, "pma_edt: Y must be <= 16384");
     c->ensure_volume(PMA_VOL_EDT2);
     uint32_t* e = c->v<uint32_t>(PMA_VOL_EDT2);
