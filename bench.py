@@ -45,7 +45,6 @@ KERNEL_BYTES_PER_VOXEL = {
     "cc_not_max": 4 + 4,
     "plateau_edges": 4 + 4,
     "write_labels": 4 + 4,      # u32 plateau index -> u32 PoreThroatIds
-    "finish_cavity": 1 + 4 + 4,  # u8 solid, f32 distance -> u32 cavity id
     "geo_init": 1 + 4 + 4,       # u8 solid -> f32 distance, u32 cavity id
 }
 # Per block visit of one relax_blocks launch: the 8^3 distances read and written once (the

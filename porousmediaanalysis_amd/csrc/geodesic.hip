@@ -216,12 +216,17 @@ __device__ __forceinline__ bool sweep_columns(float* sd, int col, float (&v)[kB]
     return changed;
 }
 
+// A voxel first lowered from FLT_MAX (unreached) belongs to the current cavity: it takes the
+// cavity id, and the workgroup adds it and the boundary planes it lies on (bits axis * 2 +
+// side) to its own slot of part_count / part_walls, which the host clears per cavity.
 template <bool D3Q7>
-__global__ void __launch_bounds__(64) relax_blocks(float* __restrict__ dist, const uint32_t* __restrict__ proc,
-                                                   uint32_t* __restrict__ next, uint32_t* __restrict__ block_tag,
-                                                   uint32_t tag, uint32_t* __restrict__ block_key,
-                                                   uint32_t* __restrict__ st, int X, int Y, int Z, int NBX, int NBY,
-                                                   int NBZ, unsigned long long src) {
+__global__ void __launch_bounds__(64) relax_blocks(float* __restrict__ dist, uint32_t* __restrict__ cav, uint32_t id,
+                                                   const uint32_t* __restrict__ proc, uint32_t* __restrict__ next,
+                                                   uint32_t* __restrict__ block_tag, uint32_t tag,
+                                                   uint32_t* __restrict__ block_key, uint32_t* __restrict__ st,
+                                                   unsigned long long* __restrict__ part_count,
+                                                   uint32_t* __restrict__ part_walls, int X, int Y, int Z, int NBX,
+                                                   int NBY, int NBZ, unsigned long long src) {
     __shared__ float sd[kHalo];
     __shared__ uint32_t pushbuf[kPushBuf];  // activated blocks, reserved in `next` a batch at a time
     const int lane = threadIdx.x;
@@ -229,6 +234,7 @@ __global__ void __launch_bounds__(64) relax_blocks(float* __restrict__ dist, con
     const int col = (cx + 1) * kH + cy + 1;  // the lane's column within a z-layer of sd
     const uint32_t count = st[2];
     uint32_t npush = 0;                      // entries in pushbuf (uniform)
+    uint32_t reached = 0, walls = 0;         // this lane's voxels new to the cavity, and their planes
     auto flush = [&]() {
         uint32_t base = 0;
         if (lane == 0) base = atomicAdd(&st[0], npush);
@@ -282,10 +288,19 @@ __global__ void __launch_bounds__(64) relax_blocks(float* __restrict__ dist, con
         uint32_t touch = 0, bmin = kFltMaxBits;
         const int xlo = cx == 0 ? -1 : 0, xhi = cx == kB - 1 ? 1 : 0;
         const int ylo = cy == 0 ? -1 : 0, yhi = cy == kB - 1 ? 1 : 0;
+        const uint32_t side_xy = (gx == 0 ? 1u : 0u) | (gx == X - 1 ? 2u : 0u) | (gy == 0 ? 4u : 0u) |
+                                 (gy == Y - 1 ? 8u : 0u);
 #pragma unroll
         for (int z = 0; z < kB; ++z) {
             if (!(v[z] < orig[z])) continue;
-            dist[((size_t)(z0 + 1 + z) * X + gx) * Y + gy] = v[z];
+            const int gz = z0 + 1 + z;
+            const size_t gi = ((size_t)gz * X + gx) * Y + gy;
+            dist[gi] = v[z];
+            if (__float_as_uint(orig[z]) == kFltMaxBits) {
+                cav[gi] = id;
+                ++reached;
+                walls |= side_xy | (gz == 0 ? 16u : 0u) | (gz == Z - 1 ? 32u : 0u);
+            }
             const int zlo = z == 0 ? -1 : 0, zhi = z == kB - 1 ? 1 : 0;
             if (!(zlo | zhi | xlo | xhi | ylo | yhi)) continue;
             for (int dz = zlo; dz <= zhi; ++dz)
@@ -313,47 +328,39 @@ __global__ void __launch_bounds__(64) relax_blocks(float* __restrict__ dist, con
         if (npush > kPushBuf - 27) flush();
     }
     if (npush) flush();
+    reached = __ockl_wfred_add_u32(reached);
+    walls = __ockl_wfred_or_u32(walls);
+    if (lane == 0) {
+        part_count[blockIdx.x] += reached;
+        part_walls[blockIdx.x] |= walls;
+    }
 }
 
-// Voxels reached in this cavity (void, cavity id unset, distance set) take the id; walls
-// touched are OR-ed as bits (axis * 2 + side).
-__global__ void __launch_bounds__(256) finish_cavity(const uint8_t* __restrict__ solid, const float* __restrict__ dist,
-                                                     uint32_t* __restrict__ cav, uint32_t id, int X, int Y, int Z,
+// Sums the workgroups' slots of the cavity just finished.
+__global__ void __launch_bounds__(256) cavity_totals(const unsigned long long* __restrict__ part_count,
+                                                     const uint32_t* __restrict__ part_walls, uint32_t n,
                                                      unsigned long long* __restrict__ count,
                                                      uint32_t* __restrict__ walls) {
-    __shared__ unsigned long long part_n[4];
-    __shared__ uint32_t part_w[4];
-    unsigned long long mine = 0;
-    uint32_t wb = 0;
-    const size_t n = (size_t)X * Y * Z;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        if (solid[i] != 0 || cav[i] != 0xFFFFFFFFu || __float_as_uint(dist[i]) == kFltMaxBits) continue;
-        cav[i] = id;
-        ++mine;
-        int y = (int)(i % Y);
-        size_t r = i / Y;
-        int x = (int)(r % X), z = (int)(r / X);
-        if (x == 0) wb |= 1u;
-        if (x == X - 1) wb |= 2u;
-        if (y == 0) wb |= 4u;
-        if (y == Y - 1) wb |= 8u;
-        if (z == 0) wb |= 16u;
-        if (z == Z - 1) wb |= 32u;
+    __shared__ unsigned long long pc[4];
+    __shared__ uint32_t pw[4];
+    unsigned long long c = 0;
+    uint32_t w = 0;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        c += part_count[i];
+        w |= part_walls[i];
     }
     for (int off = 32; off > 0; off >>= 1) {
-        mine += __shfl_xor(mine, off);
-        wb |= (uint32_t)__shfl_xor((int)wb, off);
+        c += __shfl_xor(c, off);
+        w |= (uint32_t)__shfl_xor((int)w, off);
     }
     if ((threadIdx.x & 63) == 0) {
-        part_n[threadIdx.x >> 6] = mine;
-        part_w[threadIdx.x >> 6] = wb;
+        pc[threadIdx.x >> 6] = c;
+        pw[threadIdx.x >> 6] = w;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        mine = part_n[0] + part_n[1] + part_n[2] + part_n[3];
-        wb = part_w[0] | part_w[1] | part_w[2] | part_w[3];
-        if (mine) atomicAdd(count, mine);
-        if (wb) atomicOr(walls, wb);
+        *count = pc[0] + pc[1] + pc[2] + pc[3];
+        *walls = pw[0] | pw[1] | pw[2] | pw[3];
     }
 }
 
@@ -394,6 +401,8 @@ void run_geodesic(pma_ctx* c) {
     unsigned long long* key = c->buf<unsigned long long>("geo_key", 1);
     unsigned long long* vcount = c->buf<unsigned long long>("geo_vcount", 1);
     uint32_t* walls = c->buf<uint32_t>("geo_walls", 1);
+    unsigned long long* part_count = c->buf<unsigned long long>("geo_part_count", kRelaxGrid);
+    uint32_t* part_walls = c->buf<uint32_t>("geo_part_walls", kRelaxGrid);
     PMA_HIP(hipMemsetAsync(block_tag, 0xFF, NB * sizeof(uint32_t), c->stream));
     fill_key<<<grid_for(NB, 256), 256, 0, c->stream>>>(block_key, kFltMaxBits, NB);
     PMA_LAUNCHED(c);
@@ -447,6 +456,8 @@ void run_geodesic(pma_ctx* c) {
         upload(c, fa, &b0, sizeof b0);
         const uint32_t zero = 0;
         upload(c, block_key + b0, &zero, sizeof zero);
+        PMA_HIP(hipMemsetAsync(part_count, 0, kRelaxGrid * sizeof(unsigned long long), c->stream));
+        PMA_HIP(hipMemsetAsync(part_walls, 0, kRelaxGrid * sizeof(uint32_t), c->stream));
         uint32_t nf = 1, kmin = 0;
         uint32_t* cur = fa;
         uint32_t* nxt = fb;
@@ -464,11 +475,13 @@ void run_geodesic(pma_ctx* c) {
                 const unsigned long long src = first ? (unsigned long long)si : ~0ull;
                 const uint32_t grid = std::min(nf, kRelaxGrid);
                 if (d3q7)
-                    relax_blocks<true><<<grid, 64, 0, c->stream>>>(dist, fp, nxt, block_tag, round, block_key, st, X,
-                                                                    Y, Z, NBX, NBY, NBZ, src);
+                    relax_blocks<true><<<grid, 64, 0, c->stream>>>(dist, cav, id, fp, nxt, block_tag, round, block_key,
+                                                                    st, part_count, part_walls, X, Y, Z, NBX, NBY, NBZ,
+                                                                    src);
                 else
-                    relax_blocks<false><<<grid, 64, 0, c->stream>>>(dist, fp, nxt, block_tag, round, block_key, st, X,
-                                                                     Y, Z, NBX, NBY, NBZ, src);
+                    relax_blocks<false><<<grid, 64, 0, c->stream>>>(dist, cav, id, fp, nxt, block_tag, round,
+                                                                     block_key, st, part_count, part_walls, X, Y, Z,
+                                                                     NBX, NBY, NBZ, src);
                 PMA_LAUNCHED(c);
             }
             next_min_key<<<256, 256, 0, c->stream>>>(nxt, block_key, st);
@@ -483,14 +496,8 @@ void run_geodesic(pma_ctx* c) {
             relaxations += h[2];
             std::swap(cur, nxt);
         }
-        PMA_HIP(hipMemsetAsync(vcount, 0, sizeof(unsigned long long), c->stream));
-        PMA_HIP(hipMemsetAsync(walls, 0, sizeof(uint32_t), c->stream));
-        {
-            PMA_TIMED(c, "finish_cavity");
-            finish_cavity<<<std::min(grid_for(n, 256), 8192u), 256, 0, c->stream>>>(solid, dist, cav, id, X, Y, Z,
-                                                                                     vcount, walls);
-            PMA_LAUNCHED(c);
-        }
+        cavity_totals<<<1, 256, 0, c->stream>>>(part_count, part_walls, kRelaxGrid, vcount, walls);
+        PMA_LAUNCHED(c);
         CavityRow row;
         row.id = id;
         row.x = start[0];

@@ -146,6 +146,16 @@ inline unsigned __ockl_wfred_or_u32(unsigned v) {
     shim_yield();
     return r;
 }
+inline unsigned __ockl_wfred_add_u32(unsigned v) {
+    if (!g_wave) throw std::runtime_error("__ockl_wfred_add_u32 outside a wave launch");
+    g_wave->slot[threadIdx.x] = v;
+    shim_yield();
+    unsigned r = 0;
+    const unsigned base = threadIdx.x & ~63u;
+    for (unsigned l = 0; l < 64 && base + l < g_wave->slot.size(); ++l) r += (unsigned)g_wave->slot[base + l];
+    shim_yield();
+    return r;
+}
 inline unsigned __ockl_wfred_min_u32(unsigned v) {
     if (!g_wave) throw std::runtime_error("__ockl_wfred_min_u32 outside a wave launch");
     g_wave->slot[threadIdx.x] = v;
