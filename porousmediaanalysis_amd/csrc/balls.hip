@@ -102,51 +102,58 @@ __global__ void __launch_bounds__(1024) far_prefix_max(uint32_t* __restrict__ sh
     }
 }
 
-__device__ __forceinline__ size_t flat_block() {
-    return ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-}
-
-// Block = 4 rows of 64 voxels along y, walking kPruneZ planes along z with the planes z - 1,
-// z, z + 1 (and a one-voxel halo in x and y; outside the image reads 0) in an LDS ring.  Marks
-// solid 0, void 127 and surviving centres kCandidate, and writes every plane's count of
-// surviving centres at the index emit_centres' block (same 4 x 64 rows, one plane) has.
+// Block = 4 rows of 64 voxels along y; each thread walks its (x, y) column through every plane,
+// with planes z - 1 .. z + 2 (and a one-voxel halo in x and y; outside the image reads 0) in
+// a 4-slot LDS ring: plane z + 2 is fetched into registers while plane z is tested.  Marks
+// solid 0, void 127 and surviving centres kCandidate, and writes the column's count of
+// surviving centres at col_count[x * Y + y] (the columns in the reference's scan order).
 // c is dropped when a neighbour q of offset class k has q > c and q >= far_k(c), i.e. when the
 // class maximum of q reaches max(c + 1, far_k(c)).
-constexpr int kPruneZ = 16;
 constexpr int kPruneW = 64 + 2, kPruneH = 4 + 2, kPrunePlane = kPruneW * kPruneH;
+constexpr int kPruneLoads = (kPrunePlane + 255) / 256;
 
 __global__ void __launch_bounds__(256) prune_centres(const uint32_t* __restrict__ e, uint8_t* __restrict__ mask,
                                                      const uint32_t* __restrict__ far, uint32_t L,
-                                                     uint32_t* __restrict__ block_count, int X, int Y, int Z) {
-    __shared__ uint32_t ring[3][kPrunePlane];
-    __shared__ uint32_t part[4];
+                                                     uint32_t* __restrict__ col_count, int X, int Y, int Z) {
+    __shared__ uint32_t ring[4][kPrunePlane];
     const int lane = threadIdx.x & 63, row = threadIdx.x >> 6;
     const int y = blockIdx.x * 64 + lane, x = blockIdx.y * 4 + row;
-    const int za = blockIdx.z * kPruneZ, zb = min(za + kPruneZ, Z);
-    auto load_plane = [&](int z) {
-        uint32_t* dst = ring[(z + 3) % 3];
-        for (int k = threadIdx.x; k < kPrunePlane; k += 256) {
+    auto fetch = [&](int z, uint32_t (&r)[kPruneLoads]) {
+#pragma unroll
+        for (int j = 0; j < kPruneLoads; ++j) {
+            const int k = threadIdx.x + 256 * j;
             const int xx = blockIdx.y * 4 - 1 + k / kPruneW, yy = blockIdx.x * 64 - 1 + k % kPruneW;
-            dst[k] = (z >= 0 && z < Z && xx >= 0 && xx < X && yy >= 0 && yy < Y) ? e[((size_t)z * X + xx) * Y + yy]
-                                                                                  : 0u;
+            r[j] = (k < kPrunePlane && z >= 0 && z < Z && xx >= 0 && xx < X && yy >= 0 && yy < Y)
+                       ? e[((size_t)z * X + xx) * Y + yy]
+                       : 0u;
         }
     };
-    load_plane(za - 1);
-    load_plane(za);
-    for (int z = za; z < zb; ++z) {
-        load_plane(z + 1);
-        __syncthreads();
-        bool keep = false;
+    auto store = [&](int z, const uint32_t (&r)[kPruneLoads]) {
+#pragma unroll
+        for (int j = 0; j < kPruneLoads; ++j) {
+            const int k = threadIdx.x + 256 * j;
+            if (k < kPrunePlane) ring[(z + 4) & 3][k] = r[j];
+        }
+    };
+    uint32_t r[kPruneLoads];
+    for (int z = -1; z <= 1; ++z) {
+        fetch(z, r);
+        store(z, r);
+    }
+    __syncthreads();
+    uint32_t kept = 0;
+    const int o = (row + 1) * kPruneW + lane + 1;
+    for (int z = 0; z < Z; ++z) {
+        fetch(z + 2, r);
         if (x < X && y < Y) {
             const size_t i = ((size_t)z * X + x) * Y + y;
-            const int o = (row + 1) * kPruneW + lane + 1;
-            const uint32_t c = ring[(z + 3) % 3][o];
+            const uint32_t c = ring[z & 3][o];
             if (c == 0) {
                 mask[i] = 0;
             } else {
                 uint32_t m1 = 0, m2 = 0, m3 = 0;
                 for (int dz = -1; dz <= 1; ++dz) {
-                    const uint32_t* pl = ring[(z + dz + 3) % 3];
+                    const uint32_t* pl = ring[(z + dz + 4) & 3];
                     for (int dx = -1; dx <= 1; ++dx)
                         for (int dy = -1; dy <= 1; ++dy) {
                             const int k = abs(dz) + abs(dx) + abs(dy);
@@ -156,53 +163,35 @@ __global__ void __launch_bounds__(256) prune_centres(const uint32_t* __restrict_
                             else if (k == 3) m3 = max(m3, q);
                         }
                 }
-                keep = !(m1 >= max(c + 1, far[c]) || m2 >= max(c + 1, far[L + c]) ||
-                         m3 >= max(c + 1, far[2 * (size_t)L + c]));
+                const bool keep = !(m1 >= max(c + 1, far[c]) || m2 >= max(c + 1, far[L + c]) ||
+                                    m3 >= max(c + 1, far[2 * (size_t)L + c]));
                 mask[i] = keep ? kCandidate : 127;
+                kept += keep;
             }
         }
-        const uint64_t bal = __ballot(keep);
-        if (lane == 0) part[row] = (uint32_t)__popcll(bal);
+        store(z + 2, r);  // the slot of plane z - 2, last read in the previous step
         __syncthreads();
-        if (threadIdx.x == 0)
-            block_count[((size_t)z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = part[0] + part[1] + part[2] +
-                                                                                        part[3];
     }
+    if (x < X && y < Y) col_count[(size_t)x * Y + y] = kept;
 }
 
-// Sort key of every surviving centre: (maxE - r2) << 32 | scan, ascending = the win order.
-// Written at the block's offset (exclusive scan of the block counts) plus its rank in the block.
+// Sort key of every surviving centre: (maxE - r2) << 32 | scan.  Each thread writes its column's
+// centres in z order from the column's offset (exclusive scan of col_count), so the keys come out
+// in scan order and a stable sort on the r2 bits alone gives the win order.
 __global__ void __launch_bounds__(256) emit_centres(const uint32_t* __restrict__ e, uint8_t* __restrict__ mask,
-                                                    const unsigned long long* __restrict__ block_off,
-                                                    unsigned long long* __restrict__ keys, unsigned long long cap,
-                                                    uint32_t max_e, int X, int Y, int Z) {
-    __shared__ uint32_t part[4];
-    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int y = blockIdx.x * 64 + lane;
-    int x = blockIdx.y * 4 + (threadIdx.x >> 6);
-    int z = blockIdx.z;
-    bool keep = false;
-    size_t i = 0;
-    if (x < X && y < Y) {
-        i = ((size_t)z * X + x) * Y + y;
-        keep = mask[i] == kCandidate;
-    }
-    uint64_t bal = __ballot(keep);
-    if (lane == 0) part[wave] = (uint32_t)__popcll(bal);
-    __syncthreads();
-    unsigned long long base = block_off[flat_block()];
-    for (int w = 0; w < wave; ++w) base += part[w];
-    if (keep) {
-        unsigned long long slot = base + (unsigned long long)__popcll(bal & ((1ull << lane) - 1));
-        const uint32_t scan = (uint32_t)(((uint64_t)x * Y + y) * (uint64_t)Z + z);
-        if (slot < cap) keys[slot] = ((unsigned long long)(max_e - e[i]) << 32) | scan;
+                                                    const uint32_t* __restrict__ col_off,
+                                                    unsigned long long* __restrict__ keys, uint32_t max_e, int X, int Y,
+                                                    int Z) {
+    const int y = blockIdx.x * 64 + (threadIdx.x & 63), x = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= X || y >= Y) return;
+    uint32_t pos = col_off[(size_t)x * Y + y];
+    const uint32_t scan0 = (uint32_t)(((uint64_t)x * Y + y) * (uint64_t)Z);
+    for (int z = 0; z < Z; ++z) {
+        const size_t i = ((size_t)z * X + x) * Y + y;
+        if (mask[i] != kCandidate) continue;
+        keys[pos++] = ((unsigned long long)(max_e - e[i]) << 32) | (scan0 + (uint32_t)z);
         mask[i] = 127;
     }
-}
-
-__global__ void widen_counts(const uint32_t* __restrict__ in, size_t n, unsigned long long* __restrict__ out) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = in[i];
 }
 
 __global__ void decode_centres(const unsigned long long* __restrict__ keys, size_t n, uint32_t max_e, int Y, int Z,
@@ -484,35 +473,31 @@ void run_containing_balls(pma_ctx* c) {
 
     const uint32_t* e = c->v<uint32_t>(PMA_VOL_EDT2);
     uint8_t* mask = c->v<uint8_t>(PMA_VOL_MASK);
-    const dim3 vgrid(grid_for(Y, 64), grid_for(X, 4), Z);
+    const dim3 cgrid(grid_for(Y, 64), grid_for(X, 4));  // one thread per (x, y) column
 
-    // 1-2. surviving centres, sorted into win order.
-    const size_t nblocks = (size_t)vgrid.x * vgrid.y * vgrid.z;
-    uint32_t* bcount = c->buf<uint32_t>("cb_block_count", nblocks);
-    unsigned long long* bcount64 = c->buf<unsigned long long>("cb_block_count64", nblocks);
-    unsigned long long* boff = c->buf<unsigned long long>("cb_block_off", nblocks);
+    // 1-2. surviving centres in scan order, then sorted by r2 (stable) into win order.
+    const size_t ncols = (size_t)X * Y;
+    uint32_t* col_count = c->buf<uint32_t>("cb_col_count", ncols);
+    uint32_t* col_off = c->buf<uint32_t>("cb_col_off", ncols);
     {
         PMA_TIMED(c, "prune_centres");
-        const dim3 pgrid(vgrid.x, vgrid.y, (unsigned)((Z + kPruneZ - 1) / kPruneZ));
-        prune_centres<<<pgrid, 256, 0, c->stream>>>(e, mask, far, (uint32_t)L, bcount, X, Y, Z);
+        prune_centres<<<cgrid, 256, 0, c->stream>>>(e, mask, far, (uint32_t)L, col_count, X, Y, Z);
         PMA_LAUNCHED(c);
     }
-    widen_counts<<<grid_for(nblocks, 256), 256, 0, c->stream>>>(bcount, nblocks, bcount64);
-    PMA_LAUNCHED(c);
-    const unsigned long long n_cand = exclusive_scan_u64(c, reinterpret_cast<const uint64_t*>(bcount64),
-                                                         reinterpret_cast<uint64_t*>(boff), nblocks);
+    // fewer than 2^32 voxels, so the offsets fit 32 bits
+    const unsigned long long n_cand = exclusive_scan_u32(c, col_count, col_off, ncols);
     const size_t N1 = std::max<unsigned long long>(n_cand, 1);
     unsigned long long* keys = c->buf<unsigned long long>("cb_keys", N1);
     unsigned long long* keys2 = c->buf<unsigned long long>("cb_keys2", N1);
     {
         PMA_TIMED(c, "emit_centres");
-        emit_centres<<<vgrid, 256, 0, c->stream>>>(e, mask, boff, keys, n_cand, max_e, X, Y, Z);
+        emit_centres<<<cgrid, 256, 0, c->stream>>>(e, mask, col_off, keys, max_e, X, Y, Z);
         PMA_LAUNCHED(c);
     }
     int hi_bits = 0;
     while (hi_bits < 32 && (max_e >> hi_bits) != 0) ++hi_bits;
-    radix_sort_keys_u64(c, reinterpret_cast<const uint64_t*>(keys), reinterpret_cast<uint64_t*>(keys2), n_cand, 0,
-                        32 + hi_bits);
+    radix_sort_keys_u64(c, reinterpret_cast<const uint64_t*>(keys), reinterpret_cast<uint64_t*>(keys2), n_cand, 32,
+                        32 + std::max(hi_bits, 1));
     Ball* balls = c->buf<Ball>("cb_balls", N1);
     if (n_cand) {
         decode_centres<<<grid_for(n_cand, 256), 256, 0, c->stream>>>(keys2, n_cand, max_e, Y, Z, balls);
