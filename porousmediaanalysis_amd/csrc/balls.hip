@@ -24,6 +24,8 @@
 //      (SetMaximumBallsMaskInActiveArea).
 #include "pma_internal.h"
 
+#include <type_traits>
+
 namespace pma {
 namespace {
 
@@ -36,6 +38,9 @@ constexpr int TZ = 8, TX = 8, TY = 64;  // tile: 64 rows (z, x) of 64 voxels alo
 #endif
 #ifndef PMA_CB_GX
 #define PMA_CB_GX 2  // tiles per pair cell along x
+#endif
+#ifndef PMA_CB_PACKED
+#define PMA_CB_PACKED 1  // 0 keeps ranks in the pair sort and gathers the balls afterwards
 #endif
 #ifndef PMA_CB_SMALL_R2
 #define PMA_CB_SMALL_R2 (1u << 22)  // r2 below this use isqrt_small (its exactness bound)
@@ -217,10 +222,31 @@ struct Cells {
     int NCX, NCY;  // cells along x and y
 };
 
+// A ball packed relative to the origin (z0, x0, y0) of a cell it reaches, so that the pair
+// sort carries the ball itself: r2 << 33 | (z - z0 + off) << 22 | (x - x0 + off) << 11 |
+// (y - y0 + off).  A ball that reaches a cell has its centre within sqrt(r2) <= off - 1 of it,
+// so each field is in [1, cell edge + 2 off - 2]; the host uses this form only when that is
+// below 2^11 and r2 < 2^31.
+__device__ __forceinline__ uint64_t pack_ball(const Ball& b, int z0, int x0, int y0, int off) {
+    return (uint64_t)b.r2 << 33 | (uint64_t)(b.z - z0 + off) << 22 | (uint64_t)(b.x - x0 + off) << 11 |
+           (uint64_t)(b.y - y0 + off);
+}
+
+__device__ __forceinline__ Ball unpack_ball(uint64_t v, int z0, int x0, int y0, int off) {
+    Ball b;
+    b.y = (int)(v & 2047u) - off + y0;
+    b.x = (int)((v >> 11) & 2047u) - off + x0;
+    b.z = (int)((v >> 22) & 2047u) - off + z0;
+    b.r2 = (uint32_t)(v >> 33);
+    return b;
+}
+
 // Cells of the slab [cz0, cz1) that ball b reaches (box distance^2 <= r2): counted (pk == null)
-// or listed as (cell - cell_base, rank) pairs from `at`.
+// or listed from `at` as (cell - cell_base, value) pairs, the value being the ball's rank
+// (V = uint32_t) or the ball packed relative to the cell (V = uint64_t).
+template <class V>
 __device__ uint32_t ball_tiles(const Ball& b, int X, int Y, int Z, Cells g, int cz0, int cz1, uint32_t cell_base,
-                               uint32_t rank, uint32_t* __restrict__ pk, uint32_t* __restrict__ pv,
+                               uint32_t rank, int off, uint32_t* __restrict__ pk, V* __restrict__ pv,
                                unsigned long long at) {
     const int R = (int)isqrt_u32(b.r2);
     const int za = max(b.z - R, 0) / g.sz, zb = min(b.z + R, Z - 1) / g.sz;
@@ -239,7 +265,8 @@ __device__ uint32_t ball_tiles(const Ball& b, int X, int Y, int Z, Cells g, int 
                 if ((uint32_t)(dz * dz + dx * dx + dy * dy) > b.r2) continue;
                 if (pk) {
                     pk[at + n] = (uint32_t)((tz * g.NCX + tx) * g.NCY + ty) - cell_base;
-                    pv[at + n] = rank;
+                    if constexpr (sizeof(V) == 8) pv[at + n] = pack_ball(b, z0, x0, y0, off);
+                    else pv[at + n] = rank;
                 }
                 ++n;
             }
@@ -252,25 +279,27 @@ __global__ void count_pairs(const Ball* __restrict__ balls, size_t n, int X, int
                             unsigned long long* __restrict__ cnt) {
     size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
-    cnt[r] = ball_tiles(balls[r], X, Y, Z, g, cz0, cz1, 0, 0, nullptr, nullptr, 0);
+    cnt[r] = ball_tiles<uint32_t>(balls[r], X, Y, Z, g, cz0, cz1, 0, 0, 0, nullptr, nullptr, 0);
 }
 
-__global__ void emit_pairs(const Ball* __restrict__ balls, size_t n, const unsigned long long* __restrict__ off,
-                           int X, int Y, int Z, Cells g, int cz0, int cz1, uint32_t cell_base,
-                           uint32_t* __restrict__ pk, uint32_t* __restrict__ pv) {
+template <class V>
+__global__ void emit_pairs(const Ball* __restrict__ balls, size_t n, const unsigned long long* __restrict__ at,
+                           int X, int Y, int Z, Cells g, int cz0, int cz1, uint32_t cell_base, int off,
+                           uint32_t* __restrict__ pk, V* __restrict__ pv) {
     size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
-    ball_tiles(balls[r], X, Y, Z, g, cz0, cz1, cell_base, (uint32_t)r, pk, pv, off[r]);
+    ball_tiles<V>(balls[r], X, Y, Z, g, cz0, cz1, cell_base, (uint32_t)r, off, pk, pv, at[r]);
 }
 
-// [start, end) of every cell's run in the cell-sorted pair list, and the balls themselves in
-// that order (so that a tile reads its cell's balls contiguously).
+// [start, end) of every cell's run in the cell-sorted pair list; when the pairs carry ranks,
+// also the balls themselves in that order (so that a tile reads its cell's balls contiguously).
+template <bool Gather>
 __global__ void tile_ranges(const uint32_t* __restrict__ pk, const uint32_t* __restrict__ rank, size_t P,
                             const Ball* __restrict__ balls, Ball* __restrict__ cell_balls,
                             uint32_t* __restrict__ start, uint32_t* __restrict__ end) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P) return;
-    cell_balls[i] = balls[rank[i]];
+    if (Gather) cell_balls[i] = balls[rank[i]];
     const uint32_t t = pk[i];
     if (i == 0 || pk[i - 1] != t) start[t] = (uint32_t)i;
     if (i + 1 == P || pk[i + 1] != t) end[t] = (uint32_t)(i + 1);
@@ -315,8 +344,9 @@ __global__ void __launch_bounds__(256) max_run(const uint32_t* __restrict__ star
 // a voxel is kept in LDS as the position of its ball in that run (16 bits when every run is
 // shorter than 65536).  Workgroups are dispatched round-robin over the 8 XCDs, so the block
 // index is mapped such that the tiles of one cell run on the same XCD and share its L2.
-template <class W, int Root>
+template <class W, int Root, bool Packed>
 __global__ void __launch_bounds__(64) cover_tiles(const Ball* __restrict__ cell_balls,
+                                                  const uint64_t* __restrict__ cell_packed, int off,
                                                   const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
                                                   const uint32_t* __restrict__ e, uint32_t* __restrict__ cr,
                                                   uint32_t* __restrict__ cx, uint32_t* __restrict__ cy,
@@ -335,6 +365,16 @@ __global__ void __launch_bounds__(64) cover_tiles(const Ball* __restrict__ cell_
     const int ccz = (int)(cell / ((uint32_t)g.NCY * (uint32_t)g.NCX));
     const int tz = ccz * g.gz + (int)(sub / (uint32_t)g.gx), tx = ccx * g.gx + (int)(sub % (uint32_t)g.gx);
     const int z = tz * TZ + lane / TX, x = tx * TX + lane % TX, y0 = ty * TY;
+    const int oz = ccz * g.sz, ox = ccx * g.sx;  // origin of the cell (packed balls are relative to it)
+    using Raw = std::conditional_t<Packed, uint64_t, Ball>;
+    auto raw_at = [&](uint32_t i) -> Raw {
+        if constexpr (Packed) return cell_packed[i];
+        else return cell_balls[i];
+    };
+    auto ball_of = [&](const Raw& r) -> Ball {
+        if constexpr (Packed) return unpack_ball(r, oz, ox, y0, off);
+        else return r;
+    };
     if (Root == kRootTable)
         for (int v = lane; v < kRootTableSize; v += 64) root_tab[v] = (uint8_t)isqrt_small(v);
 
@@ -360,8 +400,8 @@ __global__ void __launch_bounds__(64) cover_tiles(const Ball* __restrict__ cell_
     }
 
     const uint32_t s = start[local], en = end[local];
-    Ball next{0, 0, 0, 0};
-    if (s + lane < en) next = cell_balls[s + lane];
+    Raw next{};
+    if (s + lane < en) next = raw_at(s + lane);
     for (uint32_t base = s; base < en; base += 64) {
         const bool open_row = cov != ~0ull;
         const uint64_t rows = __ballot(open_row);
@@ -376,8 +416,8 @@ __global__ void __launch_bounds__(64) cover_tiles(const Ball* __restrict__ cell_
         const int xlo = tx * TX + __builtin_ctz(xs), xhi = tx * TX + 31 - __builtin_clz(xs);
         const int ylo = y0 + __builtin_ctzll(ys), yhi = y0 + 63 - __builtin_clzll(ys);
 
-        const Ball mb = next;  // this batch; the next one is loaded while it is processed
-        if (base + 64 + lane < en) next = cell_balls[base + 64 + lane];
+        const Ball mb = ball_of(next);  // this batch; the next one is loaded while it is processed
+        if (base + 64 + lane < en) next = raw_at(base + 64 + lane);
         bool reach = false;
         if (base + lane < en) {
             const int ez = max(max(zlo - mb.z, mb.z - zhi), 0), ex = max(max(xlo - mb.x, mb.x - xhi), 0);
@@ -435,13 +475,45 @@ __global__ void __launch_bounds__(64) cover_tiles(const Ball* __restrict__ cell_
             atomicOr(error_flag, 1u);
             continue;
         }
-        const Ball b = cell_balls[s + (uint32_t)win[r * TY + lane]];
+        const Ball b = ball_of(raw_at(s + (uint32_t)win[r * TY + lane]));
         cr[i] = b.r2;
         cx[i] = (uint32_t)b.x;
         cy[i] = (uint32_t)b.y;
         cz[i] = (uint32_t)b.z;
     }
 }
+
+struct CoverArgs {
+    uint32_t blocks;
+    const Ball* cell_balls;
+    const uint64_t* cell_packed;
+    int off;
+    const uint32_t *start, *end, *e;
+    uint32_t *cr, *cx, *cy, *cz;
+    uint8_t* mask;
+    int X, Y, Z;
+    Cells g;
+    uint32_t cell_base, slab_cells;
+    uint32_t* err;
+};
+
+template <class W, int Root, bool Packed>
+void launch_cover(hipStream_t stream, const CoverArgs& a) {
+    cover_tiles<W, Root, Packed><<<a.blocks, 64, 0, stream>>>(a.cell_balls, a.cell_packed, a.off, a.start, a.end, a.e,
+                                                              a.cr, a.cx, a.cy, a.cz, a.mask, a.X, a.Y, a.Z, a.g,
+                                                              a.cell_base, a.slab_cells, a.err);
+}
+
+using CoverLaunch = void (*)(hipStream_t, const CoverArgs&);
+// [16-bit winner slots][chord root mode][packed balls]
+constexpr CoverLaunch kCoverLaunch[2][3][2] = {
+    {{launch_cover<uint32_t, kRootGeneral, false>, launch_cover<uint32_t, kRootGeneral, true>},
+     {launch_cover<uint32_t, kRootSmall, false>, launch_cover<uint32_t, kRootSmall, true>},
+     {launch_cover<uint32_t, kRootTable, false>, launch_cover<uint32_t, kRootTable, true>}},
+    {{launch_cover<uint16_t, kRootGeneral, false>, launch_cover<uint16_t, kRootGeneral, true>},
+     {launch_cover<uint16_t, kRootSmall, false>, launch_cover<uint16_t, kRootSmall, true>},
+     {launch_cover<uint16_t, kRootTable, false>, launch_cover<uint16_t, kRootTable, true>}},
+};
 
 __global__ void __launch_bounds__(256) count_equal_u8(const uint8_t* __restrict__ v, size_t n, uint8_t value,
                                                       unsigned long long* __restrict__ out) {
@@ -521,6 +593,9 @@ void run_containing_balls(pma_ctx* c) {
     unsigned long long* poff = c->buf<unsigned long long>("cb_pair_off", N1);
     uint32_t* err = c->buf<uint32_t>("cb_error", 1);
     PMA_HIP(hipMemsetAsync(err, 0, sizeof(uint32_t), c->stream));
+    // balls travel packed in the pair sort when they fit 64 bits relative to their cell
+    const int off = (int)std::floor(std::sqrt((double)max_e)) + 1;
+    const bool packed = PMA_CB_PACKED && max_e < (1u << 31) && std::max(std::max(g.sz, g.sx), TY) + 2 * off < 2048;
     uint64_t total_pairs = 0;
     int slabs = 1;
     for (int tz0 = 0; tz0 < NCZ;) {
@@ -541,23 +616,44 @@ void run_containing_balls(pma_ctx* c) {
         const uint32_t slab_cells = (uint32_t)(tz1 - tz0) * layer_cells;
         const size_t P1 = std::max<uint64_t>(P, 1);
         uint32_t* pk = c->buf<uint32_t>("cb_pair_tile", P1);
-        uint32_t* pv = c->buf<uint32_t>("cb_pair_rank", P1);
         uint32_t* pk2 = c->buf<uint32_t>("cb_pair_tile2", P1);
-        uint32_t* pv2 = c->buf<uint32_t>("cb_pair_rank2", P1);
-        Ball* cell_balls = c->buf<Ball>("cb_cell_balls", P1);
+        uint32_t* pv = nullptr;
+        uint32_t* pv2 = nullptr;
+        Ball* cell_balls = nullptr;
+        uint64_t* pb = nullptr;
+        uint64_t* pb2 = nullptr;
+        if (packed) {
+            pb = c->buf<uint64_t>("cb_pair_ball", P1);
+            pb2 = c->buf<uint64_t>("cb_pair_ball2", P1);
+        } else {
+            pv = c->buf<uint32_t>("cb_pair_rank", P1);
+            pv2 = c->buf<uint32_t>("cb_pair_rank2", P1);
+            cell_balls = c->buf<Ball>("cb_cell_balls", P1);
+        }
         uint32_t* tstart = c->buf<uint32_t>("cb_cell_start", slab_cells);
         uint32_t* tend = c->buf<uint32_t>("cb_cell_end", slab_cells);
         PMA_HIP(hipMemsetAsync(tstart, 0, slab_cells * sizeof(uint32_t), c->stream));
         PMA_HIP(hipMemsetAsync(tend, 0, slab_cells * sizeof(uint32_t), c->stream));
         if (P) {
-            emit_pairs<<<grid_for(n_cand, 256), 256, 0, c->stream>>>(balls, n_cand, poff, X, Y, Z, g, tz0, tz1,
-                                                                     cell_base, pk, pv);
-            PMA_LAUNCHED(c);
             int cell_bits = 1;
             while (cell_bits < 32 && (slab_cells >> cell_bits) != 0) ++cell_bits;
-            radix_sort_pairs_u32(c, pk, pk2, pv, pv2, P, 0, cell_bits);
-            tile_ranges<<<grid_for(P, 256), 256, 0, c->stream>>>(pk2, pv2, P, balls, cell_balls, tstart, tend);
-            PMA_LAUNCHED(c);
+            if (packed) {
+                emit_pairs<uint64_t><<<grid_for(n_cand, 256), 256, 0, c->stream>>>(balls, n_cand, poff, X, Y, Z, g, tz0,
+                                                                                   tz1, cell_base, off, pk, pb);
+                PMA_LAUNCHED(c);
+                radix_sort_pairs_u32_u64(c, pk, pk2, pb, pb2, P, 0, cell_bits);
+                tile_ranges<false><<<grid_for(P, 256), 256, 0, c->stream>>>(pk2, nullptr, P, balls, nullptr, tstart,
+                                                                            tend);
+                PMA_LAUNCHED(c);
+            } else {
+                emit_pairs<uint32_t><<<grid_for(n_cand, 256), 256, 0, c->stream>>>(balls, n_cand, poff, X, Y, Z, g, tz0,
+                                                                                   tz1, cell_base, off, pk, pv);
+                PMA_LAUNCHED(c);
+                radix_sort_pairs_u32(c, pk, pk2, pv, pv2, P, 0, cell_bits);
+                tile_ranges<true><<<grid_for(P, 256), 256, 0, c->stream>>>(pk2, pv2, P, balls, cell_balls, tstart,
+                                                                           tend);
+                PMA_LAUNCHED(c);
+            }
         }
         unsigned int longest = 0;
         if (P) {
@@ -581,24 +677,9 @@ void run_containing_balls(pma_ctx* c) {
                              : max_e < PMA_CB_SMALL_R2                                            ? kRootSmall
                                                                                                    : kRootGeneral;
             const bool w16 = longest <= PMA_CB_WIN16_MAX;
-            if (w16 && root == kRootTable)
-                cover_tiles<uint16_t, kRootTable><<<blocks, 64, 0, c->stream>>>(
-                    cell_balls, tstart, tend, e, cr, cx, cy, cz, mask, X, Y, Z, g, cell_base, slab_cells, err);
-            else if (w16 && root == kRootSmall)
-                cover_tiles<uint16_t, kRootSmall><<<blocks, 64, 0, c->stream>>>(
-                    cell_balls, tstart, tend, e, cr, cx, cy, cz, mask, X, Y, Z, g, cell_base, slab_cells, err);
-            else if (w16)
-                cover_tiles<uint16_t, kRootGeneral><<<blocks, 64, 0, c->stream>>>(
-                    cell_balls, tstart, tend, e, cr, cx, cy, cz, mask, X, Y, Z, g, cell_base, slab_cells, err);
-            else if (root == kRootTable)
-                cover_tiles<uint32_t, kRootTable><<<blocks, 64, 0, c->stream>>>(
-                    cell_balls, tstart, tend, e, cr, cx, cy, cz, mask, X, Y, Z, g, cell_base, slab_cells, err);
-            else if (root == kRootSmall)
-                cover_tiles<uint32_t, kRootSmall><<<blocks, 64, 0, c->stream>>>(
-                    cell_balls, tstart, tend, e, cr, cx, cy, cz, mask, X, Y, Z, g, cell_base, slab_cells, err);
-            else
-                cover_tiles<uint32_t, kRootGeneral><<<blocks, 64, 0, c->stream>>>(
-                    cell_balls, tstart, tend, e, cr, cx, cy, cz, mask, X, Y, Z, g, cell_base, slab_cells, err);
+            CoverArgs a{blocks, cell_balls, pb2, off, tstart, tend, e, cr, cx, cy, cz, mask, X, Y, Z, g,
+                        cell_base, slab_cells, err};
+            kCoverLaunch[w16 ? 1 : 0][root][packed ? 1 : 0](c->stream, a);
             PMA_LAUNCHED(c);
         }
         total_pairs += P;

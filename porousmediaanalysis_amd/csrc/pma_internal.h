@@ -157,6 +157,8 @@ void radix_sort_pairs_u64(pma_ctx* c, const uint64_t* keys_in, uint64_t* keys_ou
                           uint32_t* vals_out, size_t n, int begin_bit = 0, int end_bit = 64);
 void radix_sort_pairs_u64_u64(pma_ctx* c, const uint64_t* keys_in, uint64_t* keys_out, const uint64_t* vals_in,
                               uint64_t* vals_out, size_t n, int begin_bit = 0, int end_bit = 64);
+void radix_sort_pairs_u32_u64(pma_ctx* c, const uint32_t* keys_in, uint32_t* keys_out, const uint64_t* vals_in,
+                              uint64_t* vals_out, size_t n, int begin_bit = 0, int end_bit = 32);
 void radix_sort_pairs_u32(pma_ctx* c, const uint32_t* keys_in, uint32_t* keys_out, const uint32_t* vals_in,
                           uint32_t* vals_out, size_t n, int begin_bit = 0, int end_bit = 32);
 void radix_sort_keys_u64(pma_ctx* c, const uint64_t* keys_in, uint64_t* keys_out, size_t n, int begin_bit = 0,

@@ -37,6 +37,7 @@ struct Dim3 {
 typedef Dim3 dim3;
 inline thread_local Dim3 threadIdx, blockIdx;
 inline Dim3 blockDim, gridDim;
+using hipStream_t = void*;
 struct uint2 { uint32_t x, y; };
 struct alignas(16) uint4 { uint32_t x, y, z, w; };
 inline uint2 make_uint2(uint32_t a, uint32_t b) { return uint2{a, b}; }
@@ -413,6 +414,7 @@ template <class K, class V> void rs(const K* ki, K* ko, const V* vi, V* vo, size
 }
 inline void radix_sort_pairs_u64(pma_ctx*, const uint64_t* a, uint64_t* b, const uint32_t* c, uint32_t* d, size_t n, int b0 = 0, int b1 = 64) { rs(a, b, c, d, n, b0, b1); }
 inline void radix_sort_pairs_u64_u64(pma_ctx*, const uint64_t* a, uint64_t* b, const uint64_t* c, uint64_t* d, size_t n, int b0 = 0, int b1 = 64) { rs(a, b, c, d, n, b0, b1); }
+inline void radix_sort_pairs_u32_u64(pma_ctx*, const uint32_t* a, uint32_t* b, const uint64_t* c, uint64_t* d, size_t n, int b0 = 0, int b1 = 32) { rs(a, b, c, d, n, b0, b1); }
 inline void radix_sort_pairs_u32(pma_ctx*, const uint32_t* a, uint32_t* b, const uint32_t* c, uint32_t* d, size_t n, int b0 = 0, int b1 = 32) { rs(a, b, c, d, n, b0, b1); }
 inline void radix_sort_keys_u64(pma_ctx*, const uint64_t* a, uint64_t* b, size_t n, int b0 = 0, int b1 = 64) {
     std::vector<uint8_t> none(n);

@@ -80,10 +80,12 @@ def hostemu(tmp_path_factory):
 @pytest.fixture(scope="session")
 def hostemu_slabs(tmp_path_factory):
     """Sanitized build with small work limits: containing-ball pair lists cut into slabs of at most
-    4000 pairs, 32-bit winner slots, and geodesic relaxation on 2 waves with 32-entry push buffers."""
+    4000 pairs, 32-bit winner slots, ranks (not packed balls) in the pair sort, and geodesic
+    relaxation on 2 waves with 32-entry push buffers."""
     return _build(tmp_path_factory, SANITIZED + ["-DPMA_CB_SLAB_PAIRS=4000ull", "-DPMA_CB_WIN16_MAX=0u",
                                                  "-DPMA_CB_GZ=1", "-DPMA_CB_GX=3", "-DPMA_CB_SMALL_R2=0u",
-                                                 "-DPMA_GEO_RELAX_GRID=2u", "-DPMA_GEO_PUSH_BUF=32u"], "slabs")
+                                                 "-DPMA_GEO_RELAX_GRID=2u", "-DPMA_GEO_PUSH_BUF=32u",
+                                                 "-DPMA_CB_PACKED=0"], "slabs")
 
 
 @pytest.fixture(scope="session")
@@ -216,7 +218,8 @@ def test_geodesic_small_grid(hostemu_slabs, tmp_path, name):
 
 def test_containing_balls_in_slabs(hostemu_slabs, tmp_path):
     """The same result when the (cell, ball) pairs are processed in many z-slabs, with 32-bit winner
-    slots, pair cells of 1 x 3 tiles and the general integer square root."""
+    slots, pair cells of 1 x 3 tiles, ranks gathered after the pair sort and the general integer
+    square root."""
     solid = BALL_CASES["gauss_s4"]
     e, max_e, _ = oracle.edt(solid, threads=4)
     g = emu_balls(hostemu_slabs, tmp_path, e, max_e)
