@@ -105,58 +105,74 @@ static_assert(kPushBuf > 27, "a block activates up to 26 neighbours at once");
 constexpr int kH = kB + 2;      // with halo
 constexpr int kHalo = kH * kH * kH;
 
-// Round state (device): [0] next list length, [1] smallest key of the next list (float
-// bits, from next_min_key), [2] length of this round's processing list.
+// Rounds run back to back on the device: every kernel of a round reads the list lengths it
+// needs from the round states, so the host does not wait between rounds.
+struct RoundState {
+    uint32_t count;  // length of the active list
+    uint32_t kmin;   // smallest key over it (float bits; +inf when empty)
+    uint32_t proc;   // length of the round's processing list
+    uint32_t pad;
+};
+
 __global__ void fill_key(uint32_t* __restrict__ p, uint32_t v, size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
 
-__global__ void round_init(uint32_t* __restrict__ st) {
-    st[0] = 0;
-    st[1] = 0x7F800000u;  // +inf
-    st[2] = 0;
+__global__ void round_init(RoundState* __restrict__ cur, RoundState* __restrict__ nxt) {
+    nxt->count = 0;
+    nxt->kmin = 0x7F800000u;  // +inf
+    cur->proc = 0;
 }
 
-// Splits the active list: keys within the threshold go to `proc` (their key is cleared, so that
-// activations during the round start it afresh), the others are carried into the next list
-// (tagged so that a new activation does not list them twice).
-__global__ void __launch_bounds__(256) partition_active(const uint32_t* __restrict__ active, uint32_t n,
-                                                        uint32_t* __restrict__ block_key, uint32_t threshold,
+// Splits the active list: keys within kDelta of its smallest key go to `proc` (their key is
+// cleared, so that activations during the round start it afresh), the others are carried into
+// the next list (tagged so that a new activation does not list them twice).  Grid-stride over
+// the list, whose length is cur->count.
+__global__ void __launch_bounds__(256) partition_active(const uint32_t* __restrict__ active,
+                                                        RoundState* __restrict__ cur, uint32_t* __restrict__ block_key,
                                                         uint32_t* __restrict__ proc, uint32_t* __restrict__ next,
                                                         uint32_t* __restrict__ block_tag, uint32_t tag,
-                                                        uint32_t* __restrict__ st) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+                                                        RoundState* __restrict__ nxt) {
+    const uint32_t n = cur->count;
+    const float thr = __uint_as_float(cur->kmin) + kDelta;
+    const uint32_t threshold = isfinite(thr) ? __float_as_uint(thr) : 0xFFFFFFFFu;
     const int lane = threadIdx.x & 63;
-    uint32_t b = 0, key = 0;
-    bool now = false, later = false;
-    if (t < n) {
-        b = active[t];
-        key = block_key[b];
-        now = key <= threshold;
-        later = !now;
-    }
-    if (now) block_key[b] = kFltMaxBits;
-    if (later) (void)atomicExch(&block_tag[b], tag);
-    const unsigned long long bn = __ballot(now), bl = __ballot(later);
-    uint32_t base_n = 0, base_l = 0;
-    if (lane == 0) {
-        if (bn) base_n = atomicAdd(&st[2], (uint32_t)__popcll(bn));
-        if (bl) base_l = atomicAdd(&st[0], (uint32_t)__popcll(bl));
-    }
-    base_n = (uint32_t)__shfl((int)base_n, 0);
-    base_l = (uint32_t)__shfl((int)base_l, 0);
     const unsigned long long below = (1ull << lane) - 1;
-    if (now) proc[base_n + (uint32_t)__popcll(bn & below)] = b;
-    if (later) next[base_l + (uint32_t)__popcll(bl & below)] = b;
+    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n; base += gridDim.x * blockDim.x) {
+        const uint32_t t = base + (uint32_t)lane;  // `base` is uniform per wave
+        uint32_t b = 0, key = 0;
+        bool now = false, later = false;
+        if (t < n) {
+            b = active[t];
+            key = block_key[b];
+            now = key <= threshold;
+            later = !now;
+        }
+        if (now) block_key[b] = kFltMaxBits;
+        if (later) (void)atomicExch(&block_tag[b], tag);
+        const unsigned long long bn = __ballot(now), bl = __ballot(later);
+        uint32_t base_n = 0, base_l = 0;
+        if (lane == 0) {
+            if (bn) base_n = atomicAdd(&cur->proc, (uint32_t)__popcll(bn));
+            if (bl) base_l = atomicAdd(&nxt->count, (uint32_t)__popcll(bl));
+        }
+        base_n = (uint32_t)__shfl((int)base_n, 0);
+        base_l = (uint32_t)__shfl((int)base_l, 0);
+        if (now) proc[base_n + (uint32_t)__popcll(bn & below)] = b;
+        if (later) next[base_l + (uint32_t)__popcll(bl & below)] = b;
+    }
 }
 
-// Smallest key over the next list (its length st[0] is read on the device); one atomic per
-// workgroup.
+// Smallest key over the next list (length nxt->count, read on the device), one atomic per
+// workgroup; workgroup 0 also adds the round's block visits to *visits.
 __global__ void __launch_bounds__(256) next_min_key(const uint32_t* __restrict__ next,
-                                                    const uint32_t* __restrict__ block_key, uint32_t* __restrict__ st) {
+                                                    const uint32_t* __restrict__ block_key,
+                                                    RoundState* __restrict__ nxt, const RoundState* __restrict__ cur,
+                                                    unsigned long long* __restrict__ visits) {
     __shared__ uint32_t part[4];
-    const uint32_t n = st[0];
+    const uint32_t n = nxt->count;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *visits += cur->proc;
     uint32_t m = 0x7F800000u;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         m = min(m, block_key[next[i]]);
@@ -165,12 +181,12 @@ __global__ void __launch_bounds__(256) next_min_key(const uint32_t* __restrict__
     __syncthreads();
     if (threadIdx.x == 0) {
         m = min(min(part[0], part[1]), min(part[2], part[3]));
-        if (m != 0x7F800000u) atomicMin(&st[1], m);
+        if (m != 0x7F800000u) atomicMin(&nxt->kmin, m);
     }
 }
 
-// One wave relaxes one block of this round's processing list (length st[2]) at a time, striding
-// over the list by the grid size.  Lane l owns the z-column (x, y) = (l / 8, l % 8) of the block
+// One wave relaxes one block of this round's processing list (length cur->proc) at a time,
+// striding over the list by the grid size.  Lane l owns the z-column (x, y) = (l / 8, l % 8) of the block
 // and keeps its 8 values in registers; a sweep walks every column along z (up and down in turn),
 // so a value crosses the whole column in one sweep and one column per sweep sideways.  Sweeps
 // repeat until one changes nothing.  The block and a one-voxel halo sit in LDS with +inf for
@@ -223,7 +239,8 @@ template <bool D3Q7>
 __global__ void __launch_bounds__(64) relax_blocks(float* __restrict__ dist, uint32_t* __restrict__ cav, uint32_t id,
                                                    const uint32_t* __restrict__ proc, uint32_t* __restrict__ next,
                                                    uint32_t* __restrict__ block_tag, uint32_t tag,
-                                                   uint32_t* __restrict__ block_key, uint32_t* __restrict__ st,
+                                                   uint32_t* __restrict__ block_key,
+                                                   const RoundState* __restrict__ cur, RoundState* __restrict__ nxt,
                                                    unsigned long long* __restrict__ part_count,
                                                    uint32_t* __restrict__ part_walls, int X, int Y, int Z, int NBX,
                                                    int NBY, int NBZ, unsigned long long src) {
@@ -232,12 +249,12 @@ __global__ void __launch_bounds__(64) relax_blocks(float* __restrict__ dist, uin
     const int lane = threadIdx.x;
     const int cx = lane >> 3, cy = lane & 7;
     const int col = (cx + 1) * kH + cy + 1;  // the lane's column within a z-layer of sd
-    const uint32_t count = st[2];
+    const uint32_t count = cur->proc;
     uint32_t npush = 0;                      // entries in pushbuf (uniform)
     uint32_t reached = 0, walls = 0;         // this lane's voxels new to the cavity, and their planes
     auto flush = [&]() {
         uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(&st[0], npush);
+        if (lane == 0) base = atomicAdd(&nxt->count, npush);
         base = (uint32_t)__shfl((int)base, 0);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -397,7 +414,9 @@ void run_geodesic(pma_ctx* c) {
     uint32_t* fp = c->buf<uint32_t>("geo_active_proc", NB);
     uint32_t* block_tag = c->buf<uint32_t>("geo_block_tag", NB);
     uint32_t* block_key = c->buf<uint32_t>("geo_block_key", NB);
-    uint32_t* st = c->buf<uint32_t>("geo_round_state", 4);
+    RoundState* rs = c->buf<RoundState>("geo_round_state", 2);
+    unsigned long long* visits = c->buf<unsigned long long>("geo_visits", 1);
+    PMA_HIP(hipMemsetAsync(visits, 0, sizeof(unsigned long long), c->stream));
     unsigned long long* key = c->buf<unsigned long long>("geo_key", 1);
     unsigned long long* vcount = c->buf<unsigned long long>("geo_vcount", 1);
     uint32_t* walls = c->buf<uint32_t>("geo_walls", 1);
@@ -418,7 +437,6 @@ void run_geodesic(pma_ctx* c) {
     c->cavities.clear();
     int half = 0;
     uint32_t id = 0, round = 0;
-    uint64_t relaxations = 0;
     while (true) {
         ++half;
         int start[3];
@@ -458,43 +476,43 @@ void run_geodesic(pma_ctx* c) {
         upload(c, block_key + b0, &zero, sizeof zero);
         PMA_HIP(hipMemsetAsync(part_count, 0, kRelaxGrid * sizeof(unsigned long long), c->stream));
         PMA_HIP(hipMemsetAsync(part_walls, 0, kRelaxGrid * sizeof(uint32_t), c->stream));
-        uint32_t nf = 1, kmin = 0;
-        uint32_t* cur = fa;
-        uint32_t* nxt = fb;
-        bool first = true;
-        while (nf > 0) {
-            const float thr = __builtin_bit_cast(float, kmin) + kDelta;
-            const uint32_t threshold = std::isfinite(thr) ? __builtin_bit_cast(uint32_t, thr) : 0xFFFFFFFFu;
-            round_init<<<1, 1, 0, c->stream>>>(st);
+        // round 0 processes the start block (key 0); the host checks for an empty list after
+        // rounds 1, 2, 4, 8 and then every 8 rounds, and never waits between the others
+        const RoundState start_state{1u, 0u, 0u, 0u};
+        upload(c, rs, &start_state, sizeof start_state);
+        uint32_t r = 0, check_at = 1, step = 1;
+        while (true) {
+            RoundState* cs = rs + (r & 1);
+            RoundState* ns = rs + ((r + 1) & 1);
+            uint32_t* cur = (r & 1) ? fb : fa;
+            uint32_t* nxt = (r & 1) ? fa : fb;
+            round_init<<<1, 1, 0, c->stream>>>(cs, ns);
             PMA_LAUNCHED(c);
-            partition_active<<<grid_for(nf, 256), 256, 0, c->stream>>>(cur, nf, block_key, threshold, fp, nxt,
-                                                                       block_tag, round, st);
+            partition_active<<<std::min<size_t>(grid_for(NB, 256), 1024u), 256, 0, c->stream>>>(
+                cur, cs, block_key, fp, nxt, block_tag, round, ns);
             PMA_LAUNCHED(c);
             {
                 PMA_TIMED(c, "relax_blocks");
-                const unsigned long long src = first ? (unsigned long long)si : ~0ull;
-                const uint32_t grid = std::min(nf, kRelaxGrid);
+                const unsigned long long src = r == 0 ? (unsigned long long)si : ~0ull;
                 if (d3q7)
-                    relax_blocks<true><<<grid, 64, 0, c->stream>>>(dist, cav, id, fp, nxt, block_tag, round, block_key,
-                                                                    st, part_count, part_walls, X, Y, Z, NBX, NBY, NBZ,
-                                                                    src);
+                    relax_blocks<true><<<kRelaxGrid, 64, 0, c->stream>>>(dist, cav, id, fp, nxt, block_tag, round,
+                                                                         block_key, cs, ns, part_count, part_walls, X,
+                                                                         Y, Z, NBX, NBY, NBZ, src);
                 else
-                    relax_blocks<false><<<grid, 64, 0, c->stream>>>(dist, cav, id, fp, nxt, block_tag, round,
-                                                                     block_key, st, part_count, part_walls, X, Y, Z,
-                                                                     NBX, NBY, NBZ, src);
+                    relax_blocks<false><<<kRelaxGrid, 64, 0, c->stream>>>(dist, cav, id, fp, nxt, block_tag, round,
+                                                                          block_key, cs, ns, part_count, part_walls, X,
+                                                                          Y, Z, NBX, NBY, NBZ, src);
                 PMA_LAUNCHED(c);
             }
-            next_min_key<<<256, 256, 0, c->stream>>>(nxt, block_key, st);
+            next_min_key<<<256, 256, 0, c->stream>>>(nxt, block_key, ns, cs, visits);
             PMA_LAUNCHED(c);
             ++round;
-            first = false;
-            uint32_t* h = static_cast<uint32_t*>(c->host_staging(3 * sizeof(uint32_t)));
-            PMA_HIP(hipMemcpyAsync(h, st, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-            PMA_HIP(hipStreamSynchronize(c->stream));
-            nf = h[0];
-            kmin = h[1];
-            relaxations += h[2];
-            std::swap(cur, nxt);
+            ++r;
+            if (r == check_at) {
+                if (read_scalar(c, &ns->count) == 0) break;
+                step = std::min(step * 2, 8u);
+                check_at += step;
+            }
         }
         cavity_totals<<<1, 256, 0, c->stream>>>(part_count, part_walls, kRelaxGrid, vcount, walls);
         PMA_LAUNCHED(c);
@@ -515,7 +533,7 @@ void run_geodesic(pma_ctx* c) {
     }
     c->have_cavities = true;
     c->valid[PMA_VOL_SPD] = c->valid[PMA_VOL_CAVITY] = true;
-    c->stage[PMA_STAGE_GEODESIC].work_items = relaxations;
+    c->stage[PMA_STAGE_GEODESIC].work_items = read_scalar(c, visits);
 }
 
 }  // namespace pma

@@ -38,6 +38,7 @@ typedef Dim3 dim3;
 inline thread_local Dim3 threadIdx, blockIdx;
 inline Dim3 blockDim, gridDim;
 using hipStream_t = void*;
+using std::isfinite;
 struct uint2 { uint32_t x, y; };
 struct alignas(16) uint4 { uint32_t x, y, z, w; };
 inline uint2 make_uint2(uint32_t a, uint32_t b) { return uint2{a, b}; }

@@ -74,7 +74,9 @@ def _build(tmp_path_factory, flags, tag):
 @pytest.fixture(scope="session")
 def hostemu(tmp_path_factory):
     """Sanitized build: every access is bounds-checked."""
-    return _build(tmp_path_factory, SANITIZED, "asan")
+    # 64 relaxing waves per launch: each round launches the whole grid, and the emulation runs
+    # every workgroup, even those with nothing to do
+    return _build(tmp_path_factory, SANITIZED + ["-DPMA_GEO_RELAX_GRID=64u"], "asan")
 
 
 @pytest.fixture(scope="session")
