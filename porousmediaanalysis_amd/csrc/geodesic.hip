@@ -270,14 +270,22 @@ __global__ void __launch_bounds__(64) relax_blocks(float* __restrict__ dist, uin
         const int z0 = bz * kB - 1, x0 = bx * kB - 1, y0 = by * kB - 1;
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // the previous block's LDS use is over
         __builtin_amdgcn_wave_barrier();
-        for (int k = lane; k < kHalo; k += 64) {
+        // all loads first, then all stores, so that the loads are in flight together
+        constexpr int kStage = (kHalo + 63) / 64;
+        float staged[kStage];
+#pragma unroll
+        for (int j = 0; j < kStage; ++j) {
+            const int k = lane + 64 * j;
             const int lz = k / (kH * kH), lx = (k / kH) % kH, ly = k % kH;
             const int z = z0 + lz, x = x0 + lx, y = y0 + ly;
-            const bool in = z >= 0 && z < Z && x >= 0 && x < X && y >= 0 && y < Y;
+            const bool in = k < kHalo && z >= 0 && z < Z && x >= 0 && x < X && y >= 0 && y < Y;
             const size_t gi = ((size_t)z * X + x) * Y + y;
             const float v = in ? dist[gi] : -1.0f;
-            sd[k] = (in && gi == src) ? 0.0f : (v < 0.0f ? INFINITY : v);
+            staged[j] = (in && gi == src) ? 0.0f : (v < 0.0f ? INFINITY : v);
         }
+#pragma unroll
+        for (int j = 0; j < kStage; ++j)
+            if (lane + 64 * j < kHalo) sd[lane + 64 * j] = staged[j];
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
 
