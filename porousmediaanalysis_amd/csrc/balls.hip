@@ -326,7 +326,7 @@ __device__ __forceinline__ int isqrt_small(int v) {
 enum ChordRoot : int {
     kRootGeneral = 0,  // isqrt_fast (any r2)
     kRootSmall = 1,    // isqrt_small (r2 < 2^22)
-    kRootTable = 2,    // a table of floor(sqrt(v)) in LDS (r2 < kRootTableSize)
+    kRootTable = 2,    // a table of floor(sqrt(v)) in LDS (r2 < kRootTableSize - 1)
 };
 constexpr int kRootTableSize = 2048;
 
@@ -354,7 +354,8 @@ __global__ void __launch_bounds__(64) cover_tiles(const Ball* __restrict__ cell_
                                                   int Z, Cells g, uint32_t cell_base, uint32_t slab_cells,
                                                   uint32_t* __restrict__ error_flag) {
     __shared__ W win[TZ * TX * TY];
-    __shared__ uint8_t root_tab[Root == kRootTable ? kRootTableSize : 1];
+    // root_tab[v + 1] = floor(sqrt(v)) and root_tab[0] = -1 (a row the ball misses)
+    __shared__ int8_t root_tab[Root == kRootTable ? kRootTableSize : 1];
     const int lane = threadIdx.x;
     const uint32_t per_cell = (uint32_t)(g.gz * g.gx);
     const uint32_t k = blockIdx.x / kXcds, sub = k % per_cell;
@@ -376,7 +377,9 @@ __global__ void __launch_bounds__(64) cover_tiles(const Ball* __restrict__ cell_
         else return r;
     };
     if (Root == kRootTable)
-        for (int v = lane; v < kRootTableSize; v += 64) root_tab[v] = (uint8_t)isqrt_small(v);
+        for (int v = lane; v < kRootTableSize; v += 64) root_tab[v] = v == 0 ? (int8_t)-1 : (int8_t)isqrt_small(v - 1);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
 
     // coverage of this lane's row: solid voxels and voxels outside the image count as covered
     uint64_t cov = ~0ull;
@@ -416,7 +419,8 @@ __global__ void __launch_bounds__(64) cover_tiles(const Ball* __restrict__ cell_
         const int xlo = tx * TX + __builtin_ctz(xs), xhi = tx * TX + 31 - __builtin_clz(xs);
         const int ylo = y0 + __builtin_ctzll(ys), yhi = y0 + 63 - __builtin_clzll(ys);
 
-        const Ball mb = ball_of(next);  // this batch; the next one is loaded while it is processed
+        const Raw cur = next;
+        const Ball mb = ball_of(cur);  // this batch; the next one is loaded while it is processed
         if (base + 64 + lane < en) next = raw_at(base + 64 + lane);
         bool reach = false;
         if (base + lane < en) {
@@ -428,20 +432,29 @@ __global__ void __launch_bounds__(64) cover_tiles(const Ball* __restrict__ cell_
         while (todo) {
             const int j = __builtin_ctzll(todo);
             todo &= todo - 1;
-            // the ball's fields are uniform: read them into scalar registers
-            const int bx = __builtin_amdgcn_readlane(mb.x, j), by = __builtin_amdgcn_readlane(mb.y, j);
-            const int bz = __builtin_amdgcn_readlane(mb.z, j);
-            const int r2 = __builtin_amdgcn_readlane((int)mb.r2, j);
+            // the ball's fields are uniform: read them into scalar registers (a packed ball takes
+            // two reads and is unpacked on the scalar unit)
+            int bx, by, bz, r2;
+            if constexpr (Packed) {
+                const uint64_t pv = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(cur >> 32), j) << 32 |
+                                    (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)cur, j);
+                const Ball u = unpack_ball(pv, oz, ox, y0, off);
+                bx = u.x;
+                by = u.y;
+                bz = u.z;
+                r2 = (int)u.r2;
+            } else {
+                bx = __builtin_amdgcn_readlane(mb.x, j);
+                by = __builtin_amdgcn_readlane(mb.y, j);
+                bz = __builtin_amdgcn_readlane(mb.z, j);
+                r2 = __builtin_amdgcn_readlane((int)mb.r2, j);
+            }
             const int dz = z - bz, dx = x - bx;
             const int rem = r2 - (__mul24(dz, dz) + __mul24(dx, dx));  // |dz|, |dx| < 2^16
             // chord [by - h, by + h] of the ball on this lane's row; h = -1 when the row misses it
             int h;
-            if (Root == kRootTable) {
-                const int t = root_tab[max(rem, 0)];  // rem <= r2 < kRootTableSize
-                h = rem < 0 ? -1 : t;
-            } else {
-                h = rem < 0 ? -1 : (Root == kRootSmall ? isqrt_small(rem) : (int)isqrt_fast((uint32_t)rem));
-            }
+            if (Root == kRootTable) h = root_tab[max(rem, -1) + 1];  // rem <= r2 < kRootTableSize - 1
+            else h = rem < 0 ? -1 : (Root == kRootSmall ? isqrt_small(rem) : (int)isqrt_fast((uint32_t)rem));
             const int lo = max(by - h - y0, 0), hi = min(by + h - y0, 63);
             uint64_t nw = lo <= hi ? (((~0ull) >> (63 - (hi - lo))) << lo) & ~cov : 0ull;
             if (__ballot(nw != 0) == 0) continue;
@@ -673,7 +686,7 @@ void run_containing_balls(pma_ctx* c) {
             const uint32_t blocks = (slab_cells + kXcds - 1) / kXcds * kXcds * per_cell;
             // chord widths: a table below kRootTableSize, the short integer root below its
             // exactness bound, else the general one
-            const int root = max_e < (uint32_t)std::min<uint64_t>(kRootTableSize, PMA_CB_SMALL_R2) ? kRootTable
+            const int root = max_e < (uint32_t)std::min<uint64_t>(kRootTableSize - 1, PMA_CB_SMALL_R2) ? kRootTable
                              : max_e < PMA_CB_SMALL_R2                                            ? kRootSmall
                                                                                                    : kRootGeneral;
             const bool w16 = longest <= PMA_CB_WIN16_MAX;
