@@ -18,7 +18,7 @@ import pytest
 
 import oracle
 from tests import example_fixture as ex
-from tests import volumes
+from tests import removal_inputs, volumes
 from tests.volume_utils import canonical_relabel, sha256_array
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -258,3 +258,14 @@ def test_example_partitions(hostemu_fast, tmp_path):
     gw = emu_watershed(hostemu, tmp_path, cr, max_e, o2[1], False)
     assert_same(gw, oracle.watershed(cr, max_e, o2[1], False), "watershed")
     assert sha256_array(canonical_relabel(gw[0])) == ex.digests()["ptw_partition"]
+
+
+@pytest.mark.parametrize("name", sorted(removal_inputs.CASES))
+@pytest.mark.parametrize("d3q7", [False, True])
+@pytest.mark.parametrize("every", [2, 3])
+def test_remove_crafted(hostemu, tmp_path, name, d3q7, every):
+    """Empty-pore removal with some pore volumes set to 0 (pores dropped, throats folded and merged)."""
+    pt, ptxt, ktxt = removal_inputs.removal_input(name, d3q7, every)
+    o = oracle.remove_empty_pores(pt, ptxt, ktxt)
+    assert o[1].count("\n") < ptxt.count("\n")
+    assert_same(emu_remove(hostemu, tmp_path, pt, ptxt, ktxt), o, "empty-pore removal")
