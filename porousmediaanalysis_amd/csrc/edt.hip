@@ -72,7 +72,7 @@ __global__ void __launch_bounds__(256) edt_pass_y(const uint32_t* __restrict__ g
 // Pass Z on chunks of kChunkZ planes: a block stages 64 columns (lanes along y) of its chunk and
 // kHaloZ planes either side in LDS, and each of its 4 waves takes every 4th plane.  A voxel
 // whose search reaches beyond the staged planes reads the rest from global memory.
-constexpr int kChunkZ = 128, kHaloZ = 32, kStageZ = kChunkZ + 2 * kHaloZ;
+constexpr int kChunkZ = 64, kHaloZ = 32, kStageZ = kChunkZ + 2 * kHaloZ;
 
 __global__ void __launch_bounds__(256) edt_pass_z(const uint32_t* __restrict__ h,
                                                   uint32_t* __restrict__ e, int X, int Y, int Z,
