@@ -7,6 +7,9 @@
 //                     out: CR2, CX, CY, CZ u32 | mask u8; text: maximum-ball count
 //   hostemu geodesic  <in> <out_volumes> <out_text>  in: X Y Z d3q7 | solid u8;
 //                     out: distance f32 | cavity u32; text: cavityDescriptions.txt
+//   hostemu pores_seq <list> - -                     list: lines "<in> <out_volume> <out_text>" of
+//                     pore-stage cases run one after another on ONE context, whose scratch
+//                     buffers keep the previous case's contents (as in a process that runs many)
 #include <fstream>
 #include <sstream>
 
@@ -65,30 +68,58 @@ void write_outputs(pma_ctx& c, pma_volume w, const std::vector<pma::PoreRow>& po
 
 }  // namespace
 
+void read_pores_input(pma_ctx& c, FILE* f) {
+    int32_t h[4] = {0, 0, 0, 0};
+    read_into(f, h, 4);
+    c.dims = {h[0], h[1], h[2]};
+    c.n = (size_t)h[0] * h[1] * h[2];
+    c.stencil = h[3] ? PMA_STENCIL_D3Q7 : PMA_STENCIL_D3Q27;
+    c.ensure_volume(PMA_VOL_EDT2);
+    c.ensure_volume(PMA_VOL_CR2);
+    c.ensure_volume(PMA_VOL_MASK);
+    read_into(f, c.v<uint32_t>(PMA_VOL_EDT2), c.n);
+    read_into(f, c.v<uint32_t>(PMA_VOL_CR2), c.n);
+    read_into(f, c.v<uint8_t>(PMA_VOL_MASK), c.n);
+    c.valid[PMA_VOL_EDT2] = c.valid[PMA_VOL_CR2] = c.valid[PMA_VOL_MASK] = true;
+}
+
 int main(int argc, char** argv) {
     if (argc != 5) return 2;
     const std::string stage = argv[1];
+    if (stage == "pores_seq") {
+        std::ifstream list(argv[2]);
+        pma_ctx c;
+        std::string in, vol, txt;
+        while (list >> in >> vol >> txt) {
+            FILE* f = std::fopen(in.c_str(), "rb");
+            if (!f) return 2;
+            try {
+                read_pores_input(c, f);
+                pma::run_pores(&c);
+                write_outputs(c, PMA_VOL_PT, c.pores, c.throats, vol.c_str(), txt.c_str());
+            } catch (const std::exception& ex) {
+                std::fprintf(stderr, "error: %s (%s)\n", ex.what(), in.c_str());
+                return 1;
+            }
+            std::fclose(f);
+        }
+        return 0;
+    }
     FILE* f = std::fopen(argv[2], "rb");
     if (!f) return 2;
     pma_ctx c;
     try {
-        if (stage == "pores" || stage == "watershed") {
+        if (stage == "pores") {
+            read_pores_input(c, f);
+            pma::run_pores(&c);
+            write_outputs(c, PMA_VOL_PT, c.pores, c.throats, argv[3], argv[4]);
+        } else if (stage == "watershed") {
             int32_t h[5] = {0, 0, 0, 0, 0};
-            read_into(f, h, stage == "pores" ? 4 : 5);
+            read_into(f, h, 5);
             c.dims = {h[0], h[1], h[2]};
             c.n = (size_t)h[0] * h[1] * h[2];
             c.stencil = h[3] ? PMA_STENCIL_D3Q7 : PMA_STENCIL_D3Q27;
-            if (stage == "pores") {
-                c.ensure_volume(PMA_VOL_EDT2);
-                c.ensure_volume(PMA_VOL_CR2);
-                c.ensure_volume(PMA_VOL_MASK);
-                read_into(f, c.v<uint32_t>(PMA_VOL_EDT2), c.n);
-                read_into(f, c.v<uint32_t>(PMA_VOL_CR2), c.n);
-                read_into(f, c.v<uint8_t>(PMA_VOL_MASK), c.n);
-                c.valid[PMA_VOL_EDT2] = c.valid[PMA_VOL_CR2] = c.valid[PMA_VOL_MASK] = true;
-                pma::run_pores(&c);
-                write_outputs(c, PMA_VOL_PT, c.pores, c.throats, argv[3], argv[4]);
-            } else {
+            {
                 c.max_e = (uint32_t)h[4];
                 c.have_edt_stats = true;
                 c.ensure_volume(PMA_VOL_CR2);

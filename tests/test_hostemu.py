@@ -269,3 +269,30 @@ def test_remove_crafted(hostemu, tmp_path, name, d3q7, every):
     o = oracle.remove_empty_pores(pt, ptxt, ktxt)
     assert o[1].count("\n") < ptxt.count("\n")
     assert_same(emu_remove(hostemu, tmp_path, pt, ptxt, ktxt), o, "empty-pore removal")
+
+
+def test_pores_one_context(hostemu, tmp_path):
+    """The pore stage over the first GPU pore test cases in their order (the faulting run's prefix
+    and the next case), on one context: scratch buffers keep the previous case's contents, as in
+    one process that runs them all.  (The whole sequence, both stencils, also passes: ~10 min.)"""
+    from tests import test_gpu_pores as gp
+
+    order = [(False, name) for name in sorted(gp.CASES)][:4]
+    lines, expected = [], []
+    for k, (d3q7, name) in enumerate(order):
+        solid = gp.CASES[name]
+        e, max_e, cr, mask = front(solid)
+        Z, X, Y = e.shape
+        inp = tmp_path / f"in{k}.bin"
+        inp.write_bytes(np.array([X, Y, Z, int(d3q7)], np.int32).tobytes() + e.astype(np.uint32).tobytes() +
+                        cr.astype(np.uint32).tobytes() + mask.astype(np.uint8).tobytes())
+        lines.append(f"{inp} {tmp_path / f'out{k}.bin'} {tmp_path / f'out{k}.txt'}")
+        expected.append(oracle.pores(mask, e, cr, d3q7))
+    (tmp_path / "list.txt").write_text("\n".join(lines) + "\n")
+    r = subprocess.run([hostemu, "pores_seq", str(tmp_path / "list.txt"), "-", "-"], capture_output=True, text=True,
+                       env=ENV, timeout=1200)
+    assert r.returncode == 0, f"pores_seq: rc {r.returncode}\n{r.stdout[-2000:]}\n{r.stderr[-6000:]}"
+    for k, (d3q7, name) in enumerate(order):
+        pores, throats = (tmp_path / f"out{k}.txt").read_text().split("@@\n")
+        pt = np.fromfile(tmp_path / f"out{k}.bin", np.uint32).reshape(gp.CASES[name].shape)
+        assert_same((pt, pores, throats), expected[k], f"pores {name} d3q7={d3q7}")
