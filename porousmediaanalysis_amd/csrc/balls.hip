@@ -10,11 +10,12 @@
 //      edge and corner offsets d_k.  A centre c is skipped when a stencil neighbour q has
 //      e(q) > e(c) and far_k(e(c)) <= e(q): then B(c) is a subset of B(q), every voxel of B(c)
 //      has a ball of larger r2, and c can never be taken (exact pruning).
-//   2. the surviving centres are sorted by (r2 descending, scan ascending): their rank is the
-//      order in which the reference would let them win.
+//   2. the surviving centres, emitted column by column in scan order, are sorted stably by r2
+//      descending: their rank is the order in which the reference would let them win.
 //   3. every centre lists the pair cells (4 x 2 tiles of 8 x 8 x 64 voxels along z, x) its ball
-//      reaches; the (cell, rank) pairs are sorted stably by cell, so each cell sees its balls in
-//      rank order.
+//      reaches, as (cell, ball packed relative to the cell) pairs (or (cell, rank) pairs when a
+//      ball does not fit 64 bits); the pairs are sorted stably by cell, so each cell sees its
+//      balls in rank order.
 //   4. one wave per tile, walking its cell's balls and skipping those that cannot reach the tile's
 //      still-uncovered voxels: lane l owns the row (z, x) = (l / 8, l % 8) of the tile and a 64-bit mask
 //      of the row's voxels that are solid or already covered.  For each ball in rank order a lane
@@ -31,7 +32,7 @@ namespace {
 
 constexpr int TZ = 8, TX = 8, TY = 64;  // tile: 64 rows (z, x) of 64 voxels along y
 #ifndef PMA_CB_SLAB_PAIRS
-#define PMA_CB_SLAB_PAIRS (1ull << 30)  // most (tile, ball) pairs sorted at once
+#define PMA_CB_SLAB_PAIRS (1ull << 30)  // most (cell, ball) pairs sorted at once
 #endif
 #ifndef PMA_CB_GZ
 #define PMA_CB_GZ 4  // tiles per pair cell along z (4 x 2 measured best of 1..4 x 1..4 at 1024^3)

@@ -5,15 +5,15 @@
 // in a and never below a, so the result is the least fixed point
 //     d(v) = min(d(v), min_u fl(d(u) + w)),  i.e. the minimum over paths of the left-fold
 // float32 sum, whatever the settling order.  It is computed here by block-synchronous label
-// correcting: the volume is cut into 8 x 8 x 8 blocks; an active block stages itself and a
-// one-voxel halo in LDS (solid = -1, as in the distance volume), relaxes its own voxels by
-// pulling from their neighbours until nothing changes, writes back what improved, and
-// activates the neighbour blocks across any face, edge or corner where a border voxel
-// improved.  An active block's key is the smallest improved border distance that woke it; a
-// round processes the active blocks whose key is within kDelta of the smallest key and
-// carries the others, so blocks are visited roughly in distance order (fewer re-visits).
-// Rounds repeat until no block is active.  Every add is one correctly rounded fp32 add (no
-// contraction).
+// correcting: the volume is cut into 8 x 8 x 8 blocks; an active block is taken by one wave,
+// which stages it and a one-voxel halo in LDS (+inf for solid voxels and voxels outside the
+// image), relaxes its own voxels by pulling from their neighbours until nothing changes, writes
+// back what improved, and activates the neighbour blocks across any face, edge or corner where
+// a border voxel improved.  An active block's key is the smallest improved border distance
+// that woke it; a round processes the active blocks whose key is within kDelta of the smallest
+// key and carries the others, so blocks are visited roughly in distance order (fewer
+// re-visits).  Rounds run back to back on the device until no block is active.  Every add is
+// one correctly rounded fp32 add (no contraction).
 //
 // Start voxels follow SelectStartingPosition / FindVoidPixelOnBoxBoundaries (:396-511):
 // the centre min((int)(0.5 * size + 0.5), size - 1) if void and unreached, else the first
