@@ -119,12 +119,6 @@ __global__ void fill_key(uint32_t* __restrict__ p, uint32_t v, size_t n) {
     if (i < n) p[i] = v;
 }
 
-__global__ void round_init(RoundState* __restrict__ cur, RoundState* __restrict__ nxt) {
-    nxt->count = 0;
-    nxt->kmin = 0x7F800000u;  // +inf
-    cur->proc = 0;
-}
-
 // Splits the active list: keys within kDelta of its smallest key go to `proc` (their key is
 // cleared, so that activations during the round start it afresh), the others are carried into
 // the next list (tagged so that a new activation does not list them twice).  Grid-stride over
@@ -165,14 +159,21 @@ __global__ void __launch_bounds__(256) partition_active(const uint32_t* __restri
 }
 
 // Smallest key over the next list (length nxt->count, read on the device), one atomic per
-// workgroup; workgroup 0 also adds the round's block visits to *visits.
+// workgroup.  Workgroup 0 also adds the round's block visits to *visits and prepares the two
+// states for the next round, in which they swap roles: cur (read by nothing after this
+// kernel) becomes the empty next list, and nxt's processing count starts at 0.
 __global__ void __launch_bounds__(256) next_min_key(const uint32_t* __restrict__ next,
                                                     const uint32_t* __restrict__ block_key,
-                                                    RoundState* __restrict__ nxt, const RoundState* __restrict__ cur,
+                                                    RoundState* __restrict__ nxt, RoundState* __restrict__ cur,
                                                     unsigned long long* __restrict__ visits) {
     __shared__ uint32_t part[4];
     const uint32_t n = nxt->count;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *visits += cur->proc;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *visits += cur->proc;
+        cur->count = 0;
+        cur->kmin = 0x7F800000u;  // +inf
+        nxt->proc = 0;
+    }
     uint32_t m = 0x7F800000u;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         m = min(m, block_key[next[i]]);
@@ -486,17 +487,15 @@ void run_geodesic(pma_ctx* c) {
         PMA_HIP(hipMemsetAsync(part_walls, 0, kRelaxGrid * sizeof(uint32_t), c->stream));
         // round 0 processes the start block (key 0); the host checks for an empty list after
         // rounds 1, 2, 4, 8 and then every 8 rounds, and never waits between the others
-        const RoundState start_state{1u, 0u, 0u, 0u};
-        upload(c, rs, &start_state, sizeof start_state);
+        const RoundState start_states[2] = {{1u, 0u, 0u, 0u}, {0u, 0x7F800000u, 0u, 0u}};
+        upload(c, rs, start_states, sizeof start_states);
         uint32_t r = 0, check_at = 1, step = 1;
         while (true) {
             RoundState* cs = rs + (r & 1);
             RoundState* ns = rs + ((r + 1) & 1);
             uint32_t* cur = (r & 1) ? fb : fa;
             uint32_t* nxt = (r & 1) ? fa : fb;
-            round_init<<<1, 1, 0, c->stream>>>(cs, ns);
-            PMA_LAUNCHED(c);
-            partition_active<<<std::min<size_t>(grid_for(NB, 256), 1024u), 256, 0, c->stream>>>(
+            partition_active<<<std::min<size_t>(grid_for(NB, 256), 256u), 256, 0, c->stream>>>(
                 cur, cs, block_key, fp, nxt, block_tag, round, ns);
             PMA_LAUNCHED(c);
             {
