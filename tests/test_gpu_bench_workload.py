@@ -1,0 +1,42 @@
+"""The benchmark's workload at 192^3: the GPU Gaussian-field generator (sigma 8, porosity 0.65),
+then EDT, containing balls and geodesic distances with cavities, each bit-exact against the
+oracle.  At this size the containing-ball stage takes the paths the 1024^3 run takes (balls
+packed in the pair sort, chord widths from the LDS table, 16-bit winner slots) and the
+geodesic runs a few hundred chained rounds."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+import porousmediaanalysis_amd as pma
+from tests.gpu_common import assert_front_equal, log
+
+pytestmark = pytest.mark.gpu
+
+THREADS = min(os.cpu_count() or 1, 16)
+
+
+@pytest.mark.parametrize("seed", [20261203, 7])
+def test_gaussian_192_matches_oracle(seed):
+    n = 192
+    log(f"bench workload {n}^3 seed {seed}")
+    with pma.Context(0) as ctx:
+        ctx.generate_gaussian((n, n, n), 8.0, 0.65, seed)
+        solid = ctx.read(pma.VOL_SOLID)
+        ctx.edt()
+        ctx.containing_balls()
+        ctx.geodesic()
+        m, nmax = ctx.statistics()
+        g = dict(e=ctx.read(pma.VOL_EDT2), max_e=m, hist=ctx.histogram(), cr=ctx.read(pma.VOL_CR2),
+                 cx=ctx.read(pma.VOL_CX), cy=ctx.read(pma.VOL_CY), cz=ctx.read(pma.VOL_CZ),
+                 mask=ctx.read(pma.VOL_MASK), n_max=nmax)
+        dist, cav, text = ctx.read(pma.VOL_SPD), ctx.read(pma.VOL_CAVITY), ctx.text(pma.TEXT_CAVITIES)
+    assert abs((solid == 0).mean() - 0.65) < 1e-3
+    e, max_e, hist = oracle.edt(solid, threads=THREADS)
+    cr, cx, cy, cz, mask, n_max = oracle.containing_balls(e, threads=THREADS)
+    assert_front_equal(g, dict(e=e, max_e=max_e, hist=hist, cr=cr, cx=cx, cy=cy, cz=cz, mask=mask, n_max=n_max))
+    odist, ocav, otext = oracle.geodesic(solid)
+    assert np.array_equal(dist.view(np.uint32), odist.view(np.uint32)), "distances differ"
+    assert np.array_equal(cav, ocav), "cavity ids differ"
+    assert text == otext
