@@ -1,0 +1,104 @@
+"""The benchmark's workload at its full size, 1024^3 (GPU Gaussian field, sigma 8, porosity 0.65,
+D3Q27), checked through properties that hold at any size, on a seeded sample of voxels:
+
+* EDT: zero exactly on solid voxels; at sampled void voxels the value is the smallest squared
+  distance to a solid voxel or to the virtual planes x = -X, x = 2X - 1 (brute force over the
+  cube that can hold it).
+* Containing balls: the voxel's ball covers it, its centre's EDT is the ball's r2, r2 >= the
+  voxel's own EDT, the centre is marked 255, and the 255 count is the maximum-ball count.
+* Geodesic distances: at sampled reached void voxels other than a cavity start, the distance is
+  the minimum over the 26 neighbours of fl32(d(u) + w) with w = 1, sqrt 2, sqrt 3 (the fixpoint
+  equation, evaluated in float32); solid voxels hold -1.
+
+The bit-exact comparison with the oracle runs at 192^3 in test_gpu_bench_workload.py.
+"""
+import numpy as np
+import pytest
+
+import porousmediaanalysis_amd as pma
+from tests.gpu_common import log
+
+pytestmark = pytest.mark.gpu
+
+N = 1024
+SAMPLES = 3000
+
+
+@pytest.fixture(scope="module")
+def run():
+    log(f"full size {N}^3: generate and run EDT, containing balls, geodesic")
+    with pma.Context(0) as ctx:
+        ctx.generate_gaussian((N, N, N), 8.0, 0.65, 20261203)
+        ctx.edt()
+        ctx.containing_balls()
+        ctx.geodesic()
+        max_e, n_max = ctx.statistics()
+        out = {k: ctx.read(getattr(pma, k)) for k in
+               ["VOL_SOLID", "VOL_EDT2", "VOL_CR2", "VOL_CX", "VOL_CY", "VOL_CZ", "VOL_MASK", "VOL_SPD", "VOL_CAVITY"]}
+        out["max_e"], out["n_max"] = max_e, n_max
+        out["cavities"] = ctx.text(pma.TEXT_CAVITIES)
+    return out
+
+
+def sample_void(solid, rng):
+    idx = rng.integers(0, solid.size, SAMPLES * 3)
+    idx = idx[solid.reshape(-1)[idx] == 0][:SAMPLES]
+    return np.stack(np.unravel_index(idx, solid.shape), 1)  # (z, x, y)
+
+
+def test_edt(run):
+    solid, e = run["VOL_SOLID"], run["VOL_EDT2"]
+    assert np.array_equal(e == 0, solid != 0)
+    assert int(e.max()) == run["max_e"]
+    Z, X, Y = solid.shape
+    rng = np.random.default_rng(1)
+    for z, x, y in sample_void(solid, rng):
+        r = int(np.ceil(np.sqrt(int(e[z, x, y])))) + 1
+        z0, z1, x0, x1, y0, y1 = max(z - r, 0), min(z + r + 1, Z), max(x - r, 0), min(x + r + 1, X), \
+            max(y - r, 0), min(y + r + 1, Y)
+        sz, sx, sy = np.nonzero(solid[z0:z1, x0:x1, y0:y1])
+        d2 = (sz + z0 - z) ** 2 + (sx + x0 - x) ** 2 + (sy + y0 - y) ** 2
+        best = min(int(d2.min()) if d2.size else 1 << 62, (x + X) ** 2, (2 * X - 1 - x) ** 2)
+        assert int(e[z, x, y]) == best, f"EDT at {(z, x, y)}: {e[z, x, y]} vs {best}"
+
+
+def test_containing_balls(run):
+    solid, e, cr = run["VOL_SOLID"], run["VOL_EDT2"], run["VOL_CR2"]
+    cx, cy, cz, mask = run["VOL_CX"], run["VOL_CY"], run["VOL_CZ"], run["VOL_MASK"]
+    assert int((mask == 255).sum()) == run["n_max"]
+    rng = np.random.default_rng(2)
+    for z, x, y in sample_void(solid, rng):
+        bx, by, bz, r2 = int(cx[z, x, y]), int(cy[z, x, y]), int(cz[z, x, y]), int(cr[z, x, y])
+        assert (bx - x) ** 2 + (by - y) ** 2 + (bz - z) ** 2 <= r2
+        assert int(e[bz, bx, by]) == r2
+        assert r2 >= int(e[z, x, y])
+        assert mask[bz, bx, by] == 255
+    assert np.all(cr[solid != 0] == 0)
+
+
+def test_geodesic_fixpoint(run):
+    solid, dist = run["VOL_SOLID"], run["VOL_SPD"]
+    assert np.all(dist[solid != 0] == np.float32(-1.0))
+    Z, X, Y = solid.shape
+    w = [np.float32(1.0), np.sqrt(np.float32(2.0)), np.sqrt(np.float32(3.0))]
+    rng = np.random.default_rng(3)
+    checked = 0
+    for z, x, y in sample_void(solid, rng):
+        d = dist[z, x, y]
+        if d == np.float32(np.finfo(np.float32).max) or d == 0:
+            continue  # not reached, or a cavity's start voxel
+        best = np.float32(np.inf)
+        for dz in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                for dy in (-1, 0, 1):
+                    m = abs(dz) + abs(dx) + abs(dy)
+                    u = (z + dz, x + dx, y + dy)
+                    if m == 0 or not (0 <= u[0] < Z and 0 <= u[1] < X and 0 <= u[2] < Y):
+                        continue
+                    du = dist[u]
+                    if du < 0 or du == np.float32(np.finfo(np.float32).max):
+                        continue
+                    best = min(best, np.float32(du + w[m - 1]))
+        assert d == best, f"distance at {(z, x, y)}: {d} vs {best}"
+        checked += 1
+    assert checked > SAMPLES // 2
