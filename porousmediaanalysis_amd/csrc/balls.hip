@@ -529,10 +529,26 @@ constexpr CoverLaunch kCoverLaunch[2][3][2] = {
      {launch_cover<uint16_t, kRootTable, false>, launch_cover<uint16_t, kRootTable, true>}},
 };
 
+// Bytes of w equal to the byte b (b replicated in all four bytes): exact count of the zero
+// bytes of w ^ b (a zero byte, and only a zero byte, leaves 0x80 in its place).
+__device__ __forceinline__ uint32_t equal_bytes(uint32_t w, uint32_t b) {
+    const uint32_t t = w ^ b;
+    return (uint32_t)__popc(~(((t & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | t | 0x7F7F7F7Fu));
+}
+
+// Count of bytes equal to `value`, 16 bytes per thread and step (the buffer is 16-byte aligned).
 __global__ void __launch_bounds__(256) count_equal_u8(const uint8_t* __restrict__ v, size_t n, uint8_t value,
                                                       unsigned long long* __restrict__ out) {
     unsigned long long local = 0;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    const uint32_t b = 0x01010101u * value;
+    const size_t n16 = n / 16;
+    const uint4* v4 = reinterpret_cast<const uint4*>(v);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x) {
+        const uint4 q = v4[i];
+        local += equal_bytes(q.x, b) + equal_bytes(q.y, b) + equal_bytes(q.z, b) + equal_bytes(q.w, b);
+    }
+    for (size_t i = n16 * 16 + (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
         local += v[i] == value;
     for (int off = 32; off > 0; off >>= 1) local += __shfl_xor(local, off);
     if ((threadIdx.x & 63) == 0 && local) atomicAdd(out, local);

@@ -30,13 +30,24 @@ namespace {
 
 constexpr uint32_t kFltMaxBits = 0x7F7FFFFFu;
 
+// Four voxels per thread: solid -> distance -1 and cavity 0, void -> FLT_MAX and ~0.
 __global__ void geo_init(const uint8_t* __restrict__ solid, float* __restrict__ dist, uint32_t* __restrict__ cav,
                          size_t n) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    bool s = solid[i] != 0;
-    dist[i] = s ? -1.0f : FLT_MAX;
-    cav[i] = s ? 0u : 0xFFFFFFFFu;
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t i = q * 4;
+    if (i + 4 <= n) {
+        const uchar4 s = reinterpret_cast<const uchar4*>(solid)[q];
+        reinterpret_cast<float4*>(dist)[q] = make_float4(s.x ? -1.0f : FLT_MAX, s.y ? -1.0f : FLT_MAX,
+                                                         s.z ? -1.0f : FLT_MAX, s.w ? -1.0f : FLT_MAX);
+        reinterpret_cast<uint4*>(cav)[q] = make_uint4(s.x ? 0u : 0xFFFFFFFFu, s.y ? 0u : 0xFFFFFFFFu,
+                                                      s.z ? 0u : 0xFFFFFFFFu, s.w ? 0u : 0xFFFFFFFFu);
+    } else {
+        for (size_t k = i; k < n; ++k) {
+            const bool sk = solid[k] != 0;
+            dist[k] = sk ? -1.0f : FLT_MAX;
+            cav[k] = sk ? 0u : 0xFFFFFFFFu;
+        }
+    }
 }
 
 // Smallest scan key of voxel v among the shells h in [lo_h, hi_h), or ~0 if none.
@@ -414,7 +425,7 @@ void run_geodesic(pma_ctx* c) {
     float* dist = c->v<float>(PMA_VOL_SPD);
     uint32_t* cav = c->v<uint32_t>(PMA_VOL_CAVITY);
     const size_t n = c->n;
-    geo_init<<<grid_for(n, 256), 256, 0, c->stream>>>(solid, dist, cav, n);
+    geo_init<<<grid_for((n + 3) / 4, 256), 256, 0, c->stream>>>(solid, dist, cav, n);
     PMA_LAUNCHED(c);
     const int NBZ = (Z + kB - 1) / kB, NBX = (X + kB - 1) / kB, NBY = (Y + kB - 1) / kB;
     const size_t NB = (size_t)NBZ * NBX * NBY;
