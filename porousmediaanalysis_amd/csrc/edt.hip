@@ -18,13 +18,6 @@ namespace {
 
 constexpr int kLocalBins = 2048;
 
-__device__ __forceinline__ uint32_t isqrt_u32(uint32_t v) {
-    uint32_t s = (uint32_t)sqrtf((float)v);
-    while ((uint64_t)s * s > v) --s;
-    while ((uint64_t)(s + 1) * (s + 1) <= v) ++s;
-    return s;
-}
-
 __global__ void __launch_bounds__(256) edt_pass_x(const uint8_t* __restrict__ solid,
                                                   uint32_t* __restrict__ g, int X, int Y) {
     int y = blockIdx.x * blockDim.x + threadIdx.x;
@@ -95,9 +88,16 @@ __global__ void __launch_bounds__(256) edt_pass_y(const uint32_t* __restrict__ g
 }
 
 // Pass Z on chunks of kChunkZ planes: a block stages 64 columns (lanes along y) of its chunk and
-// kHaloZ planes either side in LDS, and each of its 4 waves takes every 4th plane.  A voxel
-// whose search reaches beyond the staged planes reads the rest from global memory.
+// kHaloZ planes either side in LDS (planes outside the volume hold kFarZ, which never wins), and
+// each of its 4 waves takes every 4th plane.  The search runs outward from z over k = 1, 2, ...,
+// taking min(h(z-k), h(z+k)) + k^2, and stops once k^2 >= best on every lane of the wave: no
+// plane at distance k or beyond can then improve best, so the result is the exact minimum over
+// the column, which equals the minimum over the reference's window |dz| <= floor(sqrt(h(z))).
+// k is the same on every lane (k^2 is a constant of the unrolled loop); lanes that still need
+// k > kHaloZ continue alone from global memory.
 constexpr int kChunkZ = 64, kHaloZ = 32, kStageZ = kChunkZ + 2 * kHaloZ;
+// Inputs of pass Z are below (2X)^2 <= 2^30 for X <= 16383, and h + k^2 stays below 2^31.
+constexpr uint32_t kFarZ = 0x40000000u;
 
 __global__ void __launch_bounds__(256) edt_pass_z(const uint32_t* __restrict__ h,
                                                   uint32_t* __restrict__ e, int X, int Y, int Z,
@@ -109,39 +109,47 @@ __global__ void __launch_bounds__(256) edt_pass_z(const uint32_t* __restrict__ h
     const int lane = threadIdx.x & 63, sub = threadIdx.x >> 6;
     const int y = blockIdx.x * 64 + lane, x = blockIdx.y;
     const int z0 = blockIdx.z * kChunkZ, z1 = min(z0 + kChunkZ, Z);
-    const int s0 = max(z0 - kHaloZ, 0), s1 = min(z1 + kHaloZ, Z);  // staged planes [s0, s1)
+    const int s0 = z0 - kHaloZ;  // LDS row r holds plane s0 + r
     const size_t stride = (size_t)X * Y;
     const size_t base = (size_t)x * Y + y;
+    const bool in_y = y < Y;
     for (int i = threadIdx.x; i < kLocalBins; i += blockDim.x) lhist[i] = 0;
-    if (y < Y)
-        for (int z = s0 + sub; z < s1; z += 4) col[z - s0][lane] = h[base + (size_t)z * stride];
+    for (int r = sub; r < kStageZ; r += 4) {
+        const int z = s0 + r;
+        col[r][lane] = (in_y && z >= 0 && z < Z) ? h[base + (size_t)z * stride] : kFarZ;
+    }
     __syncthreads();
     uint32_t my_max = 0;
-    if (y < Y) {
-        const int mxy = min(min(x, y), min(X - 1 - x, Y - 1 - y));
-        for (int z = z0 + sub; z < z1; z += 4) {
-            uint32_t best = col[z - s0][lane];
-            if (best != 0) {
-                // the reference's window |dz| <= floor(sqrt(h(z))); after passes X and Y the values are
-                // short 2-D distances, so the fixed window is short and its loads pipeline
-                const int w = (int)isqrt_u32(best) + 1;
-                const int lo = min(w, z), hi = min(w, Z - z);
-                if (z - lo >= s0 && z + hi <= s1) {
-                    for (int dz = -lo; dz < hi; ++dz) best = min(best, col[z + dz - s0][lane] + (uint32_t)(dz * dz));
-                } else {
-                    for (int dz = -lo; dz < hi; ++dz)
-                        best = min(best, h[base + (size_t)(z + dz) * stride] + (uint32_t)(dz * dz));
-                }
-                my_max = max(my_max, best);
-                // sqrt(best) > m  <=>  best > m*m for integer m >= 0
-                const int m = min(mxy, min(z, Z - 1 - z));
-                if ((uint64_t)best <= (uint64_t)m * (uint64_t)m) {
-                    if (best < kLocalBins) atomicAdd(&lhist[best], 1u);
-                    else if (best < hist_len) atomicAdd(&hist[best], 1ull);
-                }
-            }
-            e[base + (size_t)z * stride] = best;
+    const int mxy = min(min(x, y), min(X - 1 - x, Y - 1 - y));
+    for (int z = z0 + sub; z < z1; z += 4) {  // the same trip count on every lane of a wave
+        const int r = z - s0;
+        uint32_t best = in_y ? col[r][lane] : 0u;
+        const uint32_t* below = &col[r - kHaloZ][lane];  // below[(kHaloZ - k) * 64] is plane z - k
+        const uint32_t* above = &col[r][lane];           // above[k * 64] is plane z + k
+#pragma unroll
+        for (int k0 = 1; k0 <= kHaloZ; k0 += 4) {
+            if (!__any((uint32_t)(k0 * k0) < best)) break;
+#pragma unroll
+            for (int k = k0; k < k0 + 4; ++k)
+                best = min(best, min(below[(kHaloZ - k) * 64], above[k * 64]) + (uint32_t)(k * k));
         }
+        if (best > (uint32_t)((kHaloZ + 1) * (kHaloZ + 1))) {
+            for (uint32_t k = kHaloZ + 1; k * k < best; ++k) {
+                const uint32_t a = z >= (int)k ? h[base + (size_t)(z - (int)k) * stride] : kFarZ;
+                const uint32_t b = z + (int)k < Z ? h[base + (size_t)(z + (int)k) * stride] : kFarZ;
+                best = min(best, min(a, b) + k * k);
+            }
+        }
+        if (best != 0) {
+            my_max = max(my_max, best);
+            // sqrt(best) > m  <=>  best > m*m for integer m >= 0
+            const int m = min(mxy, min(z, Z - 1 - z));
+            if ((uint64_t)best <= (uint64_t)m * (uint64_t)m) {
+                if (best < kLocalBins) atomicAdd(&lhist[best], 1u);
+                else if (best < hist_len) atomicAdd(&hist[best], 1ull);
+            }
+        }
+        if (in_y) e[base + (size_t)z * stride] = best;
     }
     for (int off = 32; off > 0; off >>= 1) my_max = max(my_max, (uint32_t)__shfl_xor((int)my_max, off));
     if (lane == 0) wave_max[sub] = my_max;

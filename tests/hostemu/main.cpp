@@ -3,6 +3,8 @@
 //   hostemu pores     <in> <out_volume> <out_text>   in: X Y Z d3q7 | EDT2 u32 | CR2 u32 | mask u8
 //   hostemu watershed <in> <out_volume> <out_text>   in: X Y Z d3q7 maxE | CR2 u32 | np | np x 7 i32
 //   hostemu remove    <in> <out_volume> <out_text>   in: X Y Z | PT u32 | text (pores, "@@", throats)
+//   hostemu edt       <in> <out_volume> <out_text>   in: X Y Z | solid u8; out: EDT2 u32;
+//                     text: maximum square, then the histogram counts one a line
 //   hostemu balls     <in> <out_volumes> <out_text>  in: X Y Z maxE | EDT2 u32;
 //                     out: CR2, CX, CY, CZ u32 | mask u8; text: maximum-ball count
 //   hostemu geodesic  <in> <out_volumes> <out_text>  in: X Y Z d3q7 | solid u8;
@@ -16,6 +18,7 @@
 #include "shim.h"
 
 namespace pma {
+void run_edt(pma_ctx* c);
 void run_pores(pma_ctx* c);
 void run_watershed(pma_ctx* c);
 void run_remove_empty_pores(pma_ctx* c);
@@ -144,6 +147,22 @@ int main(int argc, char** argv) {
                 pma::run_watershed(&c);
                 write_outputs(c, PMA_VOL_PTW, c.pores_ws, c.throats_ws, argv[3], argv[4]);
             }
+        } else if (stage == "edt") {
+            int32_t h[3];
+            read_into(f, h, 3);
+            c.dims = {h[0], h[1], h[2]};
+            c.n = (size_t)h[0] * h[1] * h[2];
+            c.ensure_volume(PMA_VOL_SOLID);
+            read_into(f, c.v<uint8_t>(PMA_VOL_SOLID), c.n);
+            c.valid[PMA_VOL_SOLID] = true;
+            pma::run_edt(&c);
+            FILE* o = std::fopen(argv[3], "wb");
+            std::fwrite(c.v<uint32_t>(PMA_VOL_EDT2), 4, c.n, o);
+            std::fclose(o);
+            o = std::fopen(argv[4], "wb");
+            std::fprintf(o, "%u\n", c.max_e);
+            for (uint64_t v : c.histogram) std::fprintf(o, "%llu\n", (unsigned long long)v);
+            std::fclose(o);
         } else if (stage == "balls") {
             int32_t h[4];
             read_into(f, h, 4);

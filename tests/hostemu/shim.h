@@ -72,6 +72,7 @@ struct WaveCtx {
     int cur = -1;
     std::function<void()>* body = nullptr;
     Dim3 block;
+    unsigned arrived = 0, generation = 0;  // block barrier
 };
 inline WaveCtx* g_wave = nullptr;
 inline void shim_yield() {
@@ -99,8 +100,25 @@ extern "C" inline void shim_fiber_entry() {
     w->fibers[w->cur].done = true;
     for (;;) shim_yield();
 }
+// A block barrier: a fiber waits (yielding) until every fiber of the block that has not ended
+// has arrived.  The fiber that completes the barrier also yields once, so after a barrier the
+// fibers of a wave that trail by one switch are always lower-numbered than those ahead of them,
+// which keeps the slot exchange of the cross-lane operations in order.
 inline void __syncthreads() {
-    if (g_wave) shim_yield();
+    WaveCtx* w = g_wave;
+    if (!w) return;
+    const unsigned gen = w->generation;
+    ++w->arrived;
+    for (;;) {
+        if (w->generation != gen) return;
+        unsigned live = 0;
+        for (const Fiber& f : w->fibers) live += !f.done;
+        if (w->arrived >= live) {
+            w->arrived = 0;
+            ++w->generation;
+        }
+        shim_yield();
+    }
 }
 template <class T> T wave_read(T v, int from_lane_abs) {
     uint64_t bits = 0;
@@ -142,6 +160,7 @@ inline unsigned long long __ballot(int pred) {
     shim_yield();
     return r;
 }
+inline int __any(int pred) { return __ballot(pred) != 0; }
 inline unsigned __ockl_wfred_or_u32(unsigned v) {
     if (!g_wave) throw std::runtime_error("__ockl_wfred_or_u32 outside a wave launch");
     g_wave->slot[threadIdx.x] = v;
@@ -190,10 +209,10 @@ inline float __builtin_amdgcn_sqrtf(float x) {
 inline int __syncthreads_or(int pred) {
     if (!g_wave) throw std::runtime_error("__syncthreads_or outside a wave launch");
     g_wave->slot[threadIdx.x] = pred ? 1 : 0;
-    shim_yield();
+    __syncthreads();
     int r = 0;
-    for (uint64_t v : g_wave->slot) r |= v != 0;
-    shim_yield();
+    for (size_t t = 0; t < g_wave->slot.size(); ++t) r |= !g_wave->fibers[t].done && g_wave->slot[t] != 0;
+    __syncthreads();
     return r;
 }
 // Signed 24-bit multiply: operands are the sign-extended low 24 bits, the result the low 32 bits.
@@ -247,9 +266,15 @@ inline std::vector<char*>& fiber_stacks() {
     return stacks;
 }
 
-template <class F> void launch_cpu(bool wave, Dim3 g, Dim3 b, F f) {
+// Dynamic shared memory of the current launch, allocated at its exact size so that the
+// sanitizer sees any access past its end.
+inline std::vector<uint8_t> g_dyn_shared;
+inline void* hostemu_dyn_shared() { return g_dyn_shared.data(); }
+
+template <class F> void launch_cpu(bool wave, Dim3 g, Dim3 b, F f, size_t shmem = 0) {
     gridDim = g;
     blockDim = b;
+    std::vector<uint8_t>(shmem, 0xCD).swap(g_dyn_shared);
     for (unsigned bz = 0; bz < g.z; ++bz)
         for (unsigned by = 0; by < g.y; ++by)
             for (unsigned bx = 0; bx < g.x; ++bx) {
@@ -371,6 +396,7 @@ struct pma_ctx {
     bool valid[PMA_VOL_COUNT] = {};
     std::map<std::string, std::pair<void*, size_t>> bufs;
     uint32_t max_e = 0;
+    std::vector<uint64_t> histogram;
     uint64_t n_max_balls = 0;
     bool have_edt_stats = false, have_pores = false, have_pores_ws = false;
     std::vector<pma::PoreRow> pores, pores_ws;

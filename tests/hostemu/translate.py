@@ -9,7 +9,8 @@ import sys
 
 WAVE_KERNELS = {"count_centres", "write_centres", "dfs_replay", "cr_histogram", "scan_reduce<T>", "scan_down<T>",
                 "prune_centres", "emit_centres", "cover_tiles", "count_equal_u8", "far_prefix_max",
-                "relax_blocks", "find_start", "cavity_totals", "max_run", "partition_active", "next_min_key"}
+                "relax_blocks", "find_start", "cavity_totals", "max_run", "partition_active", "next_min_key",
+                "edt_pass_y", "edt_pass_z"}
 
 
 def split_args(s):
@@ -33,14 +34,16 @@ def launch(m):
     # The arguments are evaluated once, before the launch, as a real launch does.
     name, cfg, args = m.group(1), split_args(m.group(2)), m.group(3)
     wave = "true" if name in WAVE_KERNELS or name.split("<")[0] in WAVE_KERNELS else "false"
+    shmem = cfg[2].strip() if len(cfg) > 2 else "0"
     return (f"{{ auto hostemu_args = std::make_tuple({args}); "
             f"launch_cpu({wave}, Dim3({cfg[0].strip()}), Dim3({cfg[1].strip()}), [&] {{ "
-            f"std::apply([&](auto... a) {{ {name}(a...); }}, hostemu_args); }}); }}")
+            f"std::apply([&](auto... a) {{ {name}(a...); }}, hostemu_args); }}, (size_t)({shmem})); }}")
 
 
 def translate(text):
     text = text.replace('#include "pma_internal.h"', '#include "shim.h"')
     text = re.sub(r"#include <rocprim/.*?>", "", text)
+    text = re.sub(r"extern __shared__ (\w+) (\w+)\[\];", r"\1* \2 = (\1*)hostemu_dyn_shared();", text)
     return re.sub(r"(\w+(?:<[\w, ]*>)?)<<<(.*?)>>>\((.*?)\);", launch, text, flags=re.S)
 
 

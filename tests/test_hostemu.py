@@ -1,6 +1,6 @@
-"""Host emulation of the containing-ball, pore, removal, watershed and geodesic stages.
+"""Host emulation of the EDT, containing-ball, pore, removal, watershed and geodesic stages.
 
-The library's own .hip sources (scan, libstdc++-sort emulation, prefix trie, containing balls,
+The library's own .hip sources (scan, libstdc++-sort emulation, prefix trie, EDT, containing balls,
 pores, removal, watershed, geodesic) are translated to host C++ (tests/hostemu/translate.py), built with
 AddressSanitizer and UndefinedBehaviorSanitizer and run on the oracle's inputs: every kernel
 executes its real body, thread by thread (wave kernels as fibers), with exact-size buffers, so
@@ -24,7 +24,7 @@ from tests.volume_utils import canonical_relabel, sha256_array
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(ROOT, "porousmediaanalysis_amd", "csrc")
 EMU = os.path.join(ROOT, "tests", "hostemu")
-SOURCES = ["scan", "sort", "lists", "balls", "pores", "remove", "watershed", "geodesic"]
+SOURCES = ["scan", "sort", "lists", "edt", "balls", "pores", "remove", "watershed", "geodesic"]
 SANITIZED = ["-std=c++20", "-O1", "-g", "-fsanitize=address,undefined", "-fno-omit-frame-pointer",
              "-fno-sanitize-recover=undefined", f"-I{EMU}"]
 FAST = ["-std=c++20", "-O2", f"-I{EMU}"]
@@ -169,6 +169,40 @@ CASES["noise_many_pores"] = volumes.random_noise((24, 26, 28), 0.45, 6)
 BALL_CASES = dict(CASES)
 BALL_CASES["gauss_s4"] = volumes.gaussian((48, 56, 64), 4.0, 0.7, 3)
 BALL_CASES["gauss_ragged"] = volumes.gaussian((19, 70, 130), 3.0, 0.75, 3)
+
+
+def _far_void(Y):
+    # void columns 48 voxels wide in x and solid only at z = 0: squared 2-D distances up to 71^2
+    # (the virtual planes x = -X and 2X-1), so pass Z searches beyond its staged halo
+    s = np.zeros((150, 48, Y), np.uint8)
+    s[0] = 1
+    s[120, 3, 5] = 1
+    return s
+
+
+EDT_CASES = dict(BALL_CASES)
+EDT_CASES["far_void_y12"] = _far_void(12)
+EDT_CASES["far_void_y13"] = _far_void(13)
+
+
+@pytest.mark.parametrize("name", sorted(EDT_CASES))
+def test_edt(hostemu, tmp_path, name):
+    solid = EDT_CASES[name]
+    Z, X, Y = solid.shape
+    inp, vol, txt = tmp_path / "in.bin", tmp_path / "out.bin", tmp_path / "out.txt"
+    inp.write_bytes(np.array([X, Y, Z], np.int32).tobytes() + solid.astype(np.uint8).tobytes())
+    r = subprocess.run([hostemu, "edt", str(inp), str(vol), str(txt)], capture_output=True, text=True, env=ENV,
+                       timeout=600)
+    assert r.returncode == 0, f"edt: rc {r.returncode}\n{r.stdout[-2000:]}\n{r.stderr[-6000:]}"
+    e = np.fromfile(vol, np.uint32).reshape(solid.shape)
+    words = txt.read_text().split()
+    oe, om, oh = oracle.edt(solid, threads=4)
+    if not np.array_equal(e, oe):
+        bad = np.argwhere(e != oe)
+        raise AssertionError(f"{len(bad)} squared distances differ, first {tuple(bad[0])}: "
+                             f"{e[tuple(bad[0])]} vs {oe[tuple(bad[0])]}")
+    assert int(words[0]) == om, "maximum differs"
+    assert np.array_equal(np.array(words[1:], np.uint64), oh), "histogram differs"
 
 
 @pytest.mark.parametrize("name", sorted(BALL_CASES))
