@@ -63,27 +63,45 @@ __global__ void __launch_bounds__(256) edt_pass_x4(const uint8_t* __restrict__ s
     }
 }
 
+// Pass Y: one workgroup per (x, z) row, staged in LDS with kHaloY entries of kFarY either side.
+// Each lane searches outward from its y over k = 1, 2, ..., taking min(g(y-k), g(y+k)) + k^2,
+// until k^2 >= best on every lane of its wave: no farther y' can then do better, so the result
+// is the exact minimum over the row, which equals the minimum over the reference's window
+// |dy| <= floor(sqrt(g(y))).  k is the same on every lane (k^2 is a constant of the unrolled loop);
+// lanes that still need k > kHaloY continue alone with bounds checks.
+constexpr int kHaloY = 32;
+// Inputs of pass Y are below (2X)^2 <= 2^30 for X <= 16383, and g + k^2 stays below 2^31.
+constexpr uint32_t kFarY = 0x40000000u;
+
 __global__ void __launch_bounds__(256) edt_pass_y(const uint32_t* __restrict__ g,
                                                   uint32_t* __restrict__ h, int X, int Y) {
-    extern __shared__ uint32_t line[];
-    size_t base = ((size_t)blockIdx.y * X + blockIdx.x) * Y;
-    for (int y = threadIdx.x; y < Y; y += blockDim.x) line[y] = g[base + y];
+    extern __shared__ uint32_t line[];  // line[kHaloY + y] = g(y) for 0 <= y < Y, kFarY outside
+    const size_t base = ((size_t)blockIdx.y * X + blockIdx.x) * Y;
+    for (int i = threadIdx.x; i < Y + 2 * kHaloY; i += blockDim.x) {
+        const int y = i - kHaloY;
+        line[i] = (y >= 0 && y < Y) ? g[base + y] : kFarY;
+    }
     __syncthreads();
-    for (int y = threadIdx.x; y < Y; y += blockDim.x) {
-        // outward from y: once dy^2 >= best no farther y' can do better, so the minimum over the
-        // reference's window (|dy| <= floor(sqrt(g(y)))) is reached without walking all of it.
-        // The test runs every 4 steps (steps past it add dy^2 >= best and change nothing).
-        uint32_t best = line[y];
-        for (uint32_t dy = 1; dy * dy < best; dy += 4) {  // sums fit in 32 bits (X <= 16383)
+    const int lane = threadIdx.x & 63;
+    for (int yw = (int)(threadIdx.x & ~63u); yw < Y; yw += blockDim.x) {  // the same trips on every lane of a wave
+        const int y = yw + lane;
+        const bool in_y = y < Y;
+        const uint32_t* c = &line[kHaloY + (in_y ? y : Y - 1)];  // c[-kHaloY .. kHaloY] lie in line
+        uint32_t best = in_y ? c[0] : 0u;
 #pragma unroll
-            for (uint32_t u = 0; u < 4; ++u) {
-                const uint32_t d = dy + u, dd = d * d;
-                const uint32_t a = (int)d <= y ? line[y - (int)d] + dd : best;
-                const uint32_t b = y + (int)d < Y ? line[y + (int)d] + dd : best;
-                best = min(best, min(a, b));
+        for (int k0 = 1; k0 <= kHaloY; k0 += 4) {
+            if (!__any((uint32_t)(k0 * k0) < best)) break;
+#pragma unroll
+            for (int k = k0; k < k0 + 4; ++k) best = min(best, min(c[-k], c[k]) + (uint32_t)(k * k));
+        }
+        if (best > (uint32_t)((kHaloY + 1) * (kHaloY + 1))) {
+            for (uint32_t k = kHaloY + 1; k * k < best; ++k) {
+                const uint32_t a = y >= (int)k ? line[kHaloY + y - (int)k] : kFarY;
+                const uint32_t b = y + (int)k < Y ? line[kHaloY + y + (int)k] : kFarY;
+                best = min(best, min(a, b) + k * k);
             }
         }
-        h[base + y] = best;
+        if (in_y) h[base + y] = best;
     }
 }
 
@@ -167,7 +185,7 @@ void run_edt(pma_ctx* c) {
     const int X = c->dims.x, Y = c->dims.y, Z = c->dims.z;
     PMA_REQUIRE(Z <= 65535 && X <= 16383, PMA_ERR_INVALID_ARGUMENT,
                 "pma_edt: X must be <= 16383 (squared distances and their sums in 32 bits) and Z <= 65535");
-    PMA_REQUIRE((size_t)Y * 4 <= 64 * 1024, PMA_ERR_INVALID_ARGUMENT, "pma_edt: Y must be <= 16384");
+    PMA_REQUIRE((size_t)(Y + 2 * kHaloY) * 4 <= 64 * 1024, PMA_ERR_INVALID_ARGUMENT, "pma_edt: Y must be <= 16320");
     c->ensure_volume(PMA_VOL_EDT2);
     uint32_t* e = c->v<uint32_t>(PMA_VOL_EDT2);
     uint32_t* tmp = c->buf<uint32_t>("edt_tmp", c->n);
@@ -196,7 +214,7 @@ void run_edt(pma_ctx* c) {
     }
     {
         PMA_TIMED(c, "edt_pass_y");
-        edt_pass_y<<<dim3(X, Z), 256, (size_t)Y * sizeof(uint32_t), c->stream>>>(e, tmp, X, Y);
+        edt_pass_y<<<dim3(X, Z), 256, (size_t)(Y + 2 * kHaloY) * sizeof(uint32_t), c->stream>>>(e, tmp, X, Y);
         PMA_LAUNCHED(c);
     }
     {

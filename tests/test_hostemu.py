@@ -183,6 +183,10 @@ def _far_void(Y):
 EDT_CASES = dict(BALL_CASES)
 EDT_CASES["far_void_y12"] = _far_void(12)
 EDT_CASES["far_void_y13"] = _far_void(13)
+# solid only at y = 0: rows of 100 along y whose nearest solid lies up to 99 away, so pass Y
+# searches beyond its LDS padding
+EDT_CASES["far_void_rows"] = np.zeros((6, 48, 100), np.uint8)
+EDT_CASES["far_void_rows"][:, :, 0] = 1
 
 
 @pytest.mark.parametrize("name", sorted(EDT_CASES))
