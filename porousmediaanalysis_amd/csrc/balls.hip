@@ -13,9 +13,10 @@
 //   2. the surviving centres, emitted column by column in scan order, are sorted stably by r2
 //      descending: their rank is the order in which the reference would let them win.
 //   3. every centre lists the pair cells (4 x 2 tiles of 8 x 8 x 64 voxels along z, x) its ball
-//      reaches, as (cell, ball packed relative to the cell) pairs (or (cell, rank) pairs when a
-//      ball does not fit 64 bits); the pairs are sorted stably by cell, so each cell sees its
-//      balls in rank order.
+//      reaches, as one 64-bit word holding the ball packed relative to the cell with the cell
+//      number in its middle bits (or, when these do not fit, as (cell, packed ball) pairs, or
+//      (cell, rank) pairs when a ball does not fit 64 bits); they are sorted stably by cell, so
+//      each cell sees its balls in rank order.
 //   4. one wave per tile, walking its cell's balls and skipping those that cannot reach the tile's
 //      still-uncovered voxels: lane l owns the row (z, x) = (l / 8, l % 8) of the tile and a 64-bit mask
 //      of the row's voxels that are solid or already covered.  For each ball in rank order a lane
@@ -42,6 +43,9 @@ constexpr int TZ = 8, TX = 8, TY = 64;  // tile: 64 rows (z, x) of 64 voxels alo
 #endif
 #ifndef PMA_CB_PACKED
 #define PMA_CB_PACKED 1  // 0 keeps ranks in the pair sort and gathers the balls afterwards
+#endif
+#ifndef PMA_CB_FUSED
+#define PMA_CB_FUSED 1  // 0 sorts packed balls as values of separate cell keys even where they fit one word
 #endif
 #ifndef PMA_CB_SMALL_R2
 #define PMA_CB_SMALL_R2 (1u << 22)  // r2 below this use isqrt_small (its exactness bound)
@@ -224,31 +228,34 @@ struct Cells {
 };
 
 // A ball packed relative to the origin (z0, x0, y0) of a cell it reaches, so that the pair
-// sort carries the ball itself: r2 << 33 | (z - z0 + off) << 22 | (x - x0 + off) << 11 |
+// sort carries the ball itself: r2 << r2_shift | (z - z0 + off) << 22 | (x - x0 + off) << 11 |
 // (y - y0 + off).  A ball that reaches a cell has its centre within sqrt(r2) <= off - 1 of it,
 // so each field is in [1, cell edge + 2 off - 2]; the host uses this form only when that is
-// below 2^11 and r2 < 2^31.
-__device__ __forceinline__ uint64_t pack_ball(const Ball& b, int z0, int x0, int y0, int off) {
-    return (uint64_t)b.r2 << 33 | (uint64_t)(b.z - z0 + off) << 22 | (uint64_t)(b.x - x0 + off) << 11 |
+// below 2^11 and r2 < 2^(64 - r2_shift).  Bits [33, r2_shift) hold the cell number when the word
+// is its own sort key (r2_shift = 33 + the cell bits), and are absent when the cell is a separate
+// key (r2_shift = 33).
+__device__ __forceinline__ uint64_t pack_ball(const Ball& b, int z0, int x0, int y0, int off, int r2_shift) {
+    return (uint64_t)b.r2 << r2_shift | (uint64_t)(b.z - z0 + off) << 22 | (uint64_t)(b.x - x0 + off) << 11 |
            (uint64_t)(b.y - y0 + off);
 }
 
-__device__ __forceinline__ Ball unpack_ball(uint64_t v, int z0, int x0, int y0, int off) {
+__device__ __forceinline__ Ball unpack_ball(uint64_t v, int z0, int x0, int y0, int off, int r2_shift) {
     Ball b;
     b.y = (int)(v & 2047u) - off + y0;
     b.x = (int)((v >> 11) & 2047u) - off + x0;
     b.z = (int)((v >> 22) & 2047u) - off + z0;
-    b.r2 = (uint32_t)(v >> 33);
+    b.r2 = (uint32_t)(v >> r2_shift);
     return b;
 }
 
-// Cells of the slab [cz0, cz1) that ball b reaches (box distance^2 <= r2): counted (pk == null)
+// Cells of the slab [cz0, cz1) that ball b reaches (box distance^2 <= r2): counted (pv == null)
 // or listed from `at` as (cell - cell_base, value) pairs, the value being the ball's rank
-// (V = uint32_t) or the ball packed relative to the cell (V = uint64_t).
+// (V = uint32_t) or the ball packed relative to the cell (V = uint64_t); with V = uint64_t and
+// pk null, as single words that carry the cell in bits [33, r2_shift).
 template <class V>
 __device__ uint32_t ball_tiles(const Ball& b, int X, int Y, int Z, Cells g, int cz0, int cz1, uint32_t cell_base,
-                               uint32_t rank, int off, uint32_t* __restrict__ pk, V* __restrict__ pv,
-                               unsigned long long at) {
+                               uint32_t rank, int off, int r2_shift, uint32_t* __restrict__ pk,
+                               V* __restrict__ pv, unsigned long long at) {
     const int R = (int)isqrt_u32(b.r2);
     const int za = max(b.z - R, 0) / g.sz, zb = min(b.z + R, Z - 1) / g.sz;
     const int xa = max(b.x - R, 0) / g.sx, xb = min(b.x + R, X - 1) / g.sx;
@@ -264,10 +271,20 @@ __device__ uint32_t ball_tiles(const Ball& b, int X, int Y, int Z, Cells g, int 
                 const int y0 = ty * TY, y1 = min(y0 + TY, Y) - 1;
                 const int dy = max(max(y0 - b.y, b.y - y1), 0);
                 if ((uint32_t)(dz * dz + dx * dx + dy * dy) > b.r2) continue;
-                if (pk) {
-                    pk[at + n] = (uint32_t)((tz * g.NCX + tx) * g.NCY + ty) - cell_base;
-                    if constexpr (sizeof(V) == 8) pv[at + n] = pack_ball(b, z0, x0, y0, off);
-                    else pv[at + n] = rank;
+                if (pv) {
+                    const uint32_t cell = (uint32_t)((tz * g.NCX + tx) * g.NCY + ty) - cell_base;
+                    if constexpr (sizeof(V) == 8) {
+                        const uint64_t w = pack_ball(b, z0, x0, y0, off, r2_shift);
+                        if (pk) {
+                            pk[at + n] = cell;
+                            pv[at + n] = w;
+                        } else {
+                            pv[at + n] = w | (uint64_t)cell << 33;
+                        }
+                    } else {
+                        pk[at + n] = cell;
+                        pv[at + n] = rank;
+                    }
                 }
                 ++n;
             }
@@ -280,16 +297,16 @@ __global__ void count_pairs(const Ball* __restrict__ balls, size_t n, int X, int
                             unsigned long long* __restrict__ cnt) {
     size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
-    cnt[r] = ball_tiles<uint32_t>(balls[r], X, Y, Z, g, cz0, cz1, 0, 0, 0, nullptr, nullptr, 0);
+    cnt[r] = ball_tiles<uint32_t>(balls[r], X, Y, Z, g, cz0, cz1, 0, 0, 0, 33, nullptr, nullptr, 0);
 }
 
 template <class V>
 __global__ void emit_pairs(const Ball* __restrict__ balls, size_t n, const unsigned long long* __restrict__ at,
                            int X, int Y, int Z, Cells g, int cz0, int cz1, uint32_t cell_base, int off,
-                           uint32_t* __restrict__ pk, V* __restrict__ pv) {
+                           int r2_shift, uint32_t* __restrict__ pk, V* __restrict__ pv) {
     size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
-    ball_tiles<V>(balls[r], X, Y, Z, g, cz0, cz1, cell_base, (uint32_t)r, off, pk, pv, at[r]);
+    ball_tiles<V>(balls[r], X, Y, Z, g, cz0, cz1, cell_base, (uint32_t)r, off, r2_shift, pk, pv, at[r]);
 }
 
 // [start, end) of every cell's run in the cell-sorted pair list; when the pairs carry ranks,
@@ -304,6 +321,17 @@ __global__ void tile_ranges(const uint32_t* __restrict__ pk, const uint32_t* __r
     const uint32_t t = pk[i];
     if (i == 0 || pk[i - 1] != t) start[t] = (uint32_t)i;
     if (i + 1 == P || pk[i + 1] != t) end[t] = (uint32_t)(i + 1);
+}
+
+// The same for words that carry their cell in bits [33, 33 + cell_bits).
+__global__ void tile_ranges_fused(const uint64_t* __restrict__ w, size_t P, uint32_t cell_mask,
+                                  uint32_t* __restrict__ start, uint32_t* __restrict__ end) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    auto cell = [&](size_t j) { return (uint32_t)(w[j] >> 33) & cell_mask; };
+    const uint32_t t = cell(i);
+    if (i == 0 || cell(i - 1) != t) start[t] = (uint32_t)i;
+    if (i + 1 == P || cell(i + 1) != t) end[t] = (uint32_t)(i + 1);
 }
 
 // floor(sqrt(v)) from one correctly rounded float square root and a one-step correction.
@@ -347,7 +375,7 @@ __global__ void __launch_bounds__(256) max_run(const uint32_t* __restrict__ star
 // index is mapped such that the tiles of one cell run on the same XCD and share its L2.
 template <class W, int Root, bool Packed>
 __global__ void __launch_bounds__(64) cover_tiles(const Ball* __restrict__ cell_balls,
-                                                  const uint64_t* __restrict__ cell_packed, int off,
+                                                  const uint64_t* __restrict__ cell_packed, int off, int r2_shift,
                                                   const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
                                                   const uint32_t* __restrict__ e, uint32_t* __restrict__ cr,
                                                   uint32_t* __restrict__ cx, uint32_t* __restrict__ cy,
@@ -374,7 +402,7 @@ __global__ void __launch_bounds__(64) cover_tiles(const Ball* __restrict__ cell_
         else return cell_balls[i];
     };
     auto ball_of = [&](const Raw& r) -> Ball {
-        if constexpr (Packed) return unpack_ball(r, oz, ox, y0, off);
+        if constexpr (Packed) return unpack_ball(r, oz, ox, y0, off, r2_shift);
         else return r;
     };
     if (Root == kRootTable)
@@ -439,7 +467,7 @@ __global__ void __launch_bounds__(64) cover_tiles(const Ball* __restrict__ cell_
             if constexpr (Packed) {
                 const uint64_t pv = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(cur >> 32), j) << 32 |
                                     (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)cur, j);
-                const Ball u = unpack_ball(pv, oz, ox, y0, off);
+                const Ball u = unpack_ball(pv, oz, ox, y0, off, r2_shift);
                 bx = u.x;
                 by = u.y;
                 bz = u.z;
@@ -501,7 +529,7 @@ struct CoverArgs {
     uint32_t blocks;
     const Ball* cell_balls;
     const uint64_t* cell_packed;
-    int off;
+    int off, r2_shift;
     const uint32_t *start, *end, *e;
     uint32_t *cr, *cx, *cy, *cz;
     uint8_t* mask;
@@ -513,7 +541,8 @@ struct CoverArgs {
 
 template <class W, int Root, bool Packed>
 void launch_cover(hipStream_t stream, const CoverArgs& a) {
-    cover_tiles<W, Root, Packed><<<a.blocks, 64, 0, stream>>>(a.cell_balls, a.cell_packed, a.off, a.start, a.end, a.e,
+    cover_tiles<W, Root, Packed><<<a.blocks, 64, 0, stream>>>(a.cell_balls, a.cell_packed, a.off, a.r2_shift, a.start,
+                                                              a.end, a.e,
                                                               a.cr, a.cx, a.cy, a.cz, a.mask, a.X, a.Y, a.Z, a.g,
                                                               a.cell_base, a.slab_cells, a.err);
 }
@@ -598,6 +627,7 @@ void run_containing_balls(pma_ctx* c) {
     }
     int hi_bits = 0;
     while (hi_bits < 32 && (max_e >> hi_bits) != 0) ++hi_bits;
+    const int r2_bits = std::max(hi_bits, 1);  // bits of any r2 <= max_e
     radix_sort_keys_u64(c, reinterpret_cast<const uint64_t*>(keys), reinterpret_cast<uint64_t*>(keys2), n_cand, 32,
                         32 + std::max(hi_bits, 1));
     Ball* balls = c->buf<Ball>("cb_balls", N1);
@@ -645,8 +675,17 @@ void run_containing_balls(pma_ctx* c) {
         const uint32_t cell_base = (uint32_t)tz0 * layer_cells;
         const uint32_t slab_cells = (uint32_t)(tz1 - tz0) * layer_cells;
         const size_t P1 = std::max<uint64_t>(P, 1);
-        uint32_t* pk = c->buf<uint32_t>("cb_pair_tile", P1);
-        uint32_t* pk2 = c->buf<uint32_t>("cb_pair_tile2", P1);
+        int cell_bits = 1;
+        while (cell_bits < 32 && (slab_cells >> cell_bits) != 0) ++cell_bits;
+        // one sort key per pair: the packed ball with its cell in bits [33, 33 + cell_bits)
+        const bool fused = packed && PMA_CB_FUSED && 33 + cell_bits + r2_bits <= 64;
+        const int r2_shift = fused ? 33 + cell_bits : 33;
+        uint32_t* pk = nullptr;
+        uint32_t* pk2 = nullptr;
+        if (!fused) {
+            pk = c->buf<uint32_t>("cb_pair_tile", P1);
+            pk2 = c->buf<uint32_t>("cb_pair_tile2", P1);
+        }
         uint32_t* pv = nullptr;
         uint32_t* pv2 = nullptr;
         Ball* cell_balls = nullptr;
@@ -665,11 +704,19 @@ void run_containing_balls(pma_ctx* c) {
         PMA_HIP(hipMemsetAsync(tstart, 0, slab_cells * sizeof(uint32_t), c->stream));
         PMA_HIP(hipMemsetAsync(tend, 0, slab_cells * sizeof(uint32_t), c->stream));
         if (P) {
-            int cell_bits = 1;
-            while (cell_bits < 32 && (slab_cells >> cell_bits) != 0) ++cell_bits;
-            if (packed) {
+            if (fused) {
                 emit_pairs<uint64_t><<<grid_for(n_cand, 256), 256, 0, c->stream>>>(balls, n_cand, poff, X, Y, Z, g, tz0,
-                                                                                   tz1, cell_base, off, pk, pb);
+                                                                                   tz1, cell_base, off, r2_shift,
+                                                                                   nullptr, pb);
+                PMA_LAUNCHED(c);
+                radix_sort_keys_u64(c, pb, pb2, P, 33, 33 + cell_bits);
+                tile_ranges_fused<<<grid_for(P, 256), 256, 0, c->stream>>>(pb2, P, (1u << cell_bits) - 1u, tstart,
+                                                                           tend);
+                PMA_LAUNCHED(c);
+            } else if (packed) {
+                emit_pairs<uint64_t><<<grid_for(n_cand, 256), 256, 0, c->stream>>>(balls, n_cand, poff, X, Y, Z, g, tz0,
+                                                                                   tz1, cell_base, off, r2_shift, pk,
+                                                                                   pb);
                 PMA_LAUNCHED(c);
                 radix_sort_pairs_u32_u64(c, pk, pk2, pb, pb2, P, 0, cell_bits);
                 tile_ranges<false><<<grid_for(P, 256), 256, 0, c->stream>>>(pk2, nullptr, P, balls, nullptr, tstart,
@@ -677,7 +724,8 @@ void run_containing_balls(pma_ctx* c) {
                 PMA_LAUNCHED(c);
             } else {
                 emit_pairs<uint32_t><<<grid_for(n_cand, 256), 256, 0, c->stream>>>(balls, n_cand, poff, X, Y, Z, g, tz0,
-                                                                                   tz1, cell_base, off, pk, pv);
+                                                                                   tz1, cell_base, off, r2_shift, pk,
+                                                                                   pv);
                 PMA_LAUNCHED(c);
                 radix_sort_pairs_u32(c, pk, pk2, pv, pv2, P, 0, cell_bits);
                 tile_ranges<true><<<grid_for(P, 256), 256, 0, c->stream>>>(pk2, pv2, P, balls, cell_balls, tstart,
@@ -707,7 +755,7 @@ void run_containing_balls(pma_ctx* c) {
                              : max_e < PMA_CB_SMALL_R2                                            ? kRootSmall
                                                                                                    : kRootGeneral;
             const bool w16 = longest <= PMA_CB_WIN16_MAX;
-            CoverArgs a{blocks, cell_balls, pb2, off, tstart, tend, e, cr, cx, cy, cz, mask, X, Y, Z, g,
+            CoverArgs a{blocks, cell_balls, pb2, off, r2_shift, tstart, tend, e, cr, cx, cy, cz, mask, X, Y, Z, g,
                         cell_base, slab_cells, err};
             kCoverLaunch[w16 ? 1 : 0][root][packed ? 1 : 0](c->stream, a);
             PMA_LAUNCHED(c);

@@ -92,8 +92,9 @@ def hostemu_slabs(tmp_path_factory):
 
 @pytest.fixture(scope="session")
 def hostemu_fast(tmp_path_factory):
-    """Optimised build without sanitizers, for the larger fixture volume."""
-    return _build(tmp_path_factory, FAST, "fast")
+    """Optimised build without sanitizers, for the larger fixture volume; its containing-ball pair
+    sort keeps cells and packed balls apart (the form used where they do not fit one word)."""
+    return _build(tmp_path_factory, FAST + ["-DPMA_CB_FUSED=0"], "fast")
 
 
 def _run(exe, stage, payload, tmp_path, shape):
@@ -209,10 +210,9 @@ def test_edt(hostemu, tmp_path, name):
     assert np.array_equal(np.array(words[1:], np.uint64), oh), "histogram differs"
 
 
-@pytest.mark.parametrize("name", sorted(BALL_CASES))
-def test_containing_balls(hostemu, tmp_path, name):
-    e, max_e, _ = oracle.edt(BALL_CASES[name], threads=4)
-    g = emu_balls(hostemu, tmp_path, e, max_e)
+def check_balls(binary, tmp_path, solid):
+    e, max_e, _ = oracle.edt(solid, threads=4)
+    g = emu_balls(binary, tmp_path, e, max_e)
     o = oracle.containing_balls(e, threads=4)
     for k, what in enumerate(["CR2", "CX", "CY", "CZ", "mask"]):
         if not np.array_equal(g[k], o[k]):
@@ -220,6 +220,16 @@ def test_containing_balls(hostemu, tmp_path, name):
             raise AssertionError(f"{what}: {len(bad)} voxels differ, first {tuple(bad[0])}: "
                                  f"{g[k][tuple(bad[0])]} vs {o[k][tuple(bad[0])]}")
     assert g[5] == o[5], "maximum-ball count differs"
+
+
+@pytest.mark.parametrize("name", sorted(BALL_CASES))
+def test_containing_balls(hostemu, tmp_path, name):
+    check_balls(hostemu, tmp_path, BALL_CASES[name])
+
+
+@pytest.mark.parametrize("name", ["gauss_s4", "gauss_ragged", "spheres_48", "noise_many_pores"])
+def test_containing_balls_separate_keys(hostemu_fast, tmp_path, name):
+    check_balls(hostemu_fast, tmp_path, BALL_CASES[name])
 
 
 GEO_CASES = dict(volumes.edge_cases())
