@@ -44,6 +44,9 @@ constexpr int TZ = 8, TX = 8, TY = 64;  // tile: 64 rows (z, x) of 64 voxels alo
 #ifndef PMA_CB_PACKED
 #define PMA_CB_PACKED 1  // 0 keeps ranks in the pair sort and gathers the balls afterwards
 #endif
+#ifndef PMA_CB_PRUNE_FAR_LDS
+#define PMA_CB_PRUNE_FAR_LDS 2048u  // largest far-table length L that prune_centres copies to LDS
+#endif
 #ifndef PMA_CB_FUSED
 #define PMA_CB_FUSED 1  // 0 sorts packed balls as values of separate cell keys even where they fit one word
 #endif
@@ -114,7 +117,9 @@ __global__ void __launch_bounds__(1024) far_prefix_max(uint32_t* __restrict__ sh
 
 // Block = 4 rows of 64 voxels along y; each thread walks its (x, y) column through every plane,
 // with planes z - 1 .. z + 2 (and a one-voxel halo in x and y; outside the image reads 0) in
-// a 4-slot LDS ring: plane z + 2 is fetched into registers while plane z is tested.  Marks
+// a 4-slot LDS ring: while plane z is tested, plane z + 2 (loaded one step earlier) goes into
+// the ring and plane z + 3 is loaded into registers.  With FarLds the far tables are copied
+// to LDS first (3 L words of dynamic shared memory).  Marks
 // solid 0, void 127 and surviving centres kCandidate, and writes the column's count of
 // surviving centres at col_count[x * Y + y] (the columns in the reference's scan order).
 // c is dropped when a neighbour q of offset class k has q > c and q >= far_k(c), i.e. when the
@@ -122,10 +127,17 @@ __global__ void __launch_bounds__(1024) far_prefix_max(uint32_t* __restrict__ sh
 constexpr int kPruneW = 64 + 2, kPruneH = 4 + 2, kPrunePlane = kPruneW * kPruneH;
 constexpr int kPruneLoads = (kPrunePlane + 255) / 256;
 
+constexpr uint32_t kPruneFarLds = PMA_CB_PRUNE_FAR_LDS;
+
+template <bool FarLds>
 __global__ void __launch_bounds__(256) prune_centres(const uint32_t* __restrict__ e, uint8_t* __restrict__ mask,
-                                                     const uint32_t* __restrict__ far, uint32_t L,
+                                                     const uint32_t* __restrict__ far_global, uint32_t L,
                                                      uint32_t* __restrict__ col_count, int X, int Y, int Z) {
     __shared__ uint32_t ring[4][kPrunePlane];
+    extern __shared__ uint32_t far_lds[];
+    if (FarLds)
+        for (uint32_t k = threadIdx.x; k < 3 * L; k += blockDim.x) far_lds[k] = far_global[k];
+    const uint32_t* far = FarLds ? far_lds : far_global;
     const int lane = threadIdx.x & 63, row = threadIdx.x >> 6;
     const int y = blockIdx.x * 64 + lane, x = blockIdx.y * 4 + row;
     auto fetch = [&](int z, uint32_t (&r)[kPruneLoads]) {
@@ -145,16 +157,18 @@ __global__ void __launch_bounds__(256) prune_centres(const uint32_t* __restrict_
             if (k < kPrunePlane) ring[(z + 4) & 3][k] = r[j];
         }
     };
-    uint32_t r[kPruneLoads];
+    uint32_t r[kPruneLoads], ahead[kPruneLoads];
     for (int z = -1; z <= 1; ++z) {
         fetch(z, r);
         store(z, r);
     }
+    fetch(2, r);
     __syncthreads();
     uint32_t kept = 0;
     const int o = (row + 1) * kPruneW + lane + 1;
     for (int z = 0; z < Z; ++z) {
-        fetch(z + 2, r);
+        // plane z + 3 is loaded a whole step before it is stored, so its latency is hidden
+        fetch(z + 3, ahead);
         if (x < X && y < Y) {
             const size_t i = ((size_t)z * X + x) * Y + y;
             const uint32_t c = ring[z & 3][o];
@@ -181,6 +195,8 @@ __global__ void __launch_bounds__(256) prune_centres(const uint32_t* __restrict_
         }
         store(z + 2, r);  // the slot of plane z - 2, last read in the previous step
         __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kPruneLoads; ++j) r[j] = ahead[j];
     }
     if (x < X && y < Y) col_count[(size_t)x * Y + y] = kept;
 }
@@ -196,8 +212,26 @@ __global__ void __launch_bounds__(256) emit_centres(const uint32_t* __restrict__
     if (x >= X || y >= Y) return;
     uint32_t pos = col_off[(size_t)x * Y + y];
     const uint32_t scan0 = (uint32_t)(((uint64_t)x * Y + y) * (uint64_t)Z);
-    for (int z = 0; z < Z; ++z) {
-        const size_t i = ((size_t)z * X + x) * Y + y;
+    const size_t plane = (size_t)X * Y;
+    size_t i = (size_t)x * Y + y;
+    int z = 0;
+    // 8 planes at a time, their loads issued together (the rows they touch are read anyway)
+    for (; z + 8 <= Z; z += 8, i += 8 * plane) {
+        uint8_t m[8];
+        uint32_t v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            m[j] = mask[i + j * plane];
+            v[j] = e[i + j * plane];
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (m[j] == kCandidate) {
+                keys[pos++] = ((unsigned long long)(max_e - v[j]) << 32) | (scan0 + (uint32_t)(z + j));
+                mask[i + j * plane] = 127;
+            }
+    }
+    for (; z < Z; ++z, i += plane) {
         if (mask[i] != kCandidate) continue;
         keys[pos++] = ((unsigned long long)(max_e - e[i]) << 32) | (scan0 + (uint32_t)z);
         mask[i] = 127;
@@ -612,7 +646,11 @@ void run_containing_balls(pma_ctx* c) {
     uint32_t* col_off = c->buf<uint32_t>("cb_col_off", ncols);
     {
         PMA_TIMED(c, "prune_centres");
-        prune_centres<<<cgrid, 256, 0, c->stream>>>(e, mask, far, (uint32_t)L, col_count, X, Y, Z);
+        if (L <= kPruneFarLds)
+            prune_centres<true><<<cgrid, 256, 3 * L * sizeof(uint32_t), c->stream>>>(e, mask, far, (uint32_t)L,
+                                                                                     col_count, X, Y, Z);
+        else
+            prune_centres<false><<<cgrid, 256, 0, c->stream>>>(e, mask, far, (uint32_t)L, col_count, X, Y, Z);
         PMA_LAUNCHED(c);
     }
     // fewer than 2^32 voxels, so the offsets fit 32 bits

@@ -82,12 +82,13 @@ def hostemu(tmp_path_factory):
 @pytest.fixture(scope="session")
 def hostemu_slabs(tmp_path_factory):
     """Sanitized build with small work limits: containing-ball pair lists cut into slabs of at most
-    4000 pairs, 32-bit winner slots, ranks (not packed balls) in the pair sort, and geodesic
-    relaxation on 2 waves with 32-entry push buffers."""
+    4000 pairs, 32-bit winner slots, ranks (not packed balls) in the pair sort, far tables read
+    from global memory by the centre prune, and geodesic relaxation on 2 waves with 32-entry push
+    buffers."""
     return _build(tmp_path_factory, SANITIZED + ["-DPMA_CB_SLAB_PAIRS=4000ull", "-DPMA_CB_WIN16_MAX=0u",
                                                  "-DPMA_CB_GZ=1", "-DPMA_CB_GX=3", "-DPMA_CB_SMALL_R2=0u",
                                                  "-DPMA_GEO_RELAX_GRID=2u", "-DPMA_GEO_PUSH_BUF=32u",
-                                                 "-DPMA_CB_PACKED=0"], "slabs")
+                                                 "-DPMA_CB_PACKED=0", "-DPMA_CB_PRUNE_FAR_LDS=0u"], "slabs")
 
 
 @pytest.fixture(scope="session")
