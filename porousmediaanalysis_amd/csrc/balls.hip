@@ -7,9 +7,11 @@
 //
 // Work plan:
 //   1. far tables: far_k(a) = max over lattice p with |p|^2 <= a of |p - d_k|^2 for the face,
-//      edge and corner offsets d_k.  A centre c is skipped when a stencil neighbour q has
+//      edge and corner offsets d_k and the offsets (2, 1, 0) and (2, 1, 1) (up to signs and
+//      order).  A centre c is skipped when a neighbour q at one of these offsets has
 //      e(q) > e(c) and far_k(e(c)) <= e(q): then B(c) is a subset of B(q), every voxel of B(c)
-//      has a ball of larger r2, and c can never be taken (exact pruning).
+//      has a ball of larger r2, and c can never be taken (exact pruning).  (Offsets (2, 0, 0),
+//      (2, 2, 0), (2, 2, 1) and (2, 2, 2) are left out: they remove few or no further centres.)
 //   2. the surviving centres, emitted column by column in scan order, are sorted stably by r2
 //      descending: their rank is the order in which the reference would let them win.
 //   3. every centre lists the pair cells (4 x 2 tiles of 8 x 8 x 64 voxels along z, x) its ball
@@ -82,10 +84,14 @@ __global__ void far_shells(uint32_t* __restrict__ shell, uint32_t max_e, int R) 
     for (int c = 0; ab + (uint32_t)(c * c) <= max_e; ++c) {
         uint32_t p2 = ab + (uint32_t)(c * c);
         int hi = max(a, max(b, c)), lo = min(a, min(b, c));
-        int s = a + b + c;
-        atomicMax(&shell[p2], p2 + 2u * hi + 1u);
-        atomicMax(&shell[L + p2], p2 + 2u * (uint32_t)(s - lo) + 2u);
-        atomicMax(&shell[2 * L + p2], p2 + 2u * (uint32_t)s + 3u);
+        int s = a + b + c, mid = s - hi - lo;
+        // |p' - d|^2 = p2 - 2 p'.d + |d|^2, largest for the image p' of p whose largest
+        // coordinates point away from d
+        atomicMax(&shell[p2], p2 + 2u * hi + 1u);                                    // (1, 0, 0)
+        atomicMax(&shell[L + p2], p2 + 2u * (uint32_t)(s - lo) + 2u);                // (1, 1, 0)
+        atomicMax(&shell[2 * L + p2], p2 + 2u * (uint32_t)s + 3u);                   // (1, 1, 1)
+        atomicMax(&shell[3 * L + p2], p2 + 2u * (uint32_t)(2 * hi + mid) + 5u);      // (2, 1, 0)
+        atomicMax(&shell[4 * L + p2], p2 + 2u * (uint32_t)(hi + s) + 6u);            // (2, 1, 1)
     }
 }
 
@@ -115,17 +121,25 @@ __global__ void __launch_bounds__(1024) far_prefix_max(uint32_t* __restrict__ sh
     }
 }
 
-// Block = 4 rows of 64 voxels along y; each thread walks its (x, y) column through every plane,
-// with planes z - 1 .. z + 2 (and a one-voxel halo in x and y; outside the image reads 0) in
-// a 4-slot LDS ring: while plane z is tested, plane z + 2 (loaded one step earlier) goes into
-// the ring and plane z + 3 is loaded into registers.  With FarLds the far tables are copied
-// to LDS first (3 L words of dynamic shared memory).  Marks
-// solid 0, void 127 and surviving centres kCandidate, and writes the column's count of
-// surviving centres at col_count[x * Y + y] (the columns in the reference's scan order).
+// Block = 4 rows of 64 voxels along y; each thread walks its (x, y) column through every plane.
+// Each plane passes once through a 2-slot LDS ring (with a two-voxel halo in x and y; outside the
+// image reads 0): at step z the ring holds plane z + 2 while plane z + 3, loaded one step
+// earlier, goes into the other slot and plane z + 4 is loaded into registers.  From plane z + 2
+// the thread takes, for its own column, the maxima over the in-plane offset sets
+//   M0 {(0,0)}, Ma {(0,±1),(±1,0)}, Mb {(±1,±1)}, Mc {(±1,±2),(±2,±1)}, Md {(0,±2),(±2,0)}
+// and keeps them for planes z - 2 .. z + 2 in registers; the class maxima of voxel z are then
+//   (1,0,0): Ma[z], M0[z±1]            (1,1,0): Mb[z], Ma[z±1]       (1,1,1): Mb[z±1]
+//   (2,1,0): Mc[z], Md[z±1], Ma[z±2]   (2,1,1): Mc[z±1], Mb[z±2]
+// (every offset of a class exactly once: 6, 12, 8, 24 and 24 offsets).  With FarLds the far
+// tables are copied to LDS first (kFarClasses L words of dynamic shared memory).  Marks solid 0,
+// void 127 and surviving centres kCandidate, and writes the column's count of surviving centres
+// at col_count[x * Y + y] (the columns in the reference's scan order).
 // c is dropped when a neighbour q of offset class k has q > c and q >= far_k(c), i.e. when the
 // class maximum of q reaches max(c + 1, far_k(c)).
-constexpr int kPruneW = 64 + 2, kPruneH = 4 + 2, kPrunePlane = kPruneW * kPruneH;
+constexpr int kPruneHalo = 2, kPruneW = 64 + 2 * kPruneHalo, kPruneH = 4 + 2 * kPruneHalo;
+constexpr int kPrunePlane = kPruneW * kPruneH;
 constexpr int kPruneLoads = (kPrunePlane + 255) / 256;
+constexpr int kFarClasses = 5;
 
 constexpr uint32_t kPruneFarLds = PMA_CB_PRUNE_FAR_LDS;
 
@@ -133,10 +147,10 @@ template <bool FarLds>
 __global__ void __launch_bounds__(256) prune_centres(const uint32_t* __restrict__ e, uint8_t* __restrict__ mask,
                                                      const uint32_t* __restrict__ far_global, uint32_t L,
                                                      uint32_t* __restrict__ col_count, int X, int Y, int Z) {
-    __shared__ uint32_t ring[4][kPrunePlane];
+    __shared__ uint32_t ring[2][kPrunePlane];
     extern __shared__ uint32_t far_lds[];
     if (FarLds)
-        for (uint32_t k = threadIdx.x; k < 3 * L; k += blockDim.x) far_lds[k] = far_global[k];
+        for (uint32_t k = threadIdx.x; k < kFarClasses * L; k += blockDim.x) far_lds[k] = far_global[k];
     const uint32_t* far = FarLds ? far_lds : far_global;
     const int lane = threadIdx.x & 63, row = threadIdx.x >> 6;
     const int y = blockIdx.x * 64 + lane, x = blockIdx.y * 4 + row;
@@ -144,7 +158,7 @@ __global__ void __launch_bounds__(256) prune_centres(const uint32_t* __restrict_
 #pragma unroll
         for (int j = 0; j < kPruneLoads; ++j) {
             const int k = threadIdx.x + 256 * j;
-            const int xx = blockIdx.y * 4 - 1 + k / kPruneW, yy = blockIdx.x * 64 - 1 + k % kPruneW;
+            const int xx = blockIdx.y * 4 - kPruneHalo + k / kPruneW, yy = blockIdx.x * 64 - kPruneHalo + k % kPruneW;
             r[j] = (k < kPrunePlane && z >= 0 && z < Z && xx >= 0 && xx < X && yy >= 0 && yy < Y)
                        ? e[((size_t)z * X + xx) * Y + yy]
                        : 0u;
@@ -154,46 +168,60 @@ __global__ void __launch_bounds__(256) prune_centres(const uint32_t* __restrict_
 #pragma unroll
         for (int j = 0; j < kPruneLoads; ++j) {
             const int k = threadIdx.x + 256 * j;
-            if (k < kPrunePlane) ring[(z + 4) & 3][k] = r[j];
+            if (k < kPrunePlane) ring[z & 1][k] = r[j];
         }
     };
+    // window[s][t]: set s (M0, Ma, Mb, Mc, Md) of plane z - 2 + t
+    uint32_t w0[5] = {}, wa[5] = {}, wb[5] = {}, wc[5] = {}, wd[5] = {};
     uint32_t r[kPruneLoads], ahead[kPruneLoads];
-    for (int z = -1; z <= 1; ++z) {
-        fetch(z, r);
-        store(z, r);
-    }
-    fetch(2, r);
+    fetch(-2, r);
+    store(-2, r);
+    fetch(-1, r);
     __syncthreads();
     uint32_t kept = 0;
-    const int o = (row + 1) * kPruneW + lane + 1;
-    for (int z = 0; z < Z; ++z) {
-        // plane z + 3 is loaded a whole step before it is stored, so its latency is hidden
-        fetch(z + 3, ahead);
-        if (x < X && y < Y) {
+    const int o = (row + kPruneHalo) * kPruneW + lane + kPruneHalo;
+    for (int z = -4; z < Z; ++z) {
+        // plane z + 4 is loaded a whole step before it is stored, so its latency is hidden
+        fetch(z + 4, ahead);
+        {
+            const uint32_t* pl = ring[(z + 2) & 1];
+            auto at = [&](int dx, int dy) { return pl[o + dx * kPruneW + dy]; };
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                w0[t] = w0[t + 1];
+                wa[t] = wa[t + 1];
+                wb[t] = wb[t + 1];
+                wc[t] = wc[t + 1];
+                wd[t] = wd[t + 1];
+            }
+            w0[4] = at(0, 0);
+            wa[4] = max(max(at(0, -1), at(0, 1)), max(at(-1, 0), at(1, 0)));
+            wb[4] = max(max(at(-1, -1), at(-1, 1)), max(at(1, -1), at(1, 1)));
+            wc[4] = max(max(max(at(-1, -2), at(-1, 2)), max(at(1, -2), at(1, 2))),
+                        max(max(at(-2, -1), at(-2, 1)), max(at(2, -1), at(2, 1))));
+            wd[4] = max(max(at(0, -2), at(0, 2)), max(at(-2, 0), at(2, 0)));
+        }
+        if (z >= 0 && x < X && y < Y) {
             const size_t i = ((size_t)z * X + x) * Y + y;
-            const uint32_t c = ring[z & 3][o];
+            const uint32_t c = w0[2];
             if (c == 0) {
                 mask[i] = 0;
             } else {
-                uint32_t m1 = 0, m2 = 0, m3 = 0;
-                for (int dz = -1; dz <= 1; ++dz) {
-                    const uint32_t* pl = ring[(z + dz + 4) & 3];
-                    for (int dx = -1; dx <= 1; ++dx)
-                        for (int dy = -1; dy <= 1; ++dy) {
-                            const int k = abs(dz) + abs(dx) + abs(dy);
-                            const uint32_t q = pl[o + dx * kPruneW + dy];
-                            if (k == 1) m1 = max(m1, q);
-                            else if (k == 2) m2 = max(m2, q);
-                            else if (k == 3) m3 = max(m3, q);
-                        }
-                }
-                const bool keep = !(m1 >= max(c + 1, far[c]) || m2 >= max(c + 1, far[L + c]) ||
-                                    m3 >= max(c + 1, far[2 * (size_t)L + c]));
+                const uint32_t m[kFarClasses] = {
+                    max(wa[2], max(w0[1], w0[3])),
+                    max(wb[2], max(wa[1], wa[3])),
+                    max(wb[1], wb[3]),
+                    max(max(wc[2], max(wd[1], wd[3])), max(wa[0], wa[4])),
+                    max(max(wc[1], wc[3]), max(wb[0], wb[4])),
+                };
+                bool keep = true;
+#pragma unroll
+                for (int k = 0; k < kFarClasses; ++k) keep = keep && m[k] < max(c + 1, far[(size_t)k * L + c]);
                 mask[i] = keep ? kCandidate : 127;
                 kept += keep;
             }
         }
-        store(z + 2, r);  // the slot of plane z - 2, last read in the previous step
+        store(z + 3, r);  // the slot of plane z + 1, last read in the previous step
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < kPruneLoads; ++j) r[j] = ahead[j];
@@ -629,11 +657,11 @@ void run_containing_balls(pma_ctx* c) {
     const uint32_t max_e = c->max_e;
     const int R = (int)std::floor(std::sqrt((double)max_e));
     const size_t L = (size_t)max_e + 1;
-    uint32_t* far = c->buf<uint32_t>("cb_far", 3 * L);
-    PMA_HIP(hipMemsetAsync(far, 0, 3 * L * sizeof(uint32_t), c->stream));
+    uint32_t* far = c->buf<uint32_t>("cb_far", kFarClasses * L);
+    PMA_HIP(hipMemsetAsync(far, 0, kFarClasses * L * sizeof(uint32_t), c->stream));
     far_shells<<<dim3(grid_for(R + 1, 64), R + 1), 64, 0, c->stream>>>(far, max_e, R);
     PMA_LAUNCHED(c);
-    far_prefix_max<<<3, 1024, 0, c->stream>>>(far, L);
+    far_prefix_max<<<kFarClasses, 1024, 0, c->stream>>>(far, L);
     PMA_LAUNCHED(c);
 
     const uint32_t* e = c->v<uint32_t>(PMA_VOL_EDT2);
@@ -647,7 +675,7 @@ void run_containing_balls(pma_ctx* c) {
     {
         PMA_TIMED(c, "prune_centres");
         if (L <= kPruneFarLds)
-            prune_centres<true><<<cgrid, 256, 3 * L * sizeof(uint32_t), c->stream>>>(e, mask, far, (uint32_t)L,
+            prune_centres<true><<<cgrid, 256, kFarClasses * L * sizeof(uint32_t), c->stream>>>(e, mask, far, (uint32_t)L,
                                                                                      col_count, X, Y, Z);
         else
             prune_centres<false><<<cgrid, 256, 0, c->stream>>>(e, mask, far, (uint32_t)L, col_count, X, Y, Z);
