@@ -371,29 +371,43 @@ __global__ void emit_pairs(const Ball* __restrict__ balls, size_t n, const unsig
     ball_tiles<V>(balls[r], X, Y, Z, g, cz0, cz1, cell_base, (uint32_t)r, off, r2_shift, pk, pv, at[r]);
 }
 
-// [start, end) of every cell's run in the cell-sorted pair list; when the pairs carry ranks,
-// also the balls themselves in that order (so that a tile reads its cell's balls contiguously).
-template <bool Gather>
+// [start, end) of every cell's run in the cell-sorted (cell, rank) pair list, and the balls
+// themselves in that order (so that a tile reads its cell's balls contiguously).
 __global__ void tile_ranges(const uint32_t* __restrict__ pk, const uint32_t* __restrict__ rank, size_t P,
                             const Ball* __restrict__ balls, Ball* __restrict__ cell_balls,
                             uint32_t* __restrict__ start, uint32_t* __restrict__ end) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P) return;
-    if (Gather) cell_balls[i] = balls[rank[i]];
+    cell_balls[i] = balls[rank[i]];
     const uint32_t t = pk[i];
     if (i == 0 || pk[i - 1] != t) start[t] = (uint32_t)i;
     if (i + 1 == P || pk[i + 1] != t) end[t] = (uint32_t)(i + 1);
 }
 
-// The same for words that carry their cell in bits [33, 33 + cell_bits).
-__global__ void tile_ranges_fused(const uint64_t* __restrict__ w, size_t P, uint32_t cell_mask,
-                                  uint32_t* __restrict__ start, uint32_t* __restrict__ end) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P) return;
-    auto cell = [&](size_t j) { return (uint32_t)(w[j] >> 33) & cell_mask; };
-    const uint32_t t = cell(i);
-    if (i == 0 || cell(i - 1) != t) start[t] = (uint32_t)i;
-    if (i + 1 == P || cell(i + 1) != t) end[t] = (uint32_t)(i + 1);
+// The same when no gather is needed, by binary search (one thread per cell): the list is sorted
+// by cell, so a cell's run starts at the first entry whose cell is not below it.  The cells come
+// from the words' bits [33, 33 + cell_bits) (Fused) or from separate keys.
+template <bool Fused>
+__global__ void cell_bounds(const uint64_t* __restrict__ words, const uint32_t* __restrict__ keys, size_t P,
+                            uint32_t cell_mask, uint32_t n_cells, uint32_t* __restrict__ start,
+                            uint32_t* __restrict__ end) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_cells) return;
+    auto cell = [&](size_t j) -> uint32_t {
+        if constexpr (Fused) return (uint32_t)(words[j] >> 33) & cell_mask;
+        else return keys[j];
+    };
+    auto first_not_below = [&](uint32_t v) {
+        size_t lo = 0, hi = P;
+        while (lo < hi) {
+            const size_t mid = lo + (hi - lo) / 2;
+            if (cell(mid) < v) lo = mid + 1;
+            else hi = mid;
+        }
+        return (uint32_t)lo;
+    };
+    start[t] = first_not_below(t);
+    end[t] = first_not_below(t + 1);
 }
 
 // floor(sqrt(v)) from one correctly rounded float square root and a one-step correction.
@@ -776,8 +790,8 @@ void run_containing_balls(pma_ctx* c) {
                                                                                    nullptr, pb);
                 PMA_LAUNCHED(c);
                 radix_sort_keys_u64(c, pb, pb2, P, 33, 33 + cell_bits);
-                tile_ranges_fused<<<grid_for(P, 256), 256, 0, c->stream>>>(pb2, P, (1u << cell_bits) - 1u, tstart,
-                                                                           tend);
+                cell_bounds<true><<<grid_for(slab_cells, 256), 256, 0, c->stream>>>(
+                    pb2, nullptr, P, (1u << cell_bits) - 1u, slab_cells, tstart, tend);
                 PMA_LAUNCHED(c);
             } else if (packed) {
                 emit_pairs<uint64_t><<<grid_for(n_cand, 256), 256, 0, c->stream>>>(balls, n_cand, poff, X, Y, Z, g, tz0,
@@ -785,8 +799,8 @@ void run_containing_balls(pma_ctx* c) {
                                                                                    pb);
                 PMA_LAUNCHED(c);
                 radix_sort_pairs_u32_u64(c, pk, pk2, pb, pb2, P, 0, cell_bits);
-                tile_ranges<false><<<grid_for(P, 256), 256, 0, c->stream>>>(pk2, nullptr, P, balls, nullptr, tstart,
-                                                                            tend);
+                cell_bounds<false><<<grid_for(slab_cells, 256), 256, 0, c->stream>>>(nullptr, pk2, P, 0u, slab_cells,
+                                                                                     tstart, tend);
                 PMA_LAUNCHED(c);
             } else {
                 emit_pairs<uint32_t><<<grid_for(n_cand, 256), 256, 0, c->stream>>>(balls, n_cand, poff, X, Y, Z, g, tz0,
@@ -794,8 +808,7 @@ void run_containing_balls(pma_ctx* c) {
                                                                                    pv);
                 PMA_LAUNCHED(c);
                 radix_sort_pairs_u32(c, pk, pk2, pv, pv2, P, 0, cell_bits);
-                tile_ranges<true><<<grid_for(P, 256), 256, 0, c->stream>>>(pk2, pv2, P, balls, cell_balls, tstart,
-                                                                           tend);
+                tile_ranges<<<grid_for(P, 256), 256, 0, c->stream>>>(pk2, pv2, P, balls, cell_balls, tstart, tend);
                 PMA_LAUNCHED(c);
             }
         }

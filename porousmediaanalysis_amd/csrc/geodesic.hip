@@ -133,19 +133,21 @@ __global__ void fill_key(uint32_t* __restrict__ p, uint32_t v, size_t n) {
 // Splits the active list: keys within kDelta of its smallest key go to `proc` (their key is
 // cleared, so that activations during the round start it afresh), the others are carried into
 // the next list (tagged so that a new activation does not list them twice).  Grid-stride over
-// the list, whose length is cur->count.
+// the list, whose length is cur->count, 256 entries per workgroup step; each step reserves its
+// places in the two lists with one atomic per list.
 __global__ void __launch_bounds__(256) partition_active(const uint32_t* __restrict__ active,
                                                         RoundState* __restrict__ cur, uint32_t* __restrict__ block_key,
                                                         uint32_t* __restrict__ proc, uint32_t* __restrict__ next,
                                                         uint32_t* __restrict__ block_tag, uint32_t tag,
                                                         RoundState* __restrict__ nxt) {
+    __shared__ uint32_t wave_now[4], wave_later[4], base_now, base_later;
     const uint32_t n = cur->count;
     const float thr = __uint_as_float(cur->kmin) + kDelta;
     const uint32_t threshold = isfinite(thr) ? __float_as_uint(thr) : 0xFFFFFFFFu;
-    const int lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const unsigned long long below = (1ull << lane) - 1;
-    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n; base += gridDim.x * blockDim.x) {
-        const uint32_t t = base + (uint32_t)lane;  // `base` is uniform per wave
+    for (uint32_t step = blockIdx.x * blockDim.x; step < n; step += gridDim.x * blockDim.x) {  // uniform
+        const uint32_t t = step + threadIdx.x;
         uint32_t b = 0, key = 0;
         bool now = false, later = false;
         if (t < n) {
@@ -157,15 +159,26 @@ __global__ void __launch_bounds__(256) partition_active(const uint32_t* __restri
         if (now) block_key[b] = kFltMaxBits;
         if (later) (void)atomicExch(&block_tag[b], tag);
         const unsigned long long bn = __ballot(now), bl = __ballot(later);
-        uint32_t base_n = 0, base_l = 0;
         if (lane == 0) {
-            if (bn) base_n = atomicAdd(&cur->proc, (uint32_t)__popcll(bn));
-            if (bl) base_l = atomicAdd(&nxt->count, (uint32_t)__popcll(bl));
+            wave_now[wave] = (uint32_t)__popcll(bn);
+            wave_later[wave] = (uint32_t)__popcll(bl);
         }
-        base_n = (uint32_t)__shfl((int)base_n, 0);
-        base_l = (uint32_t)__shfl((int)base_l, 0);
-        if (now) proc[base_n + (uint32_t)__popcll(bn & below)] = b;
-        if (later) next[base_l + (uint32_t)__popcll(bl & below)] = b;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint32_t tn = wave_now[0] + wave_now[1] + wave_now[2] + wave_now[3];
+            const uint32_t tl = wave_later[0] + wave_later[1] + wave_later[2] + wave_later[3];
+            base_now = tn ? atomicAdd(&cur->proc, tn) : 0u;
+            base_later = tl ? atomicAdd(&nxt->count, tl) : 0u;
+        }
+        __syncthreads();
+        uint32_t off_n = base_now, off_l = base_later;
+        for (int w = 0; w < wave; ++w) {
+            off_n += wave_now[w];
+            off_l += wave_later[w];
+        }
+        if (now) proc[off_n + (uint32_t)__popcll(bn & below)] = b;
+        if (later) next[off_l + (uint32_t)__popcll(bl & below)] = b;
+        __syncthreads();  // the counts are read before the next step overwrites them
     }
 }
 
