@@ -3,7 +3,8 @@
 // The reference computes, per void voxel, the exact squared distance to the nearest solid
 // voxel or to the virtual solid planes x = -X and x = 2X-1 that its X-pass start value
 // X-1 implies (:164, :180).  The same value is produced here by three separable passes:
-//   pass X: one lane per (y, z) column, forward + backward scan of the 1-D distance,
+//   pass X: one lane per (y, z) column, forward + backward scan of the 1-D distance, kept
+//           unsquared in 16 bits (at most 2X - 1 < 2^15),
 //   pass Y: one workgroup per (x, z) row staged in LDS, exact windowed minimum of
 //           g(y') + (y-y')^2 over |y-y'| <= floor(sqrt(g(y))) (the reference's window), found
 //           by scanning outward from y until dy^2 >= the best value so far,
@@ -19,7 +20,7 @@ namespace {
 constexpr int kLocalBins = 2048;
 
 __global__ void __launch_bounds__(256) edt_pass_x(const uint8_t* __restrict__ solid,
-                                                  uint32_t* __restrict__ g, int X, int Y) {
+                                                  uint16_t* __restrict__ g, int X, int Y) {
     int y = blockIdx.x * blockDim.x + threadIdx.x;
     if (y >= Y) return;
     size_t base = (size_t)blockIdx.y * X * Y + y;
@@ -27,21 +28,20 @@ __global__ void __launch_bounds__(256) edt_pass_x(const uint8_t* __restrict__ so
     for (int x = 0; x < X; ++x) {
         size_t i = base + (size_t)x * Y;
         d = solid[i] ? 0u : d + 1u;
-        g[i] = d;
+        g[i] = (uint16_t)d;
     }
     d = (uint32_t)(X - 1);
     for (int x = X - 1; x >= 0; --x) {
         size_t i = base + (size_t)x * Y;
         d = solid[i] ? 0u : d + 1u;
-        uint32_t f = min(g[i], d);
-        g[i] = f * f;
+        g[i] = (uint16_t)min((uint32_t)g[i], d);
     }
 }
 
 // Pass X for Y a multiple of 4: a thread takes four adjacent (y, z) columns (4-byte loads,
-// 16-byte stores), the same scans as edt_pass_x.
+// 8-byte stores), the same scans as edt_pass_x.
 __global__ void __launch_bounds__(256) edt_pass_x4(const uint8_t* __restrict__ solid,
-                                                   uint32_t* __restrict__ g, int X, int Y) {
+                                                   uint16_t* __restrict__ g, int X, int Y) {
     const int y = (blockIdx.x * blockDim.x + threadIdx.x) * 4;
     if (y >= Y) return;
     const size_t base = (size_t)blockIdx.y * X * Y + y;
@@ -50,16 +50,16 @@ __global__ void __launch_bounds__(256) edt_pass_x4(const uint8_t* __restrict__ s
         const size_t i = base + (size_t)x * Y;
         const uchar4 s = *reinterpret_cast<const uchar4*>(solid + i);
         d = make_uint4(s.x ? 0u : d.x + 1u, s.y ? 0u : d.y + 1u, s.z ? 0u : d.z + 1u, s.w ? 0u : d.w + 1u);
-        *reinterpret_cast<uint4*>(g + i) = d;
+        *reinterpret_cast<ushort4*>(g + i) = make_ushort4(d.x, d.y, d.z, d.w);
     }
     d = make_uint4(X - 1, X - 1, X - 1, X - 1);
     for (int x = X - 1; x >= 0; --x) {
         const size_t i = base + (size_t)x * Y;
         const uchar4 s = *reinterpret_cast<const uchar4*>(solid + i);
         d = make_uint4(s.x ? 0u : d.x + 1u, s.y ? 0u : d.y + 1u, s.z ? 0u : d.z + 1u, s.w ? 0u : d.w + 1u);
-        const uint4 f = *reinterpret_cast<const uint4*>(g + i);
-        const uint4 m = make_uint4(min(f.x, d.x), min(f.y, d.y), min(f.z, d.z), min(f.w, d.w));
-        *reinterpret_cast<uint4*>(g + i) = make_uint4(m.x * m.x, m.y * m.y, m.z * m.z, m.w * m.w);
+        const ushort4 f = *reinterpret_cast<const ushort4*>(g + i);
+        *reinterpret_cast<ushort4*>(g + i) = make_ushort4(min((uint32_t)f.x, d.x), min((uint32_t)f.y, d.y),
+                                                          min((uint32_t)f.z, d.z), min((uint32_t)f.w, d.w));
     }
 }
 
@@ -73,13 +73,14 @@ constexpr int kHaloY = 32;
 // Inputs of pass Y are below (2X)^2 <= 2^30 for X <= 16383, and g + k^2 stays below 2^31.
 constexpr uint32_t kFarY = 0x40000000u;
 
-__global__ void __launch_bounds__(256) edt_pass_y(const uint32_t* __restrict__ g,
+__global__ void __launch_bounds__(256) edt_pass_y(const uint16_t* __restrict__ dx,
                                                   uint32_t* __restrict__ h, int X, int Y) {
-    extern __shared__ uint32_t line[];  // line[kHaloY + y] = g(y) for 0 <= y < Y, kFarY outside
+    extern __shared__ uint32_t line[];  // line[kHaloY + y] = g(y) = dx(y)^2 for 0 <= y < Y, kFarY outside
     const size_t base = ((size_t)blockIdx.y * X + blockIdx.x) * Y;
     for (int i = threadIdx.x; i < Y + 2 * kHaloY; i += blockDim.x) {
         const int y = i - kHaloY;
-        line[i] = (y >= 0 && y < Y) ? g[base + y] : kFarY;
+        const uint32_t d = (y >= 0 && y < Y) ? dx[base + y] : 0u;
+        line[i] = (y >= 0 && y < Y) ? d * d : kFarY;
     }
     __syncthreads();
     const int lane = threadIdx.x & 63;
@@ -189,6 +190,7 @@ void run_edt(pma_ctx* c) {
     c->ensure_volume(PMA_VOL_EDT2);
     uint32_t* e = c->v<uint32_t>(PMA_VOL_EDT2);
     uint32_t* tmp = c->buf<uint32_t>("edt_tmp", c->n);
+    uint16_t* dx = reinterpret_cast<uint16_t*>(e);  // pass X's 16-bit distances, in the first half of e
     const uint8_t* solid = c->v<uint8_t>(PMA_VOL_SOLID);
 
     // Counted squares satisfy e <= m*m with m <= (min(X,Y,Z)-1)/2.
@@ -205,16 +207,16 @@ void run_edt(pma_ctx* c) {
         PMA_TIMED(c, "edt_pass_x");
 
         if (Y % 4 == 0)
-            edt_pass_x4<<<dim3(grid_for(Y / 4, 256), Z), 256, 0, c->stream>>>(solid, e, X, Y);
+            edt_pass_x4<<<dim3(grid_for(Y / 4, 256), Z), 256, 0, c->stream>>>(solid, dx, X, Y);
         else
-            edt_pass_x<<<dim3(grid_for(Y, 256), Z), 256, 0, c->stream>>>(solid, e, X, Y);
+            edt_pass_x<<<dim3(grid_for(Y, 256), Z), 256, 0, c->stream>>>(solid, dx, X, Y);
 
         PMA_LAUNCHED(c);
 
     }
     {
         PMA_TIMED(c, "edt_pass_y");
-        edt_pass_y<<<dim3(X, Z), 256, (size_t)(Y + 2 * kHaloY) * sizeof(uint32_t), c->stream>>>(e, tmp, X, Y);
+        edt_pass_y<<<dim3(X, Z), 256, (size_t)(Y + 2 * kHaloY) * sizeof(uint32_t), c->stream>>>(dx, tmp, X, Y);
         PMA_LAUNCHED(c);
     }
     {

@@ -43,6 +43,10 @@ struct uint2 { uint32_t x, y; };
 struct alignas(16) uint4 { uint32_t x, y, z, w; };
 struct alignas(16) float4 { float x, y, z, w; };
 struct alignas(4) uchar4 { uint8_t x, y, z, w; };
+struct alignas(8) ushort4 { uint16_t x, y, z, w; };
+inline ushort4 make_ushort4(uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
+    return ushort4{(uint16_t)a, (uint16_t)b, (uint16_t)c, (uint16_t)d};
+}
 inline uint4 make_uint4(uint32_t a, uint32_t b, uint32_t c, uint32_t d) { return uint4{a, b, c, d}; }
 inline float4 make_float4(float a, float b, float c, float d) { return float4{a, b, c, d}; }
 inline uint2 make_uint2(uint32_t a, uint32_t b) { return uint2{a, b}; }
