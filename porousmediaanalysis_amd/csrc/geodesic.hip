@@ -105,7 +105,9 @@ __global__ void __launch_bounds__(256) find_start(const uint8_t* __restrict__ so
 constexpr int kB = 8;           // block edge
 constexpr float kDelta = 16.0f;  // bucket width of a round (distance units; 4..64 measured within 10%)
 #ifndef PMA_GEO_RELAX_GRID
-#define PMA_GEO_RELAX_GRID 8192u  // one-wave workgroups of a relax launch (32 per CU)
+// one-wave workgroups of a relax launch: 64 per CU, twice what fits at once, so that waves that
+// finish early are replaced (16384 measured best of 2048..16384 at 1024^3)
+#define PMA_GEO_RELAX_GRID 16384u
 #endif
 #ifndef PMA_GEO_PUSH_BUF
 #define PMA_GEO_PUSH_BUF 256u  // per-wave buffer of activated blocks (more than 27)
@@ -295,12 +297,16 @@ __global__ void __launch_bounds__(64) relax_blocks(float* __restrict__ dist, uin
         const int z0 = bz * kB - 1, x0 = bx * kB - 1, y0 = by * kB - 1;
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // the previous block's LDS use is over
         __builtin_amdgcn_wave_barrier();
-        // all loads first, then all stores, so that the loads are in flight together
+        // all loads first, then all stores, so that the loads are in flight together.  The lane
+        // number is passed through an empty asm so that the compiler recomputes the staging
+        // offsets per block instead of keeping them live across the whole loop.
+        int slane = lane;
+        asm volatile("" : "+v"(slane));
         constexpr int kStage = (kHalo + 63) / 64;
         float staged[kStage];
 #pragma unroll
         for (int j = 0; j < kStage; ++j) {
-            const int k = lane + 64 * j;
+            const int k = slane + 64 * j;
             const int lz = k / (kH * kH), lx = (k / kH) % kH, ly = k % kH;
             const int z = z0 + lz, x = x0 + lx, y = y0 + ly;
             const bool in = k < kHalo && z >= 0 && z < Z && x >= 0 && x < X && y >= 0 && y < Y;
@@ -310,7 +316,7 @@ __global__ void __launch_bounds__(64) relax_blocks(float* __restrict__ dist, uin
         }
 #pragma unroll
         for (int j = 0; j < kStage; ++j)
-            if (lane + 64 * j < kHalo) sd[lane + 64 * j] = staged[j];
+            if (slane + 64 * j < kHalo) sd[slane + 64 * j] = staged[j];
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
 
