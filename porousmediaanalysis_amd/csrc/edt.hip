@@ -140,35 +140,47 @@ __global__ void __launch_bounds__(256) edt_pass_z(const uint32_t* __restrict__ h
     __syncthreads();
     uint32_t my_max = 0;
     const int mxy = min(min(x, y), min(X - 1 - x, Y - 1 - y));
-    for (int z = z0 + sub; z < z1; z += 4) {  // the same trip count on every lane of a wave
-        const int r = z - s0;
-        uint32_t best = in_y ? col[r][lane] : 0u;
-        const uint32_t* below = &col[r - kHaloZ][lane];  // below[(kHaloZ - k) * 64] is plane z - k
-        const uint32_t* above = &col[r][lane];           // above[k * 64] is plane z + k
+    // two planes, z and z + 4, per step (their searches interleave); the trip count and `has_b`
+    // are the same on every lane of a wave
+    for (int z = z0 + sub; z < z1; z += 8) {
+        const bool has_b = z + 4 < z1;
+        const int ra = z - s0, rb = ra + (has_b ? 4 : 0);
+        uint32_t best_a = in_y ? col[ra][lane] : 0u;
+        uint32_t best_b = (in_y && has_b) ? col[rb][lane] : 0u;
+        const uint32_t* below_a = &col[ra - kHaloZ][lane];  // below_a[(kHaloZ - k) * 64] is plane z - k
+        const uint32_t* above_a = &col[ra][lane];           // above_a[k * 64] is plane z + k
+        const uint32_t* below_b = &col[rb - kHaloZ][lane];
+        const uint32_t* above_b = &col[rb][lane];
 #pragma unroll
         for (int k0 = 1; k0 <= kHaloZ; k0 += 4) {
-            if (!__any((uint32_t)(k0 * k0) < best)) break;
+            if (!__any((uint32_t)(k0 * k0) < max(best_a, best_b))) break;
 #pragma unroll
-            for (int k = k0; k < k0 + 4; ++k)
-                best = min(best, min(below[(kHaloZ - k) * 64], above[k * 64]) + (uint32_t)(k * k));
-        }
-        if (best > (uint32_t)((kHaloZ + 1) * (kHaloZ + 1))) {
-            for (uint32_t k = kHaloZ + 1; k * k < best; ++k) {
-                const uint32_t a = z >= (int)k ? h[base + (size_t)(z - (int)k) * stride] : kFarZ;
-                const uint32_t b = z + (int)k < Z ? h[base + (size_t)(z + (int)k) * stride] : kFarZ;
-                best = min(best, min(a, b) + k * k);
+            for (int k = k0; k < k0 + 4; ++k) {
+                best_a = min(best_a, min(below_a[(kHaloZ - k) * 64], above_a[k * 64]) + (uint32_t)(k * k));
+                best_b = min(best_b, min(below_b[(kHaloZ - k) * 64], above_b[k * 64]) + (uint32_t)(k * k));
             }
         }
-        if (best != 0) {
-            my_max = max(my_max, best);
-            // sqrt(best) > m  <=>  best > m*m for integer m >= 0
-            const int m = min(mxy, min(z, Z - 1 - z));
-            if ((uint64_t)best <= (uint64_t)m * (uint64_t)m) {
-                if (best < kLocalBins) atomicAdd(&lhist[best], 1u);
-                else if (best < hist_len) atomicAdd(&hist[best], 1ull);
+        auto finish = [&](int zz, uint32_t best) {
+            if (best > (uint32_t)((kHaloZ + 1) * (kHaloZ + 1))) {
+                for (uint32_t k = kHaloZ + 1; k * k < best; ++k) {
+                    const uint32_t a = zz >= (int)k ? h[base + (size_t)(zz - (int)k) * stride] : kFarZ;
+                    const uint32_t b = zz + (int)k < Z ? h[base + (size_t)(zz + (int)k) * stride] : kFarZ;
+                    best = min(best, min(a, b) + k * k);
+                }
             }
-        }
-        if (in_y) e[base + (size_t)z * stride] = best;
+            if (best != 0) {
+                my_max = max(my_max, best);
+                // sqrt(best) > m  <=>  best > m*m for integer m >= 0
+                const int m = min(mxy, min(zz, Z - 1 - zz));
+                if ((uint64_t)best <= (uint64_t)m * (uint64_t)m) {
+                    if (best < kLocalBins) atomicAdd(&lhist[best], 1u);
+                    else if (best < hist_len) atomicAdd(&hist[best], 1ull);
+                }
+            }
+            if (in_y) e[base + (size_t)zz * stride] = best;
+        };
+        finish(z, best_a);
+        if (has_b) finish(z + 4, best_b);
     }
     for (int off = 32; off > 0; off >>= 1) my_max = max(my_max, (uint32_t)__shfl_xor((int)my_max, off));
     if (lane == 0) wave_max[sub] = my_max;
