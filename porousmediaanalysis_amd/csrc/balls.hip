@@ -560,7 +560,8 @@ __global__ void __launch_bounds__(64) cover_tiles(const Ball* __restrict__ cell_
             int h;
             if (Root == kRootTable) h = root_tab[max(rem, -1) + 1];  // rem <= r2 < kRootTableSize - 1
             else h = rem < 0 ? -1 : (Root == kRootSmall ? isqrt_small(rem) : (int)isqrt_fast((uint32_t)rem));
-            const int lo = max(by - h - y0, 0), hi = min(by + h - y0, 63);
+            const int by_t = __builtin_amdgcn_readfirstlane(by - y0);  // the centre's y within the tile (scalar)
+            const int lo = max(by_t - h, 0), hi = min(by_t + h, 63);
             uint64_t nw = lo <= hi ? (((~0ull) >> (63 - (hi - lo))) << lo) & ~cov : 0ull;
             if (__ballot(nw != 0) == 0) continue;
             won |= 1ull << j;

@@ -224,6 +224,12 @@ inline int __mul24(int a, int b) {
     const int64_t x = (int32_t)((uint32_t)a << 8) >> 8, y = (int32_t)((uint32_t)b << 8) >> 8;
     return (int)(uint32_t)(uint64_t)(x * y);
 }
+// The library passes only wave-uniform values to readfirstlane, so the lane's own value is it.
+inline int __builtin_amdgcn_readfirstlane(int v) { return v; }
+// Unsigned 24-bit multiply: operands are the low 24 bits, the result the low 32 bits.
+inline unsigned __umul24(unsigned a, unsigned b) {
+    return (unsigned)((uint64_t)(a & 0xFFFFFFu) * (uint64_t)(b & 0xFFFFFFu));
+}
 inline unsigned __float_as_uint(float f) { unsigned u; std::memcpy(&u, &f, 4); return u; }
 inline float __uint_as_float(unsigned u) { float f; std::memcpy(&f, &u, 4); return f; }
 inline float __fadd_rn(float a, float b) { volatile float s = a + b; return s; }
