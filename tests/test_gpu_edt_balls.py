@@ -42,6 +42,26 @@ def test_front_example_fixture(ctx):
     assert_front_equal(g, oracle_front(solid))
 
 
+def test_edt_longest_rows(ctx):
+    """Rows of the largest Y pass Y accepts (16 320 values and 32 padding entries either side in
+    64 KB of LDS); the solid lies only at y = 7, so the y searches run far past the padding.  One
+    more value along y is refused before any work."""
+    from oracle import edt as oracle_edt
+    solid = np.zeros((2, 3, 16320), np.uint8)
+    solid[:, :, 7] = 1
+    log("edt rows of 16320")
+    ctx.load(pma.VOL_SOLID, solid)
+    ctx.edt()
+    e, m, h = oracle_edt(solid)
+    assert np.array_equal(ctx.read(pma.VOL_EDT2), e)
+    assert ctx.statistics()[0] == m
+    assert np.array_equal(ctx.histogram(), h)
+    log("edt rows of 16321 (refused)")
+    ctx.load(pma.VOL_SOLID, np.zeros((2, 3, 16321), np.uint8))
+    with pytest.raises(pma.PmaError, match="Y must be <= 16320"):
+        ctx.edt()
+
+
 def test_generators_deterministic(ctx):
     log("generators")
     ctx.generate_gaussian((40, 36, 44), 3.0, 0.7, 5)
