@@ -510,19 +510,28 @@ __global__ void __launch_bounds__(64) cover_tiles(const Ball* __restrict__ cell_
     const uint32_t s = start[local], en = end[local];
     Raw next{};
     if (s + lane < en) next = raw_at(s + lane);
+    // bounding box of the voxels still uncovered (a ball that cannot reach it covers nothing new),
+    // taken again only after a batch that covered something
+    int zlo = 0, zhi = 0, xlo = 0, xhi = 0, ylo = 0, yhi = 0;
+    bool dirty = true;  // uniform
     for (uint32_t base = s; base < en; base += 64) {
-        const bool open_row = cov != ~0ull;
-        const uint64_t rows = __ballot(open_row);
-        if (rows == 0) break;  // every void voxel of the tile has its ball
-        // bounding box of the voxels still uncovered: a ball that cannot reach it covers nothing new
-        const uint64_t unc = open_row ? ~cov : 0ull;
-        const uint64_t ys = (uint64_t)__ockl_wfred_or_u32((uint32_t)unc) |
-                            (uint64_t)__ockl_wfred_or_u32((uint32_t)(unc >> 32)) << 32;
-        uint32_t xs = 0;
-        for (int r = 0; r < TZ; ++r) xs |= (uint32_t)(rows >> (r * TX)) & 0xFFu;
-        const int zlo = tz * TZ + __builtin_ctzll(rows) / TX, zhi = tz * TZ + (63 - __builtin_clzll(rows)) / TX;
-        const int xlo = tx * TX + __builtin_ctz(xs), xhi = tx * TX + 31 - __builtin_clz(xs);
-        const int ylo = y0 + __builtin_ctzll(ys), yhi = y0 + 63 - __builtin_clzll(ys);
+        if (dirty) {
+            const bool open_row = cov != ~0ull;
+            const uint64_t rows = __ballot(open_row);
+            if (rows == 0) break;  // every void voxel of the tile has its ball
+            const uint64_t unc = open_row ? ~cov : 0ull;
+            const uint64_t ys = (uint64_t)__ockl_wfred_or_u32((uint32_t)unc) |
+                                (uint64_t)__ockl_wfred_or_u32((uint32_t)(unc >> 32)) << 32;
+            uint32_t xs = 0;
+            for (int r = 0; r < TZ; ++r) xs |= (uint32_t)(rows >> (r * TX)) & 0xFFu;
+            zlo = tz * TZ + __builtin_ctzll(rows) / TX;
+            zhi = tz * TZ + (63 - __builtin_clzll(rows)) / TX;
+            xlo = tx * TX + __builtin_ctz(xs);
+            xhi = tx * TX + 31 - __builtin_clz(xs);
+            ylo = y0 + __builtin_ctzll(ys);
+            yhi = y0 + 63 - __builtin_clzll(ys);
+            dirty = false;
+        }
 
         const Raw cur = next;
         const Ball mb = ball_of(cur);  // this batch; the next one is loaded while it is processed
@@ -565,6 +574,7 @@ __global__ void __launch_bounds__(64) cover_tiles(const Ball* __restrict__ cell_
             uint64_t nw = lo <= hi ? (((~0ull) >> (63 - (hi - lo))) << lo) & ~cov : 0ull;
             if (__ballot(nw != 0) == 0) continue;
             won |= 1ull << j;
+            dirty = true;
             if (nw) {
                 cov |= nw;
                 const W slot = (W)(base - s + (uint32_t)j);
