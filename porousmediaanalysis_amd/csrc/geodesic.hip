@@ -277,6 +277,8 @@ __global__ void __launch_bounds__(64) relax_blocks(float* __restrict__ dist, uin
     const int cx = lane >> 3, cy = lane & 7;
     const int col = (cx + 1) * kH + cy + 1;  // the lane's column within a z-layer of sd
     const uint32_t count = cur->proc;
+    // the short staging path needs lz * X * Y in 24-bit operands and a 32-bit offset
+    const bool fast_plane = (uint64_t)X * (uint64_t)Y < (1ull << 24);
     uint32_t npush = 0;                      // entries in pushbuf (uniform)
     uint32_t reached = 0, walls = 0;         // this lane's voxels new to the cavity, and their planes
     auto flush = [&]() {
@@ -304,15 +306,33 @@ __global__ void __launch_bounds__(64) relax_blocks(float* __restrict__ dist, uin
         asm volatile("" : "+v"(slane));
         constexpr int kStage = (kHalo + 63) / 64;
         float staged[kStage];
+        // a block whose halo lies inside the image, in a round without a start voxel, takes the
+        // short path: 32-bit offsets from the halo's first voxel and no bounds tests (k / 100 and
+        // r / 10 by 24-bit multiplies, exact for k < 1000 and r < 100)
+        const bool interior = src == ~0ull && fast_plane && z0 >= 0 && x0 >= 0 && y0 >= 0 && z0 + kH <= Z &&
+                              x0 + kH <= X && y0 + kH <= Y;  // uniform
+        if (interior) {
+            const float* hb = dist + ((size_t)z0 * X + x0) * Y + y0;
 #pragma unroll
-        for (int j = 0; j < kStage; ++j) {
-            const int k = slane + 64 * j;
-            const int lz = k / (kH * kH), lx = (k / kH) % kH, ly = k % kH;
-            const int z = z0 + lz, x = x0 + lx, y = y0 + ly;
-            const bool in = k < kHalo && z >= 0 && z < Z && x >= 0 && x < X && y >= 0 && y < Y;
-            const size_t gi = ((size_t)z * X + x) * Y + y;
-            const float v = in ? dist[gi] : -1.0f;
-            staged[j] = (in && gi == src) ? 0.0f : (v < 0.0f ? INFINITY : v);
+            for (int j = 0; j < kStage; ++j) {
+                const uint32_t k = (uint32_t)slane + 64u * (uint32_t)j;
+                const uint32_t lz = __umul24(k, 656u) >> 16, r = k - __umul24(lz, 100u);
+                const uint32_t lx = __umul24(r, 6554u) >> 16, ly = r - __umul24(lx, 10u);
+                const uint32_t rel = __umul24(lz, (uint32_t)X * (uint32_t)Y) + __umul24(lx, (uint32_t)Y) + ly;
+                const float v = k < (uint32_t)kHalo ? hb[rel] : -1.0f;
+                staged[j] = v < 0.0f ? INFINITY : v;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < kStage; ++j) {
+                const int k = slane + 64 * j;
+                const int lz = k / (kH * kH), lx = (k / kH) % kH, ly = k % kH;
+                const int z = z0 + lz, x = x0 + lx, y = y0 + ly;
+                const bool in = k < kHalo && z >= 0 && z < Z && x >= 0 && x < X && y >= 0 && y < Y;
+                const size_t gi = ((size_t)z * X + x) * Y + y;
+                const float v = in ? dist[gi] : -1.0f;
+                staged[j] = (in && gi == src) ? 0.0f : (v < 0.0f ? INFINITY : v);
+            }
         }
 #pragma unroll
         for (int j = 0; j < kStage; ++j)
@@ -326,11 +346,16 @@ __global__ void __launch_bounds__(64) relax_blocks(float* __restrict__ dist, uin
 #pragma unroll
         for (int z = 0; z < kB; ++z) {
             const float cur = sd[(z + 1) * kH * kH + col];
-            const int gz = z0 + 1 + z;
-            const bool is_src = gz < Z && gx < X && gy < Y && ((size_t)gz * X + gx) * Y + gy == src;
             v[z] = cur;
-            orig[z] = is_src ? FLT_MAX : cur;
+            orig[z] = cur;
             open |= (cur != INFINITY ? 1u : 0u) << z;
+        }
+        if (src != ~0ull) {  // uniform: the start voxel was staged as 0 and counts as improved
+#pragma unroll
+            for (int z = 0; z < kB; ++z) {
+                const int gz = z0 + 1 + z;
+                if (gz < Z && gx < X && gy < Y && ((size_t)gz * X + gx) * Y + gy == src) orig[z] = FLT_MAX;
+            }
         }
         for (bool up = true;; up = !up) {
             const bool changed = up ? sweep_columns<D3Q7, true>(sd, col, v, open)
