@@ -396,6 +396,8 @@ bool watershed_once(pma_ctx* c, unsigned long long log_cap, unsigned long long* 
     for (long d = (long)maxE; d >= 1; --d) {
         uint32_t cnt = read_scalar(c, qcount + d);
         if (cnt == 0) continue;
+        // the kernels count past a full queue (and flag it) without writing past it
+        PMA_REQUIRE(cnt <= h[d], PMA_ERR_INTERNAL, "pma_watershed: FIFO capacity exceeded");
         unsigned long long* items = qitem + qoff_h[d];
         if (cnt > 1) {
             radix_sort_pairs_u64_u64(c, reinterpret_cast<const uint64_t*>(qkey + qoff_h[d]),
@@ -424,8 +426,7 @@ bool watershed_once(pma_ctx* c, unsigned long long log_cap, unsigned long long* 
                 PMA_LAUNCHED(c);
             }
             uint64_t added = exclusive_scan_u32(c, same_cnt, same_off, b - a);
-            PMA_REQUIRE(b + added <= h[d] && read_scalar(c, ws_err) == 0, PMA_ERR_INTERNAL,
-                        "pma_watershed: FIFO capacity exceeded");
+            PMA_REQUIRE(b + added <= h[d], PMA_ERR_INTERNAL, "pma_watershed: FIFO capacity exceeded");
             if (added) {
                 ws_append<<<g, 256, 0, c->stream>>>(items, a, b, same_mask, same_off, X, Y);
                 PMA_LAUNCHED(c);
@@ -434,6 +435,8 @@ bool watershed_once(pma_ctx* c, unsigned long long log_cap, unsigned long long* 
             b = (uint32_t)(b + added);
         }
     }
+    // a queue that filled up was flagged by ws_resolve (which never writes past it)
+    PMA_REQUIRE(read_scalar(c, ws_err) == 0, PMA_ERR_INTERNAL, "pma_watershed: FIFO capacity exceeded");
     const unsigned long long nlog = read_scalar(c, log_count);
     *needed = nlog;
     if (nlog > log_cap) return false;

@@ -40,3 +40,39 @@ def test_gaussian_192_matches_oracle(seed):
     assert np.array_equal(dist.view(np.uint32), odist.view(np.uint32)), "distances differ"
     assert np.array_equal(cav, ocav), "cavity ids differ"
     assert text == otext
+
+
+def test_gaussian_192_removal_watershed():
+    """The same workload through empty-pore removal and the watershed on the GPU, fed the oracle's
+    pore output (the pore stage itself does not run on the GPU): labels and text bit-exact against
+    the oracle.  Each stage runs twice on one context; the second run's device time is logged."""
+    n = 192
+    log(f"removal and watershed on the bench workload {n}^3")
+    with pma.Context(0) as ctx:
+        ctx.generate_gaussian((n, n, n), 8.0, 0.65, 20261203)
+        solid = ctx.read(pma.VOL_SOLID)
+    e, max_e, _ = oracle.edt(solid, threads=THREADS)
+    cr, _, _, _, mask, n_max = oracle.containing_balls(e, threads=THREADS)
+    pt, ptxt, ktxt = oracle.pores(mask, e, cr)
+    opt, optxt, oktxt = oracle.remove_empty_pores(pt, ptxt, ktxt)
+    optw, owp, owk = oracle.watershed(cr, max_e, optxt)
+    times = {}
+    with pma.Context(0) as ctx:
+        for _ in range(2):
+            ctx.load(pma.VOL_PT, pt)
+            ctx.set_text(pma.TEXT_PORES, ptxt)
+            ctx.set_text(pma.TEXT_THROATS, ktxt)
+            ctx.remove_empty_pores()
+            times["remove_empty_pores"] = ctx.stage_info(pma.STAGE_REMOVE)["device_ms"]
+            assert np.array_equal(ctx.read(pma.VOL_PT), opt), "removal labels differ"
+            assert ctx.text(pma.TEXT_PORES) == optxt and ctx.text(pma.TEXT_THROATS) == oktxt
+        for _ in range(2):
+            ctx.load(pma.VOL_CR2, cr)
+            ctx.set_text(pma.TEXT_STATS, f"Max Euclidean distance square: {max_e}\nMaximum balls count: {n_max}\n")
+            ctx.set_text(pma.TEXT_PORES, optxt)
+            ctx.watershed()
+            times["watershed"] = ctx.stage_info(pma.STAGE_WATERSHED)["device_ms"]
+            assert np.array_equal(ctx.read(pma.VOL_PTW), optw), "watershed labels differ"
+            assert ctx.text(pma.TEXT_PORES_WS) == owp and ctx.text(pma.TEXT_THROATS_WS) == owk
+    log(f"{n}^3: {optxt.count(chr(10)) - 1} pores; device ms: " +
+        ", ".join(f"{k} {v:.2f}" for k, v in times.items()))
