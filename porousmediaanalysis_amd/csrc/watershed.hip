@@ -282,9 +282,22 @@ __global__ void widen_u32(const uint32_t* __restrict__ in, uint64_t* __restrict_
     if (i < n) out[i] = in[i];
 }
 
+// Voxels per label.  Neighbouring voxels mostly share a label, so a wave adds its lanes of one
+// label with one atomic (a few distinct labels per wave) rather than one atomic per voxel on a
+// handful of counters.
 __global__ void owned_counts(const uint32_t* __restrict__ label, size_t n, unsigned long long* __restrict__ counts) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && label[i]) atomicAdd(&counts[label[i] - 1], 1ull);
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t lab = i < n ? label[i] : 0u;  // every lane stays for the wave operations
+    const int lane = (int)(threadIdx.x & 63);
+    while (true) {
+        const unsigned long long pending = __ballot(lab != 0);
+        if (!pending) break;
+        const int leader = __ffsll((long long)pending) - 1;
+        const uint32_t l = (uint32_t)__shfl((int)lab, leader);
+        const unsigned long long same = __ballot(lab == l);
+        if (lane == leader) atomicAdd(&counts[l - 1], (unsigned long long)__popcll(same));
+        if (lab == l) lab = 0;
+    }
 }
 
 __global__ void boundary_owners(const uint32_t* __restrict__ label, int X, int Y, int Z, uint32_t* __restrict__ flag) {
