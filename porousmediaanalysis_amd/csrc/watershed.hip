@@ -74,12 +74,17 @@ __global__ void push_seeds(const uint32_t* __restrict__ seed_voxel, const uint32
     qkey[qoff[lvl] + slot] = k;  // initial pushes precede every flood push (top field 0)
 }
 
-__global__ void __launch_bounds__(256) ws_bid(const unsigned long long* __restrict__ items, uint32_t a, uint32_t b,
-                                              const uint32_t* __restrict__ cr, const uint32_t* __restrict__ label,
-                                              unsigned long long* __restrict__ claim, unsigned long long hi,
-                                              uint32_t d, int X, int Y, int Z, int d3q7) {
-    uint32_t pos = a + blockIdx.x * blockDim.x + threadIdx.x;
-    if (pos >= b) return;
+// The bounds [a, b) of a round's FIFO wave, kept on the device: the kernels of a round read
+// them, and ws_scan_parts writes the next round's (rounds alternate between two states).
+struct WsRound {
+    uint32_t a, b;
+};
+constexpr uint32_t kWsGrid = 512;  // workgroups of the round kernels (grid-stride over the wave)
+
+__device__ __forceinline__ void bid_item(const unsigned long long* __restrict__ items, uint32_t pos,
+                                         const uint32_t* __restrict__ cr, const uint32_t* __restrict__ label,
+                                         unsigned long long* __restrict__ claim, unsigned long long hi, uint32_t d,
+                                         int X, int Y, int Z, int d3q7) {
     uint32_t v = (uint32_t)items[pos];
     int x, y, z;
     decode(v, X, Y, x, y, z);
@@ -96,19 +101,26 @@ __global__ void __launch_bounds__(256) ws_bid(const unsigned long long* __restri
     }
 }
 
-__global__ void __launch_bounds__(256) ws_resolve(const unsigned long long* __restrict__ items, uint32_t a, uint32_t b,
-                                                  const uint32_t* __restrict__ cr, uint32_t* __restrict__ label,
-                                                  const unsigned long long* __restrict__ claim, unsigned long long hi,
-                                                  unsigned long long tlevel, uint32_t d, int X, int Y, int Z, int d3q7,
-                                                  uint32_t* __restrict__ same_mask, uint32_t* __restrict__ same_cnt,
-                                                  const uint64_t* __restrict__ qoff, uint32_t* __restrict__ qcount,
-                                                  unsigned long long* __restrict__ qitem,
-                                                  unsigned long long* __restrict__ qkey, LogEntry* __restrict__ log,
-                                                  unsigned long long* __restrict__ log_count, unsigned long long log_cap,
-                                                  uint32_t* __restrict__ error) {
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t pos = a + t;
-    if (pos >= b) return;
+__global__ void __launch_bounds__(256) ws_bid(const unsigned long long* __restrict__ items,
+                                              const WsRound* __restrict__ round_state,
+                                              const uint32_t* __restrict__ cr, const uint32_t* __restrict__ label,
+                                              unsigned long long* __restrict__ claim, unsigned long long hi,
+                                              uint32_t d, int X, int Y, int Z, int d3q7) {
+    const uint32_t a = round_state->a, b = round_state->b;
+    for (uint32_t pos = a + blockIdx.x * blockDim.x + threadIdx.x; pos < b; pos += gridDim.x * blockDim.x)
+        bid_item(items, pos, cr, label, claim, hi, d, X, Y, Z, d3q7);
+}
+
+// The neighbours item `pos` claimed at its own level, as a mask over the 27 offsets.
+__device__ __forceinline__ uint32_t resolve_item(const unsigned long long* __restrict__ items, uint32_t pos,
+                                                 const uint32_t* __restrict__ cr, uint32_t* __restrict__ label,
+                                                 const unsigned long long* __restrict__ claim, unsigned long long hi,
+                                                 unsigned long long tlevel, uint32_t d, int X, int Y, int Z, int d3q7,
+                                                 const uint64_t* __restrict__ qoff, uint32_t* __restrict__ qcount,
+                                                 unsigned long long* __restrict__ qitem,
+                                                 unsigned long long* __restrict__ qkey, LogEntry* __restrict__ log,
+                                                 unsigned long long* __restrict__ log_count,
+                                                 unsigned long long log_cap, uint32_t* __restrict__ error) {
     unsigned long long it = items[pos];
     uint32_t v = (uint32_t)it, p = (uint32_t)(it >> 32);
     int x, y, z;
@@ -152,30 +164,135 @@ __global__ void __launch_bounds__(256) ws_resolve(const unsigned long long* __re
             if (s < log_cap) log[s] = LogEntry{tlevel | ((unsigned long long)pos << 5) | (unsigned long long)k, j, p};
         }
     }
-    same_mask[t] = mask;
-    same_cnt[t] = (uint32_t)__popc(mask);
+    return mask;
 }
 
-__global__ void __launch_bounds__(256) ws_append(unsigned long long* __restrict__ items, uint32_t a, uint32_t b,
+__global__ void __launch_bounds__(256) ws_resolve(const unsigned long long* __restrict__ items,
+                                                  const WsRound* __restrict__ round_state,
+                                                  const uint32_t* __restrict__ cr, uint32_t* __restrict__ label,
+                                                  const unsigned long long* __restrict__ claim, unsigned long long hi,
+                                                  unsigned long long tlevel, uint32_t d, int X, int Y, int Z, int d3q7,
+                                                  uint32_t* __restrict__ same_mask, uint32_t* __restrict__ same_cnt,
+                                                  const uint64_t* __restrict__ qoff, uint32_t* __restrict__ qcount,
+                                                  unsigned long long* __restrict__ qitem,
+                                                  unsigned long long* __restrict__ qkey, LogEntry* __restrict__ log,
+                                                  unsigned long long* __restrict__ log_count, unsigned long long log_cap,
+                                                  uint32_t* __restrict__ error) {
+    const uint32_t a = round_state->a, b = round_state->b;
+    for (uint32_t pos = a + blockIdx.x * blockDim.x + threadIdx.x; pos < b; pos += gridDim.x * blockDim.x) {
+        const uint32_t mask = resolve_item(items, pos, cr, label, claim, hi, tlevel, d, X, Y, Z, d3q7, qoff, qcount,
+                                           qitem, qkey, log, log_count, log_cap, error);
+        same_mask[pos - a] = mask;
+        same_cnt[pos - a] = (uint32_t)__popc(mask);
+    }
+}
+
+// Workgroup w of the round kernels below takes the slice [w * chunk, min(n, (w + 1) * chunk)) of
+// the wave's n items, chunk = ceil(n / gridDim.x).
+__device__ __forceinline__ void ws_slice(uint32_t n, uint32_t& lo, uint32_t& hi) {
+    const uint32_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    lo = min(n, blockIdx.x * chunk);
+    hi = min(n, lo + chunk);
+}
+
+// Exclusive scan of one value per thread over a 256-thread workgroup; returns the total.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t& excl, uint32_t* __restrict__ lds) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;  // inclusive scan within the wave
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)x, (unsigned)off);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) lds[wave] = x;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+    for (int w = 0; w < 4; ++w) {
+        if (w < wave) before += lds[w];
+        total += lds[w];
+    }
+    excl = before + x - v;
+    __syncthreads();  // lds is reused by the next call
+    return total;
+}
+
+// Appended items per workgroup slice.
+__global__ void __launch_bounds__(256) ws_count_parts(const WsRound* __restrict__ round_state,
+                                                      const uint32_t* __restrict__ same_cnt,
+                                                      uint32_t* __restrict__ part) {
+    __shared__ uint32_t lds[4];
+    uint32_t lo, hi;
+    ws_slice(round_state->b - round_state->a, lo, hi);
+    uint32_t sum = 0;
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) sum += same_cnt[i];
+    uint32_t excl;
+    const uint32_t total = block_exclusive_scan(sum, excl, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = total;
+}
+
+// One workgroup: the slices' offsets (exclusive scan of part, in place) and the next round's
+// bounds [b, b + appended).  A wave that would overflow its level's queue (capacity cap) is
+// flagged, and ws_append skips what does not fit.
+__global__ void __launch_bounds__(256) ws_scan_parts(const WsRound* __restrict__ round_state,
+                                                     WsRound* __restrict__ next_state, uint32_t* __restrict__ part,
+                                                     uint32_t nparts, uint32_t cap, uint32_t* __restrict__ error) {
+    __shared__ uint32_t lds[4];
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < nparts; base += blockDim.x) {  // the same trips on every thread
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t v = i < nparts ? part[i] : 0u;
+        uint32_t excl;
+        const uint32_t total = block_exclusive_scan(v, excl, lds);
+        if (i < nparts) part[i] = carry + excl;
+        carry += total;
+    }
+    if (threadIdx.x == 0) {
+        const uint32_t b = round_state->b;
+        uint64_t nb = (uint64_t)b + carry;
+        if (nb > cap) {
+            atomicOr(error, 1u);
+            nb = b;  // stop the level; the stage fails at its end
+        }
+        next_state->a = b;
+        next_state->b = (uint32_t)nb;
+    }
+}
+
+// Appends each item's same-level claims after the wave, in item order (slice offset from
+// ws_scan_parts plus a scan within the slice), skipping places past the queue's capacity.
+__global__ void __launch_bounds__(256) ws_append(unsigned long long* __restrict__ items,
+                                                 const WsRound* __restrict__ round_state,
+                                                 const uint32_t* __restrict__ part,
                                                  const uint32_t* __restrict__ same_mask,
-                                                 const uint32_t* __restrict__ same_off, int X, int Y) {
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t pos = a + t;
-    if (pos >= b) return;
-    uint32_t mask = same_mask[t];
-    if (!mask) return;
-    unsigned long long it = items[pos];
-    uint32_t v = (uint32_t)it;
-    unsigned long long p = it >> 32;
-    int x, y, z;
-    decode(v, X, Y, x, y, z);
-    uint32_t o = b + same_off[t];
-    while (mask) {
-        int k = __ffs(mask) - 1;
-        mask &= mask - 1;
-        const int dx = k / 9 - 1, dy = (k / 3) % 3 - 1, dz = k % 3 - 1;
-        const uint32_t j = ((uint32_t)(z + dz) * (uint32_t)X + (uint32_t)(x + dx)) * (uint32_t)Y + (uint32_t)(y + dy);
-        items[o++] = (unsigned long long)j | (p << 32);
+                                                 const uint32_t* __restrict__ same_cnt, uint32_t cap, int X, int Y) {
+    __shared__ uint32_t lds[4];
+    const uint32_t a = round_state->a, b = round_state->b;
+    uint32_t lo, hi;
+    ws_slice(b - a, lo, hi);
+    uint32_t carry = b + part[blockIdx.x];
+    for (uint32_t base = lo; base < hi; base += blockDim.x) {  // the same trips on every thread
+        const uint32_t t = base + threadIdx.x;
+        const uint32_t cnt = t < hi ? same_cnt[t] : 0u;
+        uint32_t excl;
+        const uint32_t total = block_exclusive_scan(cnt, excl, lds);
+        if (cnt) {
+            uint32_t mask = same_mask[t];
+            const unsigned long long it = items[a + t];
+            const uint32_t v = (uint32_t)it;
+            const unsigned long long p = it >> 32;
+            int x, y, z;
+            decode(v, X, Y, x, y, z);
+            uint32_t o = carry + excl;
+            while (mask) {
+                const int k = __ffs(mask) - 1;
+                mask &= mask - 1;
+                const int dx = k / 9 - 1, dy = (k / 3) % 3 - 1, dz = k % 3 - 1;
+                const uint32_t j = ((uint32_t)(z + dz) * (uint32_t)X + (uint32_t)(x + dx)) * (uint32_t)Y +
+                                   (uint32_t)(y + dy);
+                if (o < cap) items[o] = (unsigned long long)j | (p << 32);
+                ++o;
+            }
+        }
+        carry += total;
     }
 }
 
@@ -402,7 +519,8 @@ bool watershed_once(pma_ctx* c, unsigned long long log_cap, unsigned long long* 
     PMA_HIP(hipMemsetAsync(ws_err, 0, 4, c->stream));
     uint32_t* same_mask = c->buf<uint32_t>("ws_same_mask", std::max<uint64_t>(qtotal, 1));
     uint32_t* same_cnt = c->buf<uint32_t>("ws_same_cnt", std::max<uint64_t>(qtotal, 1));
-    uint32_t* same_off = c->buf<uint32_t>("ws_same_off", std::max<uint64_t>(qtotal, 1));
+    uint32_t* part = c->buf<uint32_t>("ws_parts", kWsGrid);
+    WsRound* rstate = c->buf<WsRound>("ws_round_state", 2);
 
     uint64_t rounds = 0;
     uint32_t gr = 0;  // global round counter for claim stamps
@@ -419,36 +537,48 @@ bool watershed_once(pma_ctx* c, unsigned long long log_cap, unsigned long long* 
             PMA_HIP(hipMemcpyAsync(items, sort_v, cnt * sizeof(uint64_t), hipMemcpyDeviceToDevice, c->stream));
         }
         const unsigned long long tlevel = (unsigned long long)(maxE - d + 1) << 40;
-        uint32_t a = 0, b = cnt;
-        while (a < b) {
+        // the level's rounds run back to back on the device, each reading its bounds from the
+        // previous one's; the host looks for an empty round only after rounds 1, 2, 4, 8 and
+        // then every 8 (the rounds after the last are empty and change nothing)
+        const WsRound first{0u, cnt};
+        upload(c, rstate, &first, sizeof first);
+        const uint32_t g = std::min(grid_for(h[d], 256), kWsGrid);
+        uint32_t r = 0, check_at = 1, step = 1;
+        while (true) {
             PMA_REQUIRE(gr < (1u << 24) - 1, PMA_ERR_INTERNAL, "pma_watershed: too many rounds");
             const unsigned long long hi = (unsigned long long)((1u << 24) - 1 - gr) << 40;
             ++gr;
             ++rounds;
-            const unsigned g = grid_for(b - a, 256);
+            const WsRound* cur = rstate + (r & 1);
+            WsRound* nxt = rstate + ((r + 1) & 1);
             {
                 PMA_TIMED(c, "ws_bid");
-                ws_bid<<<g, 256, 0, c->stream>>>(items, a, b, cr, label, claim, hi, (uint32_t)d, X, Y, Z, d3q7);
+                ws_bid<<<g, 256, 0, c->stream>>>(items, cur, cr, label, claim, hi, (uint32_t)d, X, Y, Z, d3q7);
                 PMA_LAUNCHED(c);
             }
             {
                 PMA_TIMED(c, "ws_resolve");
-                ws_resolve<<<g, 256, 0, c->stream>>>(items, a, b, cr, label, claim, hi, tlevel, (uint32_t)d, X, Y, Z, d3q7,
-                                                     same_mask, same_cnt, qoff, qcount, qitem, qkey, log, log_count,
-                                                     log_cap, ws_err);
+                ws_resolve<<<g, 256, 0, c->stream>>>(items, cur, cr, label, claim, hi, tlevel, (uint32_t)d, X, Y, Z,
+                                                     d3q7, same_mask, same_cnt, qoff, qcount, qitem, qkey, log,
+                                                     log_count, log_cap, ws_err);
                 PMA_LAUNCHED(c);
             }
-            uint64_t added = exclusive_scan_u32(c, same_cnt, same_off, b - a);
-            PMA_REQUIRE(b + added <= h[d], PMA_ERR_INTERNAL, "pma_watershed: FIFO capacity exceeded");
-            if (added) {
-                ws_append<<<g, 256, 0, c->stream>>>(items, a, b, same_mask, same_off, X, Y);
-                PMA_LAUNCHED(c);
+            ws_count_parts<<<g, 256, 0, c->stream>>>(cur, same_cnt, part);
+            PMA_LAUNCHED(c);
+            ws_scan_parts<<<1, 256, 0, c->stream>>>(cur, nxt, part, g, h[d], ws_err);
+            PMA_LAUNCHED(c);
+            ws_append<<<g, 256, 0, c->stream>>>(items, cur, part, same_mask, same_cnt, h[d], X, Y);
+            PMA_LAUNCHED(c);
+            ++r;
+            if (r == check_at) {
+                const WsRound next = read_scalar(c, nxt);
+                if (next.a == next.b) break;
+                step = std::min(step * 2, 8u);
+                check_at += step;
             }
-            a = b;
-            b = (uint32_t)(b + added);
         }
     }
-    // a queue that filled up was flagged by ws_resolve (which never writes past it)
+    // a queue that filled up was flagged by ws_resolve or ws_scan_parts (neither writes past it)
     PMA_REQUIRE(read_scalar(c, ws_err) == 0, PMA_ERR_INTERNAL, "pma_watershed: FIFO capacity exceeded");
     const unsigned long long nlog = read_scalar(c, log_count);
     *needed = nlog;

@@ -10,7 +10,8 @@ import sys
 WAVE_KERNELS = {"count_centres", "write_centres", "dfs_replay", "cr_histogram", "scan_reduce<T>", "scan_down<T>",
                 "prune_centres", "emit_centres", "cover_tiles", "count_equal_u8", "far_prefix_max",
                 "relax_blocks", "find_start", "cavity_totals", "max_run", "partition_active", "next_min_key",
-                "edt_pass_y", "edt_pass_z", "owned_counts"}
+                "edt_pass_y", "edt_pass_z", "owned_counts",
+                "ws_count_parts", "ws_scan_parts", "ws_append"}
 
 
 def split_args(s):
