@@ -79,26 +79,29 @@ __global__ void push_seeds(const uint32_t* __restrict__ seed_voxel, const uint32
 struct WsRound {
     uint32_t a, b;
 };
-constexpr uint32_t kWsGrid = 512;  // workgroups of the round kernels (grid-stride over the wave)
+constexpr uint32_t kWsGrid = 512;       // workgroups of the round's scan kernels (grid-stride over the wave)
+constexpr uint32_t kWsItemGrid = 1024;  // workgroups of ws_bid and ws_resolve (grid-stride over the lanes)
 
-__device__ __forceinline__ void bid_item(const unsigned long long* __restrict__ items, uint32_t pos,
-                                         const uint32_t* __restrict__ cr, const uint32_t* __restrict__ label,
-                                         unsigned long long* __restrict__ claim, unsigned long long hi, uint32_t d,
-                                         int X, int Y, int Z, int d3q7) {
-    uint32_t v = (uint32_t)items[pos];
+// A round's items are spread over lanes, one lane per (item, neighbour): the item's lanes form
+// an aligned group of 32 (the 26 neighbours of D3Q27) or 8 (the 6 of D3Q7), so a group never
+// straddles a wave and its loads run side by side instead of one neighbour after another.
+__device__ __forceinline__ uint32_t ws_group_shift(int d3q7) { return d3q7 ? 3u : 5u; }
+
+// Offset index k = 9 (dx + 1) + 3 (dy + 1) + (dz + 1) of lane kk of a group, or -1 for a spare
+// lane; k increases with kk.
+__device__ __forceinline__ int ws_offset(uint32_t kk, int d3q7) {
+    if (d3q7) return kk < 6 ? (int)((0x16100E0C0A04ull >> (8 * kk)) & 0xFFu) : -1;  // 4 10 12 14 16 22
+    return kk < 26 ? (int)(kk + (kk >= 13 ? 1u : 0u)) : -1;
+}
+
+// Neighbour k of voxel v, when it lies in the image.
+__device__ __forceinline__ bool ws_neighbour(uint32_t v, int k, int X, int Y, int Z, uint32_t& j) {
     int x, y, z;
     decode(v, X, Y, x, y, z);
-    for (int k = 0; k < 27; ++k) {
-        const int dx = k / 9 - 1, dy = (k / 3) % 3 - 1, dz = k % 3 - 1;
-        const int m = abs(dx) + abs(dy) + abs(dz);
-        if (m == 0 || (d3q7 && m > 1)) continue;
-        const int nx = x + dx, ny = y + dy, nz = z + dz;
-        if (nx < 0 || nx >= X || ny < 0 || ny >= Y || nz < 0 || nz >= Z) continue;
-        const uint32_t j = ((uint32_t)nz * (uint32_t)X + (uint32_t)nx) * (uint32_t)Y + (uint32_t)ny;
-        const uint32_t c = cr[j];
-        if (c == 0 || c > d || label[j] != 0) continue;
-        atomicMin(&claim[j], hi | ((unsigned long long)pos << 5) | (unsigned long long)k);
-    }
+    const int nx = x + k / 9 - 1, ny = y + (k / 3) % 3 - 1, nz = z + k % 3 - 1;
+    if (nx < 0 || nx >= X || ny < 0 || ny >= Y || nz < 0 || nz >= Z) return false;
+    j = ((uint32_t)nz * (uint32_t)X + (uint32_t)nx) * (uint32_t)Y + (uint32_t)ny;
+    return true;
 }
 
 __global__ void __launch_bounds__(256) ws_bid(const unsigned long long* __restrict__ items,
@@ -107,66 +110,24 @@ __global__ void __launch_bounds__(256) ws_bid(const unsigned long long* __restri
                                               unsigned long long* __restrict__ claim, unsigned long long hi,
                                               uint32_t d, int X, int Y, int Z, int d3q7) {
     const uint32_t a = round_state->a, b = round_state->b;
-    for (uint32_t pos = a + blockIdx.x * blockDim.x + threadIdx.x; pos < b; pos += gridDim.x * blockDim.x)
-        bid_item(items, pos, cr, label, claim, hi, d, X, Y, Z, d3q7);
-}
-
-// The neighbours item `pos` claimed at its own level, as a mask over the 27 offsets.
-__device__ __forceinline__ uint32_t resolve_item(const unsigned long long* __restrict__ items, uint32_t pos,
-                                                 const uint32_t* __restrict__ cr, uint32_t* __restrict__ label,
-                                                 const unsigned long long* __restrict__ claim, unsigned long long hi,
-                                                 unsigned long long tlevel, uint32_t d, int X, int Y, int Z, int d3q7,
-                                                 const uint64_t* __restrict__ qoff, uint32_t* __restrict__ qcount,
-                                                 unsigned long long* __restrict__ qitem,
-                                                 unsigned long long* __restrict__ qkey, LogEntry* __restrict__ log,
-                                                 unsigned long long* __restrict__ log_count,
-                                                 unsigned long long log_cap, uint32_t* __restrict__ error) {
-    unsigned long long it = items[pos];
-    uint32_t v = (uint32_t)it, p = (uint32_t)(it >> 32);
-    int x, y, z;
-    decode(v, X, Y, x, y, z);
-    uint32_t mask = 0;
-    for (int k = 0; k < 27; ++k) {
-        const int dx = k / 9 - 1, dy = (k / 3) % 3 - 1, dz = k % 3 - 1;
-        const int m = abs(dx) + abs(dy) + abs(dz);
-        if (m == 0 || (d3q7 && m > 1)) continue;
-        const int nx = x + dx, ny = y + dy, nz = z + dz;
-        if (nx < 0 || nx >= X || ny < 0 || ny >= Y || nz < 0 || nz >= Z) continue;
-        const uint32_t j = ((uint32_t)nz * (uint32_t)X + (uint32_t)nx) * (uint32_t)Y + (uint32_t)ny;
+    const uint32_t sh = ws_group_shift(d3q7);
+    const size_t n = (size_t)(b - a) << sh, stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) {
+        const int k = ws_offset((uint32_t)t & ((1u << sh) - 1u), d3q7);
+        if (k < 0) continue;
+        const uint32_t pos = a + (uint32_t)(t >> sh);
+        uint32_t j;
+        if (!ws_neighbour((uint32_t)items[pos], k, X, Y, Z, j)) continue;
         const uint32_t c = cr[j];
-        if (c == 0 || c > d) continue;
-        const unsigned long long mine = hi | ((unsigned long long)pos << 5) | (unsigned long long)k;
-        const uint32_t lab = label[j];
-        uint32_t owner;
-        if (lab == 0) {
-            const unsigned long long w = claim[j];
-            if (w == mine) {
-                label[j] = p + 1;
-                if (c == d) {
-                    mask |= 1u << k;
-                } else {
-                    uint32_t slot = atomicAdd(&qcount[c], 1u);
-                    if (qoff[c] + slot < qoff[c + 1]) {
-                        qitem[qoff[c] + slot] = (unsigned long long)j | ((unsigned long long)p << 32);
-                        qkey[qoff[c] + slot] = tlevel | ((unsigned long long)pos << 5) | (unsigned long long)k;
-                    } else {
-                        atomicOr(error, 1u);
-                    }
-                }
-                continue;
-            }
-            owner = (uint32_t)(items[(w >> 5) & kPosMask] >> 32);
-        } else {
-            owner = lab - 1;
-        }
-        if (owner != p) {
-            unsigned long long s = atomicAdd(log_count, 1ull);
-            if (s < log_cap) log[s] = LogEntry{tlevel | ((unsigned long long)pos << 5) | (unsigned long long)k, j, p};
-        }
+        if (c == 0 || c > d || label[j] != 0) continue;
+        atomicMin(&claim[j], hi | ((unsigned long long)pos << 5) | (unsigned long long)k);
     }
-    return mask;
 }
 
+// Each contact of a round: a won claim labels the neighbour and queues it (same level: marked in
+// the item's mask, appended in order by ws_append; lower level: queued with its push time); a
+// contact with another pore's voxel is logged.  A group's same-level claims become its item's
+// mask over the 27 offsets (one ballot), and a wave's log entries take one atomic.
 __global__ void __launch_bounds__(256) ws_resolve(const unsigned long long* __restrict__ items,
                                                   const WsRound* __restrict__ round_state,
                                                   const uint32_t* __restrict__ cr, uint32_t* __restrict__ label,
@@ -179,11 +140,73 @@ __global__ void __launch_bounds__(256) ws_resolve(const unsigned long long* __re
                                                   unsigned long long* __restrict__ log_count, unsigned long long log_cap,
                                                   uint32_t* __restrict__ error) {
     const uint32_t a = round_state->a, b = round_state->b;
-    for (uint32_t pos = a + blockIdx.x * blockDim.x + threadIdx.x; pos < b; pos += gridDim.x * blockDim.x) {
-        const uint32_t mask = resolve_item(items, pos, cr, label, claim, hi, tlevel, d, X, Y, Z, d3q7, qoff, qcount,
-                                           qitem, qkey, log, log_count, log_cap, error);
-        same_mask[pos - a] = mask;
-        same_cnt[pos - a] = (uint32_t)__popc(mask);
+    const uint32_t sh = ws_group_shift(d3q7), span = 1u << sh;
+    const int lane = (int)(threadIdx.x & 63);
+    const size_t n = (size_t)(b - a) << sh, stride = (size_t)gridDim.x * blockDim.x;
+    // n and the stride are multiples of the group size, so a group is wholly in or out of the loop
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) {
+        const uint32_t kk = (uint32_t)t & (span - 1u);
+        const int k = ws_offset(kk, d3q7);
+        const uint32_t pos = a + (uint32_t)(t >> sh);
+        const unsigned long long it = items[pos];
+        const uint32_t p = (uint32_t)(it >> 32);
+        const unsigned long long when = (unsigned long long)(pos) << 5 | (unsigned long long)(k < 0 ? 0 : k);
+        bool same = false, logged = false;
+        uint32_t j = 0;
+        if (k >= 0 && ws_neighbour((uint32_t)it, k, X, Y, Z, j)) {
+            const uint32_t c = cr[j];
+            if (c != 0 && c <= d) {
+                const uint32_t lab = label[j];
+                uint32_t owner = p;
+                if (lab == 0) {
+                    const unsigned long long w = claim[j];
+                    if (w == (hi | when)) {
+                        label[j] = p + 1;
+                        if (c == d) {
+                            same = true;
+                        } else {
+                            const uint32_t slot = atomicAdd(&qcount[c], 1u);
+                            if (qoff[c] + slot < qoff[c + 1]) {
+                                qitem[qoff[c] + slot] = (unsigned long long)j | ((unsigned long long)p << 32);
+                                qkey[qoff[c] + slot] = tlevel | when;
+                            } else {
+                                atomicOr(error, 1u);
+                            }
+                        }
+                    } else {
+                        owner = (uint32_t)(items[(w >> 5) & kPosMask] >> 32);
+                    }
+                } else {
+                    owner = lab - 1;
+                }
+                logged = owner != p;
+            }
+        }
+        const unsigned long long logs = __ballot(logged);
+        if (logs) {
+            const int leader = __ffsll((long long)logs) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(log_count, (unsigned long long)__popcll(logs));
+            base = __shfl(base, leader);
+            if (logged) {
+                const unsigned long long s = base + (unsigned long long)__popcll(logs & ((1ull << lane) - 1ull));
+                if (s < log_cap) log[s] = LogEntry{tlevel | when, j, p};
+            }
+        }
+        const unsigned long long sames = __ballot(same);
+        if (kk == 0) {
+            const uint32_t g = (uint32_t)((sames >> (lane & ~(int)(span - 1u))) & ((1ull << span) - 1ull));
+            uint32_t mask;
+            if (d3q7) {
+                mask = 0;
+                for (uint32_t i = 0; i < 6; ++i)
+                    if (g >> i & 1u) mask |= 1u << ws_offset(i, 1);
+            } else {
+                mask = (g & 0x1FFFu) | ((g >> 13) << 14);
+            }
+            same_mask[pos - a] = mask;
+            same_cnt[pos - a] = (uint32_t)__popc(mask);
+        }
     }
 }
 
@@ -543,6 +566,7 @@ bool watershed_once(pma_ctx* c, unsigned long long log_cap, unsigned long long* 
         const WsRound first{0u, cnt};
         upload(c, rstate, &first, sizeof first);
         const uint32_t g = std::min(grid_for(h[d], 256), kWsGrid);
+        const uint32_t gi = std::min(grid_for((size_t)h[d] << (d3q7 ? 3 : 5), 256), kWsItemGrid);
         uint32_t r = 0, check_at = 1, step = 1;
         while (true) {
             PMA_REQUIRE(gr < (1u << 24) - 1, PMA_ERR_INTERNAL, "pma_watershed: too many rounds");
@@ -553,12 +577,12 @@ bool watershed_once(pma_ctx* c, unsigned long long log_cap, unsigned long long* 
             WsRound* nxt = rstate + ((r + 1) & 1);
             {
                 PMA_TIMED(c, "ws_bid");
-                ws_bid<<<g, 256, 0, c->stream>>>(items, cur, cr, label, claim, hi, (uint32_t)d, X, Y, Z, d3q7);
+                ws_bid<<<gi, 256, 0, c->stream>>>(items, cur, cr, label, claim, hi, (uint32_t)d, X, Y, Z, d3q7);
                 PMA_LAUNCHED(c);
             }
             {
                 PMA_TIMED(c, "ws_resolve");
-                ws_resolve<<<g, 256, 0, c->stream>>>(items, cur, cr, label, claim, hi, tlevel, (uint32_t)d, X, Y, Z,
+                ws_resolve<<<gi, 256, 0, c->stream>>>(items, cur, cr, label, claim, hi, tlevel, (uint32_t)d, X, Y, Z,
                                                      d3q7, same_mask, same_cnt, qoff, qcount, qitem, qkey, log,
                                                      log_count, log_cap, ws_err);
                 PMA_LAUNCHED(c);
