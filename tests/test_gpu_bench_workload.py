@@ -1,6 +1,6 @@
-"""The benchmark's workload at 192^3: the GPU Gaussian-field generator (sigma 8, porosity 0.65),
-then EDT, containing balls and geodesic distances with cavities, each bit-exact against the
-oracle.  At this size the containing-ball stage takes the paths the 1024^3 run takes (balls
+"""The benchmark's workload at 192^3 (and 160^3 for the 6-neighbour watershed): the GPU
+Gaussian-field generator (sigma 8, porosity 0.65), then EDT, containing balls and geodesic
+distances with cavities, each bit-exact against the oracle.  At this size the containing-ball stage takes the paths the 1024^3 run takes (balls
 packed in the pair sort, chord widths from the LDS table, 16-bit winner slots) and the
 geodesic runs a few hundred chained rounds."""
 import os
@@ -76,3 +76,30 @@ def test_gaussian_192_removal_watershed():
             assert ctx.text(pma.TEXT_PORES_WS) == owp and ctx.text(pma.TEXT_THROATS_WS) == owk
     log(f"{n}^3: {optxt.count(chr(10)) - 1} pores; device ms: " +
         ", ".join(f"{k} {v:.2f}" for k, v in times.items()))
+
+
+def test_gaussian_160_watershed_d3q7():
+    """The watershed on the same generator at 160^3 with the 6-neighbour stencil (8 lanes per
+    item in the round kernels), fed the oracle's D3Q7 pores after empty-pore removal: labels and
+    text bit-exact against the oracle, on two runs in one context."""
+    n = 160
+    log(f"watershed d3q7 on the bench workload {n}^3")
+    with pma.Context(0) as ctx:
+        ctx.generate_gaussian((n, n, n), 8.0, 0.65, 20261203)
+        solid = ctx.read(pma.VOL_SOLID)
+    e, max_e, _ = oracle.edt(solid, threads=THREADS)
+    cr, _, _, _, mask, n_max = oracle.containing_balls(e, threads=THREADS)
+    pt, ptxt, ktxt = oracle.pores(mask, e, cr, True)
+    _, optxt, _ = oracle.remove_empty_pores(pt, ptxt, ktxt)
+    optw, owp, owk = oracle.watershed(cr, max_e, optxt, True)
+    with pma.Context(0) as ctx:
+        ctx.set_stencil(True)
+        for _ in range(2):
+            ctx.load(pma.VOL_CR2, cr)
+            ctx.set_text(pma.TEXT_STATS, f"Max Euclidean distance square: {max_e}\nMaximum balls count: {n_max}\n")
+            ctx.set_text(pma.TEXT_PORES, optxt)
+            ctx.watershed()
+            assert np.array_equal(ctx.read(pma.VOL_PTW), optw), "watershed labels differ"
+            assert ctx.text(pma.TEXT_PORES_WS) == owp and ctx.text(pma.TEXT_THROATS_WS) == owk
+        ms = ctx.stage_info(pma.STAGE_WATERSHED)["device_ms"]
+    log(f"{n}^3 d3q7: {optxt.count(chr(10)) - 1} pores; watershed device ms {ms:.2f}")
