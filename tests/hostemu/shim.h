@@ -153,6 +153,8 @@ template <class T> T __shfl_up(T v, unsigned d) {
     const int src = lane >= (int)d ? (int)threadIdx.x - (int)d : (int)threadIdx.x;
     return wave_read(v, src);
 }
+// Lanes whose thread has ended take no part in the wave operations below (on the GPU they are
+// masked off), so their last slot values are skipped.
 inline unsigned long long __ballot(int pred) {
     if (!g_wave) throw std::runtime_error("__ballot outside a wave launch");
     g_wave->slot[threadIdx.x] = pred ? 1 : 0;
@@ -160,7 +162,7 @@ inline unsigned long long __ballot(int pred) {
     unsigned long long r = 0;
     const unsigned base = threadIdx.x & ~63u;
     for (unsigned l = 0; l < 64 && base + l < g_wave->slot.size(); ++l)
-        if (g_wave->slot[base + l]) r |= 1ull << l;
+        if (!g_wave->fibers[base + l].done && g_wave->slot[base + l]) r |= 1ull << l;
     shim_yield();
     return r;
 }
@@ -171,7 +173,8 @@ inline unsigned __ockl_wfred_or_u32(unsigned v) {
     shim_yield();
     unsigned r = 0;
     const unsigned base = threadIdx.x & ~63u;
-    for (unsigned l = 0; l < 64 && base + l < g_wave->slot.size(); ++l) r |= (unsigned)g_wave->slot[base + l];
+    for (unsigned l = 0; l < 64 && base + l < g_wave->slot.size(); ++l)
+        if (!g_wave->fibers[base + l].done) r |= (unsigned)g_wave->slot[base + l];
     shim_yield();
     return r;
 }
@@ -181,7 +184,8 @@ inline unsigned __ockl_wfred_add_u32(unsigned v) {
     shim_yield();
     unsigned r = 0;
     const unsigned base = threadIdx.x & ~63u;
-    for (unsigned l = 0; l < 64 && base + l < g_wave->slot.size(); ++l) r += (unsigned)g_wave->slot[base + l];
+    for (unsigned l = 0; l < 64 && base + l < g_wave->slot.size(); ++l)
+        if (!g_wave->fibers[base + l].done) r += (unsigned)g_wave->slot[base + l];
     shim_yield();
     return r;
 }
@@ -191,7 +195,8 @@ inline unsigned __ockl_wfred_min_u32(unsigned v) {
     shim_yield();
     unsigned r = ~0u;
     const unsigned base = threadIdx.x & ~63u;
-    for (unsigned l = 0; l < 64 && base + l < g_wave->slot.size(); ++l) r = std::min(r, (unsigned)g_wave->slot[base + l]);
+    for (unsigned l = 0; l < 64 && base + l < g_wave->slot.size(); ++l)
+        if (!g_wave->fibers[base + l].done) r = std::min(r, (unsigned)g_wave->slot[base + l]);
     shim_yield();
     return r;
 }
