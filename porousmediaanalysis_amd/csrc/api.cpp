@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cinttypes>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 
@@ -281,6 +282,8 @@ pma_status pma_create(pma_ctx** out, int device) {
         PMA_HIP(hipSetDevice(device));
         pma_ctx* c = new pma_ctx();
         c->device = device;
+        const char* dbg = std::getenv("PMA_DEBUG_SYNC");
+        c->debug_sync = dbg && dbg[0] == '1';
         try {
             PMA_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
             PMA_HIP(hipEventCreate(&c->ev0));

@@ -33,19 +33,37 @@ __global__ void length_histogram(const uint32_t* __restrict__ len, size_t S, uin
     if (s < S && len[s] <= max_len) atomicAdd(&hist[len[s]], 1u);
 }
 
-__global__ void trie_start(const uint32_t* __restrict__ act, size_t A, const uint32_t* __restrict__ elems,
-                           const uint64_t* __restrict__ off, uint32_t* __restrict__ cur) {
-    size_t a = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (a < A) cur[a] = elems[off[act[a]]];
-}
+// Kernel ids of the in-kernel checks (code = id << 8 | check).
+enum : uint32_t { kTrieStart = 0x41, kTrieKeys, kTrieAssign };
 
-__global__ void trie_keys(const uint32_t* __restrict__ act, size_t A, const uint32_t* __restrict__ elems,
-                          const uint64_t* __restrict__ off, const uint32_t* __restrict__ cur, uint32_t L,
-                          uint64_t* __restrict__ key, uint32_t* __restrict__ val) {
+__global__ void trie_start(const uint32_t* __restrict__ act, size_t A, size_t S, const uint32_t* __restrict__ elems,
+                           const uint64_t* __restrict__ off, uint64_t elem_cap, uint32_t* __restrict__ cur,
+                           uint32_t* __restrict__ err) {
     size_t a = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= A) return;
-    key[a] = ((uint64_t)cur[a] << 32) | elems[off[act[a]] + L];
+    const uint32_t s = act[a];
+    if (s >= S || off[s] >= elem_cap) {
+        fail_check(err, kTrieStart << 8 | 1);
+        cur[a] = 0;
+        return;
+    }
+    cur[a] = elems[off[s]];
+}
+
+__global__ void trie_keys(const uint32_t* __restrict__ act, size_t A, size_t S, const uint32_t* __restrict__ elems,
+                          const uint64_t* __restrict__ off, uint64_t elem_cap, const uint32_t* __restrict__ cur,
+                          uint32_t L, uint64_t* __restrict__ key, uint32_t* __restrict__ val,
+                          uint32_t* __restrict__ err) {
+    size_t a = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= A) return;
+    const uint32_t s = act[a];
     val[a] = (uint32_t)a;
+    if (s >= S || off[s] + L >= elem_cap) {
+        fail_check(err, kTrieKeys << 8 | 1);
+        key[a] = 0;
+        return;
+    }
+    key[a] = ((uint64_t)cur[a] << 32) | elems[off[s] + L];
 }
 
 __global__ void trie_heads(const uint64_t* __restrict__ key, size_t A, uint32_t* __restrict__ head) {
@@ -54,21 +72,25 @@ __global__ void trie_heads(const uint64_t* __restrict__ key, size_t A, uint32_t*
 }
 
 __global__ void trie_assign(const uint64_t* __restrict__ key, const uint32_t* __restrict__ val,
-                            const uint32_t* __restrict__ head, const uint32_t* __restrict__ hscan, size_t A,
+                            const uint32_t* __restrict__ head, const uint32_t* __restrict__ hscan, size_t A, size_t S,
                             uint32_t dbase, uint32_t L, const uint32_t* __restrict__ act,
-                            const uint32_t* __restrict__ len, const uint64_t* __restrict__ rbase,
+                            const uint32_t* __restrict__ len, const uint64_t* __restrict__ rbase, uint64_t R,
                             const uint64_t* __restrict__ rec_time, const uint32_t* __restrict__ weight,
                             uint32_t* __restrict__ cur, uint32_t* __restrict__ rec_node,
                             uint64_t* __restrict__ node_key, unsigned long long* __restrict__ node_time,
-                            unsigned long long* __restrict__ node_final) {
+                            unsigned long long* __restrict__ node_final, uint32_t* __restrict__ err) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= A) return;
     const uint32_t a = val[i];
-    const uint32_t d = dbase + hscan[i] + head[i] - 1;
-    const uint32_t s = act[a];
-    const uint64_t r = rbase[s] + (L - 1);  // record of the prefix of length L + 1
-    cur[a] = kNodeBit | d;
-    rec_node[r] = d;
+    const uint64_t d = (uint64_t)dbase + hscan[i] + head[i] - 1;
+    const uint32_t s = a < A ? act[a] : kNodeBit;
+    const uint64_t r = s < S ? rbase[s] + (L - 1) : R;  // record of the prefix of length L + 1
+    if (d >= R || r >= R) {
+        fail_check(err, kTrieAssign << 8 | 1);
+        return;
+    }
+    cur[a] = kNodeBit | (uint32_t)d;
+    rec_node[r] = (uint32_t)d;
     if (head[i]) node_key[d] = key[i];
     if (rec_time) atomicMin(&node_time[d], (unsigned long long)rec_time[r]);
     if (L + 1 == len[s] && weight) atomicAdd(&node_final[d], (unsigned long long)weight[s]);
@@ -92,9 +114,9 @@ std::vector<uint32_t> PrefixTrie::members(uint32_t d) const {
     return rev;
 }
 
-PrefixTrie build_prefix_trie(pma_ctx* c, const uint32_t* elems, const uint64_t* seq_off, const uint32_t* seq_len,
-                             const uint64_t* rbase, const uint64_t* rec_time, const uint32_t* seq_weight, size_t S,
-                             uint64_t R, uint32_t* rec_node) {
+PrefixTrie build_prefix_trie(pma_ctx* c, const uint32_t* elems, uint64_t elem_cap, const uint64_t* seq_off,
+                             const uint32_t* seq_len, const uint64_t* rbase, const uint64_t* rec_time,
+                             const uint32_t* seq_weight, size_t S, uint64_t R, uint32_t* rec_node) {
     PrefixTrie t;
     if (S == 0 || R == 0) return t;
     PMA_REQUIRE(R < kNodeBit, PMA_ERR_CAPACITY, "more than 2^31 throat-key formations");
@@ -104,15 +126,17 @@ PrefixTrie build_prefix_trie(pma_ctx* c, const uint32_t* elems, const uint64_t* 
     uint32_t* lval = c->buf<uint32_t>("trie_lval", S);
     uint32_t* act = c->buf<uint32_t>("trie_act", S);
     unsigned int* dmax = c->buf<unsigned int>("trie_max_len", 1);
+    uint32_t* err = c->buf<uint32_t>("trie_err", 1);
     PMA_HIP(hipMemsetAsync(dmax, 0, 4, c->stream));
+    PMA_HIP(hipMemsetAsync(err, 0, 4, c->stream));
     length_keys<<<grid_for(S, 256), 256, 0, c->stream>>>(seq_len, S, lkey, lval, dmax);
-    PMA_LAUNCHED(c);
+    PMA_CHECKED(c, "length_keys", err);
     radix_sort_pairs_u32(c, lkey, lkey2, lval, act, S);
     const uint32_t max_len = read_scalar(c, dmax);
     unsigned int* dhist = c->buf<unsigned int>("trie_hist", (size_t)max_len + 1);
     PMA_HIP(hipMemsetAsync(dhist, 0, ((size_t)max_len + 1) * 4, c->stream));
     length_histogram<<<grid_for(S, 256), 256, 0, c->stream>>>(seq_len, S, max_len, dhist);
-    PMA_LAUNCHED(c);
+    PMA_CHECKED(c, "length_histogram", err);
     std::vector<uint32_t> hist((size_t)max_len + 1);
     PMA_HIP(hipMemcpyAsync(hist.data(), dhist, hist.size() * 4, hipMemcpyDeviceToHost, c->stream));
     PMA_HIP(hipStreamSynchronize(c->stream));
@@ -134,25 +158,28 @@ PrefixTrie build_prefix_trie(pma_ctx* c, const uint32_t* elems, const uint64_t* 
     unsigned long long* node_final = c->buf<unsigned long long>("trie_node_final", R);
     PMA_HIP(hipMemsetAsync(node_time, 0xFF, R * 8, c->stream));
     PMA_HIP(hipMemsetAsync(node_final, 0, R * 8, c->stream));
-    trie_start<<<grid_for(A1, 256), 256, 0, c->stream>>>(act, A1, elems, seq_off, cur);
-    PMA_LAUNCHED(c);
+    PMA_REQUIRE(A1 <= S, PMA_ERR_INTERNAL, "prefix trie: more active sequences than sequences");
+    trie_start<<<grid_for(A1, 256), 256, 0, c->stream>>>(act, A1, S, elems, seq_off, elem_cap, cur, err);
+    PMA_CHECKED(c, "trie_start", err);
     uint64_t D = 0;
     for (uint32_t L = 1; L < max_len; ++L) {
         const size_t A = longer[L];
         if (A == 0) break;
-        trie_keys<<<grid_for(A, 256), 256, 0, c->stream>>>(act, A, elems, seq_off, cur, L, key, val);
-        PMA_LAUNCHED(c);
+        trie_keys<<<grid_for(A, 256), 256, 0, c->stream>>>(act, A, S, elems, seq_off, elem_cap, cur, L, key, val, err);
+        PMA_CHECKED(c, "trie_keys", err);
         radix_sort_pairs_u64(c, key, key2, val, val2, A);
         trie_heads<<<grid_for(A, 256), 256, 0, c->stream>>>(key2, A, head);
-        PMA_LAUNCHED(c);
+        PMA_CHECKED(c, "trie_heads", err);
         const uint64_t DL = exclusive_scan_u32(c, head, hscan, A);
         PMA_REQUIRE(D + DL <= R, PMA_ERR_INTERNAL, "prefix trie: more nodes than formations");
-        trie_assign<<<grid_for(A, 256), 256, 0, c->stream>>>(key2, val2, head, hscan, A, (uint32_t)D, L, act,
-                                                             seq_len, rbase, rec_time, seq_weight, cur, rec_node,
-                                                             node_key, node_time, node_final);
-        PMA_LAUNCHED(c);
+        trie_assign<<<grid_for(A, 256), 256, 0, c->stream>>>(key2, val2, head, hscan, A, S, (uint32_t)D, L, act,
+                                                             seq_len, rbase, R, rec_time, seq_weight, cur, rec_node,
+                                                             node_key, node_time, node_final, err);
+        PMA_CHECKED(c, "trie_assign", err);
         D += DL;
     }
+    PMA_REQUIRE(read_scalar(c, err) == 0, PMA_ERR_INTERNAL,
+                "prefix trie: in-kernel check " + std::to_string(read_scalar(c, err)) + " failed");
     t.distinct = (uint32_t)D;
     t.key.resize(D);
     t.min_time.resize(D);

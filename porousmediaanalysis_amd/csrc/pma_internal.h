@@ -42,6 +42,11 @@ struct Error : std::runtime_error {
         if (!(cond)) throw ::pma::Error((status), (msg));                                     \
     } while (0)
 
+// A failed in-kernel check: the first code stored wins (kernel id << 8 | check id).  Kernels
+// that index buffers with values read from memory test each such index against the buffer's
+// length and report here instead of accessing it.
+__device__ __forceinline__ void fail_check(uint32_t* err, uint32_t code) { atomicCAS(err, 0u, code); }
+
 // Host tables (Types.h:157-180 of the reference).
 struct PoreRow {
     uint32_t id = 0;
@@ -105,6 +110,9 @@ struct pma_ctx {
 
     pma_stage_info stage[PMA_STAGE_COUNT] = {};
     uint64_t launches = 0;
+    // PMA_DEBUG_SYNC=1 at pma_create: every checked launch is named on stderr, waited for and
+    // its stage's error flag read, so that a failing kernel is identified.
+    bool debug_sync = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
     // per-kernel event timing (pma_set_kernel_timing)
@@ -181,9 +189,10 @@ struct PrefixTrie {
     std::vector<uint32_t> members(uint32_t d) const;  // the list, in sequence order
 };
 // rec_time and seq_weight may be null.  rec_node[r] receives the node of record r (device).
-PrefixTrie build_prefix_trie(pma_ctx* c, const uint32_t* elems, const uint64_t* seq_off, const uint32_t* seq_len,
-                             const uint64_t* rbase, const uint64_t* rec_time, const uint32_t* seq_weight, size_t S,
-                             uint64_t R, uint32_t* rec_node);
+// elems holds elem_cap entries; every sequence must lie inside it.
+PrefixTrie build_prefix_trie(pma_ctx* c, const uint32_t* elems, uint64_t elem_cap, const uint64_t* seq_off,
+                             const uint32_t* seq_len, const uint64_t* rbase, const uint64_t* rec_time,
+                             const uint32_t* seq_weight, size_t S, uint64_t R, uint32_t* rec_node);
 
 // Host -> device copy that returns only when the source may be released.
 inline void upload(pma_ctx* c, void* dst, const void* src, size_t bytes) {
@@ -201,6 +210,29 @@ template <class T> T read_scalar(pma_ctx* c, const T* dptr) {
 }
 
 inline unsigned grid_for(size_t n, unsigned block) { return (unsigned)((n + block - 1) / block); }
+
+// After a launch of `kernel`: checks the launch and, in debug-sync mode, names the kernel on
+// stderr, waits for it and reads the stage's error flag `err` (may be null).
+inline void launched(pma_ctx* c, const char* kernel, const uint32_t* err) {
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw Error(PMA_ERR_HIP, std::string("launch of ") + kernel + ": " + hipGetErrorString(e));
+    c->launches++;
+    if (!c->debug_sync) return;
+    std::fprintf(stderr, "[pma] %s\n", kernel);
+    std::fflush(stderr);
+    const hipError_t s = hipStreamSynchronize(c->stream);
+    if (s != hipSuccess) throw Error(PMA_ERR_HIP, std::string(kernel) + ": " + hipGetErrorString(s));
+    if (err) {
+        const uint32_t v = read_scalar(c, err);
+        if (v) throw Error(PMA_ERR_INTERNAL, std::string(kernel) + ": in-kernel check " + std::to_string(v) + " failed");
+    }
+}
+#define PMA_CHECKED(ctx, name, err) ::pma::launched((ctx), (name), (err))
+
+// Host-side guard of a copy or fill against the buffer it targets.
+inline void require_fits(size_t bytes, size_t capacity_bytes, const char* what) {
+    PMA_REQUIRE(bytes <= capacity_bytes, PMA_ERR_INTERNAL, std::string(what) + ": copy larger than its buffer");
+}
 
 // Brackets one kernel launch with events when kernel timing is enabled.
 struct KernelScope {

@@ -26,6 +26,12 @@
 //     been seen.
 //   * volumes = voxels per final set; boundary = some voxel of the pore's reach lies on the
 //     image border.
+//
+// Every kernel is total: an index read from memory is tested against the length of the buffer
+// it selects from (lengths are kernel arguments), and a failed test reports a code
+// (kernel << 8 | check) to the stage's error flag instead of making the access.  Loops over
+// parent chains and merge cursors are bounded by the counts that bound them in a consistent
+// state.  So whatever values a kernel reads, it touches only its own buffers.
 #include "pma_internal.h"
 
 #include <algorithm>
@@ -36,6 +42,25 @@ namespace pma {
 namespace {
 
 constexpr uint32_t kInv = 0xFFFFFFFFu;
+
+// Kernel ids of the in-kernel checks (the code's high bits).
+enum : uint32_t {
+    kWriteCentres = 1, kCcUnion, kCcFlatten, kCcNotMax, kFirstInSorted, kSeedFlags, kWriteSeeds, kPlateauIndex,
+    kPlateauOfVoxels, kSeedPlateaus, kPlateauEdges, kWriteEdges, kEdgeRanges, kLevelStarts, kSetUnion,
+    kReachSizes, kDfsReplay, kWriteLabels, kSingleVolumes
+};
+const char* const kKernelNames[] = {"?", "write_centres", "cc_union", "cc_flatten", "cc_not_max",
+                                    "first_in_sorted", "seed_flags", "write_seeds", "plateau_index",
+                                    "plateau_of_voxels", "seed_plateaus", "plateau_edges", "write_edges",
+                                    "edge_ranges", "level_starts", "set_union_level", "reach_sizes",
+                                    "dfs_replay", "write_labels", "single_volumes"};
+
+__device__ __forceinline__ uint32_t ld_relaxed(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_relaxed(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 __device__ __forceinline__ void decode(uint32_t i, int X, int Y, int& x, int& y, int& z) {
     y = (int)(i % (uint32_t)Y);
@@ -79,7 +104,8 @@ __global__ void __launch_bounds__(256) count_centres(const uint8_t* __restrict__
 
 __global__ void __launch_bounds__(256) write_centres(const uint8_t* __restrict__ mask, const uint32_t* __restrict__ e,
                                                      size_t n, const uint32_t* __restrict__ block_off,
-                                                     unsigned long long* __restrict__ rec) {
+                                                     unsigned long long* __restrict__ rec, size_t M,
+                                                     uint32_t* __restrict__ err) {
     __shared__ uint32_t wsum[4];
     size_t base = (size_t)blockIdx.x * 256 * kChunk + (size_t)threadIdx.x * kChunk;
     uint32_t c = 0;
@@ -93,11 +119,17 @@ __global__ void __launch_bounds__(256) write_centres(const uint8_t* __restrict__
     }
     if (lane == 63) wsum[wave] = incl;
     __syncthreads();
-    uint32_t o = block_off[blockIdx.x] + incl - c;
+    size_t o = (size_t)block_off[blockIdx.x] + incl - c;
     for (int w = 0; w < wave; ++w) o += wsum[w];
     for (int k = 0; k < kChunk; ++k) {
         size_t i = base + k;
-        if (i < n && mask[i] == 255) rec[o++] = ((unsigned long long)e[i] << 32) | (unsigned long long)i;
+        if (i < n && mask[i] == 255) {
+            if (o >= M) {
+                fail_check(err, kWriteCentres << 8 | 1);
+                return;
+            }
+            rec[o++] = ((unsigned long long)e[i] << 32) | (unsigned long long)i;
+        }
     }
 }
 
@@ -106,36 +138,54 @@ __global__ void cc_init(const uint32_t* __restrict__ cr, size_t n, uint32_t* __r
     if (i < n) parent[i] = cr[i] ? (uint32_t)i : kInv;
 }
 
-__device__ uint32_t find_root(uint32_t* parent, uint32_t x) {
-    while (true) {
-        uint32_t p = parent[x];
+// Root of x in the plateau forest.  Parents always point to smaller indices (a root is hooked
+// under a smaller root, and path halving replaces a parent by one of its ancestors), so a
+// chain ends within n links; a parent that does not decrease is reported and ends the walk
+// with kInv.  Parents are read and written with agent-scope atomics, so no value is taken
+// from a cache line another compute unit has since changed.
+__device__ uint32_t find_root(uint32_t* parent, uint32_t x, uint32_t n, uint32_t* err, uint32_t code) {
+    for (uint32_t step = 0; step <= n; ++step) {
+        const uint32_t p = ld_relaxed(&parent[x]);
         if (p == x) return x;
-        uint32_t g = parent[p];
-        if (g != p) parent[x] = g;  // path halving: g is an ancestor of x
+        if (p > x) {
+            fail_check(err, code);
+            return kInv;
+        }
+        const uint32_t g = ld_relaxed(&parent[p]);
+        if (g > p) {
+            fail_check(err, code);
+            return kInv;
+        }
+        if (g != p) st_relaxed(&parent[x], g);  // path halving: g is an ancestor of x
         x = p;
     }
+    fail_check(err, code | 1);
+    return kInv;
 }
 
-__device__ void unite(uint32_t* parent, uint32_t a, uint32_t b) {
-    while (true) {
-        a = find_root(parent, a);
-        b = find_root(parent, b);
-        if (a == b) return;
+__device__ void unite(uint32_t* parent, uint32_t a, uint32_t b, uint32_t n, uint32_t* err) {
+    // Every failed hook means another thread hooked the root, which happens at most n times.
+    for (uint32_t step = 0; step <= n; ++step) {
+        a = find_root(parent, a, n, err, kCcUnion << 8 | 2);
+        b = find_root(parent, b, n, err, kCcUnion << 8 | 2);
+        if (a == kInv || b == kInv || a == b) return;
         if (a < b) {
-            uint32_t t = a;
+            const uint32_t t = a;
             a = b;
             b = t;
         }
-        uint32_t old = atomicCAS(&parent[a], a, b);  // hook the larger root under the smaller
+        const uint32_t old = atomicCAS(&parent[a], a, b);  // hook the larger root under the smaller
         if (old == a) return;
         a = old;
     }
+    fail_check(err, kCcUnion << 8 | 4);
 }
 
 __global__ void __launch_bounds__(256) cc_union(const uint32_t* __restrict__ cr, uint32_t* parent, int X, int Y, int Z,
-                                                int d3q7) {
+                                                int d3q7, uint32_t* __restrict__ err) {
+    const size_t n = (size_t)X * Y * Z;
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)X * Y * Z) return;
+    if (i >= n) return;
     uint32_t c = cr[i];
     if (!c) return;
     int x, y, z;
@@ -145,19 +195,23 @@ __global__ void __launch_bounds__(256) cc_union(const uint32_t* __restrict__ cr,
         // k < 13 in loop order means dx < 0, or dx == 0 and (dy < 0 or (dy == 0 and dz < 0))
         uint32_t j;
         if (!neighbour(k, x, y, z, X, Y, Z, j)) continue;
-        if (cr[j] == c) unite(parent, (uint32_t)i, j);
+        if (cr[j] == c) unite(parent, (uint32_t)i, j, (uint32_t)n, err);
     }
 }
 
-__global__ void cc_flatten(uint32_t* parent, size_t n) {
+__global__ void cc_flatten(uint32_t* parent, size_t n, uint32_t* __restrict__ err) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && parent[i] != kInv) parent[i] = find_root(parent, (uint32_t)i);
+    if (i >= n || ld_relaxed(&parent[i]) == kInv) return;
+    const uint32_t r = find_root(parent, (uint32_t)i, (uint32_t)n, err, kCcFlatten << 8 | 2);
+    if (r != kInv) st_relaxed(&parent[i], r);
 }
 
 __global__ void __launch_bounds__(256) cc_not_max(const uint32_t* __restrict__ cr, const uint32_t* __restrict__ root,
-                                                  uint8_t* __restrict__ not_max, int X, int Y, int Z, int d3q7) {
+                                                  uint8_t* __restrict__ not_max, int X, int Y, int Z, int d3q7,
+                                                  uint32_t* __restrict__ err) {
+    const size_t n = (size_t)X * Y * Z;
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)X * Y * Z) return;
+    if (i >= n) return;
     uint32_t c = cr[i];
     if (!c) return;
     int x, y, z;
@@ -167,31 +221,50 @@ __global__ void __launch_bounds__(256) cc_not_max(const uint32_t* __restrict__ c
         uint32_t j;
         if (!neighbour(k, x, y, z, X, Y, Z, j)) continue;
         if (cr[j] > c) {
-            not_max[root[i]] = 1;
+            const uint32_t r = root[i];
+            if (r >= n) fail_check(err, kCcNotMax << 8 | 1);
+            else not_max[r] = 1;
             return;
         }
     }
 }
 
 __global__ void first_in_sorted(const unsigned long long* __restrict__ rec, size_t M, const uint32_t* __restrict__ root,
-                                const uint8_t* __restrict__ not_max, uint32_t* __restrict__ seedpos) {
+                                size_t n, const uint8_t* __restrict__ not_max, uint32_t* __restrict__ seedpos,
+                                uint32_t* __restrict__ err) {
     size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= M) return;
-    uint32_t r = root[(uint32_t)rec[s]];
+    const uint32_t v = (uint32_t)rec[s];
+    const uint32_t r = v < n ? root[v] : kInv;
+    if (r >= n) {
+        fail_check(err, kFirstInSorted << 8 | 1);
+        return;
+    }
     if (!not_max[r]) atomicMin(&seedpos[r], (uint32_t)s);
 }
 
 __global__ void seed_flags(const unsigned long long* __restrict__ rec, size_t M, const uint32_t* __restrict__ root,
-                           const uint32_t* __restrict__ seedpos, uint32_t* __restrict__ flag) {
+                           size_t n, const uint32_t* __restrict__ seedpos, uint32_t* __restrict__ flag,
+                           uint32_t* __restrict__ err) {
     size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= M) return;
-    flag[s] = seedpos[root[(uint32_t)rec[s]]] == (uint32_t)s;
+    const uint32_t v = (uint32_t)rec[s];
+    const uint32_t r = v < n ? root[v] : kInv;
+    if (r >= n) {
+        fail_check(err, kSeedFlags << 8 | 1);
+        flag[s] = 0;
+        return;
+    }
+    flag[s] = seedpos[r] == (uint32_t)s;
 }
 
 __global__ void write_seeds(const unsigned long long* __restrict__ rec, size_t M, const uint32_t* __restrict__ flag,
-                            const uint32_t* __restrict__ off, unsigned long long* __restrict__ seeds) {
+                            const uint32_t* __restrict__ off, unsigned long long* __restrict__ seeds,
+                            uint32_t* __restrict__ err) {
     size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < M && flag[s]) seeds[off[s]] = rec[s];
+    if (s >= M || !flag[s]) return;
+    if (off[s] >= M) fail_check(err, kWriteSeeds << 8 | 1);
+    else seeds[off[s]] = rec[s];
 }
 
 // ---- plateaus ----------------------------------------------------------------------------
@@ -202,24 +275,33 @@ __global__ void root_flags(const uint32_t* __restrict__ root, size_t n, uint32_t
 
 __global__ void plateau_index(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ off, size_t n,
                               uint32_t* __restrict__ pidx, const uint32_t* __restrict__ cr,
-                              uint32_t* __restrict__ plat_cr) {
+                              uint32_t* __restrict__ plat_cr, size_t P, uint32_t* __restrict__ err) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && flag[i]) {
-        pidx[i] = off[i];
-        plat_cr[off[i]] = cr[i];
+    if (i >= n || !flag[i]) return;
+    if (off[i] >= P) {
+        fail_check(err, kPlateauIndex << 8 | 1);
+        return;
     }
+    pidx[i] = off[i];
+    plat_cr[off[i]] = cr[i];
 }
 
 // root -> plateau index for every void voxel (in place), plus plateau size, border flag and
 // the voxel's rank inside its plateau.
-__global__ void plateau_of_voxels(uint32_t* root, const uint32_t* __restrict__ pidx, int X, int Y, int Z,
+__global__ void plateau_of_voxels(uint32_t* root, const uint32_t* __restrict__ pidx, int X, int Y, int Z, size_t P,
                                   uint32_t* __restrict__ plat_size, uint32_t* __restrict__ plat_border,
-                                  uint32_t* __restrict__ vrank) {
+                                  uint32_t* __restrict__ vrank, uint32_t* __restrict__ err) {
+    const size_t n = (size_t)X * Y * Z;
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)X * Y * Z) return;
+    if (i >= n) return;
     uint32_t r = root[i];
     if (r == kInv) return;
-    uint32_t q = pidx[r];  // only this thread reads or writes root[i]
+    const uint32_t q = r < n ? pidx[r] : kInv;  // only this thread reads or writes root[i]
+    if (q >= P) {
+        fail_check(err, kPlateauOfVoxels << 8 | 1);
+        root[i] = kInv;
+        return;
+    }
     root[i] = q;
     vrank[i] = atomicAdd(&plat_size[q], 1u);  // rank of the voxel inside its plateau (any order)
     int x, y, z;
@@ -236,15 +318,22 @@ __device__ __forceinline__ unsigned long long mix64(unsigned long long k) {
     return k;
 }
 
-// Distinct plateau edges (higher CR -> lower CR) in an open-addressing set.
+// Distinct plateau edges (higher CR -> lower CR) in an open-addressing set.  A voxel skips a
+// source plateau it has already inserted for its previous neighbours (at most 26).
 __global__ void __launch_bounds__(256) plateau_edges(const uint32_t* __restrict__ cr, const uint32_t* __restrict__ plat,
-                                                     int X, int Y, int Z, int d3q7, unsigned long long* __restrict__ table,
-                                                     unsigned long long mask, uint32_t* __restrict__ overflow) {
+                                                     int X, int Y, int Z, int d3q7, size_t P,
+                                                     unsigned long long* __restrict__ table, unsigned long long mask,
+                                                     uint32_t* __restrict__ overflow, uint32_t* __restrict__ err) {
+    const size_t n = (size_t)X * Y * Z;
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)X * Y * Z) return;
+    if (i >= n) return;
     uint32_t c = cr[i];
     if (!c) return;
     const uint32_t dst = plat[i];
+    if (dst >= P) {
+        fail_check(err, kPlateauEdges << 8 | 1);
+        return;
+    }
     int x, y, z;
     decode((uint32_t)i, X, Y, x, y, z);
     uint32_t done[26];
@@ -255,10 +344,14 @@ __global__ void __launch_bounds__(256) plateau_edges(const uint32_t* __restrict_
         if (!neighbour(k, x, y, z, X, Y, Z, j)) continue;
         if (cr[j] <= c) continue;
         const uint32_t src = plat[j];
+        if (src >= P) {
+            fail_check(err, kPlateauEdges << 8 | 2);
+            return;
+        }
         bool seen = false;
         for (int q = 0; q < nd; ++q) seen |= done[q] == src;
         if (seen) continue;
-        done[nd++] = src;
+        if (nd < 26) done[nd++] = src;
         const unsigned long long key = ((unsigned long long)src << 32) | dst;
         unsigned long long h = mix64(key) & mask;
         for (unsigned long long probe = 0;; ++probe) {
@@ -284,19 +377,28 @@ __global__ void slot_flags(const unsigned long long* __restrict__ table, size_t 
 
 // Edge keys re-packed dst-major so a sort groups them by target.
 __global__ void write_edges(const unsigned long long* __restrict__ table, size_t cap, const uint32_t* __restrict__ flag,
-                            const uint32_t* __restrict__ off, unsigned long long* __restrict__ edges) {
+                            const uint32_t* __restrict__ off, size_t E, unsigned long long* __restrict__ edges,
+                            uint32_t* __restrict__ err) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < cap && flag[i]) {
-        unsigned long long k = table[i];
-        edges[off[i]] = (k << 32) | (k >> 32);
+    if (i >= cap || !flag[i]) return;
+    if (off[i] >= E) {
+        fail_check(err, kWriteEdges << 8 | 1);
+        return;
     }
+    unsigned long long k = table[i];
+    edges[off[i]] = (k << 32) | (k >> 32);
 }
 
-__global__ void edge_ranges(const unsigned long long* __restrict__ edges, size_t E, uint32_t* __restrict__ in_start,
-                            uint32_t* __restrict__ in_count) {
+__global__ void edge_ranges(const unsigned long long* __restrict__ edges, size_t E, size_t P,
+                            uint32_t* __restrict__ in_start, uint32_t* __restrict__ in_count,
+                            uint32_t* __restrict__ err) {
     size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= E) return;
     uint32_t dst = (uint32_t)(edges[e] >> 32);
+    if (dst >= P) {
+        fail_check(err, kEdgeRanges << 8 | 1);
+        return;
+    }
     if (e == 0 || (uint32_t)(edges[e - 1] >> 32) != dst) in_start[dst] = (uint32_t)e;
     atomicAdd(&in_count[dst], 1u);
 }
@@ -315,36 +417,61 @@ __global__ void level_heads(const unsigned long long* __restrict__ keys, size_t 
 }
 
 __global__ void level_starts(const uint32_t* __restrict__ head, const uint32_t* __restrict__ off, size_t P,
-                             uint32_t* __restrict__ start) {
+                             uint32_t* __restrict__ start, size_t L, uint32_t* __restrict__ err) {
     size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < P && head[s]) start[off[s]] = (uint32_t)s;
+    if (s >= P || !head[s]) return;
+    if (off[s] >= L) fail_check(err, kLevelStarts << 8 | 1);
+    else start[off[s]] = (uint32_t)s;
 }
 
+// Sizes the pore-set stage reads its inputs with (each index read from memory is tested
+// against one of them).
+struct SetBounds {
+    unsigned long long E, P, npores, pool_cap;
+};
+
 // k-way merge of the sorted pore sets of a plateau's higher neighbours and its own pore.
-// cur/end hold one cursor per in-edge; out (if not null) receives the union in ascending order.
+// cur/end hold one cursor per in-edge; out (if not null, with room for out_cap) receives the
+// union in ascending order.  Returns the union's length, or kInv after a failed check.  Each
+// step emits one member and advances at least one cursor or retires the own pore, so the
+// loop ends within the total length of the inputs plus two steps.
 __device__ uint32_t merge_sets(uint32_t own, uint32_t e0, uint32_t ne, const unsigned long long* __restrict__ edges,
                                const unsigned long long* __restrict__ set_off, const uint32_t* __restrict__ set_len,
                                const uint32_t* __restrict__ pool, unsigned long long* __restrict__ cur,
-                               unsigned long long* __restrict__ end, uint32_t* __restrict__ out) {
+                               unsigned long long* __restrict__ end, uint32_t* __restrict__ out, uint32_t out_cap,
+                               const SetBounds& b, uint32_t* __restrict__ err) {
+    unsigned long long total = 1;
     for (uint32_t e = e0; e < e0 + ne; ++e) {
         const uint32_t src = (uint32_t)edges[e];
+        if (src >= b.P || set_off[src] + set_len[src] > b.pool_cap) {
+            fail_check(err, kSetUnion << 8 | 3);
+            return kInv;
+        }
         cur[e] = set_off[src];
         end[e] = set_off[src] + set_len[src];
+        total += set_len[src];
     }
     uint32_t m = 0;
     uint32_t own_left = own;
-    while (true) {
+    for (unsigned long long step = 0; step <= total; ++step) {
         uint32_t best = own_left;
         for (uint32_t e = e0; e < e0 + ne; ++e)
             if (cur[e] < end[e]) best = min(best, pool[cur[e]]);
-        if (best == kInv) break;
-        if (out) out[m] = best;
+        if (best == kInv) return m;
+        if (out) {
+            if (m >= out_cap) {
+                fail_check(err, kSetUnion << 8 | 4);
+                return kInv;
+            }
+            out[m] = best;
+        }
         ++m;
         if (own_left == best) own_left = kInv;
         for (uint32_t e = e0; e < e0 + ne; ++e)
             if (cur[e] < end[e] && pool[cur[e]] == best) ++cur[e];
     }
-    return m;
+    fail_check(err, kSetUnion << 8 | 5);
+    return kInv;
 }
 
 // Pore sets of the plateaus of one CR level (all their sources are on higher levels).
@@ -353,22 +480,37 @@ __global__ void set_union_level(const uint32_t* __restrict__ order, uint32_t lo,
                                 const unsigned long long* __restrict__ edges, const uint32_t* __restrict__ plat_pore,
                                 unsigned long long* __restrict__ set_off, uint32_t* __restrict__ set_len,
                                 uint32_t* __restrict__ pool, unsigned long long* __restrict__ pool_top,
-                                unsigned long long pool_cap, unsigned long long* __restrict__ cur,
-                                unsigned long long* __restrict__ end, uint32_t* __restrict__ overflow) {
+                                unsigned long long* __restrict__ cur, unsigned long long* __restrict__ end,
+                                SetBounds b, uint32_t* __restrict__ overflow, uint32_t* __restrict__ err) {
     const uint32_t s = lo + blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= hi) return;
     const uint32_t q = order[s];
+    if (q >= b.P) {
+        fail_check(err, kSetUnion << 8 | 1);
+        return;
+    }
     const uint32_t own = plat_pore[q];
     const uint32_t e0 = in_start[q], ne = in_count[q];
-    const uint32_t m = merge_sets(own, e0, ne, edges, set_off, set_len, pool, cur, end, nullptr);
+    if ((own != kInv && own >= b.npores) || (unsigned long long)e0 + ne > b.E) {
+        fail_check(err, kSetUnion << 8 | 2);
+        return;
+    }
+    const uint32_t m = merge_sets(own, e0, ne, edges, set_off, set_len, pool, cur, end, nullptr, 0, b, err);
+    if (m == kInv) return;
     const unsigned long long o = atomicAdd(pool_top, (unsigned long long)m);
-    if (o + m > pool_cap) {
+    if (o + m > b.pool_cap) {
         atomicOr(overflow, 2u);
         set_off[q] = 0;
         set_len[q] = 0;
         return;
     }
-    merge_sets(own, e0, ne, edges, set_off, set_len, pool, cur, end, pool + o);
+    const uint32_t m2 = merge_sets(own, e0, ne, edges, set_off, set_len, pool, cur, end, pool + o, m, b, err);
+    if (m2 != m) {
+        fail_check(err, kSetUnion << 8 | 6);
+        set_off[q] = 0;
+        set_len[q] = 0;
+        return;
+    }
     set_off[q] = o;
     set_len[q] = m;
 }
@@ -383,16 +525,25 @@ __global__ void prefix_counts(const uint32_t* __restrict__ set_len, size_t P, un
 
 __global__ void reach_sizes(const unsigned long long* __restrict__ set_off, const uint32_t* __restrict__ set_len,
                             const uint32_t* __restrict__ pool, const uint32_t* __restrict__ plat_size,
-                            const uint32_t* __restrict__ plat_border, size_t P, unsigned long long* __restrict__ reach,
-                            uint32_t* __restrict__ boundary, unsigned long long* __restrict__ plat_bits) {
+                            const uint32_t* __restrict__ plat_border, SetBounds b,
+                            unsigned long long* __restrict__ reach, uint32_t* __restrict__ boundary,
+                            unsigned long long* __restrict__ plat_bits, uint32_t* __restrict__ err) {
     size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= P) return;
+    if (q >= b.P) return;
+    plat_bits[q] = (unsigned long long)plat_size[q] * set_len[q];
+    if (set_off[q] + set_len[q] > b.pool_cap) {
+        fail_check(err, kReachSizes << 8 | 1);
+        return;
+    }
     const uint32_t* s = pool + set_off[q];
     for (uint32_t t = 0; t < set_len[q]; ++t) {
+        if (s[t] >= b.npores) {
+            fail_check(err, kReachSizes << 8 | 2);
+            return;
+        }
         atomicAdd(&reach[s[t]], (unsigned long long)plat_size[q]);
         if (plat_border[q]) boundary[s[t]] = 1;
     }
-    plat_bits[q] = (unsigned long long)plat_size[q] * set_len[q];
 }
 
 // ---- DFS replay: one wave per pore ------------------------------------------------------------
@@ -401,6 +552,11 @@ struct ReplayTask {
     unsigned long long stack_off;
     unsigned long long stack_cap;  // frames available to this task
     unsigned long long seen_off;   // first word of this pore's class bitmap
+};
+
+// Lengths of the buffers the replay selects from with values read from memory.
+struct ReplayBounds {
+    unsigned long long n, P, pool_cap, R, D, bits, frames, seen_words;
 };
 
 // Position of pore p in the ascending set pool[off .. off + len), or -1.
@@ -435,7 +591,7 @@ __global__ void __launch_bounds__(64) dfs_replay(const ReplayTask* __restrict__ 
                                                  uint32_t* __restrict__ seen_words, uint32_t* __restrict__ stack_voxel,
                                                  uint8_t* __restrict__ stack_k, uint32_t* __restrict__ rank,
                                                  uint32_t* __restrict__ task_status, int X, int Y, int Z, int d3q7,
-                                                 uint32_t* __restrict__ error) {
+                                                 ReplayBounds b, uint32_t* __restrict__ err) {
     // One wave per pore.  Lane 0 owns the stack, the seen-class bitmap and the counters and
     // broadcasts every decision; the other lanes evaluate the 26 neighbours in parallel.
     // Frames are (voxel, next neighbour index); a recursive DFS that tries neighbours in
@@ -443,27 +599,52 @@ __global__ void __launch_bounds__(64) dfs_replay(const ReplayTask* __restrict__ 
     const ReplayTask task = tasks[blockIdx.x];
     const int lane = threadIdx.x;
     const uint32_t p = task.pore;
+    if (task.stack_off + task.stack_cap > b.frames || task.seen_off + (task.ncls + 31) / 32 > b.seen_words ||
+        task.seed >= b.n) {
+        if (lane == 0) fail_check(err, kDfsReplay << 8 | 1);
+        return;
+    }
     uint32_t* sv = stack_voxel + task.stack_off;
     uint8_t* sk = stack_k + task.stack_off;
     uint32_t* seen = seen_words + task.seen_off;
 
-    // Marks v visited by p and, if its earlier-pore class is new, ranks that class.
-    // Executed by lane 0 only; returns the new count of classes found.
-    auto visit = [&](uint32_t v, uint32_t found) -> uint32_t {
-        const uint32_t q = plat[v];
-        const int j = member_index(pool, set_off[q], set_len[q], p);
-        if (j < 0) {
-            atomicOr(error, 1u);
-            return 0xFFFFFFFFu;
+    // Plateau q of void voxel v and the bit of (v, pore p) in the visited bitmap; false (with a
+    // reported code) if an index is out of range or p does not reach v's plateau.
+    auto locate = [&](uint32_t v, uint32_t& q, int& j, unsigned long long& bit, uint32_t code) -> bool {
+        q = plat[v];
+        if (q >= b.P || set_off[q] + set_len[q] > b.pool_cap) {
+            fail_check(err, kDfsReplay << 8 | code);
+            return false;
         }
-        const unsigned long long bit = visit_bit(bit_base, vrank, set_len, q, v, (uint32_t)j);
+        j = member_index(pool, set_off[q], set_len[q], p);
+        if (j < 0) {
+            fail_check(err, kDfsReplay << 8 | (code + 1));
+            return false;
+        }
+        bit = visit_bit(bit_base, vrank, set_len, q, v, (uint32_t)j);
+        if (bit >= b.bits) {
+            fail_check(err, kDfsReplay << 8 | (code + 2));
+            return false;
+        }
+        return true;
+    };
+
+    // Marks v visited by p and, if its earlier-pore class is new, ranks that class.
+    // Executed by lane 0 only; returns the new count of classes found, or kInv after a
+    // failed check.
+    auto visit = [&](uint32_t v, uint32_t found) -> uint32_t {
+        uint32_t q;
+        int j;
+        unsigned long long bit;
+        if (!locate(v, q, j, bit, 2)) return kInv;
         atomicOr(&visited[bit >> 6], 1ull << (bit & 63));
         if (j >= 1) {
-            const uint32_t e = rec_node[rbase[q] + (uint32_t)j - 1];
-            const uint32_t l = event_local[e];
+            const unsigned long long r = rbase[q] + (uint32_t)j - 1;
+            const uint32_t e = r < b.R ? rec_node[r] : kInv;
+            const uint32_t l = e < b.D ? event_local[e] : kInv;
             if (l >= task.ncls) {
-                atomicOr(error, 4u);
-                return 0xFFFFFFFFu;
+                fail_check(err, kDfsReplay << 8 | 5);
+                return kInv;
             }
             if (!(seen[l >> 5] & (1u << (l & 31)))) {
                 seen[l >> 5] |= 1u << (l & 31);
@@ -477,7 +658,7 @@ __global__ void __launch_bounds__(64) dfs_replay(const ReplayTask* __restrict__ 
     uint32_t found = 0;
     if (lane == 0) found = visit(task.seed, 0);
     found = (uint32_t)__shfl((int)found, 0);
-    if (found == 0xFFFFFFFFu) return;
+    if (found == kInv) return;
     unsigned long long top = 0;
     uint32_t u = task.seed;
     int k = 27;
@@ -490,12 +671,10 @@ __global__ void __launch_bounds__(64) dfs_replay(const ReplayTask* __restrict__ 
         if (lane < k && lane < 27 && in_stencil(lane, d3q7) && neighbour(lane, x, y, z, X, Y, Z, j)) {
             const uint32_t cn = cr[j];
             if (cn != 0 && cn <= cu) {
-                const uint32_t q = plat[j];
-                const int b = member_index(pool, set_off[q], set_len[q], p);
-                if (b < 0) {
-                    atomicOr(error, 2u);
-                } else {
-                    const unsigned long long bit = visit_bit(bit_base, vrank, set_len, q, j, (uint32_t)b);
+                uint32_t q;
+                int m;
+                unsigned long long bit;
+                if (locate(j, q, m, bit, 6)) {
                     const unsigned long long w =
                         __hip_atomic_load(&visited[bit >> 6], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     cand = !(w & (1ull << (bit & 63)));
@@ -513,6 +692,10 @@ __global__ void __launch_bounds__(64) dfs_replay(const ReplayTask* __restrict__ 
             }
             u = (uint32_t)__shfl((int)fu, 0);
             k = __shfl((int)fk, 0);
+            if (u >= b.n || k > 27) {
+                if (lane == 0) fail_check(err, kDfsReplay << 8 | 9);
+                return;
+            }
             continue;
         }
         if (top + 1 >= task.stack_cap) {  // no room for this frame: the host repeats the pore
@@ -527,48 +710,64 @@ __global__ void __launch_bounds__(64) dfs_replay(const ReplayTask* __restrict__ 
             found = visit(nvox, found);
         }
         found = (uint32_t)__shfl((int)found, 0);
-        if (found == 0xFFFFFFFFu) return;
+        if (found == kInv) return;
         ++top;
         u = nvox;
         k = 27;
     }
-    if (lane == 0 && found < task.ncls) atomicOr(error, 8u);  // reach exhausted before every class was seen
+    if (lane == 0 && found < task.ncls) fail_check(err, kDfsReplay << 8 | 10);  // reach exhausted too early
 }
 
 // ---- labels ------------------------------------------------------------------------------------
+struct LabelBounds {
+    unsigned long long P, pool_cap, R, D, npores;
+};
+
 __global__ void write_labels(const uint32_t* __restrict__ plat, const unsigned long long* __restrict__ set_off,
                              const uint32_t* __restrict__ set_len, const uint32_t* __restrict__ pool,
                              const unsigned long long* __restrict__ rbase, const uint32_t* __restrict__ rec_node,
-                             const uint32_t* __restrict__ key_id, size_t n, uint32_t* __restrict__ pt) {
+                             const uint32_t* __restrict__ key_id, size_t n, LabelBounds b, uint32_t* __restrict__ pt,
+                             uint32_t* __restrict__ err) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint32_t q = plat[i];
-    if (q == kInv) {
-        pt[i] = 0;
+    pt[i] = 0;
+    if (q == kInv) return;
+    if (q >= b.P) {
+        fail_check(err, kWriteLabels << 8 | 1);
         return;
     }
-    uint32_t m = set_len[q];
-    if (m == 0) pt[i] = 0;
-    else if (m == 1) pt[i] = pore_id_of(pool[set_off[q]]);
-    else pt[i] = key_id[rec_node[rbase[q] + m - 2]];
+    const uint32_t m = set_len[q];
+    if (m == 1) {
+        if (set_off[q] >= b.pool_cap || pool[set_off[q]] >= b.npores) fail_check(err, kWriteLabels << 8 | 2);
+        else pt[i] = pore_id_of(pool[set_off[q]]);
+    } else if (m >= 2) {
+        const unsigned long long r = rbase[q] + m - 2;
+        const uint32_t d = r < b.R ? rec_node[r] : kInv;
+        if (d >= b.D) fail_check(err, kWriteLabels << 8 | 3);
+        else pt[i] = key_id[d];
+    }
 }
 
 __global__ void single_volumes(const unsigned long long* __restrict__ set_off, const uint32_t* __restrict__ set_len,
-                               const uint32_t* __restrict__ pool, const uint32_t* __restrict__ plat_size, size_t P,
-                               unsigned long long* __restrict__ vol) {
+                               const uint32_t* __restrict__ pool, const uint32_t* __restrict__ plat_size, SetBounds b,
+                               unsigned long long* __restrict__ vol, uint32_t* __restrict__ err) {
     size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < P && set_len[q] == 1) atomicAdd(&vol[pool[set_off[q]]], (unsigned long long)plat_size[q]);
+    if (q >= b.P || set_len[q] != 1) return;
+    const uint32_t k = set_off[q] < b.pool_cap ? pool[set_off[q]] : kInv;
+    if (k >= b.npores) fail_check(err, kSingleVolumes << 8 | 1);
+    else atomicAdd(&vol[k], (unsigned long long)plat_size[q]);
 }
 
 __global__ void seed_plateaus(const uint32_t* __restrict__ plat, const uint32_t* __restrict__ seed_voxel, uint32_t n,
                               size_t nvox, uint32_t nplat, uint32_t* __restrict__ out, uint32_t* __restrict__ plat_pore,
-                              uint32_t* __restrict__ error) {
+                              uint32_t* __restrict__ err) {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const uint32_t v = seed_voxel[k];
     const uint32_t q = v < nvox ? plat[v] : kInv;
     if (q >= nplat) {
-        atomicOr(error, 8u);
+        fail_check(err, kSeedPlateaus << 8 | 1);
         return;
     }
     out[k] = q;
@@ -580,16 +779,19 @@ __global__ void fill_u32(uint32_t* p, uint32_t v, size_t n) {
     if (i < n) p[i] = v;
 }
 
+// Raises PMA_ERR_INTERNAL naming the kernel whose check failed, if any did.
+void check_stage(pma_ctx* c, const uint32_t* err) {
+    const uint32_t v = read_scalar(c, err);
+    if (!v) return;
+    const uint32_t kid = v >> 8;
+    const char* name = kid < sizeof kKernelNames / sizeof kKernelNames[0] ? kKernelNames[kid] : "?";
+    throw Error(PMA_ERR_INTERNAL, std::string("pma_pores: in-kernel check ") + std::to_string(v & 0xFF) + " of " +
+                                      name + " failed (inconsistent intermediate state)");
+}
+
 }  // namespace
 
 void run_pores(pma_ctx* c) {
-    // Switched off on the device: on MI355X this stage has made illegal memory accesses whose
-    // cause is not found; its kernels are verified bit for bit against the oracle only in a
-    // host emulation (tests/hostemu).  PMA_UNSAFE_GPU_PORES=1 runs it anyway.
-    const char* unsafe = std::getenv("PMA_UNSAFE_GPU_PORES");
-    PMA_REQUIRE(unsafe && unsafe[0] == '1', PMA_ERR_STATE,
-                "pma_pores: switched off on the GPU (an unexplained illegal memory access on MI355X); "
-                "set PMA_UNSAFE_GPU_PORES=1 to run it anyway");
     PMA_REQUIRE(c->valid[PMA_VOL_MASK] && c->valid[PMA_VOL_EDT2] && c->valid[PMA_VOL_CR2], PMA_ERR_STATE,
                 "pma_pores: the maximum-ball mask, EDT and containing radii are needed");
     const int X = c->dims.x, Y = c->dims.y, Z = c->dims.z;
@@ -602,17 +804,20 @@ void run_pores(pma_ctx* c) {
     c->ensure_volume(PMA_VOL_PT);
     uint32_t* pt = c->v<uint32_t>(PMA_VOL_PT);
     const unsigned nb = grid_for(n, 256);
+    uint32_t* err = c->buf<uint32_t>("pt_err", 1);
+    uint32_t* of = c->buf<uint32_t>("pt_overflow", 1);
+    PMA_HIP(hipMemsetAsync(err, 0, 4, c->stream));
 
     // 1. maximum balls in z, x, y order (memory order), then the libstdc++ permutation.
     const size_t cblocks = (n + 256 * kChunk - 1) / (256 * kChunk);
     uint32_t* bcount = c->buf<uint32_t>("pt_bcount", cblocks);
     uint32_t* boff = c->buf<uint32_t>("pt_boff", cblocks);
     count_centres<<<(unsigned)cblocks, 256, 0, c->stream>>>(mask, n, bcount);
-    PMA_LAUNCHED(c);
+    PMA_CHECKED(c, "count_centres", err);
     const uint64_t M = exclusive_scan_u32(c, bcount, boff, cblocks);
     unsigned long long* rec = c->buf<unsigned long long>("pt_rec", std::max<uint64_t>(M, 1));
-    write_centres<<<(unsigned)cblocks, 256, 0, c->stream>>>(mask, e, n, boff, rec);
-    PMA_LAUNCHED(c);
+    write_centres<<<(unsigned)cblocks, 256, 0, c->stream>>>(mask, e, n, boff, rec, M, err);
+    PMA_CHECKED(c, "write_centres", err);
     sort_like_libstdcxx_desc(c, reinterpret_cast<uint64_t*>(rec), M);
 
     // 2. plateaus of CR and regional maxima.
@@ -620,62 +825,68 @@ void run_pores(pma_ctx* c) {
     uint8_t* not_max = c->buf<uint8_t>("pt_not_max", n);
     uint32_t* seedpos = c->buf<uint32_t>("pt_seedpos", n);
     cc_init<<<nb, 256, 0, c->stream>>>(cr, n, root);
-    PMA_LAUNCHED(c);
+    PMA_CHECKED(c, "cc_init", err);
     {
         PMA_TIMED(c, "cc_union");
-        cc_union<<<nb, 256, 0, c->stream>>>(cr, root, X, Y, Z, d3q7);
-        PMA_LAUNCHED(c);
+        cc_union<<<nb, 256, 0, c->stream>>>(cr, root, X, Y, Z, d3q7, err);
+        PMA_CHECKED(c, "cc_union", err);
     }
-    cc_flatten<<<nb, 256, 0, c->stream>>>(root, n);
-    PMA_LAUNCHED(c);
+    cc_flatten<<<nb, 256, 0, c->stream>>>(root, n, err);
+    PMA_CHECKED(c, "cc_flatten", err);
     PMA_HIP(hipMemsetAsync(not_max, 0, n, c->stream));
     PMA_HIP(hipMemsetAsync(seedpos, 0xFF, n * sizeof(uint32_t), c->stream));
     {
         PMA_TIMED(c, "cc_not_max");
-        cc_not_max<<<nb, 256, 0, c->stream>>>(cr, root, not_max, X, Y, Z, d3q7);
-        PMA_LAUNCHED(c);
+        cc_not_max<<<nb, 256, 0, c->stream>>>(cr, root, not_max, X, Y, Z, d3q7, err);
+        PMA_CHECKED(c, "cc_not_max", err);
     }
 
     // 3. seeds: the first sorted maximum ball of every regional-maximum plateau.
     uint32_t npores = 0;
     unsigned long long* seeds = c->buf<unsigned long long>("pt_seeds", std::max<uint64_t>(M, 1));
     if (M) {
-        first_in_sorted<<<grid_for(M, 256), 256, 0, c->stream>>>(rec, M, root, not_max, seedpos);
-        PMA_LAUNCHED(c);
+        first_in_sorted<<<grid_for(M, 256), 256, 0, c->stream>>>(rec, M, root, n, not_max, seedpos, err);
+        PMA_CHECKED(c, "first_in_sorted", err);
         uint32_t* sflag = c->buf<uint32_t>("pt_sflag", M);
         uint32_t* soff = c->buf<uint32_t>("pt_soff", M);
-        seed_flags<<<grid_for(M, 256), 256, 0, c->stream>>>(rec, M, root, seedpos, sflag);
-        PMA_LAUNCHED(c);
+        seed_flags<<<grid_for(M, 256), 256, 0, c->stream>>>(rec, M, root, n, seedpos, sflag, err);
+        PMA_CHECKED(c, "seed_flags", err);
         npores = (uint32_t)exclusive_scan_u32(c, sflag, soff, M);
-        write_seeds<<<grid_for(M, 256), 256, 0, c->stream>>>(rec, M, sflag, soff, seeds);
-        PMA_LAUNCHED(c);
+        write_seeds<<<grid_for(M, 256), 256, 0, c->stream>>>(rec, M, sflag, soff, seeds, err);
+        PMA_CHECKED(c, "write_seeds", err);
     }
+    check_stage(c, err);
+    PMA_REQUIRE(npores <= M, PMA_ERR_INTERNAL, "pma_pores: more seeds than maximum balls");
     std::vector<unsigned long long> seeds_h(npores);
-    if (npores)
+    if (npores) {
         PMA_HIP(hipMemcpyAsync(seeds_h.data(), seeds, npores * 8ull, hipMemcpyDeviceToHost, c->stream));
+        PMA_HIP(hipStreamSynchronize(c->stream));
+    }
+    for (uint32_t k = 0; k < npores; ++k)
+        PMA_REQUIRE((uint32_t)seeds_h[k] < n, PMA_ERR_INTERNAL, "pma_pores: a seed lies outside the volume");
 
     // 4. plateau indices, sizes, border flags.
     uint32_t* rflag = c->buf<uint32_t>("pt_rflag", n);
     uint32_t* roff = c->buf<uint32_t>("pt_roff", n);
     root_flags<<<nb, 256, 0, c->stream>>>(root, n, rflag);
-    PMA_LAUNCHED(c);
+    PMA_CHECKED(c, "root_flags", err);
     const uint64_t P = exclusive_scan_u32(c, rflag, roff, n);
+    PMA_REQUIRE(P <= n, PMA_ERR_INTERNAL, "pma_pores: more plateaus than voxels");
     uint32_t* pidx = seedpos;  // seed positions are no longer needed
     const size_t P1 = std::max<uint64_t>(P, 1);
     uint32_t* plat_cr = c->buf<uint32_t>("pt_plat_cr", P1);
     uint32_t* plat_size = c->buf<uint32_t>("pt_plat_size", P1);
     uint32_t* plat_border = c->buf<uint32_t>("pt_plat_border", P1);
     uint32_t* plat_pore = c->buf<uint32_t>("pt_plat_pore", P1);
-    plateau_index<<<nb, 256, 0, c->stream>>>(rflag, roff, n, pidx, cr, plat_cr);
-    PMA_LAUNCHED(c);
+    plateau_index<<<nb, 256, 0, c->stream>>>(rflag, roff, n, pidx, cr, plat_cr, P, err);
+    PMA_CHECKED(c, "plateau_index", err);
     PMA_HIP(hipMemsetAsync(plat_size, 0, P1 * 4, c->stream));
     PMA_HIP(hipMemsetAsync(plat_border, 0, P1 * 4, c->stream));
     fill_u32<<<grid_for(P1, 256), 256, 0, c->stream>>>(plat_pore, kInv, P1);
-    PMA_LAUNCHED(c);
-    PMA_HIP(hipStreamSynchronize(c->stream));
+    PMA_CHECKED(c, "fill_u32", err);
     uint32_t* vrank = c->buf<uint32_t>("pt_vrank", n);
-    plateau_of_voxels<<<nb, 256, 0, c->stream>>>(root, pidx, X, Y, Z, plat_size, plat_border, vrank);
-    PMA_LAUNCHED(c);
+    plateau_of_voxels<<<nb, 256, 0, c->stream>>>(root, pidx, X, Y, Z, P, plat_size, plat_border, vrank, err);
+    PMA_CHECKED(c, "plateau_of_voxels", err);
     uint32_t* plat = root;  // now the plateau index of every void voxel
     if (npores) {
         // plat_pore[plat[seed_k]] = k
@@ -684,29 +895,26 @@ void run_pores(pma_ctx* c) {
         uint32_t* d_sv = c->buf<uint32_t>("pt_seed_voxels", npores);
         upload(c, d_sv, sv.data(), npores * 4ull);
         uint32_t* d_sp = c->buf<uint32_t>("pt_seed_plat", npores);
-        uint32_t* err = c->buf<uint32_t>("pt_overflow", 1);
-        PMA_HIP(hipMemsetAsync(err, 0, 4, c->stream));
         seed_plateaus<<<grid_for(npores, 256), 256, 0, c->stream>>>(plat, d_sv, npores, n, (uint32_t)P, d_sp,
                                                                     plat_pore, err);
-        PMA_LAUNCHED(c);
-        PMA_REQUIRE(read_scalar(c, err) == 0, PMA_ERR_INTERNAL, "pma_pores: a seed lies outside the plateaus");
+        PMA_CHECKED(c, "seed_plateaus", err);
     }
+    check_stage(c, err);
 
     // 5. plateau DAG edges (distinct), grouped by target.
     size_t cap = 1024;
     while (cap < P * 8) cap <<= 1;
-    unsigned long long E = 0;
     unsigned long long* table = nullptr;
-    uint32_t* of = c->buf<uint32_t>("pt_overflow", 1);
     for (int attempt = 0;; ++attempt) {
         table = c->buf<unsigned long long>("pt_edge_table", cap);
         PMA_HIP(hipMemsetAsync(table, 0xFF, cap * 8, c->stream));
         PMA_HIP(hipMemsetAsync(of, 0, 4, c->stream));
         {
             PMA_TIMED(c, "plateau_edges");
-            plateau_edges<<<nb, 256, 0, c->stream>>>(cr, plat, X, Y, Z, d3q7, table, cap - 1, of);
-            PMA_LAUNCHED(c);
+            plateau_edges<<<nb, 256, 0, c->stream>>>(cr, plat, X, Y, Z, d3q7, P, table, cap - 1, of, err);
+            PMA_CHECKED(c, "plateau_edges", err);
         }
+        check_stage(c, err);
         if (read_scalar(c, of) == 0) break;
         PMA_REQUIRE(attempt < 4, PMA_ERR_INTERNAL, "pma_pores: plateau edge table overflow");
         cap <<= 2;
@@ -714,13 +922,14 @@ void run_pores(pma_ctx* c) {
     uint32_t* tflag = c->buf<uint32_t>("pt_tflag", cap);
     uint32_t* toff = c->buf<uint32_t>("pt_toff", cap);
     slot_flags<<<grid_for(cap, 256), 256, 0, c->stream>>>(table, cap, tflag);
-    PMA_LAUNCHED(c);
-    E = exclusive_scan_u32(c, tflag, toff, cap);
-    const size_t E1 = std::max<unsigned long long>(E, 1);
+    PMA_CHECKED(c, "slot_flags", err);
+    const uint64_t E = exclusive_scan_u32(c, tflag, toff, cap);
+    PMA_REQUIRE(E <= cap && E < 0xFFFFFFFFull, PMA_ERR_INTERNAL, "pma_pores: edge count out of range");
+    const size_t E1 = std::max<uint64_t>(E, 1);
     unsigned long long* edges = c->buf<unsigned long long>("pt_edges", E1);
     unsigned long long* edges2 = c->buf<unsigned long long>("pt_edges2", E1);
-    write_edges<<<grid_for(cap, 256), 256, 0, c->stream>>>(table, cap, tflag, toff, edges);
-    PMA_LAUNCHED(c);
+    write_edges<<<grid_for(cap, 256), 256, 0, c->stream>>>(table, cap, tflag, toff, E, edges, err);
+    PMA_CHECKED(c, "write_edges", err);
     uint32_t* dummy = c->buf<uint32_t>("pt_dummy", E1);
     uint32_t* dummy2 = c->buf<uint32_t>("pt_dummy2", E1);
     radix_sort_pairs_u64(c, reinterpret_cast<uint64_t*>(edges), reinterpret_cast<uint64_t*>(edges2), dummy, dummy2, E);
@@ -730,8 +939,8 @@ void run_pores(pma_ctx* c) {
     PMA_HIP(hipMemsetAsync(in_start, 0, P1 * 4, c->stream));
     PMA_HIP(hipMemsetAsync(in_count, 0, P1 * 4, c->stream));
     if (E) {
-        edge_ranges<<<grid_for(E, 256), 256, 0, c->stream>>>(edges, E, in_start, in_count);
-        PMA_LAUNCHED(c);
+        edge_ranges<<<grid_for(E, 256), 256, 0, c->stream>>>(edges, E, P, in_start, in_count, err);
+        PMA_CHECKED(c, "edge_ranges", err);
     }
 
     // 6. pore sets, level by level in descending CR.
@@ -748,20 +957,24 @@ void run_pores(pma_ctx* c) {
     std::vector<uint32_t> lstart_h;
     if (P) {
         level_keys<<<grid_for(P, 256), 256, 0, c->stream>>>(plat_cr, P, lkeys, lvals);
-        PMA_LAUNCHED(c);
+        PMA_CHECKED(c, "level_keys", err);
         radix_sort_pairs_u64(c, reinterpret_cast<uint64_t*>(lkeys), reinterpret_cast<uint64_t*>(lkeys2), lvals, order, P);
         uint32_t* lhead = c->buf<uint32_t>("pt_lhead", P);
         uint32_t* loff = c->buf<uint32_t>("pt_loff", P);
         level_heads<<<grid_for(P, 256), 256, 0, c->stream>>>(lkeys2, P, lhead);
-        PMA_LAUNCHED(c);
-        uint32_t L = (uint32_t)exclusive_scan_u32(c, lhead, loff, P);
+        PMA_CHECKED(c, "level_heads", err);
+        const uint64_t L = exclusive_scan_u32(c, lhead, loff, P);
+        PMA_REQUIRE(L >= 1 && L <= P, PMA_ERR_INTERNAL, "pma_pores: level count out of range");
         uint32_t* lstart = c->buf<uint32_t>("pt_lstart", L);
-        level_starts<<<grid_for(P, 256), 256, 0, c->stream>>>(lhead, loff, P, lstart);
-        PMA_LAUNCHED(c);
+        level_starts<<<grid_for(P, 256), 256, 0, c->stream>>>(lhead, loff, P, lstart, L, err);
+        PMA_CHECKED(c, "level_starts", err);
+        check_stage(c, err);
         lstart_h.resize(L + 1);
         PMA_HIP(hipMemcpyAsync(lstart_h.data(), lstart, L * 4ull, hipMemcpyDeviceToHost, c->stream));
         PMA_HIP(hipStreamSynchronize(c->stream));
         lstart_h[L] = (uint32_t)P;
+        for (size_t l = 0; l < L; ++l)
+            PMA_REQUIRE(lstart_h[l] < lstart_h[l + 1], PMA_ERR_INTERNAL, "pma_pores: level starts out of order");
     }
     unsigned long long pool_cap = std::max<unsigned long long>(4 * P, 1024);
     uint32_t* pool = nullptr;
@@ -770,22 +983,24 @@ void run_pores(pma_ctx* c) {
         pool = c->buf<uint32_t>("pt_pool", pool_cap);
         PMA_HIP(hipMemsetAsync(pool_top, 0, 8, c->stream));
         PMA_HIP(hipMemsetAsync(of, 0, 4, c->stream));
+        const SetBounds sb{E, P, npores, pool_cap};
         for (size_t l = 0; l + 1 < lstart_h.size(); ++l) {
             uint32_t lo = lstart_h[l], hi = lstart_h[l + 1];
             {
                 PMA_TIMED(c, "set_union_level");
-                set_union_level<<<grid_for(hi - lo, 64), 64, 0, c->stream>>>(order, lo, hi, in_start, in_count, edges,
-                                                                             plat_pore, set_off, set_len, pool, pool_top,
-                                                                             pool_cap, cur, cend, of);
-                PMA_LAUNCHED(c);
+                set_union_level<<<grid_for(hi - lo, 64), 64, 0, c->stream>>>(
+                    order, lo, hi, in_start, in_count, edges, plat_pore, set_off, set_len, pool, pool_top, cur, cend,
+                    sb, of, err);
+                PMA_CHECKED(c, "set_union_level", err);
             }
         }
+        check_stage(c, err);
         const uint32_t o = read_scalar(c, of);
         if (!(o & 2u)) break;
         PMA_REQUIRE(attempt < 3, PMA_ERR_INTERNAL, "pma_pores: pore set pool overflow");
         pool_cap = read_scalar(c, pool_top) + 1024;
     }
-    const unsigned long long pool_used = read_scalar(c, pool_top);
+    const SetBounds sb{E, P, npores, pool_cap};
 
     // 7. key formations (prefix records) and their distinct lists.
     unsigned long long* pcnt = c->buf<unsigned long long>("pt_pcnt", P1);
@@ -793,13 +1008,14 @@ void run_pores(pma_ctx* c) {
     uint64_t R = 0;
     if (P) {
         prefix_counts<<<grid_for(P, 256), 256, 0, c->stream>>>(set_len, P, pcnt);
-        PMA_LAUNCHED(c);
+        PMA_CHECKED(c, "prefix_counts", err);
         R = exclusive_scan_u64(c, reinterpret_cast<const uint64_t*>(pcnt), reinterpret_cast<uint64_t*>(rbase), P);
     }
+    PMA_REQUIRE(R <= pool_cap, PMA_ERR_INTERNAL, "pma_pores: more key formations than set members");
     uint32_t* rd = c->buf<uint32_t>("pt_rd", std::max<uint64_t>(R, 1));
     PrefixTrie trie;
     if (R)
-        trie = build_prefix_trie(c, pool, reinterpret_cast<const uint64_t*>(set_off), set_len,
+        trie = build_prefix_trie(c, pool, pool_cap, reinterpret_cast<const uint64_t*>(set_off), set_len,
                                  reinterpret_cast<const uint64_t*>(rbase), nullptr, plat_size, P, R, rd);
     // the pore that forms a list is its largest member, the last one in ascending order
     const uint32_t D = trie.distinct;
@@ -819,12 +1035,13 @@ void run_pores(pma_ctx* c) {
     PMA_HIP(hipMemsetAsync(bflag, 0, std::max(npores, 1u) * 4ull, c->stream));
     uint64_t total_bits = 0;
     if (P) {
-        reach_sizes<<<grid_for(P, 256), 256, 0, c->stream>>>(set_off, set_len, pool, plat_size, plat_border, P, reach,
-                                                             bflag, plat_bits);
-        PMA_LAUNCHED(c);
+        reach_sizes<<<grid_for(P, 256), 256, 0, c->stream>>>(set_off, set_len, pool, plat_size, plat_border, sb, reach,
+                                                             bflag, plat_bits, err);
+        PMA_CHECKED(c, "reach_sizes", err);
         total_bits = exclusive_scan_u64(c, reinterpret_cast<const uint64_t*>(plat_bits),
                                         reinterpret_cast<uint64_t*>(bit_base), P);
     }
+    check_stage(c, err);
     std::vector<unsigned long long> reach_h(npores);
     std::vector<uint32_t> bflag_h(npores);
     if (npores) {
@@ -832,6 +1049,8 @@ void run_pores(pma_ctx* c) {
         PMA_HIP(hipMemcpyAsync(bflag_h.data(), bflag, npores * 4ull, hipMemcpyDeviceToHost, c->stream));
         PMA_HIP(hipStreamSynchronize(c->stream));
     }
+    for (uint32_t k = 0; k < npores; ++k)
+        PMA_REQUIRE(reach_h[k] >= 1 && reach_h[k] <= n, PMA_ERR_INTERNAL, "pma_pores: pore reach out of range");
 
     // 8. DFS replays for pores that form two or more lists: rank of each list inside its pore.
     std::vector<uint32_t> rank_h(D, 0);
@@ -892,17 +1111,15 @@ void run_pores(pma_ctx* c) {
                     uint8_t* stack_k = c->buf<uint8_t>("pt_stack_k", frames);
                     upload(c, d_tasks, batch.data(), batch.size() * sizeof(ReplayTask));
                     PMA_HIP(hipMemsetAsync(status, 0, batch.size() * 4, c->stream));
-                    PMA_HIP(hipMemsetAsync(of, 0, 4, c->stream));
+                    const ReplayBounds rb{n, P, pool_cap, R, D, (unsigned long long)vwords * 64, frames, seen_total};
                     {
                         PMA_TIMED(c, "dfs_replay");
                         dfs_replay<<<(unsigned)batch.size(), 64, 0, c->stream>>>(
                             d_tasks, cr, plat, set_off, set_len, pool, rbase, rd, d_local, bit_base, vrank, visited,
-                            seen, stack_voxel, stack_k, d_rank, status, X, Y, Z, d3q7, of);
-                        PMA_LAUNCHED(c);
+                            seen, stack_voxel, stack_k, d_rank, status, X, Y, Z, d3q7, rb, err);
+                        PMA_CHECKED(c, "dfs_replay", err);
                     }
-                    const uint32_t rerr = read_scalar(c, of);
-                    PMA_REQUIRE(rerr == 0, PMA_ERR_INTERNAL,
-                                "pma_pores: DFS replay invariant broken (code " + std::to_string(rerr) + ")");
+                    check_stage(c, err);
                     std::vector<uint32_t> st(batch.size());
                     PMA_HIP(hipMemcpyAsync(st.data(), status, batch.size() * 4ull, hipMemcpyDeviceToHost, c->stream));
                     PMA_HIP(hipStreamSynchronize(c->stream));
@@ -933,19 +1150,20 @@ void run_pores(pma_ctx* c) {
     upload(c, d_kid, key_id.data(), D * 4ull);
     {
         PMA_TIMED(c, "write_labels");
-        write_labels<<<nb, 256, 0, c->stream>>>(plat, set_off, set_len, pool, rbase, rd, d_kid, n, pt);
-        PMA_LAUNCHED(c);
+        const LabelBounds lb{P, pool_cap, R, D, npores};
+        write_labels<<<nb, 256, 0, c->stream>>>(plat, set_off, set_len, pool, rbase, rd, d_kid, n, lb, pt, err);
+        PMA_CHECKED(c, "write_labels", err);
     }
     unsigned long long* pvol = c->buf<unsigned long long>("pt_pvol", std::max(npores, 1u));
     PMA_HIP(hipMemsetAsync(pvol, 0, std::max(npores, 1u) * 8ull, c->stream));
     if (P) {
-        single_volumes<<<grid_for(P, 256), 256, 0, c->stream>>>(set_off, set_len, pool, plat_size, P, pvol);
-        PMA_LAUNCHED(c);
+        single_volumes<<<grid_for(P, 256), 256, 0, c->stream>>>(set_off, set_len, pool, plat_size, sb, pvol, err);
+        PMA_CHECKED(c, "single_volumes", err);
     }
+    check_stage(c, err);
     std::vector<unsigned long long> pvol_h(npores);
     if (npores) PMA_HIP(hipMemcpyAsync(pvol_h.data(), pvol, npores * 8ull, hipMemcpyDeviceToHost, c->stream));
     PMA_HIP(hipStreamSynchronize(c->stream));
-    (void)pool_used;
 
     // 10. tables: pores in ascending ID, keys with volume in ascending ID, lists in the
     // reference's order [p2, p1, p3, ...].
@@ -966,6 +1184,7 @@ void run_pores(pma_ctx* c) {
         uint32_t d = ord[r];
         if (trie.finals[d] == 0) continue;
         std::vector<uint32_t> mem = trie.members(d);
+        PMA_REQUIRE(mem.size() >= 2, PMA_ERR_INTERNAL, "pma_pores: a throat key with fewer than two pores");
         ThroatRow t;
         t.id = key_id[d];
         t.volume = trie.finals[d];

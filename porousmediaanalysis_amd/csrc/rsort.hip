@@ -1,4 +1,5 @@
-// Device-wide stable radix sorts (rocPRIM) with temporary storage kept in the context.
+// Device-wide stable radix sorts (rocPRIM) with temporary storage kept in the context.  In
+// debug-sync mode each sort is waited for and named, like the library's own kernels.
 #include <cstring>
 
 #include <rocprim/rocprim.hpp>
@@ -16,7 +17,7 @@ void radix_sort_pairs_u64(pma_ctx* c, const uint64_t* keys_in, uint64_t* keys_ou
     void* tmp = c->buf("rsort_tmp", bytes);
     PMA_HIP(rocprim::radix_sort_pairs(tmp, bytes, keys_in, keys_out, vals_in, vals_out, n, begin_bit, end_bit,
                                       c->stream));
-    c->launches++;
+    PMA_CHECKED(c, "rocprim::radix_sort", nullptr);
 }
 
 void radix_sort_pairs_u64_u64(pma_ctx* c, const uint64_t* keys_in, uint64_t* keys_out, const uint64_t* vals_in,
@@ -28,7 +29,7 @@ void radix_sort_pairs_u64_u64(pma_ctx* c, const uint64_t* keys_in, uint64_t* key
     void* tmp = c->buf("rsort_tmp", bytes);
     PMA_HIP(rocprim::radix_sort_pairs(tmp, bytes, keys_in, keys_out, vals_in, vals_out, n, begin_bit, end_bit,
                                       c->stream));
-    c->launches++;
+    PMA_CHECKED(c, "rocprim::radix_sort", nullptr);
 }
 
 void radix_sort_pairs_u32(pma_ctx* c, const uint32_t* keys_in, uint32_t* keys_out, const uint32_t* vals_in,
@@ -40,7 +41,7 @@ void radix_sort_pairs_u32(pma_ctx* c, const uint32_t* keys_in, uint32_t* keys_ou
     void* tmp = c->buf("rsort_tmp", bytes);
     PMA_HIP(rocprim::radix_sort_pairs(tmp, bytes, keys_in, keys_out, vals_in, vals_out, n, begin_bit, end_bit,
                                       c->stream));
-    c->launches++;
+    PMA_CHECKED(c, "rocprim::radix_sort", nullptr);
 }
 
 void radix_sort_pairs_u32_u64(pma_ctx* c, const uint32_t* keys_in, uint32_t* keys_out, const uint64_t* vals_in,
@@ -52,7 +53,7 @@ void radix_sort_pairs_u32_u64(pma_ctx* c, const uint32_t* keys_in, uint32_t* key
     void* tmp = c->buf("rsort_tmp", bytes);
     PMA_HIP(rocprim::radix_sort_pairs(tmp, bytes, keys_in, keys_out, vals_in, vals_out, n, begin_bit, end_bit,
                                       c->stream));
-    c->launches++;
+    PMA_CHECKED(c, "rocprim::radix_sort", nullptr);
 }
 
 void radix_sort_keys_u64(pma_ctx* c, const uint64_t* keys_in, uint64_t* keys_out, size_t n, int begin_bit,
@@ -62,7 +63,7 @@ void radix_sort_keys_u64(pma_ctx* c, const uint64_t* keys_in, uint64_t* keys_out
     PMA_HIP(rocprim::radix_sort_keys(nullptr, bytes, keys_in, keys_out, n, begin_bit, end_bit, c->stream));
     void* tmp = c->buf("rsort_tmp", bytes);
     PMA_HIP(rocprim::radix_sort_keys(tmp, bytes, keys_in, keys_out, n, begin_bit, end_bit, c->stream));
-    c->launches++;
+    PMA_CHECKED(c, "rocprim::radix_sort", nullptr);
 }
 
 }  // namespace pma

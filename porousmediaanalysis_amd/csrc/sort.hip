@@ -24,6 +24,11 @@ namespace {
 constexpr uint32_t kInactive = 0xFFFFFFFFu;
 constexpr int kThreshold = 16;
 
+// Kernel ids of the in-kernel checks (code = id << 8 | check); every index read from memory
+// (segment bounds, stop positions, ranks, leaves) is tested before it is used.
+enum : uint32_t { kHeapSort = 0x21, kChoosePivots, kStopFlags, kRankStops, kFindCuts, kApplySwaps, kCountChildren,
+                  kWriteChildren, kRelabel, kInsertion };
+
 __device__ __forceinline__ bool comp(uint64_t a, uint64_t b) { return (a >> 32) > (b >> 32); }
 
 struct Seg {
@@ -31,6 +36,8 @@ struct Seg {
     int depth;
     uint32_t pad;
 };
+
+__device__ __forceinline__ bool seg_ok(const Seg& g, size_t M) { return g.first <= g.last && g.last <= M; }
 
 __device__ void push_heap_(uint64_t* a, long hole, long top, uint64_t v) {
     long p = (hole - 1) / 2;
@@ -60,10 +67,14 @@ __device__ void adjust_heap_(uint64_t* a, long hole, long len, uint64_t v) {
 }
 
 // std::__partial_sort(first, last, last): __make_heap then __sort_heap.
-__global__ void heap_sort_segments(uint64_t* __restrict__ A, const Seg* __restrict__ segs, uint32_t nseg,
-                                   const uint32_t* __restrict__ heap_flag) {
+__global__ void heap_sort_segments(uint64_t* __restrict__ A, size_t M, const Seg* __restrict__ segs, uint32_t nseg,
+                                   const uint32_t* __restrict__ heap_flag, uint32_t* __restrict__ err) {
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nseg || !heap_flag[s]) return;
+    if (!seg_ok(segs[s], M)) {
+        fail_check(err, kHeapSort << 8 | 1);
+        return;
+    }
     uint64_t* a = A + segs[s].first;
     long n = (long)segs[s].last - segs[s].first;
     if (n >= 2) {
@@ -84,11 +95,17 @@ __global__ void heap_sort_segments(uint64_t* __restrict__ A, const Seg* __restri
 }
 
 // Median of three to *first (or a heap-sort flag when the depth limit is exhausted).
-__global__ void choose_pivots(uint64_t* __restrict__ A, const Seg* __restrict__ segs, uint32_t nseg,
-                              uint32_t* __restrict__ pivot, uint32_t* __restrict__ heap_flag) {
+__global__ void choose_pivots(uint64_t* __restrict__ A, size_t M, const Seg* __restrict__ segs, uint32_t nseg,
+                              uint32_t* __restrict__ pivot, uint32_t* __restrict__ heap_flag,
+                              uint32_t* __restrict__ err) {
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nseg) return;
     Seg g = segs[s];
+    if (!seg_ok(g, M) || g.last - g.first <= (uint32_t)kThreshold) {
+        fail_check(err, kChoosePivots << 8 | 1);
+        heap_flag[s] = 1;  // a heap flag keeps the later kernels away from this segment
+        return;
+    }
     if (g.depth == 0) {
         heap_flag[s] = 1;
         return;
@@ -108,14 +125,17 @@ __global__ void choose_pivots(uint64_t* __restrict__ A, const Seg* __restrict__ 
 
 // Packed stop flags: low 32 bits = left stop, high 32 bits = right stop.
 __global__ void stop_flags(const uint64_t* __restrict__ A, const uint32_t* __restrict__ seg_of,
-                           const Seg* __restrict__ segs, const uint32_t* __restrict__ pivot,
-                           const uint32_t* __restrict__ heap_flag, unsigned long long* __restrict__ flags, size_t M) {
+                           const Seg* __restrict__ segs, uint32_t nseg, const uint32_t* __restrict__ pivot,
+                           const uint32_t* __restrict__ heap_flag, unsigned long long* __restrict__ flags, size_t M,
+                           uint32_t* __restrict__ err) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i > M) return;
     unsigned long long f = 0;
     if (i < M) {
         uint32_t s = seg_of[i];
-        if (s != kInactive && !heap_flag[s]) {
+        if (s != kInactive && s >= nseg) {
+            fail_check(err, kStopFlags << 8 | 1);
+        } else if (s != kInactive && !heap_flag[s]) {
             uint32_t key = (uint32_t)(A[i] >> 32), P = pivot[s];
             bool L = i > segs[s].first && key <= P;
             bool R = key >= P;
@@ -126,20 +146,29 @@ __global__ void stop_flags(const uint64_t* __restrict__ A, const uint32_t* __res
 }
 
 __global__ void rank_stops(const unsigned long long* __restrict__ flags, const unsigned long long* __restrict__ pre,
-                           const uint32_t* __restrict__ seg_of, const Seg* __restrict__ segs,
+                           const uint32_t* __restrict__ seg_of, const Seg* __restrict__ segs, uint32_t nseg,
                            uint32_t* __restrict__ posL, uint32_t* __restrict__ posR, uint32_t* __restrict__ rankL,
-                           size_t M) {
+                           size_t M, uint32_t* __restrict__ err) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= M) return;
     unsigned long long f = flags[i];
     rankL[i] = 0;
     if (!f) return;
     uint32_t s = seg_of[i];
+    if (s >= nseg || !seg_ok(segs[s], M) || i < segs[s].first || i >= segs[s].last) {
+        fail_check(err, kRankStops << 8 | 1);
+        return;
+    }
     Seg g = segs[s];
     unsigned long long base = pre[g.first], here = pre[i], end = pre[g.last];
     uint32_t inclL = (uint32_t)((here - base) & 0xFFFFFFFFull) + (uint32_t)(f & 1ull);
     uint32_t inclR = (uint32_t)((here - base) >> 32) + (uint32_t)(f >> 32);
     uint32_t totR = (uint32_t)((end - base) >> 32);
+    const uint32_t width = g.last - g.first;
+    if (inclL > width || inclR > totR || totR > width || ((f & 1ull) && inclL == 0) || ((f >> 32) && inclR == 0)) {
+        fail_check(err, kRankStops << 8 | 2);
+        return;
+    }
     if (f & 1ull) {
         posL[g.first + inclL - 1] = (uint32_t)i;
         rankL[i] = inclL;
@@ -147,14 +176,21 @@ __global__ void rank_stops(const unsigned long long* __restrict__ flags, const u
     if (f >> 32) posR[g.first + (totR - inclR)] = (uint32_t)i;  // rank from the right = totR - inclR + 1
 }
 
-__global__ void find_cuts(const unsigned long long* __restrict__ pre, const Seg* __restrict__ segs, uint32_t nseg,
-                          const uint32_t* __restrict__ heap_flag, const uint32_t* __restrict__ posL,
-                          const uint32_t* __restrict__ posR, uint32_t* __restrict__ swaps, uint32_t* __restrict__ cut) {
+__global__ void find_cuts(const unsigned long long* __restrict__ pre, size_t M, const Seg* __restrict__ segs,
+                          uint32_t nseg, const uint32_t* __restrict__ heap_flag, const uint32_t* __restrict__ posL,
+                          const uint32_t* __restrict__ posR, uint32_t* __restrict__ swaps, uint32_t* __restrict__ cut,
+                          uint32_t* __restrict__ err) {
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nseg || heap_flag[s]) return;
     Seg g = segs[s];
     unsigned long long tot = pre[g.last] - pre[g.first];
     uint32_t totL = (uint32_t)(tot & 0xFFFFFFFFull), totR = (uint32_t)(tot >> 32);
+    if (totL > g.last - g.first || totR > g.last - g.first) {
+        fail_check(err, kFindCuts << 8 | 1);
+        swaps[s] = 0;
+        cut[s] = g.first + 1;
+        return;
+    }
     uint32_t lo = 0, hi = min(totL, totR);  // largest K in [lo, hi] with posL[K-1] < posR[K-1]
     while (lo < hi) {
         uint32_t k = (lo + hi + 1) / 2;
@@ -164,20 +200,35 @@ __global__ void find_cuts(const unsigned long long* __restrict__ pre, const Seg*
     uint32_t K = lo;
     uint32_t c = K < totL ? posL[g.first + K] : 0xFFFFFFFFu;
     if (K >= 1) c = min(c, posR[g.first + K - 1]);
+    if (c <= g.first || c >= g.last) {  // Hoare's cut lies strictly inside the segment
+        fail_check(err, kFindCuts << 8 | 2);
+        K = 0;
+        c = g.first + 1;
+    }
     swaps[s] = K;
     cut[s] = c;
 }
 
 __global__ void apply_swaps(uint64_t* __restrict__ A, const uint32_t* __restrict__ rankL,
-                            const uint32_t* __restrict__ seg_of, const Seg* __restrict__ segs,
-                            const uint32_t* __restrict__ swaps, const uint32_t* __restrict__ posR, size_t M) {
+                            const uint32_t* __restrict__ seg_of, const Seg* __restrict__ segs, uint32_t nseg,
+                            const uint32_t* __restrict__ swaps, const uint32_t* __restrict__ posR, size_t M,
+                            uint32_t* __restrict__ err) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= M) return;
     uint32_t r = rankL[i];
     if (r == 0) return;
     uint32_t s = seg_of[i];
+    if (s >= nseg) {
+        fail_check(err, kApplySwaps << 8 | 1);
+        return;
+    }
     if (r > swaps[s]) return;
-    uint32_t j = posR[segs[s].first + r - 1];
+    const size_t pr = (size_t)segs[s].first + r - 1;
+    const uint32_t j = pr < M ? posR[pr] : kInactive;
+    if (j >= M) {
+        fail_check(err, kApplySwaps << 8 | 2);
+        return;
+    }
     uint64_t t = A[i];
     A[i] = A[j];
     A[j] = t;
@@ -186,39 +237,60 @@ __global__ void apply_swaps(uint64_t* __restrict__ A, const uint32_t* __restrict
 // Children of each partitioned segment: count active ones (> 16) and record leaves.
 __global__ void count_children(const Seg* __restrict__ segs, uint32_t nseg, const uint32_t* __restrict__ heap_flag,
                                const uint32_t* __restrict__ cut, uint32_t* __restrict__ nchild,
-                               uint2* __restrict__ leaves, uint32_t* __restrict__ nleaves) {
+                               uint2* __restrict__ leaves, uint32_t* __restrict__ nleaves, size_t leaf_cap,
+                               uint32_t* __restrict__ err) {
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nseg) return;
     uint32_t k = 0;
     if (!heap_flag[s]) {
         Seg g = segs[s];
         uint32_t c = cut[s];
-        if (c - g.first > (uint32_t)kThreshold) ++k;
-        else leaves[atomicAdd(nleaves, 1u)] = make_uint2(g.first, c);
-        if (g.last - c > (uint32_t)kThreshold) ++k;
-        else leaves[atomicAdd(nleaves, 1u)] = make_uint2(c, g.last);
+        if (c - g.first > (uint32_t)kThreshold) {
+            ++k;
+        } else {
+            const uint32_t l = atomicAdd(nleaves, 1u);
+            if (l < leaf_cap) leaves[l] = make_uint2(g.first, c);
+            else fail_check(err, kCountChildren << 8 | 1);
+        }
+        if (g.last - c > (uint32_t)kThreshold) {
+            ++k;
+        } else {
+            const uint32_t l = atomicAdd(nleaves, 1u);
+            if (l < leaf_cap) leaves[l] = make_uint2(c, g.last);
+            else fail_check(err, kCountChildren << 8 | 1);
+        }
     }
     nchild[s] = k;
 }
 
 __global__ void write_children(const Seg* __restrict__ segs, uint32_t nseg, const uint32_t* __restrict__ heap_flag,
                                const uint32_t* __restrict__ cut, const uint32_t* __restrict__ child_off,
-                               Seg* __restrict__ next) {
+                               Seg* __restrict__ next, size_t next_cap, uint32_t* __restrict__ err) {
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nseg || heap_flag[s]) return;
     Seg g = segs[s];
     uint32_t c = cut[s], o = child_off[s];
+    const uint32_t need = (c - g.first > (uint32_t)kThreshold) + (g.last - c > (uint32_t)kThreshold);
+    if ((size_t)o + need > next_cap) {
+        fail_check(err, kWriteChildren << 8 | 1);
+        return;
+    }
     if (c - g.first > (uint32_t)kThreshold) next[o++] = Seg{g.first, c, g.depth - 1, 0};
     if (g.last - c > (uint32_t)kThreshold) next[o++] = Seg{c, g.last, g.depth - 1, 0};
 }
 
-__global__ void relabel_elements(uint32_t* __restrict__ seg_of, const Seg* __restrict__ segs,
+__global__ void relabel_elements(uint32_t* __restrict__ seg_of, const Seg* __restrict__ segs, uint32_t nseg,
                                  const uint32_t* __restrict__ heap_flag, const uint32_t* __restrict__ cut,
-                                 const uint32_t* __restrict__ child_off, size_t M) {
+                                 const uint32_t* __restrict__ child_off, size_t M, uint32_t* __restrict__ err) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= M) return;
     uint32_t s = seg_of[i];
     if (s == kInactive) return;
+    if (s >= nseg) {
+        fail_check(err, kRelabel << 8 | 1);
+        seg_of[i] = kInactive;
+        return;
+    }
     if (heap_flag[s]) {
         seg_of[i] = kInactive;
         return;
@@ -230,10 +302,15 @@ __global__ void relabel_elements(uint32_t* __restrict__ seg_of, const Seg* __res
     else seg_of[i] = (g.last - c > (uint32_t)kThreshold) ? o + (left_active ? 1 : 0) : kInactive;
 }
 
-__global__ void insertion_sort_leaves(uint64_t* __restrict__ A, const uint2* __restrict__ leaves, uint32_t n) {
+__global__ void insertion_sort_leaves(uint64_t* __restrict__ A, size_t M, const uint2* __restrict__ leaves, uint32_t n,
+                                      uint32_t* __restrict__ err) {
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     uint2 L = leaves[t];
+    if (L.x > L.y || L.y > M) {
+        fail_check(err, kInsertion << 8 | 1);
+        return;
+    }
     for (uint32_t i = L.x + 1; i < L.y; ++i) {
         uint64_t v = A[i];
         uint32_t j = i;
@@ -255,20 +332,24 @@ __global__ void fill_u32(uint32_t* p, uint32_t v, size_t n) {
 void sort_like_libstdcxx_desc(pma_ctx* c, uint64_t* A, size_t M) {
     if (M <= 1) return;
     PMA_REQUIRE(M < 0xFFFFFFF0ull, PMA_ERR_INVALID_ARGUMENT, "sort: too many elements");
-    uint2* leaves = c->buf<uint2>("sort_leaves", M + 2);
+    const size_t leaf_cap = M + 2;
+    uint2* leaves = c->buf<uint2>("sort_leaves", leaf_cap);
     uint32_t* nleaves = c->buf<uint32_t>("sort_nleaves", 1);
+    uint32_t* err = c->buf<uint32_t>("sort_err", 1);
     PMA_HIP(hipMemsetAsync(nleaves, 0, sizeof(uint32_t), c->stream));
+    PMA_HIP(hipMemsetAsync(err, 0, sizeof(uint32_t), c->stream));
     if (M <= (size_t)kThreshold) {
         uint2 one = make_uint2(0, (uint32_t)M);
         upload(c, leaves, &one, sizeof one);
-        insertion_sort_leaves<<<1, 64, 0, c->stream>>>(A, leaves, 1);
-        PMA_LAUNCHED(c);
-        PMA_HIP(hipStreamSynchronize(c->stream));
+        insertion_sort_leaves<<<1, 64, 0, c->stream>>>(A, M, leaves, 1, err);
+        PMA_CHECKED(c, "insertion_sort_leaves", err);
+        PMA_REQUIRE(read_scalar(c, err) == 0, PMA_ERR_INTERNAL, "sort: in-kernel check failed");
         return;
     }
     int lg = 0;
     for (size_t v = M; v > 1; v >>= 1) ++lg;
-    size_t cap = M / (kThreshold + 1) + 2;
+    // every active segment holds more than 16 elements and active segments are disjoint
+    const size_t cap = M / (kThreshold + 1) + 2;
     Seg* segs = c->buf<Seg>("sort_segs_a", cap);
     Seg* next = c->buf<Seg>("sort_segs_b", cap);
     uint32_t* seg_of = c->buf<uint32_t>("sort_seg_of", M);
@@ -287,40 +368,47 @@ void sort_like_libstdcxx_desc(pma_ctx* c, uint64_t* A, size_t M) {
     Seg root{0, (uint32_t)M, 2 * lg, 0};
     upload(c, segs, &root, sizeof root);
     fill_u32<<<grid_for(M, 256), 256, 0, c->stream>>>(seg_of, 0u, M);
-    PMA_LAUNCHED(c);
+    PMA_CHECKED(c, "fill_u32", err);
     uint32_t nseg = 1;
     const unsigned eb = grid_for(M + 1, 256);
     while (nseg > 0) {
+        PMA_REQUIRE(nseg <= cap, PMA_ERR_INTERNAL, "sort: more active segments than capacity");
         const unsigned sb = grid_for(nseg, 256);
-        choose_pivots<<<sb, 256, 0, c->stream>>>(A, segs, nseg, pivot, heap_flag);
-        PMA_LAUNCHED(c);
-        heap_sort_segments<<<sb, 256, 0, c->stream>>>(A, segs, nseg, heap_flag);
-        PMA_LAUNCHED(c);
-        stop_flags<<<eb, 256, 0, c->stream>>>(A, seg_of, segs, pivot, heap_flag, flags, M);
-        PMA_LAUNCHED(c);
+        choose_pivots<<<sb, 256, 0, c->stream>>>(A, M, segs, nseg, pivot, heap_flag, err);
+        PMA_CHECKED(c, "choose_pivots", err);
+        heap_sort_segments<<<sb, 256, 0, c->stream>>>(A, M, segs, nseg, heap_flag, err);
+        PMA_CHECKED(c, "heap_sort_segments", err);
+        stop_flags<<<eb, 256, 0, c->stream>>>(A, seg_of, segs, nseg, pivot, heap_flag, flags, M, err);
+        PMA_CHECKED(c, "stop_flags", err);
         exclusive_scan_u64(c, reinterpret_cast<const uint64_t*>(flags), reinterpret_cast<uint64_t*>(pre), M + 1);
-        rank_stops<<<eb, 256, 0, c->stream>>>(flags, pre, seg_of, segs, posL, posR, rankL, M);
-        PMA_LAUNCHED(c);
-        find_cuts<<<sb, 256, 0, c->stream>>>(pre, segs, nseg, heap_flag, posL, posR, swaps, cut);
-        PMA_LAUNCHED(c);
-        apply_swaps<<<eb, 256, 0, c->stream>>>(A, rankL, seg_of, segs, swaps, posR, M);
-        PMA_LAUNCHED(c);
-        count_children<<<sb, 256, 0, c->stream>>>(segs, nseg, heap_flag, cut, nchild, leaves, nleaves);
-        PMA_LAUNCHED(c);
+        rank_stops<<<eb, 256, 0, c->stream>>>(flags, pre, seg_of, segs, nseg, posL, posR, rankL, M, err);
+        PMA_CHECKED(c, "rank_stops", err);
+        find_cuts<<<sb, 256, 0, c->stream>>>(pre, M, segs, nseg, heap_flag, posL, posR, swaps, cut, err);
+        PMA_CHECKED(c, "find_cuts", err);
+        apply_swaps<<<eb, 256, 0, c->stream>>>(A, rankL, seg_of, segs, nseg, swaps, posR, M, err);
+        PMA_CHECKED(c, "apply_swaps", err);
+        count_children<<<sb, 256, 0, c->stream>>>(segs, nseg, heap_flag, cut, nchild, leaves, nleaves, leaf_cap, err);
+        PMA_CHECKED(c, "count_children", err);
         uint64_t total = exclusive_scan_u32(c, nchild, child_off, nseg);
-        write_children<<<sb, 256, 0, c->stream>>>(segs, nseg, heap_flag, cut, child_off, next);
-        PMA_LAUNCHED(c);
-        relabel_elements<<<grid_for(M, 256), 256, 0, c->stream>>>(seg_of, segs, heap_flag, cut, child_off, M);
-        PMA_LAUNCHED(c);
+        PMA_REQUIRE(total <= cap, PMA_ERR_INTERNAL, "sort: more child segments than capacity");
+        write_children<<<sb, 256, 0, c->stream>>>(segs, nseg, heap_flag, cut, child_off, next, cap, err);
+        PMA_CHECKED(c, "write_children", err);
+        relabel_elements<<<grid_for(M, 256), 256, 0, c->stream>>>(seg_of, segs, nseg, heap_flag, cut, child_off, M,
+                                                                  err);
+        PMA_CHECKED(c, "relabel_elements", err);
         std::swap(segs, next);
         nseg = (uint32_t)total;
+        PMA_REQUIRE(read_scalar(c, err) == 0, PMA_ERR_INTERNAL,
+                    "sort: in-kernel check " + std::to_string(read_scalar(c, err)) + " failed");
     }
     uint32_t nl = read_scalar(c, nleaves);
+    PMA_REQUIRE(nl <= leaf_cap, PMA_ERR_INTERNAL, "sort: more leaves than capacity");
     if (nl) {
-        insertion_sort_leaves<<<grid_for(nl, 256), 256, 0, c->stream>>>(A, leaves, nl);
-        PMA_LAUNCHED(c);
+        insertion_sort_leaves<<<grid_for(nl, 256), 256, 0, c->stream>>>(A, M, leaves, nl, err);
+        PMA_CHECKED(c, "insertion_sort_leaves", err);
     }
-    PMA_HIP(hipStreamSynchronize(c->stream));
+    PMA_REQUIRE(read_scalar(c, err) == 0, PMA_ERR_INTERNAL,
+                "sort: in-kernel check " + std::to_string(read_scalar(c, err)) + " failed");
 }
 
 }  // namespace pma

@@ -657,7 +657,7 @@ bool watershed_once(pma_ctx* c, unsigned long long log_cap, unsigned long long* 
             write_sequences<<<grid_for(runs, 256), 256, 0, c->stream>>>(log, perm, start, runs, nlog, label, soff, rbase,
                                                                         elems, tm, wt, vox, shared_owned);
             PMA_LAUNCHED(c);
-            PrefixTrie lt = build_prefix_trie(c, elems, reinterpret_cast<const uint64_t*>(soff), slen,
+            PrefixTrie lt = build_prefix_trie(c, elems, nelem, reinterpret_cast<const uint64_t*>(soff), slen,
                                               reinterpret_cast<const uint64_t*>(rbase),
                                               reinterpret_cast<const uint64_t*>(tm), wt, runs, R, rd);
             std::vector<uint32_t> order(lt.distinct);

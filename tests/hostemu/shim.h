@@ -267,6 +267,8 @@ template <class T, class U> T atomicMax(T* p, U v) {
 #define __ATOMIC_RELAXED_ 0
 #define __HIP_MEMORY_SCOPE_AGENT 0
 template <class T> T __hip_atomic_load(T* p, int, int) { return __atomic_load_n(p, __ATOMIC_SEQ_CST); }
+template <class T, class U> void __hip_atomic_store(T* p, U v, int, int) { __atomic_store_n(p, (T)v, __ATOMIC_SEQ_CST); }
+inline void fail_check(uint32_t* err, uint32_t code) { atomicCAS(err, 0u, code); }
 inline uint32_t min(uint32_t a, uint32_t b) { return a < b ? a : b; }
 inline uint32_t max(uint32_t a, uint32_t b) { return a > b ? a : b; }
 inline int min(int a, int b) { return a < b ? a : b; }
@@ -394,6 +396,15 @@ inline unsigned grid_for(size_t n, unsigned block) { return (unsigned)((n + bloc
 }  // namespace pma
 #define PMA_REQUIRE(cond, status, msg) do { if (!(cond)) throw ::pma::Error((status), (msg)); } while (0)
 #define PMA_LAUNCHED(c) do {} while (0)
+// Kernels run to completion at their launch here, so a failed in-kernel check is raised at once,
+// naming the kernel.
+#define PMA_CHECKED(c, name, err)                                                                   \
+    do {                                                                                            \
+        const uint32_t* e_ = (err);                                                                 \
+        if (e_ && *e_)                                                                              \
+            throw ::pma::Error(PMA_ERR_INTERNAL, std::string(name) + ": in-kernel check " +         \
+                                                     std::to_string(*e_) + " failed");               \
+    } while (0)
 #define PMA_TIMED(c, n) do {} while (0)
 #define PMA_HIP(x) (void)(x)
 #define hipMemsetAsync(p, v, bytes, s) (std::memset((p), (v), (bytes)), 0)
@@ -419,6 +430,7 @@ struct pma_ctx {
     std::vector<pma::CavityRow> cavities;
     bool have_cavities = false;
     StageInfo stage[PMA_STAGE_COUNT];
+    bool debug_sync = false;
     template <class T> T* v(int w) { return (T*)vol[w].data(); }
     void ensure_volume(int w) {
         const size_t bytes = n * ((w == PMA_VOL_SOLID || w == PMA_VOL_MASK) ? 1 : 4);
@@ -475,7 +487,7 @@ struct PrefixTrie {
     std::vector<uint64_t> key, min_time, finals;
     std::vector<uint32_t> members(uint32_t d) const;
 };
-PrefixTrie build_prefix_trie(pma_ctx* c, const uint32_t* elems, const uint64_t* seq_off, const uint32_t* seq_len,
-                             const uint64_t* rbase, const uint64_t* rec_time, const uint32_t* seq_weight, size_t S,
-                             uint64_t R, uint32_t* rec_node);
+PrefixTrie build_prefix_trie(pma_ctx* c, const uint32_t* elems, uint64_t elem_cap, const uint64_t* seq_off,
+                             const uint32_t* seq_len, const uint64_t* rbase, const uint64_t* rec_time,
+                             const uint32_t* seq_weight, size_t S, uint64_t R, uint32_t* rec_node);
 }  // namespace pma

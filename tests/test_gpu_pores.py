@@ -14,12 +14,12 @@ from tests import volumes
 from tests.gpu_common import log
 from tests.volume_utils import canonical_relabel, sha256_array
 
-pytestmark = [
-    pytest.mark.gpu,
-    pytest.mark.skip(reason="pore stage switched off on the GPU: test_pores_match_oracle[False-gauss_s3_low] "
-                            "made an illegal memory access on MI355X (cause not found on the CPU; the same kernels "
-                            "are parity-checked in tests/test_hostemu.py)"),
-]
+pytestmark = pytest.mark.gpu
+
+# The one case of this module that made an illegal memory access on MI355X with the earlier,
+# unchecked pore kernels (cause not found).  The kernels now check every index they read
+# from memory; this exact case stays off the GPU all the same.
+FAULTED = {("gauss_s3_low", False)}
 
 
 @pytest.fixture(scope="module")
@@ -68,6 +68,9 @@ CASES["noise_many_pores"] = volumes.random_noise((40, 48, 56), 0.45, 6)  # > 129
 @pytest.mark.parametrize("name", sorted(CASES))
 @pytest.mark.parametrize("d3q7", [False, True])
 def test_pores_match_oracle(ctx, name, d3q7):
+    if (name, d3q7) in FAULTED:
+        pytest.skip("made an illegal memory access with the earlier unchecked pore kernels; cause not found, "
+                    "kept off the GPU")
     log(f"pores case {name} d3q7={d3q7} shape {CASES[name].shape}")
     e, cr, mask = oracle_front(CASES[name])
     first, removed = gpu_pores(ctx, e, cr, mask, d3q7)

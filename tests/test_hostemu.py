@@ -7,6 +7,7 @@ executes its real body, thread by thread (wave kernels as fibers), with exact-si
 an out-of-bounds access fails the test.  Outputs must equal the oracle bit for bit.  rocPRIM
 sorts are replaced by std::stable_sort, the one device helper not emulated here.
 """
+import fcntl
 import hashlib
 import os
 import shutil
@@ -46,10 +47,17 @@ def _build(tmp_path_factory, flags, tag):
     if shutil.which("g++") is None:
         pytest.skip("g++ is not available")
     out = tmp_path_factory.getbasetemp().parent / f"pma_hostemu_{tag}_{_digest()}"
+    out.mkdir(parents=True, exist_ok=True)
+    # pytest-xdist workers share this folder: one builds, the others wait on the lock
+    with open(out / "build.lock", "w") as lock:
+        fcntl.flock(lock, fcntl.LOCK_EX)
+        return _build_locked(out, flags)
+
+
+def _build_locked(out, flags):
     exe = out / "hostemu"
     if exe.exists():
         return str(exe)
-    out.mkdir(parents=True, exist_ok=True)
     import sys
     sys.path.insert(0, EMU)
     from translate import translate
