@@ -1,27 +1,29 @@
 """Benchmark of the pore-throat + geodesic pipeline on one MI355X.
 
-One step = the stages of the reference's -all mode and -spd that run on the GPU, on one
-synthetic volume resident in HBM: EDT -> containing balls -> geodesic distances with
-cavities.  The pore-propagation stage (and with it empty-pore removal and the watershed, which
-consume its pores) is switched off on the GPU (see the pores.hip gate), so it is not in the
-step.  Default workload: 1024^3 voxels, i.i.d. N(0,1) noise blurred with sigma = 8 (truncated
-at 4 sigma), void = lowest 65 % of the field (porosity 0.65), generated on the GPU (not
-timed).
+One step = the reference's -all mode followed by -spd, all on the GPU, on one synthetic
+volume resident in HBM: EDT -> containing balls -> pore seeds and pore propagation ->
+empty-pore removal -> watershed -> geodesic distances with cavities.  Default workload:
+1024^3 voxels, i.i.d. N(0,1) noise blurred with sigma = 8 (truncated at 4 sigma), void = lowest
+65 % of the field (porosity 0.65), generated on the GPU (not timed).
 
 Prints one JSON line:
   value      voxels per second of whole steps (input resident in HBM before timing),
+  stages_ms  device time of each stage (HIP events around the stage on the context stream),
+  stage_roofline  per stage: compulsory bytes (each input read once, each output written
+             once, per voxel) / stage time, against the MI355X HBM peak of 8 TB/s,
   roofline   the dominant instrumented kernel: algorithmic bytes per launch / its average
              launch time from HIP events recorded on the context's stream in the timed
-             region, against the MI355X HBM peak of 8 TB/s,
-  cpu_baseline  the CPU oracle (a restatement of the reference with its OpenMP threading in
-             the EDT and containing-ball loops, serial Dijkstra for the distances) timed on the
-             same stages over a bounded sample of the same generator (--cpu-size^3).
+             region,
+  cpu_baseline  the CPU oracle (a restatement of the reference: OpenMP in the EDT and
+             containing-ball loops as in the reference, serial elsewhere) running all six
+             stages on bounded samples of the same generators.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 1024] [--no-cpu]
 """
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 
@@ -31,26 +33,30 @@ sys.path.insert(0, ROOT)
 METRIC = "voxels/s, 1024³ EDT→watershed→geodesic; batched & surrogate vs plain at MAE"
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic (compulsory) bytes per voxel of each instrumented kernel that sweeps the whole
-# volume once per step (in one launch, or in z-slabs over several): each input read once,
-# each output written once.
+STAGES = ("edt", "containing_balls", "pores", "remove_empty_pores", "watershed", "geodesic")
+# Compulsory bytes per voxel of each stage (inputs read once, outputs written once):
+#   EDT: u8 solid in, u32 EDT square out; containing balls: EDT in, CR, CX, CY, CZ (u32) and
+#   the u8 mask out; pores: mask, EDT and CR in for the seeds (9), CR in and PT out for the
+#   propagation (8); removal: PT in and out; watershed: CR in, PTW out; geodesic: solid in,
+#   f32 distance and u32 cavity id out.
+STAGE_BYTES_PER_VOXEL = {"edt": 5, "containing_balls": 21, "pores": 17, "remove_empty_pores": 8,
+                         "watershed": 8, "geodesic": 9}
+# Instrumented kernels that sweep the whole volume once per launch.
 KERNEL_BYTES_PER_VOXEL = {
     "edt_pass_x": 1 + 4,        # u8 solid -> u32 1-D distance square
     "edt_pass_y": 4 + 4,        # u32 -> u32
     "edt_pass_z": 4 + 4,        # u32 -> u32 EDT square (+ histogram)
-    "prune_centres": 4 + 1,     # u32 EDT -> u8 mask / candidate flag
-    "cover_tiles": 4 + 16,      # u32 EDT -> u32 CR, CX, CY, CZ (over all the launches of a step)
+    "prune_centres": 4 + 1,     # u32 EDT -> u8 candidate flag
+    "cover_tiles": 4 + 16 + 1,  # u32 EDT -> u32 CR, CX, CY, CZ and the u8 mask
     "count_equal_u8": 1,
     "cc_union": 4 + 4,          # u32 CR -> u32 plateau parent
     "cc_not_max": 4 + 4,
     "plateau_edges": 4 + 4,
     "write_labels": 4 + 4,      # u32 plateau index -> u32 PoreThroatIds
-    "geo_init": 1 + 4 + 4,       # u8 solid -> f32 distance, u32 cavity id
 }
 # Per block visit of one relax_blocks launch: the 8^3 distances read and written once (the
 # one-voxel halo is shared with the neighbouring blocks and not counted).
 RELAX_BYTES_PER_BLOCK = 512 * (4 + 4)
-STAGES = ("edt", "containing_balls", "geodesic")
 
 
 def parse():
@@ -73,134 +79,70 @@ def progress(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def oracle_stages(oracle, solid, d3q7, threads):
+    """All six oracle stages on one volume; returns per-stage wall seconds."""
+    t = {}
+    t0 = time.perf_counter()
+    e, max_e, _ = oracle.edt(solid, threads)
+    t["edt"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    cr, _, _, _, mask, _ = oracle.containing_balls(e, threads)
+    t["containing_balls"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    pt, ptxt, ktxt = oracle.pores(mask, e, cr, d3q7)
+    t["pores"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    pt, ptxt, ktxt = oracle.remove_empty_pores(pt, ptxt, ktxt)
+    t["remove_empty_pores"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    oracle.watershed(cr, max_e, ptxt, d3q7)
+    t["watershed"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    oracle.geodesic(solid, d3q7)
+    t["geodesic"] = time.perf_counter() - t0
+    return t
+
+
 def cpu_baseline(args):
-    """Oracle EDT, containing balls and geodesic distances on a bounded sample, timed here."""
+    """The oracle's six stages on a bounded sample of the benchmark's generator (the value) and
+    on the 128^3 sphere-union configuration, timed on this host."""
     import oracle
     import porousmediaanalysis_amd as pma
 
     n = args.cpu_size
+    d3q7 = args.stencil == "d3q7"
     with pma.Context(0) as ctx:
         ctx.generate_gaussian((n, n, n), args.sigma, args.porosity, args.seed + 1)
-        solid = ctx.read(pma.VOL_SOLID)
-    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-    progress(f"cpu baseline: oracle EDT, containing balls, geodesic on {n}^3 with {threads} threads")
-    t0 = time.perf_counter()
-    e, _, _ = oracle.edt(solid, threads)
-    oracle.containing_balls(e, threads)
-    oracle.geodesic(solid, args.stencil == "d3q7")
-    dt = time.perf_counter() - t0
-    return {"value": n ** 3 / dt, "unit": "voxels/s", "cores": threads, "kind": "port",
-            "sample": f"{n}^3 voxels of the same Gaussian-field generator (sigma={args.sigma:g}, "
-                      f"porosity={args.porosity:g}): EDT + containing balls (OpenMP) + geodesic (serial "
-                      f"Dijkstra), {dt:.1f} s"}
-
-
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path.",
-              file=sys.stderr)
-        sys.exit(2)
-    import porousmediaanalysis_amd as pma
-
-    S = args.size
-    shape = (S, S, S)
-    nvox = S ** 3
-    with pma.Context(0) as ctx:
-        progress(f"generating {S}^3 Gaussian field sigma={args.sigma} porosity={args.porosity}")
-        ctx.generate_gaussian(shape, args.sigma, args.porosity, args.seed)
-        ctx.set_stencil(args.stencil == "d3q7")
-        def step():
-            ctx.edt()
-            ctx.containing_balls()
-            ctx.geodesic()
-
-        for w in range(args.warmup):
-            t0 = time.perf_counter()
-            step()
-            progress(f"warmup step {w + 1}/{args.warmup}: {time.perf_counter() - t0:.3f} s")
-        pma.lib().pma_set_kernel_timing(ctx._h, 1)
-        stage_ms = {name: 0.0 for name in STAGES}
-        stage_items = {name: 0 for name in STAGES}
-        t0 = time.perf_counter()
-        for s in range(args.steps):
-            ts = time.perf_counter()
-            step()
-            for k, name in enumerate(pma.STAGE_NAMES):
-                if name in stage_ms:
-                    info = ctx.stage_info(k)
-                    stage_ms[name] += info["device_ms"]
-                    stage_items[name] += info["work_items"]
-            progress(f"step {s + 1}/{args.steps}: {time.perf_counter() - ts:.3f} s")
-        elapsed = time.perf_counter() - t0
-        kernels = kernel_times(pma, ctx)
-        pma.lib().pma_set_kernel_timing(ctx._h, 0)
-        max_e, n_max = ctx.statistics()
-
-    ms_per_step = 1000.0 * elapsed / args.steps
-    dom = max(kernels.items(), key=lambda kv: kv[1][0]) if kernels else None
-    roofline = None
-    if dom:
-        name, (tot_ms, launches) = dom
-        avg_ms = tot_ms / max(launches, 1)
-        bpv = KERNEL_BYTES_PER_VOXEL.get(name)
-        per_launch = bpv * nvox * args.steps / launches if bpv is not None and launches else None
-        if name == "relax_blocks" and launches:
-            per_launch = RELAX_BYTES_PER_BLOCK * stage_items["geodesic"] / launches
-        if per_launch is not None:
-            achieved = per_launch / (avg_ms * 1e-3) / 1e9
-            roofline = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                        "traffic": measured_traffic(name, S),
-                        "bytes_per_launch": round(per_launch), "avg_launch_ms": round(avg_ms, 4)}
-        else:
-            roofline = {"bound": "hbm", "kernel": name, "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": None, "traffic": None, "avg_launch_ms": round(avg_ms, 4), "launches": launches}
-    # algorithmic bytes of the step: solid in (1 B), EDT2, CR2, CX, CY, CZ, mask, distance and
-    # cavity id out (4 + 16 + 1 + 8 B) per voxel
-    pipeline_gbs = 30.0 * nvox / (ms_per_step * 1e-3) / 1e9
-    result = {
-        "metric": METRIC,
-        "value": nvox * args.steps / elapsed,
-        "unit": "voxels/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u32+f32",
-        "data": "synthetic",
-        "config": {"workload": f"{S}^3 Gaussian field sigma={args.sigma:g} porosity={args.porosity:g} "
-                               f"({args.stencil}): EDT, containing balls, geodesic distances with cavities",
-                   "stages": list(STAGES),
-                   "not_in_step": "pore propagation, empty-pore removal, watershed (pore stage switched off on "
-                                  "the GPU after an unexplained illegal memory access)",
-                   "size": S, "sigma": args.sigma, "porosity": args.porosity,
-                   "seed": args.seed, "max_edt2": max_e, "maximum_balls": n_max},
-        "stages_ms": {k: round(v / args.steps, 3) for k, v in stage_ms.items()},
-        # EDT: voxels; containing balls: (ball, tile) pairs; geodesic: block relaxations
-        "stage_work_items": {k: v // args.steps for k, v in stage_items.items()},
-        "kernels_ms": {k: {"total_ms_per_step": round(v[0] / args.steps, 3), "launches_per_step": v[1] / args.steps}
-                       for k, v in sorted(kernels.items(), key=lambda kv: -kv[1][0])},
-        "pipeline_algorithmic_GBs": round(pipeline_gbs, 1),
-        "roofline": roofline,
-        "cpu_baseline": None,
-    }
-    if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(args)
-    print(json.dumps(result))
-
-
-def measured_traffic(kernel, size):
-    """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc (profiles/traffic.json)."""
-    path = os.path.join(ROOT, "profiles", "traffic.json")
-    if size != 1024 or not os.path.exists(path):
-        return None
-    with open(path) as f:
-        entry = json.load(f).get(kernel)
-    return round(entry["traffic_bytes"]) if entry else None
+        gauss = ctx.read(pma.VOL_SOLID)
+        ctx.generate_spheres((128, 128, 128), 300, 6.0, 12.0, 2024)
+        spheres = ctx.read(pma.VOL_SOLID)
+    # OpenMP threads as the host allows this process (OMP_NUM_THREADS, else every core)
+    threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", 0) or os.cpu_count() or 1)
+    progress(f"cpu baseline: oracle, all six stages, {n}^3 Gaussian sample, {threads} threads")
+    tg = oracle_stages(oracle, gauss, d3q7, threads)
+    progress("cpu baseline: oracle, all six stages, 128^3 sphere union")
+    ts = oracle_stages(oracle, spheres, d3q7, threads)
+    total = sum(tg.values())
+    return {"value": n ** 3 / total, "unit": "voxels/s", "cores": threads, "kind": "port",
+            "sample": f"{n}^3 voxels of the benchmark's Gaussian-field generator (sigma={args.sigma:g}, "
+                      f"porosity={args.porosity:g}), all six stages: EDT and containing balls with OpenMP, "
+                      f"pores, removal, watershed and geodesic serial (as in the reference), {total:.1f} s",
+            "stages_s": {k: round(v, 3) for k, v in tg.items()},
+            "sphere_union_128": {"value": 128 ** 3 / sum(ts.values()), "unit": "voxels/s",
+                                 "stages_s": {k: round(v, 3) for k, v in ts.items()}},
+            "host": {"cpu_model": cpu_model(), "os_cpu_count": os.cpu_count(),
+                     "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS"), "threads_used": threads}}
 
 
 def kernel_times(pma, ctx):
@@ -223,6 +165,128 @@ def kernel_times(pma, ctx):
         L.pma_get_kernel_time(ctx._h, name.encode(), ctypes.byref(ms), ctypes.byref(cnt))
         out[name] = (ms.value, cnt.value)
     return out
+
+
+def measured_traffic(kernel, size):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc (profiles/traffic.json)."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if size != 1024 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        entry = json.load(f).get(kernel)
+    return round(entry["traffic_bytes"]) if entry else None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path.",
+              file=sys.stderr)
+        sys.exit(2)
+    import porousmediaanalysis_amd as pma
+
+    S = args.size
+    shape = (S, S, S)
+    nvox = S ** 3
+    calls = [("edt", "edt"), ("containing_balls", "containing_balls"), ("pores", "pores"),
+             ("remove_empty_pores", "remove_empty_pores"), ("watershed", "watershed"), ("geodesic", "geodesic")]
+    with pma.Context(0) as ctx:
+        progress(f"generating {S}^3 Gaussian field sigma={args.sigma} porosity={args.porosity}")
+        ctx.generate_gaussian(shape, args.sigma, args.porosity, args.seed)
+        ctx.set_stencil(args.stencil == "d3q7")
+
+        def step(tag):
+            for stage, fn in calls:
+                t0 = time.perf_counter()
+                getattr(ctx, fn)()
+                progress(f"{tag} {stage}: {time.perf_counter() - t0:.3f} s")
+
+        for w in range(args.warmup):
+            t0 = time.perf_counter()
+            step(f"warmup {w + 1}/{args.warmup}")
+            progress(f"warmup step {w + 1}/{args.warmup}: {time.perf_counter() - t0:.3f} s")
+        pma.lib().pma_set_kernel_timing(ctx._h, 1)
+        stage_ms = {name: 0.0 for name in STAGES}
+        stage_items = {name: 0 for name in STAGES}
+        t0 = time.perf_counter()
+        for s in range(args.steps):
+            ts = time.perf_counter()
+            step(f"step {s + 1}/{args.steps}")
+            for k, name in enumerate(pma.STAGE_NAMES):
+                info = ctx.stage_info(k)
+                stage_ms[name] += info["device_ms"]
+                stage_items[name] += info["work_items"]
+            progress(f"step {s + 1}/{args.steps}: {time.perf_counter() - ts:.3f} s")
+        elapsed = time.perf_counter() - t0
+        kernels = kernel_times(pma, ctx)
+        pma.lib().pma_set_kernel_timing(ctx._h, 0)
+        max_e, n_max = ctx.statistics()
+        n_pores = ctx.text(pma.TEXT_PORES).count("\n") - 1
+        n_throats = ctx.text(pma.TEXT_THROATS).count("\n") - 1
+        n_throats_ws = ctx.text(pma.TEXT_THROATS_WS).count("\n") - 1
+
+    ms_per_step = 1000.0 * elapsed / args.steps
+    stages_ms = {k: round(v / args.steps, 3) for k, v in stage_ms.items()}
+    stage_roofline = {}
+    for k, ms in stages_ms.items():
+        gbs = STAGE_BYTES_PER_VOXEL[k] * nvox / (ms * 1e-3) / 1e9 if ms > 0 else None
+        stage_roofline[k] = {"bytes_per_voxel": STAGE_BYTES_PER_VOXEL[k],
+                             "achieved_GBs": round(gbs, 1) if gbs else None,
+                             "frac": round(gbs / HBM_PEAK_GBS, 4) if gbs else None}
+    dom = max(kernels.items(), key=lambda kv: kv[1][0]) if kernels else None
+    roofline = None
+    if dom:
+        name, (tot_ms, launches) = dom
+        avg_ms = tot_ms / max(launches, 1)
+        bpv = KERNEL_BYTES_PER_VOXEL.get(name)
+        per_launch = bpv * nvox * args.steps / launches if bpv is not None and launches else None
+        if name == "relax_blocks" and launches:
+            per_launch = RELAX_BYTES_PER_BLOCK * stage_items["geodesic"] / launches
+        if per_launch is not None:
+            achieved = per_launch / (avg_ms * 1e-3) / 1e9
+            roofline = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                        "traffic": measured_traffic(name, S),
+                        "bytes_per_launch": round(per_launch), "avg_launch_ms": round(avg_ms, 4)}
+        else:
+            roofline = {"bound": "hbm", "kernel": name, "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": None, "traffic": None, "avg_launch_ms": round(avg_ms, 4), "launches": launches,
+                        "note": "no whole-volume algorithmic byte count for this kernel"}
+    pipeline_bytes = sum(STAGE_BYTES_PER_VOXEL.values()) * nvox
+    result = {
+        "metric": METRIC,
+        "value": nvox * args.steps / elapsed,
+        "unit": "voxels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32+f32",
+        "data": "synthetic",
+        "config": {"workload": f"{S}^3 Gaussian field sigma={args.sigma:g} porosity={args.porosity:g} "
+                               f"({args.stencil}): EDT, containing balls, pores, empty-pore removal, watershed, "
+                               f"geodesic distances with cavities",
+                   "stages": list(STAGES),
+                   "size": S, "sigma": args.sigma, "porosity": args.porosity,
+                   "seed": args.seed, "max_edt2": max_e, "maximum_balls": n_max, "pores": n_pores,
+                   "throat_keys": n_throats, "watershed_throat_keys": n_throats_ws},
+        "stages_ms": stages_ms,
+        "stage_roofline": stage_roofline,
+        # EDT: voxels; containing balls: (ball, tile) pairs; pores: DFS replay frames;
+        # geodesic: block relaxations
+        "stage_work_items": {k: v // args.steps for k, v in stage_items.items()},
+        "kernels_ms": {k: {"total_ms_per_step": round(v[0] / args.steps, 3), "launches_per_step": v[1] / args.steps}
+                       for k, v in sorted(kernels.items(), key=lambda kv: -kv[1][0])},
+        "pipeline_algorithmic_GBs": round(pipeline_bytes / (ms_per_step * 1e-3) / 1e9, 1),
+        "roofline": roofline,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(args)
+    print(json.dumps(result))
 
 
 if __name__ == "__main__":

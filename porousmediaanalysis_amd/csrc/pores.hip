@@ -199,10 +199,30 @@ __global__ void __launch_bounds__(256) cc_union(const uint32_t* __restrict__ cr,
     }
 }
 
+// Root of x without writing any parent.  cc_flatten walks with this: there, thread i stores
+// the root into parent[i] while other threads still walk chains through i, and a path-halving
+// store by such a thread (parent[i] := the grandparent it read before thread i's store) could
+// land after it and put a non-root back into parent[i].  Concurrent root stores only shorten
+// the chains this walk follows, since each replaces a parent by the same root.
+__device__ uint32_t root_of(const uint32_t* parent, uint32_t x, uint32_t n, uint32_t* err, uint32_t code) {
+    for (uint32_t step = 0; step <= n; ++step) {
+        const uint32_t p = ld_relaxed(&parent[x]);
+        if (p == x) return x;
+        if (p > x) {
+            fail_check(err, code);
+            return kInv;
+        }
+        x = p;
+    }
+    fail_check(err, code | 1);
+    return kInv;
+}
+
+// parent[i] := root of i; only thread i writes parent[i] in this launch.
 __global__ void cc_flatten(uint32_t* parent, size_t n, uint32_t* __restrict__ err) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n || ld_relaxed(&parent[i]) == kInv) return;
-    const uint32_t r = find_root(parent, (uint32_t)i, (uint32_t)n, err, kCcFlatten << 8 | 2);
+    const uint32_t r = root_of(parent, (uint32_t)i, (uint32_t)n, err, kCcFlatten << 8 | 2);
     if (r != kInv) st_relaxed(&parent[i], r);
 }
 
@@ -976,7 +996,7 @@ void run_pores(pma_ctx* c) {
         for (size_t l = 0; l < L; ++l)
             PMA_REQUIRE(lstart_h[l] < lstart_h[l + 1], PMA_ERR_INTERNAL, "pma_pores: level starts out of order");
     }
-    unsigned long long pool_cap = std::max<unsigned long long>(4 * P, 1024);
+    unsigned long long pool_cap = std::max<unsigned long long>(8 * P, 1024);
     uint32_t* pool = nullptr;
     unsigned long long* pool_top = c->buf<unsigned long long>("pt_pool_top", 1);
     for (int attempt = 0;; ++attempt) {
@@ -997,8 +1017,10 @@ void run_pores(pma_ctx* c) {
         check_stage(c, err);
         const uint32_t o = read_scalar(c, of);
         if (!(o & 2u)) break;
-        PMA_REQUIRE(attempt < 3, PMA_ERR_INTERNAL, "pma_pores: pore set pool overflow");
-        pool_cap = read_scalar(c, pool_top) + 1024;
+        // After an overflow the lower levels merged truncated sets, so the space taken is only a
+        // lower bound of what is needed: grow geometrically and start again.
+        PMA_REQUIRE(attempt < 8, PMA_ERR_INTERNAL, "pma_pores: pore set pool overflow");
+        pool_cap = std::max<unsigned long long>(4 * pool_cap, 2 * read_scalar(c, pool_top));
     }
     const SetBounds sb{E, P, npores, pool_cap};
 
