@@ -1,6 +1,7 @@
-"""The benchmark's workload at 192^3 (and 160^3 for the 6-neighbour watershed): the GPU
-Gaussian-field generator (sigma 8, porosity 0.65), then EDT, containing balls and geodesic
-distances with cavities, each bit-exact against the oracle.  At this size the containing-ball stage takes the paths the 1024^3 run takes (balls
+"""The benchmark's workload at 192^3 and 256^3 (and 160^3 for the 6-neighbour watershed): the
+GPU Gaussian-field generator (sigma 8, porosity 0.65), then EDT, containing balls, pores,
+empty-pore removal, watershed and geodesic distances with cavities, each bit-exact against the
+oracle.  At this size the containing-ball stage takes the paths the 1024^3 run takes (balls
 packed in the pair sort, chord widths from the LDS table, 16-bit winner slots) and the
 geodesic runs a few hundred chained rounds."""
 import os
@@ -44,8 +45,7 @@ def test_gaussian_192_matches_oracle(seed):
 
 def test_gaussian_192_removal_watershed():
     """The same workload through empty-pore removal and the watershed on the GPU, fed the oracle's
-    pore output (the pore stage itself does not run on the GPU): labels and text bit-exact against
-    the oracle.  Each stage runs twice on one context; the second run's device time is logged."""
+    pore output: labels and text bit-exact against the oracle.  Each stage runs twice on one context; the second run's device time is logged."""
     n = 192
     log(f"removal and watershed on the bench workload {n}^3")
     with pma.Context(0) as ctx:
@@ -103,3 +103,38 @@ def test_gaussian_160_watershed_d3q7():
             assert ctx.text(pma.TEXT_PORES_WS) == owp and ctx.text(pma.TEXT_THROATS_WS) == owk
         ms = ctx.stage_info(pma.STAGE_WATERSHED)["device_ms"]
     log(f"{n}^3 d3q7: {optxt.count(chr(10)) - 1} pores; watershed device ms {ms:.2f}")
+
+
+def test_gaussian_256_pores_watershed():
+    """The pore stage, empty-pore removal and the watershed of the GPU pipeline on the benchmark's
+    generator at 256^3, against the oracle run on the GPU's own EDT, containing radii and
+    maximum-ball mask (the stage inputs of the reference's -apt mode).  At this size plateau
+    chains are long enough that the flattening of the plateau forest meets concurrent walks
+    through the voxels it writes (a flatten that path-halves there left non-root parents)."""
+    n = 256
+    log(f"pores, removal and watershed on the bench workload {n}^3")
+    with pma.Context(0) as ctx:
+        ctx.generate_gaussian((n, n, n), 8.0, 0.65, 20261203)
+        ctx.edt()
+        ctx.containing_balls()
+        ctx.pores()
+        pt, ptxt, ktxt = ctx.read(pma.VOL_PT), ctx.text(pma.TEXT_PORES), ctx.text(pma.TEXT_THROATS)
+        ctx.remove_empty_pores()
+        rpt, rptxt, rktxt = ctx.read(pma.VOL_PT), ctx.text(pma.TEXT_PORES), ctx.text(pma.TEXT_THROATS)
+        ctx.watershed()
+        ptw, wp, wk = ctx.read(pma.VOL_PTW), ctx.text(pma.TEXT_PORES_WS), ctx.text(pma.TEXT_THROATS_WS)
+        e, cr, mask = ctx.read(pma.VOL_EDT2), ctx.read(pma.VOL_CR2), ctx.read(pma.VOL_MASK)
+        max_e, _ = ctx.statistics()
+        ms = {k: ctx.stage_info(s)["device_ms"] for k, s in
+              [("pores", pma.STAGE_PORES), ("watershed", pma.STAGE_WATERSHED)]}
+    opt, optxt, oktxt = oracle.pores(mask, e, cr)
+    assert ptxt == optxt, "pores.txt differs"
+    assert ktxt == oktxt, "sharedPixelTypes.txt differs"
+    assert np.array_equal(pt, opt), "PoreThroatIds differ"
+    opt, optxt, oktxt = oracle.remove_empty_pores(opt, optxt, oktxt)
+    assert np.array_equal(rpt, opt) and rptxt == optxt and rktxt == oktxt, "after removal"
+    optw, owp, owk = oracle.watershed(cr, max_e, optxt)
+    assert wp == owp and wk == owk, "watershed text differs"
+    assert np.array_equal(ptw, optw), "PoreThroatIdsWatershed differ"
+    log(f"{n}^3: {optxt.count(chr(10)) - 1} pores, {oktxt.count(chr(10)) - 1} throat keys; device ms " +
+        ", ".join(f"{k} {v:.1f}" for k, v in ms.items()))

@@ -65,9 +65,6 @@ CASES["noise_many_pores"] = volumes.random_noise((40, 48, 56), 0.45, 6)  # > 129
 @pytest.mark.parametrize("name", sorted(CASES))
 @pytest.mark.parametrize("d3q7", [False, True])
 def test_pipeline_matches_oracle(ctx, name, d3q7):
-    if name == "gauss_s2" and not d3q7:
-        pytest.skip("made an illegal memory access with the earlier fixed 9-pore-list pore kernels; cause not "
-                    "found, kept off the GPU")
     log(f"pipeline case {name} d3q7={d3q7} shape {CASES[name].shape}")
     compare(gpu_pipeline(ctx, CASES[name], d3q7), oracle.pipeline(CASES[name], d3q7, threads=4))
 

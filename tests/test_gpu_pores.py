@@ -16,11 +16,6 @@ from tests.volume_utils import canonical_relabel, sha256_array
 
 pytestmark = pytest.mark.gpu
 
-# The one case of this module that made an illegal memory access on MI355X with the earlier,
-# unchecked pore kernels (cause not found).  The kernels now check every index they read
-# from memory; this exact case stays off the GPU all the same.
-FAULTED = {("gauss_s3_low", False)}
-
 
 @pytest.fixture(scope="module")
 def ctx():
@@ -68,9 +63,6 @@ CASES["noise_many_pores"] = volumes.random_noise((40, 48, 56), 0.45, 6)  # > 129
 @pytest.mark.parametrize("name", sorted(CASES))
 @pytest.mark.parametrize("d3q7", [False, True])
 def test_pores_match_oracle(ctx, name, d3q7):
-    if (name, d3q7) in FAULTED:
-        pytest.skip("made an illegal memory access with the earlier unchecked pore kernels; cause not found, "
-                    "kept off the GPU")
     log(f"pores case {name} d3q7={d3q7} shape {CASES[name].shape}")
     e, cr, mask = oracle_front(CASES[name])
     first, removed = gpu_pores(ctx, e, cr, mask, d3q7)

@@ -29,7 +29,7 @@ SOURCES = ["scan", "sort", "lists", "edt", "balls", "pores", "remove", "watershe
 SANITIZED = ["-std=c++20", "-O1", "-g", "-fsanitize=address,undefined", "-fno-omit-frame-pointer",
              "-fno-sanitize-recover=undefined", f"-I{EMU}"]
 FAST = ["-std=c++20", "-O2", f"-I{EMU}"]
-ENV = {"PATH": "/usr/bin:/bin", "PMA_UNSAFE_GPU_PORES": "1",
+ENV = {"PATH": "/usr/bin:/bin",
        "ASAN_OPTIONS": "detect_leaks=0:detect_stack_use_after_return=0",
        "UBSAN_OPTIONS": "print_stacktrace=1"}
 
