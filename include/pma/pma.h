@@ -137,6 +137,32 @@ pma_status pma_geodesic(pma_ctx* ctx);
 /* The reference's -all mode followed by -spd (ImageProcessingManager.cpp:63-108). */
 pma_status pma_run_all(pma_ctx* ctx);
 
+/* ---- allocation-free geodesic ---------------------------------------------------------- */
+/* One row of cavityDescriptions.txt (ShortestPathComputer.cpp:143-164, Serializer.cpp:205-221). */
+typedef struct pma_cavity {
+    uint32_t id;
+    int32_t x, y, z;           /* start voxel */
+    double distance_to_center; /* to the image centre voxel */
+    uint64_t void_count;       /* void voxels reached */
+    int32_t reached_walls;     /* distinct boundary planes reached */
+    int32_t reaches_all;       /* 1 if every boundary plane was reached */
+} pma_cavity;
+/* Bytes of the static per-device scratch of pma_geodesic_noalloc: room for up to
+ * (48 MiB - 0.2 MiB) / 20 B = 2.5 M blocks of 8^3 voxels (a 1024^3 volume has 2 097 152). */
+#define PMA_GEODESIC_NOALLOC_SCRATCH_BYTES (48u << 20)
+/* No counterpart in the reference: pma_geodesic for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid it).  It
+ * computes the same distances, cavity ids and cavity rows as pma_geodesic, from and into the
+ * caller's device buffers (solid u8, dist f32, cavity u32, [z][x][y] layout), on `stream` (a
+ * hipStream_t; NULL = the default stream), and blocks until done.  It calls no hipMalloc or
+ * hipFree: its scratch is one static device array per GPU of
+ * PMA_GEODESIC_NOALLOC_SCRATCH_BYTES, held until the process ends.  Volumes whose scratch would
+ * not fit are rejected with PMA_ERR_CAPACITY before any device work; more than `cap` cavities
+ * end the call with PMA_ERR_CAPACITY (*count rows written).  Calls that overlap on one GPU share
+ * that array and race: serialise them.  Makes `device` current. */
+pma_status pma_geodesic_noalloc(int device, const uint8_t* solid, pma_dims dims, pma_stencil stencil, float* dist,
+                                uint32_t* cavity, pma_cavity* cavities, size_t cap, size_t* count, void* stream);
+
 /* ---- statistics and tables ------------------------------------------------------------ */
 pma_status pma_get_statistics(pma_ctx* ctx, uint32_t* max_edt2, uint64_t* maximum_balls_count);
 /* counts[0..maxE] of non-boundary-crossing EDT squares; *len receives maxE + 1. */

@@ -31,7 +31,7 @@ EXPORTS = [
     "pma_pores", "pma_remove_empty_pores", "pma_watershed", "pma_geodesic", "pma_run_all",
     "pma_get_statistics", "pma_get_histogram", "pma_get_text", "pma_set_text", "pma_get_stage_info",
     "pma_generate_spheres", "pma_generate_gaussian", "pma_set_kernel_timing", "pma_get_kernel_time",
-    "pma_list_kernel_times",
+    "pma_list_kernel_times", "pma_geodesic_noalloc",
 ]
 
 

@@ -402,6 +402,14 @@ pma_status pma_run_all(pma_ctx* c) {
     });
 }
 
+pma_status pma_geodesic_noalloc(int device, const uint8_t* solid, pma_dims dims, pma_stencil stencil, float* dist,
+                                uint32_t* cavity, pma_cavity* cavities, size_t cap, size_t* count, void* stream) {
+    return guard([&] {
+        run_geodesic_noalloc(device, solid, dims, stencil, dist, cavity, cavities, cap, count,
+                             static_cast<hipStream_t>(stream));
+    });
+}
+
 pma_status pma_get_statistics(pma_ctx* c, uint32_t* max_e, uint64_t* n_max) {
     return guard([&] {
         check_ctx(c);

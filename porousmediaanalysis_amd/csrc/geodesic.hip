@@ -24,6 +24,7 @@
 
 #include <cfloat>
 #include <cmath>
+#include <mutex>
 
 namespace pma {
 namespace {
@@ -456,39 +457,68 @@ uint32_t plane_bits(uint32_t wb, const int S[3]) {
     return out;
 }
 
-}  // namespace
+__global__ void fill_u64(unsigned long long* __restrict__ p, unsigned long long v, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
 
-void run_geodesic(pma_ctx* c) {
-    PMA_REQUIRE(c->valid[PMA_VOL_SOLID], PMA_ERR_STATE, "pma_geodesic: the solid volume has not been loaded");
-    const int X = c->dims.x, Y = c->dims.y, Z = c->dims.z;
-    PMA_REQUIRE((double)X * Y * Z < 4294967295.0 && X < (1 << 20) && Y < (1 << 20) && Z < (1 << 20),
-                PMA_ERR_INVALID_ARGUMENT, "pma_geodesic: volume too large");
-    c->ensure_volume(PMA_VOL_SPD);
-    c->ensure_volume(PMA_VOL_CAVITY);
-    const uint8_t* solid = c->v<uint8_t>(PMA_VOL_SOLID);
-    float* dist = c->v<float>(PMA_VOL_SPD);
-    uint32_t* cav = c->v<uint32_t>(PMA_VOL_CAVITY);
-    const size_t n = c->n;
-    geo_init<<<grid_for((n + 3) / 4, 256), 256, 0, c->stream>>>(solid, dist, cav, n);
-    PMA_LAUNCHED(c);
+// Scratch of one run: per 8^3 block the two active lists, the processing list, the round tag
+// and the wake key (NB entries each); the two round states; counters; one cavity slot per
+// relaxing workgroup.
+struct GeoScratch {
+    uint32_t *fa, *fb, *fp, *block_tag, *block_key;
+    RoundState* rs;
+    unsigned long long *visits, *key, *vcount;
+    uint32_t* walls;
+    unsigned long long* part_count;
+    uint32_t* part_walls;
+};
+
+// How the run reaches the host: a context (its staging buffer, kernel timing and launch
+// counts), or, without one, plain copies on the caller's stream.
+struct GeoHost {
+    hipStream_t stream;
+    pma_ctx* ctx;  // null for the allocation-free entry point
+    template <class T> T read(const T* p) {
+        if (ctx) return read_scalar(ctx, p);
+        T v;
+        PMA_HIP(hipMemcpyAsync(&v, p, sizeof(T), hipMemcpyDeviceToHost, stream));
+        PMA_HIP(hipStreamSynchronize(stream));
+        return v;
+    }
+    void put(void* dst, const void* src, size_t bytes) {
+        PMA_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
+        PMA_HIP(hipStreamSynchronize(stream));
+    }
+    void launched(const char* name) {
+        if (ctx) {
+            PMA_CHECKED(ctx, name, nullptr);
+            return;
+        }
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) throw Error(PMA_ERR_HIP, std::string("launch of ") + name + ": " + hipGetErrorString(e));
+    }
+};
+
+// ShortestPathComputer::ComputeShortestPathDistances on device buffers: distances and cavity ids
+// into dist and cav, one row per cavity handed to emit(row) (which returns false to stop with
+// PMA_ERR_CAPACITY).  Returns the number of block relaxations.  Clears its scratch with kernels
+// (no memset nodes), so the launches it makes are all kernels and small copies.
+template <class Emit>
+unsigned long long geodesic_core(GeoHost& h, const uint8_t* solid, float* dist, uint32_t* cav, int X, int Y, int Z,
+                                 int d3q7, const GeoScratch& g, Emit emit) {
+    const size_t n = (size_t)X * Y * Z;
+    hipStream_t st = h.stream;
+    geo_init<<<grid_for((n + 3) / 4, 256), 256, 0, st>>>(solid, dist, cav, n);
+    h.launched("geo_init");
     const int NBZ = (Z + kB - 1) / kB, NBX = (X + kB - 1) / kB, NBY = (Y + kB - 1) / kB;
     const size_t NB = (size_t)NBZ * NBX * NBY;
-    uint32_t* fa = c->buf<uint32_t>("geo_active_a", NB);
-    uint32_t* fb = c->buf<uint32_t>("geo_active_b", NB);
-    uint32_t* fp = c->buf<uint32_t>("geo_active_proc", NB);
-    uint32_t* block_tag = c->buf<uint32_t>("geo_block_tag", NB);
-    uint32_t* block_key = c->buf<uint32_t>("geo_block_key", NB);
-    RoundState* rs = c->buf<RoundState>("geo_round_state", 2);
-    unsigned long long* visits = c->buf<unsigned long long>("geo_visits", 1);
-    PMA_HIP(hipMemsetAsync(visits, 0, sizeof(unsigned long long), c->stream));
-    unsigned long long* key = c->buf<unsigned long long>("geo_key", 1);
-    unsigned long long* vcount = c->buf<unsigned long long>("geo_vcount", 1);
-    uint32_t* walls = c->buf<uint32_t>("geo_walls", 1);
-    unsigned long long* part_count = c->buf<unsigned long long>("geo_part_count", kRelaxGrid);
-    uint32_t* part_walls = c->buf<uint32_t>("geo_part_walls", kRelaxGrid);
-    PMA_HIP(hipMemsetAsync(block_tag, 0xFF, NB * sizeof(uint32_t), c->stream));
-    fill_key<<<grid_for(NB, 256), 256, 0, c->stream>>>(block_key, kFltMaxBits, NB);
-    PMA_LAUNCHED(c);
+    fill_u64<<<1, 64, 0, st>>>(g.visits, 0ull, 1);
+    h.launched("fill_u64");
+    fill_key<<<grid_for(NB, 256), 256, 0, st>>>(g.block_tag, 0xFFFFFFFFu, NB);
+    h.launched("fill_key");
+    fill_key<<<grid_for(NB, 256), 256, 0, st>>>(g.block_key, kFltMaxBits, NB);
+    h.launched("fill_key");
 
     const int S[3] = {X, Y, Z};
     int ctr[3];
@@ -496,9 +526,7 @@ void run_geodesic(pma_ctx* c) {
     int max_half = 0;
     for (int a = 0; a < 3; ++a) max_half = std::max(max_half, std::max(S[a] - 1 - ctr[a], ctr[a]));
     uint32_t all_walls = plane_bits(0x3Fu, S);
-    const int d3q7 = c->stencil == PMA_STENCIL_D3Q7;
 
-    c->cavities.clear();
     int half = 0;
     uint32_t id = 0, round = 0;
     while (true) {
@@ -506,25 +534,25 @@ void run_geodesic(pma_ctx* c) {
         int start[3];
         bool found = false;
         const size_t ci = ((size_t)ctr[2] * X + ctr[0]) * Y + ctr[1];
-        uint8_t cs = read_scalar(c, solid + ci);
-        float cd = read_scalar(c, dist + ci);
+        uint8_t cs = h.read(solid + ci);
+        float cd = h.read(dist + ci);
         if (cs == 0 && cd == FLT_MAX) {
             for (int a = 0; a < 3; ++a) start[a] = ctr[a];
             found = true;
         } else if (half < max_half) {
-            PMA_HIP(hipMemsetAsync(key, 0xFF, sizeof(unsigned long long), c->stream));
-            find_start<<<std::min(grid_for(n, 256), 8192u), 256, 0, c->stream>>>(solid, dist, X, Y, Z, ctr[0], ctr[1],
-                                                                                 ctr[2], half,
-                                                                  max_half, key);
-            PMA_LAUNCHED(c);
-            unsigned long long k = read_scalar(c, key);
+            fill_u64<<<1, 64, 0, st>>>(g.key, ~0ull, 1);
+            h.launched("fill_u64");
+            find_start<<<std::min(grid_for(n, 256), 8192u), 256, 0, st>>>(solid, dist, X, Y, Z, ctr[0], ctr[1],
+                                                                        ctr[2], half, max_half, g.key);
+            h.launched("find_start");
+            unsigned long long k = h.read(g.key);
             if (k != ~0ull) {
-                int h = (int)(k >> 44), ax = (int)((k >> 42) & 3), fi = (int)((k >> 41) & 1);
+                int hh = (int)(k >> 44), ax = (int)((k >> 42) & 3), fi = (int)((k >> 41) & 1);
                 int a0 = ax == 0 ? 1 : 0, a1 = ax == 2 ? 1 : 2;
                 start[a0] = (int)((k >> 20) & 0xFFFFF);
                 start[a1] = (int)(k & 0xFFFFF);
-                start[ax] = fi == 0 ? std::max(ctr[ax] - h, 0) : std::min(ctr[ax] + h, S[ax] - 1);
-                half = h;
+                start[ax] = fi == 0 ? std::max(ctr[ax] - hh, 0) : std::min(ctr[ax] + hh, S[ax] - 1);
+                half = hh;
                 found = true;
             } else {
                 half = max_half;
@@ -535,49 +563,51 @@ void run_geodesic(pma_ctx* c) {
         const size_t si = ((size_t)start[2] * X + start[0]) * Y + start[1];
         const uint32_t b0 = ((uint32_t)(start[2] / kB) * (uint32_t)NBX + (uint32_t)(start[0] / kB)) * (uint32_t)NBY +
                             (uint32_t)(start[1] / kB);
-        upload(c, fa, &b0, sizeof b0);
+        h.put(g.fa, &b0, sizeof b0);
         const uint32_t zero = 0;
-        upload(c, block_key + b0, &zero, sizeof zero);
-        PMA_HIP(hipMemsetAsync(part_count, 0, kRelaxGrid * sizeof(unsigned long long), c->stream));
-        PMA_HIP(hipMemsetAsync(part_walls, 0, kRelaxGrid * sizeof(uint32_t), c->stream));
+        h.put(g.block_key + b0, &zero, sizeof zero);
+        fill_u64<<<grid_for(kRelaxGrid, 256), 256, 0, st>>>(g.part_count, 0ull, kRelaxGrid);
+        h.launched("fill_u64");
+        fill_key<<<grid_for(kRelaxGrid, 256), 256, 0, st>>>(g.part_walls, 0u, kRelaxGrid);
+        h.launched("fill_key");
         // round 0 processes the start block (key 0); the host checks for an empty list after
         // rounds 1, 2, 4, 8 and then every 8 rounds, and never waits between the others
         const RoundState start_states[2] = {{1u, 0u, 0u, 0u}, {0u, 0x7F800000u, 0u, 0u}};
-        upload(c, rs, start_states, sizeof start_states);
+        h.put(g.rs, start_states, sizeof start_states);
         uint32_t r = 0, check_at = 1, step = 1;
         while (true) {
-            RoundState* cs = rs + (r & 1);
-            RoundState* ns = rs + ((r + 1) & 1);
-            uint32_t* cur = (r & 1) ? fb : fa;
-            uint32_t* nxt = (r & 1) ? fa : fb;
-            partition_active<<<std::min<size_t>(grid_for(NB, 256), 256u), 256, 0, c->stream>>>(
-                cur, cs, block_key, fp, nxt, block_tag, round, ns);
-            PMA_LAUNCHED(c);
+            RoundState* cur_s = g.rs + (r & 1);
+            RoundState* nxt_s = g.rs + ((r + 1) & 1);
+            uint32_t* cur = (r & 1) ? g.fb : g.fa;
+            uint32_t* nxt = (r & 1) ? g.fa : g.fb;
+            partition_active<<<std::min<size_t>(grid_for(NB, 256), 256u), 256, 0, st>>>(
+                cur, cur_s, g.block_key, g.fp, nxt, g.block_tag, round, nxt_s);
+            h.launched("partition_active");
             {
-                PMA_TIMED(c, "relax_blocks");
+                PMA_TIMED(h.ctx, "relax_blocks");
                 const unsigned long long src = r == 0 ? (unsigned long long)si : ~0ull;
                 if (d3q7)
-                    relax_blocks<true><<<kRelaxGrid, 64, 0, c->stream>>>(dist, cav, id, fp, nxt, block_tag, round,
-                                                                         block_key, cs, ns, part_count, part_walls, X,
-                                                                         Y, Z, NBX, NBY, NBZ, src);
+                    relax_blocks<true><<<kRelaxGrid, 64, 0, st>>>(dist, cav, id, g.fp, nxt, g.block_tag, round,
+                                                                  g.block_key, cur_s, nxt_s, g.part_count,
+                                                                  g.part_walls, X, Y, Z, NBX, NBY, NBZ, src);
                 else
-                    relax_blocks<false><<<kRelaxGrid, 64, 0, c->stream>>>(dist, cav, id, fp, nxt, block_tag, round,
-                                                                          block_key, cs, ns, part_count, part_walls, X,
-                                                                          Y, Z, NBX, NBY, NBZ, src);
-                PMA_LAUNCHED(c);
+                    relax_blocks<false><<<kRelaxGrid, 64, 0, st>>>(dist, cav, id, g.fp, nxt, g.block_tag, round,
+                                                                   g.block_key, cur_s, nxt_s, g.part_count,
+                                                                   g.part_walls, X, Y, Z, NBX, NBY, NBZ, src);
+                h.launched("relax_blocks");
             }
-            next_min_key<<<256, 256, 0, c->stream>>>(nxt, block_key, ns, cs, visits);
-            PMA_LAUNCHED(c);
+            next_min_key<<<256, 256, 0, st>>>(nxt, g.block_key, nxt_s, cur_s, g.visits);
+            h.launched("next_min_key");
             ++round;
             ++r;
             if (r == check_at) {
-                if (read_scalar(c, &ns->count) == 0) break;
+                if (h.read(&nxt_s->count) == 0) break;
                 step = std::min(step * 2, 8u);
                 check_at += step;
             }
         }
-        cavity_totals<<<1, 256, 0, c->stream>>>(part_count, part_walls, kRelaxGrid, vcount, walls);
-        PMA_LAUNCHED(c);
+        cavity_totals<<<1, 256, 0, st>>>(g.part_count, g.part_walls, kRelaxGrid, g.vcount, g.walls);
+        h.launched("cavity_totals");
         CavityRow row;
         row.id = id;
         row.x = start[0];
@@ -586,16 +616,144 @@ void run_geodesic(pma_ctx* c) {
         int d2 = 0;
         for (int a = 0; a < 3; ++a) d2 += (start[a] - ctr[a]) * (start[a] - ctr[a]);
         row.distance_to_center = std::sqrt((double)d2);
-        row.void_count = read_scalar(c, vcount);
-        uint32_t wb = plane_bits(read_scalar(c, walls), S);
+        row.void_count = h.read(g.vcount);
+        uint32_t wb = plane_bits(h.read(g.walls), S);
         row.reached_walls = __builtin_popcount(wb);
         row.reaches_all = wb == all_walls;
-        c->cavities.push_back(row);
+        PMA_REQUIRE(emit(row), PMA_ERR_CAPACITY, "pma_geodesic: more cavities than the caller's table holds");
         if (row.reaches_all) break;
     }
+    return h.read(g.visits);
+}
+
+void check_geodesic_dims(int X, int Y, int Z) {
+    PMA_REQUIRE(X > 0 && Y > 0 && Z > 0, PMA_ERR_INVALID_ARGUMENT, "pma_geodesic: dimensions must be positive");
+    PMA_REQUIRE((double)X * Y * Z < 4294967295.0 && X < (1 << 20) && Y < (1 << 20) && Z < (1 << 20),
+                PMA_ERR_INVALID_ARGUMENT, "pma_geodesic: volume too large");
+}
+
+size_t geodesic_blocks(int X, int Y, int Z) {
+    return (size_t)((Z + kB - 1) / kB) * ((X + kB - 1) / kB) * ((Y + kB - 1) / kB);
+}
+
+// ---- the allocation-free entry point's scratch ---------------------------------------------
+// One array per device for pma_geodesic_noalloc, held until the process ends.  Host code uses
+// the per-device address that hipGetSymbolAddress returns, never the array's name.
+constexpr size_t kNoAllocScratch = (size_t)PMA_GEODESIC_NOALLOC_SCRATCH_BYTES;
+__device__ __align__(256) unsigned char scratch[kNoAllocScratch];
+
+constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Bytes of scratch a run over NB blocks takes, in the order carve() lays it out.
+size_t geodesic_scratch_bytes(size_t NB) {
+    return 5 * align256(NB * 4) + align256(2 * sizeof(RoundState)) + 3 * align256(8) + align256(4) +
+           align256(kRelaxGrid * 8ull) + align256(kRelaxGrid * 4ull);
+}
+
+GeoScratch carve(unsigned char* base, size_t NB) {
+    GeoScratch g;
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        unsigned char* p = base + o;
+        o += align256(bytes);
+        return p;
+    };
+    g.fa = (uint32_t*)take(NB * 4);
+    g.fb = (uint32_t*)take(NB * 4);
+    g.fp = (uint32_t*)take(NB * 4);
+    g.block_tag = (uint32_t*)take(NB * 4);
+    g.block_key = (uint32_t*)take(NB * 4);
+    g.rs = (RoundState*)take(2 * sizeof(RoundState));
+    g.visits = (unsigned long long*)take(8);
+    g.key = (unsigned long long*)take(8);
+    g.vcount = (unsigned long long*)take(8);
+    g.walls = (uint32_t*)take(4);
+    g.part_count = (unsigned long long*)take(kRelaxGrid * 8ull);
+    g.part_walls = (uint32_t*)take(kRelaxGrid * 4ull);
+    return g;
+}
+
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[64] = {};
+
+// The device address of this device's scratch array (looked up once per device, with the
+// device current).
+void* device_scratch(int device) {
+    PMA_REQUIRE(device >= 0 && device < 64, PMA_ERR_INVALID_ARGUMENT, "pma_geodesic_noalloc: device out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        PMA_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        PMA_REQUIRE(p != nullptr, PMA_ERR_HIP, "pma_geodesic_noalloc: no device address for the scratch array");
+        g_scratch_ptr[device] = p;
+    }
+    return g_scratch_ptr[device];
+}
+
+}  // namespace
+
+void run_geodesic(pma_ctx* c) {
+    PMA_REQUIRE(c->valid[PMA_VOL_SOLID], PMA_ERR_STATE, "pma_geodesic: the solid volume has not been loaded");
+    const int X = c->dims.x, Y = c->dims.y, Z = c->dims.z;
+    check_geodesic_dims(X, Y, Z);
+    c->ensure_volume(PMA_VOL_SPD);
+    c->ensure_volume(PMA_VOL_CAVITY);
+    const size_t NB = geodesic_blocks(X, Y, Z);
+    GeoScratch g;
+    g.fa = c->buf<uint32_t>("geo_active_a", NB);
+    g.fb = c->buf<uint32_t>("geo_active_b", NB);
+    g.fp = c->buf<uint32_t>("geo_active_proc", NB);
+    g.block_tag = c->buf<uint32_t>("geo_block_tag", NB);
+    g.block_key = c->buf<uint32_t>("geo_block_key", NB);
+    g.rs = c->buf<RoundState>("geo_round_state", 2);
+    g.visits = c->buf<unsigned long long>("geo_visits", 1);
+    g.key = c->buf<unsigned long long>("geo_key", 1);
+    g.vcount = c->buf<unsigned long long>("geo_vcount", 1);
+    g.walls = c->buf<uint32_t>("geo_walls", 1);
+    g.part_count = c->buf<unsigned long long>("geo_part_count", kRelaxGrid);
+    g.part_walls = c->buf<uint32_t>("geo_part_walls", kRelaxGrid);
+    GeoHost h{c->stream, c};
+    c->cavities.clear();
+    const unsigned long long visits =
+        geodesic_core(h, c->v<uint8_t>(PMA_VOL_SOLID), c->v<float>(PMA_VOL_SPD), c->v<uint32_t>(PMA_VOL_CAVITY), X, Y,
+                      Z, c->stencil == PMA_STENCIL_D3Q7, g, [&](const CavityRow& r) {
+                          c->cavities.push_back(r);
+                          return true;
+                      });
     c->have_cavities = true;
     c->valid[PMA_VOL_SPD] = c->valid[PMA_VOL_CAVITY] = true;
-    c->stage[PMA_STAGE_GEODESIC].work_items = read_scalar(c, visits);
+    c->stage[PMA_STAGE_GEODESIC].work_items = visits;
+}
+
+void run_geodesic_noalloc(int device, const uint8_t* solid, pma_dims dims, pma_stencil stencil, float* dist,
+                          uint32_t* cavity, pma_cavity* rows, size_t cap, size_t* count, hipStream_t stream) {
+    PMA_REQUIRE(solid && dist && cavity && count && (rows || cap == 0), PMA_ERR_INVALID_ARGUMENT,
+                "pma_geodesic_noalloc: null pointer");
+    PMA_REQUIRE(stencil == PMA_STENCIL_D3Q27 || stencil == PMA_STENCIL_D3Q7, PMA_ERR_INVALID_ARGUMENT,
+                "pma_geodesic_noalloc: unknown stencil");
+    const int X = dims.x, Y = dims.y, Z = dims.z;
+    check_geodesic_dims(X, Y, Z);
+    const size_t NB = geodesic_blocks(X, Y, Z);
+    PMA_REQUIRE(geodesic_scratch_bytes(NB) <= kNoAllocScratch, PMA_ERR_CAPACITY,
+                "pma_geodesic_noalloc: the volume needs " + std::to_string(geodesic_scratch_bytes(NB)) +
+                    " bytes of scratch, more than the " + std::to_string(kNoAllocScratch) + " it has");
+    PMA_HIP(hipSetDevice(device));
+    GeoScratch g = carve(static_cast<unsigned char*>(device_scratch(device)), NB);
+    GeoHost h{stream, nullptr};
+    *count = 0;
+    geodesic_core(h, solid, dist, cavity, X, Y, Z, stencil == PMA_STENCIL_D3Q7, g, [&](const CavityRow& r) {
+        if (*count >= cap) return false;
+        pma_cavity& o = rows[(*count)++];
+        o.id = r.id;
+        o.x = r.x;
+        o.y = r.y;
+        o.z = r.z;
+        o.distance_to_center = r.distance_to_center;
+        o.void_count = r.void_count;
+        o.reached_walls = r.reached_walls;
+        o.reaches_all = r.reaches_all ? 1 : 0;
+        return true;
+    });
 }
 
 }  // namespace pma

@@ -153,6 +153,9 @@ void run_pores(pma_ctx* c);
 void run_remove_empty_pores(pma_ctx* c);
 void run_watershed(pma_ctx* c);
 void run_geodesic(pma_ctx* c);
+// pma_geodesic_noalloc: the geodesic on caller buffers with the library's static scratch.
+void run_geodesic_noalloc(int device, const uint8_t* solid, pma_dims dims, pma_stencil stencil, float* dist,
+                          uint32_t* cavity, pma_cavity* rows, size_t cap, size_t* count, hipStream_t stream);
 void run_generate_spheres(pma_ctx* c, int32_t n, float rmin, float rmax, uint64_t seed);
 void run_generate_gaussian(pma_ctx* c, float sigma, float porosity, uint64_t seed);
 
@@ -240,7 +243,7 @@ struct KernelScope {
     const char* name;
     hipEvent_t start = nullptr;
     KernelScope(pma_ctx* ctx, const char* n) : c(ctx), name(n) {
-        if (c->kernel_timing) start = c->kernel_mark();
+        if (c && c->kernel_timing) start = c->kernel_mark();
     }
     ~KernelScope() {
         if (start) c->kernel_done(name, start);

@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -376,6 +377,22 @@ enum pma_stencil { PMA_STENCIL_D3Q27 = 0, PMA_STENCIL_D3Q7 = 1 };
 enum pma_stage { PMA_STAGE_EDT, PMA_STAGE_CONTAINING_BALLS, PMA_STAGE_PORES, PMA_STAGE_REMOVE_EMPTY_PORES,
                  PMA_STAGE_WATERSHED, PMA_STAGE_GEODESIC, PMA_STAGE_COUNT };
 struct pma_dims { int x, y, z; };
+struct pma_cavity {
+    uint32_t id;
+    int32_t x, y, z;
+    double distance_to_center;
+    uint64_t void_count;
+    int32_t reached_walls;
+    int32_t reaches_all;
+};
+#define PMA_GEODESIC_NOALLOC_SCRATCH_BYTES (48u << 20)
+#define __align__(n) alignas(n)
+using hipError_t = int;
+constexpr hipError_t hipSuccess = 0;
+inline hipError_t hipGetLastError() { return hipSuccess; }
+inline const char* hipGetErrorString(hipError_t) { return "no error"; }
+#define HIP_SYMBOL(x) ((void*)&(x))
+inline hipError_t hipGetSymbolAddress(void** p, void* sym) { *p = sym; return hipSuccess; }
 struct StageInfo { uint64_t work_items = 0; };
 namespace pma {
 struct Error : std::runtime_error {
