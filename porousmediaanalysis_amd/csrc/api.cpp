@@ -234,6 +234,14 @@ void* pma_ctx::host_staging(size_t bytes) {
 
 void pma_ctx::ensure_volume(pma_volume w) { vol[w].reserve(n * volume_element_size(w)); }
 
+void pma_ctx::drop(const std::string& name) {
+    auto it = scratch.find(name);
+    if (it == scratch.end()) return;
+    PMA_HIP(hipDeviceSynchronize());
+    it->second.release();
+    scratch.erase(it);
+}
+
 void pma_ctx::sync() { PMA_HIP(hipStreamSynchronize(stream)); }
 
 hipEvent_t pma_ctx::kernel_mark() {

@@ -137,6 +137,8 @@ struct pma_ctx {
     template <class T> T* buf(const std::string& name, size_t count) {
         return static_cast<T*>(buf(name, count * sizeof(T)));
     }
+    // Frees a named scratch buffer after the device is idle (for the large, short-lived ones).
+    void drop(const std::string& name);
     void* host_staging(size_t bytes);
     void ensure_volume(pma_volume w);
     void sync();

@@ -42,6 +42,9 @@ namespace pma {
 namespace {
 
 constexpr uint32_t kInv = 0xFFFFFFFFu;
+#ifndef PMA_PORES_STACK_CAP
+#define PMA_PORES_STACK_CAP (1ull << 24)  // DFS frames per replaying wave in the first pass
+#endif
 
 // Kernel ids of the in-kernel checks (the code's high bits).
 enum : uint32_t {
@@ -547,10 +550,9 @@ __global__ void reach_sizes(const unsigned long long* __restrict__ set_off, cons
                             const uint32_t* __restrict__ pool, const uint32_t* __restrict__ plat_size,
                             const uint32_t* __restrict__ plat_border, SetBounds b,
                             unsigned long long* __restrict__ reach, uint32_t* __restrict__ boundary,
-                            unsigned long long* __restrict__ plat_bits, uint32_t* __restrict__ err) {
+                            uint32_t* __restrict__ err) {
     size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= b.P) return;
-    plat_bits[q] = (unsigned long long)plat_size[q] * set_len[q];
     if (set_off[q] + set_len[q] > b.pool_cap) {
         fail_check(err, kReachSizes << 8 | 1);
         return;
@@ -566,17 +568,15 @@ __global__ void reach_sizes(const unsigned long long* __restrict__ set_off, cons
     }
 }
 
-// ---- DFS replay: one wave per pore ------------------------------------------------------------
+// ---- DFS replay: a persistent pool of waves, one pore at a time per wave --------------------
 struct ReplayTask {
-    uint32_t pore, seed, ncls, slot;
-    unsigned long long stack_off;
-    unsigned long long stack_cap;  // frames available to this task
-    unsigned long long seen_off;   // first word of this pore's class bitmap
+    uint32_t pore, seed, ncls, pad;
+    unsigned long long seen_off;  // first word of this pore's class bitmap
 };
 
 // Lengths of the buffers the replay selects from with values read from memory.
 struct ReplayBounds {
-    unsigned long long n, P, pool_cap, R, D, bits, frames, seen_words;
+    unsigned long long n, P, pool_cap, R, D, seen_words;
 };
 
 // Position of pore p in the ascending set pool[off .. off + len), or -1.
@@ -591,151 +591,160 @@ __device__ __forceinline__ int member_index(const uint32_t* __restrict__ pool, u
     return (lo < len && pool[off + lo] == p) ? (int)lo : -1;
 }
 
-// Bit of (voxel v, member j of its plateau's set) in the visited bitmap.
-__device__ __forceinline__ unsigned long long visit_bit(const unsigned long long* __restrict__ bit_base,
-                                                        const uint32_t* __restrict__ vrank,
-                                                        const uint32_t* __restrict__ set_len, uint32_t q, uint32_t v,
-                                                        uint32_t j) {
-    return bit_base[q] + (unsigned long long)vrank[v] * set_len[q] + j;
+// Lane `src`'s value as a wave-uniform scalar: the loop conditions built from broadcasts are then
+// uniform to the compiler as well.
+__device__ __forceinline__ uint32_t bcast(uint32_t v, int src) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, src);
 }
 
+// One workgroup (one wave) per pore of a batch; workgroup b uses slot b of the per-slot buffers:
+// a visited bitmap over all voxels and a bitmap of entered plateaus, both cleared first, and a
+// stack.  (There is no loop over pores inside the kernel: with one, the compiler let lanes 1..63
+// start the next pore while lane 0, which owns the stack, was still fetching it.)  Lanes 0..26 test the 26 neighbours of the current voxel with independent loads
+// (radius, plateau, visited word); lane 0 owns the stack and the counters.  Frames are (voxel,
+// next neighbour index); a recursive DFS that tries neighbours in reverse loop order visits
+// voxels in the reference's stack-DFS order.  A voxel's earlier-pore class depends only on its
+// plateau, so the class is looked up on the pore's first entry into each plateau.  The replay
+// of a pore stops once all but one of its classes are ranked: the last one is necessarily
+// found after them.
 __global__ void __launch_bounds__(64) dfs_replay(const ReplayTask* __restrict__ tasks, const uint32_t* __restrict__ cr,
                                                  const uint32_t* __restrict__ plat,
                                                  const unsigned long long* __restrict__ set_off,
                                                  const uint32_t* __restrict__ set_len, const uint32_t* __restrict__ pool,
                                                  const unsigned long long* __restrict__ rbase,
                                                  const uint32_t* __restrict__ rec_node,
-                                                 const uint32_t* __restrict__ event_local,
-                                                 const unsigned long long* __restrict__ bit_base,
-                                                 const uint32_t* __restrict__ vrank, unsigned long long* visited,
-                                                 uint32_t* __restrict__ seen_words, uint32_t* __restrict__ stack_voxel,
-                                                 uint8_t* __restrict__ stack_k, uint32_t* __restrict__ rank,
+                                                 const uint32_t* __restrict__ event_local, unsigned long long* vis_all,
+                                                 unsigned long long vis_words, unsigned long long* pent_all,
+                                                 unsigned long long pent_words, uint32_t* __restrict__ seen_words,
+                                                 uint32_t* __restrict__ stack_all, uint8_t* __restrict__ stack_k_all,
+                                                 unsigned long long stack_cap, uint32_t* __restrict__ rank,
                                                  uint32_t* __restrict__ task_status, int X, int Y, int Z, int d3q7,
                                                  ReplayBounds b, uint32_t* __restrict__ err) {
-    // One wave per pore.  Lane 0 owns the stack, the seen-class bitmap and the counters and
-    // broadcasts every decision; the other lanes evaluate the 26 neighbours in parallel.
-    // Frames are (voxel, next neighbour index); a recursive DFS that tries neighbours in
-    // reverse loop order visits voxels in the reference's stack-DFS order.
-    const ReplayTask task = tasks[blockIdx.x];
     const int lane = threadIdx.x;
-    const uint32_t p = task.pore;
-    if (task.stack_off + task.stack_cap > b.frames || task.seen_off + (task.ncls + 31) / 32 > b.seen_words ||
-        task.seed >= b.n) {
-        if (lane == 0) fail_check(err, kDfsReplay << 8 | 1);
-        return;
-    }
-    uint32_t* sv = stack_voxel + task.stack_off;
-    uint8_t* sk = stack_k + task.stack_off;
-    uint32_t* seen = seen_words + task.seen_off;
+    unsigned long long* vis = vis_all + (size_t)blockIdx.x * vis_words;
+    unsigned long long* pent = pent_all + (size_t)blockIdx.x * pent_words;
+    uint32_t* sv = stack_all + (size_t)blockIdx.x * stack_cap;
+    uint8_t* sk = stack_k_all + (size_t)blockIdx.x * stack_cap;
 
-    // Plateau q of void voxel v and the bit of (v, pore p) in the visited bitmap; false (with a
-    // reported code) if an index is out of range or p does not reach v's plateau.
-    auto locate = [&](uint32_t v, uint32_t& q, int& j, unsigned long long& bit, uint32_t code) -> bool {
-        q = plat[v];
-        if (q >= b.P || set_off[q] + set_len[q] > b.pool_cap) {
-            fail_check(err, kDfsReplay << 8 | code);
-            return false;
-        }
-        j = member_index(pool, set_off[q], set_len[q], p);
-        if (j < 0) {
-            fail_check(err, kDfsReplay << 8 | (code + 1));
-            return false;
-        }
-        bit = visit_bit(bit_base, vrank, set_len, q, v, (uint32_t)j);
-        if (bit >= b.bits) {
-            fail_check(err, kDfsReplay << 8 | (code + 2));
-            return false;
-        }
-        return true;
-    };
-
-    // Marks v visited by p and, if its earlier-pore class is new, ranks that class.
-    // Executed by lane 0 only; returns the new count of classes found, or kInv after a
-    // failed check.
-    auto visit = [&](uint32_t v, uint32_t found) -> uint32_t {
-        uint32_t q;
-        int j;
-        unsigned long long bit;
-        if (!locate(v, q, j, bit, 2)) return kInv;
-        atomicOr(&visited[bit >> 6], 1ull << (bit & 63));
-        if (j >= 1) {
-            const unsigned long long r = rbase[q] + (uint32_t)j - 1;
-            const uint32_t e = r < b.R ? rec_node[r] : kInv;
-            const uint32_t l = e < b.D ? event_local[e] : kInv;
-            if (l >= task.ncls) {
-                fail_check(err, kDfsReplay << 8 | 5);
-                return kInv;
-            }
-            if (!(seen[l >> 5] & (1u << (l & 31)))) {
-                seen[l >> 5] |= 1u << (l & 31);
-                rank[e] = found;
-                ++found;
-            }
-        }
-        return found;
-    };
-
-    uint32_t found = 0;
-    if (lane == 0) found = visit(task.seed, 0);
-    found = (uint32_t)__shfl((int)found, 0);
-    if (found == kInv) return;
-    unsigned long long top = 0;
-    uint32_t u = task.seed;
-    int k = 27;
-    while (found < task.ncls) {
-        int x, y, z;
-        decode(u, X, Y, x, y, z);
-        const uint32_t cu = cr[u];
-        bool cand = false;
-        uint32_t j = 0;
-        if (lane < k && lane < 27 && in_stencil(lane, d3q7) && neighbour(lane, x, y, z, X, Y, Z, j)) {
-            const uint32_t cn = cr[j];
-            if (cn != 0 && cn <= cu) {
-                uint32_t q;
-                int m;
-                unsigned long long bit;
-                if (locate(j, q, m, bit, 6)) {
-                    const unsigned long long w =
-                        __hip_atomic_load(&visited[bit >> 6], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    cand = !(w & (1ull << (bit & 63)));
-                }
-            }
-        }
-        const unsigned long long bal = __ballot(cand);
-        if (!bal) {
-            if (top == 0) break;  // stack empty: the pore's reach is exhausted
-            --top;                // pop; the frame below is read back by lane 0
-            uint32_t fu = 0, fk = 0;
-            if (lane == 0) {
-                fu = sv[top];
-                fk = sk[top];
-            }
-            u = (uint32_t)__shfl((int)fu, 0);
-            k = __shfl((int)fk, 0);
-            if (u >= b.n || k > 27) {
-                if (lane == 0) fail_check(err, kDfsReplay << 8 | 9);
-                return;
-            }
-            continue;
-        }
-        if (top + 1 >= task.stack_cap) {  // no room for this frame: the host repeats the pore
-            if (lane == 0) task_status[task.slot] = 1u;
+    {
+        const uint32_t t = blockIdx.x;
+        const ReplayTask task = tasks[t];
+        const uint32_t p = task.pore;
+        const uint32_t nsw = (task.ncls + 31) / 32;
+        if (task.seen_off + nsw > b.seen_words || task.seed >= b.n) {
+            if (lane == 0) fail_check(err, kDfsReplay << 8 | 1);
             return;
         }
-        const int kk = 63 - __clzll((long long)bal);  // highest neighbour index below k
-        const uint32_t nvox = (uint32_t)__shfl((int)j, kk);
+        uint32_t* seen = seen_words + task.seen_off;
+        for (unsigned long long i = lane; i < vis_words; i += 64) vis[i] = 0;
+        for (unsigned long long i = lane; i < pent_words; i += 64) pent[i] = 0;
+        for (uint32_t i = lane; i < nsw; i += 64) seen[i] = 0;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");  // the cleared words are in L2 before the atomics
+        __builtin_amdgcn_wave_barrier();  // every lane has cleared its share before lane 0 marks the seed
+
+        // Lane 0: on the pore's first entry into plateau q, ranks q's class if it is new.
+        // Returns the new count of ranked classes, or kInv after a failed check.
+        auto enter = [&](uint32_t q, uint32_t found) -> uint32_t {
+            if (q >= b.P || set_off[q] + set_len[q] > b.pool_cap) {
+                fail_check(err, kDfsReplay << 8 | 2);
+                return kInv;
+            }
+            const unsigned long long pw = pent[q >> 6];
+            if (pw & (1ull << (q & 63))) return found;
+            pent[q >> 6] = pw | (1ull << (q & 63));
+            const int j = member_index(pool, set_off[q], set_len[q], p);
+            if (j < 0) {
+                fail_check(err, kDfsReplay << 8 | 3);  // p does not reach a plateau its DFS entered
+                return kInv;
+            }
+            if (j >= 1) {
+                const unsigned long long r = rbase[q] + (uint32_t)j - 1;
+                const uint32_t e = r < b.R ? rec_node[r] : kInv;
+                const uint32_t l = e < b.D ? event_local[e] : kInv;
+                if (l >= task.ncls) {
+                    fail_check(err, kDfsReplay << 8 | 4);
+                    return kInv;
+                }
+                if (!(seen[l >> 5] & (1u << (l & 31)))) {
+                    seen[l >> 5] |= 1u << (l & 31);
+                    rank[e] = found;
+                    ++found;
+                }
+            }
+            return found;
+        };
+
+        uint32_t u = task.seed;
+        uint32_t uq = plat[u];
+        uint32_t cu = cr[u];
+        uint32_t found = 0;
         if (lane == 0) {
-            sv[top] = u;  // save this frame, resuming below kk
-            sk[top] = (uint8_t)kk;
-            found = visit(nvox, found);
+            atomicOr(&vis[u >> 6], 1ull << (u & 63));
+            found = enter(uq, 0);
         }
-        found = (uint32_t)__shfl((int)found, 0);
+        found = bcast(found, 0);
         if (found == kInv) return;
-        ++top;
-        u = nvox;
-        k = 27;
+        unsigned long long top = 0;
+        int k = 27;
+        bool overflow = false;
+        while (found + 1 < task.ncls) {
+            int x, y, z;
+            decode(u, X, Y, x, y, z);
+            bool cand = false;
+            uint32_t j = 0, qn = 0, cn = 0;
+            if (lane < k && lane < 27 && in_stencil(lane, d3q7) && neighbour(lane, x, y, z, X, Y, Z, j)) {
+                cn = cr[j];
+                qn = plat[j];
+                const unsigned long long w =
+                    __hip_atomic_load(&vis[j >> 6], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                cand = cn != 0 && cn <= cu && !(w & (1ull << (j & 63)));
+            }
+            const unsigned long long bal = __ballot(cand);
+            if (!bal) {
+                if (top == 0) break;  // stack empty: the pore's reach is exhausted
+                --top;                // pop; the frame below is read back by lane 0
+                uint32_t fu = 0, fk = 0;
+                if (lane == 0) {
+                    fu = sv[top];
+                    fk = sk[top];
+                }
+                u = bcast(fu, 0);
+                k = (int)bcast(fk, 0);
+                if (u >= b.n || k > 27) {
+                    if (lane == 0) fail_check(err, kDfsReplay << 8 | 5);
+                    return;
+                }
+                uq = plat[u];
+                cu = cr[u];
+                continue;
+            }
+            if (top + 1 >= stack_cap) {  // no room for this frame: the host repeats the pore
+                overflow = true;
+                break;
+            }
+            const int kk = 63 - __clzll((long long)bal);  // highest neighbour index below k
+            const uint32_t nvox = bcast(j, kk);
+            const uint32_t nq = bcast(qn, kk);
+            const uint32_t ncr = bcast(cn, kk);
+            if (lane == 0) {
+                sv[top] = u;  // save this frame, resuming below kk
+                sk[top] = (uint8_t)kk;
+                atomicOr(&vis[nvox >> 6], 1ull << (nvox & 63));
+                if (nq != uq) found = enter(nq, found);
+            }
+            found = bcast(found, 0);
+            if (found == kInv) return;
+            ++top;
+            u = nvox;
+            uq = nq;
+            cu = ncr;
+            k = 27;
+        }
+        if (lane == 0) {
+            if (overflow) task_status[t] = 1u;
+            else if (found + 1 < task.ncls) fail_check(err, kDfsReplay << 8 | 6);  // reach exhausted too early
+        }
     }
-    if (lane == 0 && found < task.ncls) fail_check(err, kDfsReplay << 8 | 10);  // reach exhausted too early
 }
 
 // ---- labels ------------------------------------------------------------------------------------
@@ -1048,20 +1057,15 @@ void run_pores(pma_ctx* c) {
         ev_local[d] = ncls[ev_owner[d]]++;
     }
 
-    // reach sizes, boundary flags and the visited-bitmap layout (one bit per voxel and member)
+    // reach sizes and boundary flags
     unsigned long long* reach = c->buf<unsigned long long>("pt_reach", std::max(npores, 1u));
     uint32_t* bflag = c->buf<uint32_t>("pt_bflag", std::max(npores, 1u));
-    unsigned long long* plat_bits = c->buf<unsigned long long>("pt_plat_bits", P1);
-    unsigned long long* bit_base = c->buf<unsigned long long>("pt_bit_base", P1);
     PMA_HIP(hipMemsetAsync(reach, 0, std::max(npores, 1u) * 8ull, c->stream));
     PMA_HIP(hipMemsetAsync(bflag, 0, std::max(npores, 1u) * 4ull, c->stream));
-    uint64_t total_bits = 0;
     if (P) {
         reach_sizes<<<grid_for(P, 256), 256, 0, c->stream>>>(set_off, set_len, pool, plat_size, plat_border, sb, reach,
-                                                             bflag, plat_bits, err);
+                                                             bflag, err);
         PMA_CHECKED(c, "reach_sizes", err);
-        total_bits = exclusive_scan_u64(c, reinterpret_cast<const uint64_t*>(plat_bits),
-                                        reinterpret_cast<uint64_t*>(bit_base), P);
     }
     check_stage(c, err);
     std::vector<unsigned long long> reach_h(npores);
@@ -1075,8 +1079,8 @@ void run_pores(pma_ctx* c) {
         PMA_REQUIRE(reach_h[k] >= 1 && reach_h[k] <= n, PMA_ERR_INTERNAL, "pma_pores: pore reach out of range");
 
     // 8. DFS replays for pores that form two or more lists: rank of each list inside its pore.
-    std::vector<uint32_t> rank_h(D, 0);
-    uint64_t replay_frames = 0;
+    std::vector<uint32_t> rank_h(D, kInv);
+    uint64_t replay_tasks = 0;
     {
         std::vector<ReplayTask> all;
         unsigned long long seen_total = 0;
@@ -1090,72 +1094,78 @@ void run_pores(pma_ctx* c) {
             seen_total += (ncls[k] + 31) / 32;
             all.push_back(t);
         }
+        // largest reach first, so that the longest replays start at once
+        std::stable_sort(all.begin(), all.end(),
+                         [&](const ReplayTask& a, const ReplayTask& b2) { return reach_h[a.pore] > reach_h[b2.pore]; });
+        replay_tasks = all.size();
         if (!all.empty()) {
             const size_t T = all.size();
             uint32_t* d_local = c->buf<uint32_t>("pt_ev_local", std::max(D, 1u));
             uint32_t* d_rank = c->buf<uint32_t>("pt_rank", std::max(D, 1u));
             uint32_t* seen = c->buf<uint32_t>("pt_seen", seen_total);
-            const size_t vwords = total_bits / 64 + 1;
-            unsigned long long* visited = c->buf<unsigned long long>("pt_visited", vwords);
             uint32_t* status = c->buf<uint32_t>("pt_task_status", T);
             ReplayTask* d_tasks = c->buf<ReplayTask>("pt_tasks", T);
             upload(c, d_local, ev_local.data(), D * 4ull);
-            PMA_HIP(hipMemsetAsync(d_rank, 0, D * 4ull, c->stream));
-            // Stack frames: a first pass gives every task an equal share of a fixed budget;
-            // tasks that run out are repeated with room for their whole reach.
-            const unsigned long long budget = std::max<unsigned long long>(1ull << 26, n);
-            const unsigned long long share = std::max<unsigned long long>(1024, budget / T);
+            PMA_HIP(hipMemsetAsync(d_rank, 0xFF, D * 4ull, c->stream));
+            // Per wave: a visited bitmap over all voxels, a bitmap of entered plateaus and a stack.
+            // The number of waves follows from the device memory free for them.
+            const unsigned long long vis_words = n / 64 + 1, pent_words = P / 64 + 1;
+            unsigned long long max_reach = 0;
+            for (const ReplayTask& t : all) max_reach = std::max<unsigned long long>(max_reach, reach_h[t.pore]);
+            size_t free_b = 0, total_b = 0;
+            PMA_HIP(hipMemGetInfo(&free_b, &total_b));
+            const unsigned long long budget =
+                free_b > (24ull << 30) ? std::min<unsigned long long>(free_b - (16ull << 30), 64ull << 30) : free_b / 2;
             std::vector<ReplayTask> pending = all;
             for (int pass = 0; !pending.empty(); ++pass) {
                 PMA_REQUIRE(pass < 2, PMA_ERR_INTERNAL, "pma_pores: DFS replay stack overflow with full capacity");
-                PMA_HIP(hipMemsetAsync(visited, 0, vwords * 8, c->stream));
-                PMA_HIP(hipMemsetAsync(seen, 0, seen_total * 4, c->stream));
-                std::vector<ReplayTask> failed;
-                size_t first = 0;
-                while (first < pending.size()) {
-                    // one batch: as many tasks as fit the budget (at least one)
-                    unsigned long long frames = 0;
-                    size_t last = first;
-                    std::vector<ReplayTask> batch;
-                    while (last < pending.size()) {
-                        ReplayTask t = pending[last];
-                        const unsigned long long need =
-                            pass == 0 ? std::min<unsigned long long>(reach_h[t.pore] + 1, share) : reach_h[t.pore] + 1;
-                        if (!batch.empty() && frames + need > budget && pass > 0) break;
-                        t.stack_off = frames;
-                        t.stack_cap = need;
-                        t.slot = (uint32_t)batch.size();
-                        frames += need;
-                        batch.push_back(t);
-                        ++last;
-                    }
-                    uint32_t* stack_voxel = c->buf<uint32_t>("pt_stack_voxel", frames);
-                    uint8_t* stack_k = c->buf<uint8_t>("pt_stack_k", frames);
-                    upload(c, d_tasks, batch.data(), batch.size() * sizeof(ReplayTask));
-                    PMA_HIP(hipMemsetAsync(status, 0, batch.size() * 4, c->stream));
-                    const ReplayBounds rb{n, P, pool_cap, R, D, (unsigned long long)vwords * 64, frames, seen_total};
-                    {
-                        PMA_TIMED(c, "dfs_replay");
-                        dfs_replay<<<(unsigned)batch.size(), 64, 0, c->stream>>>(
-                            d_tasks, cr, plat, set_off, set_len, pool, rbase, rd, d_local, bit_base, vrank, visited,
-                            seen, stack_voxel, stack_k, d_rank, status, X, Y, Z, d3q7, rb, err);
-                        PMA_CHECKED(c, "dfs_replay", err);
-                    }
-                    check_stage(c, err);
-                    std::vector<uint32_t> st(batch.size());
-                    PMA_HIP(hipMemcpyAsync(st.data(), status, batch.size() * 4ull, hipMemcpyDeviceToHost, c->stream));
-                    PMA_HIP(hipStreamSynchronize(c->stream));
-                    for (size_t i = 0; i < batch.size(); ++i)
-                        if (st[i]) failed.push_back(batch[i]);
-                    replay_frames += frames;
-                    first = last;
+                // pass 0: stacks of up to PMA_PORES_STACK_CAP frames; pass 1: room for the whole reach
+                const unsigned long long cap =
+                    pass == 0 ? std::min<unsigned long long>(max_reach + 2, PMA_PORES_STACK_CAP) : max_reach + 2;
+                const unsigned long long per_wave = (vis_words + pent_words) * 8 + cap * 5;
+                const unsigned long long W =
+                    std::max<unsigned long long>(1, std::min<unsigned long long>({pending.size(), budget / per_wave, 1024}));
+                unsigned long long* vis = c->buf<unsigned long long>("pt_replay_vis", W * vis_words);
+                unsigned long long* pent = c->buf<unsigned long long>("pt_replay_plateaus", W * pent_words);
+                uint32_t* stack_voxel = c->buf<uint32_t>("pt_stack_voxel", W * cap);
+                uint8_t* stack_k = c->buf<uint8_t>("pt_stack_k", W * cap);
+                upload(c, d_tasks, pending.data(), pending.size() * sizeof(ReplayTask));
+                PMA_HIP(hipMemsetAsync(status, 0, pending.size() * 4, c->stream));
+                const ReplayBounds rb{n, P, pool_cap, R, D, seen_total};
+                // batches of W pores, largest reach first: each launch waits for its longest replay
+                for (size_t first = 0; first < pending.size(); first += W) {
+                    const size_t cnt = std::min<size_t>(W, pending.size() - first);
+                    PMA_TIMED(c, "dfs_replay");
+                    dfs_replay<<<(unsigned)cnt, 64, 0, c->stream>>>(
+                        d_tasks + first, cr, plat, set_off, set_len, pool, rbase, rd, d_local, vis, vis_words, pent,
+                        pent_words, seen, stack_voxel, stack_k, cap, d_rank, status + first, X, Y, Z, d3q7, rb, err);
+                    PMA_CHECKED(c, "dfs_replay", err);
                 }
+                check_stage(c, err);
+                std::vector<uint32_t> st(pending.size());
+                PMA_HIP(hipMemcpyAsync(st.data(), status, pending.size() * 4ull, hipMemcpyDeviceToHost, c->stream));
+                PMA_HIP(hipStreamSynchronize(c->stream));
+                std::vector<ReplayTask> failed;
+                for (size_t i = 0; i < pending.size(); ++i)
+                    if (st[i]) failed.push_back(pending[i]);
                 pending.swap(failed);
             }
             PMA_HIP(hipMemcpyAsync(rank_h.data(), d_rank, D * 4ull, hipMemcpyDeviceToHost, c->stream));
             PMA_HIP(hipStreamSynchronize(c->stream));
+            // the per-slot bitmaps and stacks are sized by the free memory: give it back
+            for (const char* name : {"pt_replay_vis", "pt_replay_plateaus", "pt_stack_voxel", "pt_stack_k"}) c->drop(name);
         }
-        c->stage[PMA_STAGE_PORES].work_items = replay_frames;
+        c->stage[PMA_STAGE_PORES].work_items = replay_tasks;
+    }
+    // A replay stops with one class of its pore unranked: that class comes last.  Pores with
+    // one class rank it 0.
+    {
+        std::vector<uint32_t> unranked(npores, 0);
+        for (uint32_t d = 0; d < D; ++d)
+            if (rank_h[d] == kInv) {
+                PMA_REQUIRE(++unranked[ev_owner[d]] == 1, PMA_ERR_INTERNAL, "pma_pores: a replay left two classes unranked");
+                rank_h[d] = ncls[ev_owner[d]] - 1;
+            }
     }
     // creation order: by pore, then by first visit inside the pore's DFS
     std::vector<uint32_t> ord(D);

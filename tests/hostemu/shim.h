@@ -393,6 +393,11 @@ inline hipError_t hipGetLastError() { return hipSuccess; }
 inline const char* hipGetErrorString(hipError_t) { return "no error"; }
 #define HIP_SYMBOL(x) ((void*)&(x))
 inline hipError_t hipGetSymbolAddress(void** p, void* sym) { *p = sym; return hipSuccess; }
+inline hipError_t hipMemGetInfo(size_t* free_b, size_t* total_b) {
+    *free_b = (size_t)1 << 30;
+    *total_b = (size_t)1 << 31;
+    return hipSuccess;
+}
 struct StageInfo { uint64_t work_items = 0; };
 namespace pma {
 struct Error : std::runtime_error {
@@ -469,6 +474,12 @@ struct pma_ctx {
         return b.first;
     }
     template <class T> T* buf(const std::string& name, size_t count) { return (T*)buf(name, count * sizeof(T)); }
+    void drop(const std::string& name) {
+        auto it = bufs.find(name);
+        if (it == bufs.end()) return;
+        std::free(it->second.first);
+        bufs.erase(it);
+    }
     void* host_staging(size_t bytes) { return buf("host_staging", bytes); }
 };
 

@@ -91,12 +91,13 @@ def hostemu(tmp_path_factory):
 def hostemu_slabs(tmp_path_factory):
     """Sanitized build with small work limits: containing-ball pair lists cut into slabs of at most
     4000 pairs, 32-bit winner slots, ranks (not packed balls) in the pair sort, far tables read
-    from global memory by the centre prune, and geodesic relaxation on 2 waves with 32-entry push
-    buffers."""
+    from global memory by the centre prune, geodesic relaxation on 2 waves with 32-entry push
+    buffers, and DFS replay stacks of 64 frames (so replays overflow and are repeated)."""
     return _build(tmp_path_factory, SANITIZED + ["-DPMA_CB_SLAB_PAIRS=4000ull", "-DPMA_CB_WIN16_MAX=0u",
                                                  "-DPMA_CB_GZ=1", "-DPMA_CB_GX=3", "-DPMA_CB_SMALL_R2=0u",
                                                  "-DPMA_GEO_RELAX_GRID=2u", "-DPMA_GEO_PUSH_BUF=32u",
-                                                 "-DPMA_CB_PACKED=0", "-DPMA_CB_PRUNE_FAR_LDS=0u"], "slabs")
+                                                 "-DPMA_CB_PACKED=0", "-DPMA_CB_PRUNE_FAR_LDS=0u",
+                                                 "-DPMA_PORES_STACK_CAP=64ull"], "slabs")
 
 
 @pytest.fixture(scope="session")
@@ -286,6 +287,14 @@ def test_containing_balls_in_slabs(hostemu_slabs, tmp_path):
     for k in range(5):
         assert np.array_equal(g[k], o[k])
     assert g[5] == o[5]
+
+
+@pytest.mark.parametrize("name", ["gauss_s2", "gauss_s3_low", "noise_many_pores", "spheres_48"])
+def test_pores_small_replay_stacks(hostemu_slabs, tmp_path, name):
+    """DFS replay stacks of 64 frames: replays overflow, are repeated with room for their whole
+    reach, and the labels and tables are still the oracle's."""
+    e, max_e, cr, mask = front(CASES[name])
+    assert_same(emu_pores(hostemu_slabs, tmp_path, e, cr, mask, False), oracle.pores(mask, e, cr, False), "pores")
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
