@@ -597,17 +597,20 @@ __device__ __forceinline__ uint32_t bcast(uint32_t v, int src) {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, src);
 }
 
-// One workgroup (one wave) per pore of a batch; workgroup b uses slot b of the per-slot buffers:
-// a visited bitmap over all voxels and a bitmap of entered plateaus, both cleared first, and a
-// stack.  (There is no loop over pores inside the kernel: with one, the compiler let lanes 1..63
-// start the next pore while lane 0, which owns the stack, was still fetching it.)  Lanes 0..26 test the 26 neighbours of the current voxel with independent loads
+// A pool of waves (workgroup b uses slot b of the per-slot buffers: a visited bitmap over all
+// voxels and a bitmap of entered plateaus, both cleared per pore, and a stack) takes pores from a
+// queue, largest reach first.  The code at the boundary of the loop over pores is executed by
+// every lane (the fetch adds 1 in lane 0 and 0 elsewhere, the last lane's value is read back):
+// with lane-0-only blocks there the compiler merged the end of one pore with the fetch of the
+// next and let lanes 1..63 start a pore while lane 0, which owns the stack, was still fetching.  Lanes 0..26 test the 26 neighbours of the current voxel with independent loads
 // (radius, plateau, visited word); lane 0 owns the stack and the counters.  Frames are (voxel,
 // next neighbour index); a recursive DFS that tries neighbours in reverse loop order visits
 // voxels in the reference's stack-DFS order.  A voxel's earlier-pore class depends only on its
 // plateau, so the class is looked up on the pore's first entry into each plateau.  The replay
 // of a pore stops once all but one of its classes are ranked: the last one is necessarily
 // found after them.
-__global__ void __launch_bounds__(64) dfs_replay(const ReplayTask* __restrict__ tasks, const uint32_t* __restrict__ cr,
+__global__ void __launch_bounds__(64) dfs_replay(const ReplayTask* __restrict__ tasks, uint32_t T,
+                                                 uint32_t* __restrict__ queue, const uint32_t* __restrict__ cr,
                                                  const uint32_t* __restrict__ plat,
                                                  const unsigned long long* __restrict__ set_off,
                                                  const uint32_t* __restrict__ set_len, const uint32_t* __restrict__ pool,
@@ -618,16 +621,20 @@ __global__ void __launch_bounds__(64) dfs_replay(const ReplayTask* __restrict__ 
                                                  unsigned long long pent_words, uint32_t* __restrict__ seen_words,
                                                  uint32_t* __restrict__ stack_all, uint8_t* __restrict__ stack_k_all,
                                                  unsigned long long stack_cap, uint32_t* __restrict__ rank,
-                                                 uint32_t* __restrict__ task_status, int X, int Y, int Z, int d3q7,
-                                                 ReplayBounds b, uint32_t* __restrict__ err) {
+                                                 uint32_t* __restrict__ task_status,
+                                                 unsigned long long* __restrict__ task_steps, int X, int Y, int Z,
+                                                 int d3q7, ReplayBounds b, uint32_t* __restrict__ err) {
     const int lane = threadIdx.x;
     unsigned long long* vis = vis_all + (size_t)blockIdx.x * vis_words;
     unsigned long long* pent = pent_all + (size_t)blockIdx.x * pent_words;
     uint32_t* sv = stack_all + (size_t)blockIdx.x * stack_cap;
     uint8_t* sk = stack_k_all + (size_t)blockIdx.x * stack_cap;
 
-    {
-        const uint32_t t = blockIdx.x;
+    while (true) {
+        // every lane adds (lane 0 adds 1): the returned values are the old count and old count + 1
+        const uint32_t mine = atomicAdd(queue, lane == 0 ? 1u : 0u);
+        const uint32_t t = bcast(mine, 0) == bcast(mine, 63) ? bcast(mine, 0) : bcast(mine, 63) - 1;
+        if (t >= T) return;
         const ReplayTask task = tasks[t];
         const uint32_t p = task.pore;
         const uint32_t nsw = (task.ncls + 31) / 32;
@@ -684,7 +691,7 @@ __global__ void __launch_bounds__(64) dfs_replay(const ReplayTask* __restrict__ 
         }
         found = bcast(found, 0);
         if (found == kInv) return;
-        unsigned long long top = 0;
+        unsigned long long top = 0, steps = 0;
         int k = 27;
         bool overflow = false;
         while (found + 1 < task.ncls) {
@@ -735,15 +742,16 @@ __global__ void __launch_bounds__(64) dfs_replay(const ReplayTask* __restrict__ 
             found = bcast(found, 0);
             if (found == kInv) return;
             ++top;
+            ++steps;
             u = nvox;
             uq = nq;
             cu = ncr;
             k = 27;
         }
-        if (lane == 0) {
-            if (overflow) task_status[t] = 1u;
-            else if (found + 1 < task.ncls) fail_check(err, kDfsReplay << 8 | 6);  // reach exhausted too early
-        }
+        // every lane stores the same values
+        task_steps[t] = steps;
+        task_status[t] = overflow ? 1u : 0u;
+        if (!overflow && found + 1 < task.ncls) fail_check(err, kDfsReplay << 8 | 6);  // reach exhausted too early
     }
 }
 
@@ -1080,7 +1088,7 @@ void run_pores(pma_ctx* c) {
 
     // 8. DFS replays for pores that form two or more lists: rank of each list inside its pore.
     std::vector<uint32_t> rank_h(D, kInv);
-    uint64_t replay_tasks = 0;
+    uint64_t replay_tasks = 0, replay_steps = 0;
     {
         std::vector<ReplayTask> all;
         unsigned long long seen_total = 0;
@@ -1104,6 +1112,8 @@ void run_pores(pma_ctx* c) {
             uint32_t* d_rank = c->buf<uint32_t>("pt_rank", std::max(D, 1u));
             uint32_t* seen = c->buf<uint32_t>("pt_seen", seen_total);
             uint32_t* status = c->buf<uint32_t>("pt_task_status", T);
+            unsigned long long* steps = c->buf<unsigned long long>("pt_task_steps", T);
+            uint32_t* queue = c->buf<uint32_t>("pt_queue", 1);
             ReplayTask* d_tasks = c->buf<ReplayTask>("pt_tasks", T);
             upload(c, d_local, ev_local.data(), D * 4ull);
             PMA_HIP(hipMemsetAsync(d_rank, 0xFF, D * 4ull, c->stream));
@@ -1132,19 +1142,33 @@ void run_pores(pma_ctx* c) {
                 upload(c, d_tasks, pending.data(), pending.size() * sizeof(ReplayTask));
                 PMA_HIP(hipMemsetAsync(status, 0, pending.size() * 4, c->stream));
                 const ReplayBounds rb{n, P, pool_cap, R, D, seen_total};
-                // batches of W pores, largest reach first: each launch waits for its longest replay
-                for (size_t first = 0; first < pending.size(); first += W) {
-                    const size_t cnt = std::min<size_t>(W, pending.size() - first);
+                PMA_HIP(hipMemsetAsync(queue, 0, 4, c->stream));
+                const auto t0 = std::chrono::steady_clock::now();
+                {
                     PMA_TIMED(c, "dfs_replay");
-                    dfs_replay<<<(unsigned)cnt, 64, 0, c->stream>>>(
-                        d_tasks + first, cr, plat, set_off, set_len, pool, rbase, rd, d_local, vis, vis_words, pent,
-                        pent_words, seen, stack_voxel, stack_k, cap, d_rank, status + first, X, Y, Z, d3q7, rb, err);
+                    dfs_replay<<<(unsigned)W, 64, 0, c->stream>>>(
+                        d_tasks, (uint32_t)pending.size(), queue, cr, plat, set_off, set_len, pool, rbase, rd, d_local,
+                        vis, vis_words, pent, pent_words, seen, stack_voxel, stack_k, cap, d_rank, status, steps, X, Y,
+                        Z, d3q7, rb, err);
                     PMA_CHECKED(c, "dfs_replay", err);
                 }
+                if (c->debug_sync)
+                    std::fprintf(stderr, "[pma] dfs_replay pass %d on %llu waves: %.1f ms\n", pass, W,
+                                 std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
                 check_stage(c, err);
                 std::vector<uint32_t> st(pending.size());
+                std::vector<unsigned long long> sp(pending.size());
                 PMA_HIP(hipMemcpyAsync(st.data(), status, pending.size() * 4ull, hipMemcpyDeviceToHost, c->stream));
+                PMA_HIP(hipMemcpyAsync(sp.data(), steps, pending.size() * 8ull, hipMemcpyDeviceToHost, c->stream));
                 PMA_HIP(hipStreamSynchronize(c->stream));
+                unsigned long long smax = 0;
+                for (size_t i = 0; i < pending.size(); ++i) {
+                    replay_steps += sp[i];
+                    smax = std::max(smax, sp[i]);
+                }
+                if (c->debug_sync)
+                    std::fprintf(stderr, "[pma] dfs_replay pass %d: %zu pores, %llu steps, longest %llu\n", pass,
+                                 pending.size(), (unsigned long long)replay_steps, smax);
                 std::vector<ReplayTask> failed;
                 for (size_t i = 0; i < pending.size(); ++i)
                     if (st[i]) failed.push_back(pending[i]);
@@ -1155,7 +1179,8 @@ void run_pores(pma_ctx* c) {
             // the per-slot bitmaps and stacks are sized by the free memory: give it back
             for (const char* name : {"pt_replay_vis", "pt_replay_plateaus", "pt_stack_voxel", "pt_stack_k"}) c->drop(name);
         }
-        c->stage[PMA_STAGE_PORES].work_items = replay_tasks;
+        c->stage[PMA_STAGE_PORES].work_items = replay_steps;
+        (void)replay_tasks;
     }
     // A replay stops with one class of its pore unranked: that class comes last.  Pores with
     // one class rank it 0.
