@@ -656,9 +656,8 @@ __global__ void __launch_bounds__(64) dfs_replay(const ReplayTask* __restrict__ 
                 fail_check(err, kDfsReplay << 8 | 2);
                 return kInv;
             }
-            const unsigned long long pw = pent[q >> 6];
+            const unsigned long long pw = atomicOr(&pent[q >> 6], 1ull << (q & 63));
             if (pw & (1ull << (q & 63))) return found;
-            pent[q >> 6] = pw | (1ull << (q & 63));
             const int j = member_index(pool, set_off[q], set_len[q], p);
             if (j < 0) {
                 fail_check(err, kDfsReplay << 8 | 3);  // p does not reach a plateau its DFS entered
@@ -697,7 +696,7 @@ __global__ void __launch_bounds__(64) dfs_replay(const ReplayTask* __restrict__ 
         while (found + 1 < task.ncls) {
             int x, y, z;
             decode(u, X, Y, x, y, z);
-            bool cand = false;
+            bool cand = false, entered = true;
             uint32_t j = 0, qn = 0, cn = 0;
             if (lane < k && lane < 27 && in_stencil(lane, d3q7) && neighbour(lane, x, y, z, X, Y, Z, j)) {
                 cn = cr[j];
@@ -705,6 +704,11 @@ __global__ void __launch_bounds__(64) dfs_replay(const ReplayTask* __restrict__ 
                 const unsigned long long w =
                     __hip_atomic_load(&vis[j >> 6], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 cand = cn != 0 && cn <= cu && !(w & (1ull << (j & 63)));
+                // the entered-plateau test, for the lane whose neighbour is taken
+                // (an index out of range reads as not entered, so that enter() reports it)
+                if (cand && qn != uq)
+                    entered = qn < b.P && ((__hip_atomic_load(&pent[qn >> 6], __ATOMIC_RELAXED,
+                                                              __HIP_MEMORY_SCOPE_AGENT) >> (qn & 63)) & 1ull);
             }
             const unsigned long long bal = __ballot(cand);
             if (!bal) {
@@ -733,14 +737,17 @@ __global__ void __launch_bounds__(64) dfs_replay(const ReplayTask* __restrict__ 
             const uint32_t nvox = bcast(j, kk);
             const uint32_t nq = bcast(qn, kk);
             const uint32_t ncr = bcast(cn, kk);
+            const bool is_new = !bcast(entered ? 1u : 0u, kk);  // first entry into plateau nq
             if (lane == 0) {
                 sv[top] = u;  // save this frame, resuming below kk
                 sk[top] = (uint8_t)kk;
                 atomicOr(&vis[nvox >> 6], 1ull << (nvox & 63));
-                if (nq != uq) found = enter(nq, found);
+                if (is_new) found = enter(nq, found);
             }
-            found = bcast(found, 0);
-            if (found == kInv) return;
+            if (is_new) {
+                found = bcast(found, 0);
+                if (found == kInv) return;
+            }
             ++top;
             ++steps;
             u = nvox;
@@ -1125,7 +1132,7 @@ void run_pores(pma_ctx* c) {
             size_t free_b = 0, total_b = 0;
             PMA_HIP(hipMemGetInfo(&free_b, &total_b));
             const unsigned long long budget =
-                free_b > (24ull << 30) ? std::min<unsigned long long>(free_b - (16ull << 30), 64ull << 30) : free_b / 2;
+                free_b > (40ull << 30) ? std::min<unsigned long long>(free_b - (24ull << 30), 64ull << 30) : free_b / 2;
             std::vector<ReplayTask> pending = all;
             for (int pass = 0; !pending.empty(); ++pass) {
                 PMA_REQUIRE(pass < 2, PMA_ERR_INTERNAL, "pma_pores: DFS replay stack overflow with full capacity");
