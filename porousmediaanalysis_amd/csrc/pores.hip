@@ -45,18 +45,24 @@ constexpr uint32_t kInv = 0xFFFFFFFFu;
 #ifndef PMA_PORES_STACK_CAP
 #define PMA_PORES_STACK_CAP (1ull << 24)  // DFS frames per replaying wave in the first pass
 #endif
+#ifndef PMA_PORES_TRAV_CAP
+#define PMA_PORES_TRAV_CAP (1ull << 22)  // plateaus one skip traversal may list (then it gives up)
+#endif
+#ifndef PMA_UNION_LDS_MAX
+#define PMA_UNION_LDS_MAX 65536u  // LDS bytes a workgroup of the bitmap set union may use
+#endif
 
 // Kernel ids of the in-kernel checks (the code's high bits).
 enum : uint32_t {
     kWriteCentres = 1, kCcUnion, kCcFlatten, kCcNotMax, kFirstInSorted, kSeedFlags, kWriteSeeds, kPlateauIndex,
     kPlateauOfVoxels, kSeedPlateaus, kPlateauEdges, kWriteEdges, kEdgeRanges, kLevelStarts, kSetUnion,
-    kReachSizes, kDfsReplay, kWriteLabels, kSingleVolumes
+    kReachSizes, kDfsReplay, kWriteLabels, kSingleVolumes, kBelowMin, kClassEvents
 };
 const char* const kKernelNames[] = {"?", "write_centres", "cc_union", "cc_flatten", "cc_not_max",
                                     "first_in_sorted", "seed_flags", "write_seeds", "plateau_index",
                                     "plateau_of_voxels", "seed_plateaus", "plateau_edges", "write_edges",
                                     "edge_ranges", "level_starts", "set_union_level", "reach_sizes",
-                                    "dfs_replay", "write_labels", "single_volumes"};
+                                    "dfs_replay", "write_labels", "single_volumes", "below_min", "class_events"};
 
 __device__ __forceinline__ uint32_t ld_relaxed(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -538,6 +544,187 @@ __global__ void set_union_level(const uint32_t* __restrict__ order, uint32_t lo,
     set_len[q] = m;
 }
 
+// The same sets with one wave per plateau: the members of the sets of the plateau's in-edges
+// (lane per in-edge) and its own pore are or-ed into a bitmap over pores in LDS, then written
+// in ascending order from the bitmap (lane per run of words, offsets by a wave prefix sum).
+// Work per plateau is the total length of its inputs plus the bitmap, not their product with the
+// in-degree as in the merge.  Waves use only their own LDS slice (no workgroup barrier).
+constexpr int kUnionWaves = 4;
+__global__ void __launch_bounds__(64 * kUnionWaves)
+    set_union_bitmap(const uint32_t* __restrict__ order, uint32_t lo, uint32_t hi,
+                     const uint32_t* __restrict__ in_start, const uint32_t* __restrict__ in_count,
+                     const unsigned long long* __restrict__ edges, const uint32_t* __restrict__ plat_pore,
+                     unsigned long long* __restrict__ set_off, uint32_t* __restrict__ set_len,
+                     uint32_t* __restrict__ pool, unsigned long long* __restrict__ pool_top, SetBounds b,
+                     uint32_t words, uint32_t* __restrict__ overflow, uint32_t* __restrict__ err) {
+    extern __shared__ uint32_t union_bits[];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t* bm = union_bits + (size_t)wave * words;
+    const uint32_t s = lo + blockIdx.x * kUnionWaves + wave;
+    if (s >= hi) return;  // the whole wave leaves
+    const uint32_t q = order[s];
+    if (q >= b.P) {
+        if (lane == 0) fail_check(err, kSetUnion << 8 | 1);
+        return;
+    }
+    const uint32_t own = plat_pore[q];
+    const uint32_t e0 = in_start[q], ne = in_count[q];
+    if ((own != kInv && own >= b.npores) || (unsigned long long)e0 + ne > b.E) {
+        if (lane == 0) fail_check(err, kSetUnion << 8 | 2);
+        return;
+    }
+    for (uint32_t w = lane; w < words; w += 64) bm[w] = 0;
+    __builtin_amdgcn_wave_barrier();
+    bool bad = false;
+    if (lane == 0 && own != kInv) atomicOr(&bm[own >> 5], 1u << (own & 31));
+    for (uint32_t e = e0 + lane; e < e0 + ne; e += 64) {
+        const uint32_t src = (uint32_t)edges[e];
+        if (src >= b.P || set_off[src] + set_len[src] > b.pool_cap) {
+            bad = true;
+            break;
+        }
+        const unsigned long long off = set_off[src];
+        const uint32_t len = set_len[src];
+        for (uint32_t t = 0; t < len; ++t) {
+            const uint32_t v = pool[off + t];
+            if (v >= b.npores) {
+                bad = true;
+                break;
+            }
+            atomicOr(&bm[v >> 5], 1u << (v & 31));
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (__ballot(bad)) {
+        if (lane == 0) fail_check(err, kSetUnion << 8 | 3);
+        return;
+    }
+    // lane l owns words [l * per, (l + 1) * per)
+    const uint32_t per = (words + 63) / 64;
+    const uint32_t w0 = min(words, lane * per), w1 = min(words, w0 + per);
+    uint32_t cnt = 0;
+    for (uint32_t w = w0; w < w1; ++w) cnt += __popc(bm[w]);
+    uint32_t incl = cnt;
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(incl, d);
+        if (lane >= d) incl += o;
+    }
+    const uint32_t total = __shfl(incl, 63);
+    unsigned long long o = 0;
+    if (lane == 0 && total) o = atomicAdd(pool_top, (unsigned long long)total);
+    o = __shfl(o, 0);
+    if (o + total > b.pool_cap) {
+        if (lane == 0) {
+            atomicOr(overflow, 2u);
+            set_off[q] = 0;
+            set_len[q] = 0;
+        }
+        return;
+    }
+    unsigned long long at = o + incl - cnt;
+    for (uint32_t w = w0; w < w1; ++w) {
+        uint32_t bits = bm[w];
+        while (bits) {
+            const int k = __ffs(bits) - 1;
+            bits &= bits - 1;
+            pool[at++] = w * 32 + k;
+        }
+    }
+    if (lane == 0) {
+        set_off[q] = total ? o : 0;
+        set_len[q] = total;
+    }
+}
+
+// ---- replay skip data ----------------------------------------------------------------------
+// Edge words with their halves swapped: the dst-major words become src-major, so that a sort
+// groups a plateau's out-edges (to lower plateaus).
+__global__ void swap_halves(const unsigned long long* __restrict__ in, size_t E, unsigned long long* __restrict__ out) {
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < E) out[e] = (in[e] << 32) | (in[e] >> 32);
+}
+
+// below[q] = the smallest pore that reaches some plateau below q (q included) without reaching q,
+// or kInv.  Sets only grow along an edge, so for a pore p of set(q) with below[q] > p, every plateau
+// below q has the same pores smaller than p as q: all of them hold q's class of p.  Computed in
+// ascending CR (a level's out-edges lead to levels already done): below[q] is the minimum over
+// its out-edges d of below[d] and the first member of set(d) that set(q) lacks.  One wave per
+// plateau, lane per out-edge.
+__global__ void __launch_bounds__(256) below_min(const uint32_t* __restrict__ order, uint32_t lo, uint32_t hi,
+                                                 const uint32_t* __restrict__ out_start,
+                                                 const uint32_t* __restrict__ out_count,
+                                                 const unsigned long long* __restrict__ oedges,
+                                                 const unsigned long long* __restrict__ set_off,
+                                                 const uint32_t* __restrict__ set_len, const uint32_t* __restrict__ pool,
+                                                 SetBounds b, uint32_t* __restrict__ below, uint32_t* __restrict__ err) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t s = lo + blockIdx.x * 4 + wave;
+    if (s >= hi) return;
+    const uint32_t q = order[s];
+    const bool okq = q < b.P && set_off[q] + set_len[q] <= b.pool_cap && out_start[q] + (unsigned long long)out_count[q] <= b.E;
+    if (!okq) {
+        if (lane == 0) fail_check(err, kBelowMin << 8 | 1);
+        return;
+    }
+    const unsigned long long offq = set_off[q];
+    const uint32_t lenq = set_len[q];
+    const uint32_t e0 = out_start[q], ne = out_count[q];
+    uint32_t m = kInv;
+    bool bad = false;
+    for (uint32_t e = e0 + lane; e < e0 + ne; e += 64) {
+        const uint32_t d = (uint32_t)oedges[e];
+        if (d >= b.P || set_off[d] + set_len[d] > b.pool_cap) {
+            bad = true;
+            break;
+        }
+        m = min(m, below[d]);
+        const unsigned long long offd = set_off[d];
+        const uint32_t lend = set_len[d];
+        uint32_t i = 0;
+        for (uint32_t t = 0; t < lend; ++t) {  // set(d) contains set(q): the first mismatch is the first extra
+            const uint32_t v = pool[offd + t];
+            if (i < lenq && pool[offq + i] == v) {
+                ++i;
+            } else {
+                m = min(m, v);
+                break;
+            }
+        }
+    }
+    if (__ballot(bad)) {
+        if (lane == 0) fail_check(err, kBelowMin << 8 | 2);
+        return;
+    }
+    for (int d = 32; d >= 1; d >>= 1) m = min(m, (uint32_t)__shfl_xor(m, d));
+    if (lane == 0) below[q] = m;
+}
+
+// Per set member: the key-formation event of the plateau's class of that pore (the trie node of
+// the member's prefix), kInv for the first member (it forms no key on the plateau).
+__global__ void class_events(const unsigned long long* __restrict__ set_off, const uint32_t* __restrict__ set_len,
+                             const unsigned long long* __restrict__ rbase, const uint32_t* __restrict__ rec_node,
+                             SetBounds b, unsigned long long R, uint32_t D, uint32_t* __restrict__ cls_at,
+                             uint32_t* __restrict__ err) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= b.P) return;
+    const unsigned long long off = set_off[q];
+    const uint32_t len = set_len[q];
+    if (off + len > b.pool_cap) {
+        fail_check(err, kClassEvents << 8 | 1);
+        return;
+    }
+    if (len) cls_at[off] = kInv;
+    for (uint32_t j = 1; j < len; ++j) {
+        const unsigned long long r = rbase[q] + j - 1;
+        const uint32_t d = r < R ? rec_node[r] : kInv;
+        if (d >= D) {
+            fail_check(err, kClassEvents << 8 | 2);
+            return;
+        }
+        cls_at[off + j] = d;
+    }
+}
+
 // ---- key records -------------------------------------------------------------------------
 // A plateau whose set is s0 < s1 < ... < s(m-1) is first visited by pore s_j (j >= 1) while
 // its earlier-pore class is {s0 .. s(j-1)}: that visit forms the list of {s0 .. s_j}.
@@ -569,14 +756,29 @@ __global__ void reach_sizes(const unsigned long long* __restrict__ set_off, cons
 }
 
 // ---- DFS replay: a persistent pool of waves, one pore at a time per wave --------------------
+// What the replay reproduces.  Plateaus (connected sets of equal CR) are strongly connected and
+// edges between plateaus lead to lower CR, so the pore's DFS enters each plateau it reaches once,
+// at one voxel, and the whole plateau is visited inside that voxel's subtree.  A step from a
+// voxel into a plateau c not yet entered starts an excursion that visits exactly the plateaus
+// below c that are not yet visited (a plateau visited earlier was closed together with
+// everything below it), before the DFS goes on where it was.  Only the first entry into each
+// class matters, and the classes met in such an excursion that are not ranked yet are exactly
+// the unranked classes of the plateaus below c (a plateau of an unranked class has never been
+// entered).  So when at most one unranked class lies below c, the excursion ranks that class
+// and nothing else: it is skipped, its plateaus are marked entered and closed (their voxels are
+// visited), and the walk goes on.  Two tests decide this:
+//   * static: below[c] > p means every plateau below c holds c's class (O(1));
+//   * dynamic: a wave-parallel traversal of the plateaus below c that are not entered, not
+//     descending below static plateaus, stops at the second distinct unranked class.  A
+//     traversal that runs out of room answers "two" (the step is taken: always exact).
+// Plateaus below a static plateau are left unmarked; when the walk meets one it is static too.
 struct ReplayTask {
     uint32_t pore, seed, ncls, pad;
-    unsigned long long seen_off;  // first word of this pore's class bitmap
 };
 
 // Lengths of the buffers the replay selects from with values read from memory.
 struct ReplayBounds {
-    unsigned long long n, P, pool_cap, R, D, seen_words;
+    unsigned long long n, P, pool_cap, D, E, trav_cap;
 };
 
 // Position of pore p in the ascending set pool[off .. off + len), or -1.
@@ -597,38 +799,61 @@ __device__ __forceinline__ uint32_t bcast(uint32_t v, int src) {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, src);
 }
 
+__device__ __forceinline__ bool bit_of(const unsigned long long* words, uint32_t i) {
+    return (__hip_atomic_load(&words[i >> 6], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> (i & 63)) & 1ull;
+}
+
+// The class event of pore p on plateau q (kInv where p is q's first member); false when the
+// plateau index or p's membership is inconsistent.
+__device__ __forceinline__ bool class_of(uint32_t q, uint32_t p, const unsigned long long* __restrict__ set_off,
+                                         const uint32_t* __restrict__ set_len, const uint32_t* __restrict__ pool,
+                                         const uint32_t* __restrict__ cls_at, const ReplayBounds& b, uint32_t& ev) {
+    ev = kInv;
+    if (q >= b.P || set_off[q] + set_len[q] > b.pool_cap) return false;
+    const int j = member_index(pool, set_off[q], set_len[q], p);
+    if (j < 0) return false;
+    if (j >= 1) {
+        ev = cls_at[set_off[q] + j];
+        if (ev >= b.D) return false;
+    }
+    return true;
+}
+
 // A pool of waves (workgroup b uses slot b of the per-slot buffers: a visited bitmap over all
-// voxels and a bitmap of entered plateaus, both cleared per pore, and a stack) takes pores from a
-// queue, largest reach first.  The code at the boundary of the loop over pores is executed by
-// every lane (the fetch adds 1 in lane 0 and 0 elsewhere, the last lane's value is read back):
-// with lane-0-only blocks there the compiler merged the end of one pore with the fetch of the
-// next and let lanes 1..63 start a pore while lane 0, which owns the stack, was still fetching.  Lanes 0..26 test the 26 neighbours of the current voxel with independent loads
-// (radius, plateau, visited word); lane 0 owns the stack and the counters.  Frames are (voxel,
-// next neighbour index); a recursive DFS that tries neighbours in reverse loop order visits
-// voxels in the reference's stack-DFS order.  A voxel's earlier-pore class depends only on its
-// plateau, so the class is looked up on the pore's first entry into each plateau.  The replay
-// of a pore stops once all but one of its classes are ranked: the last one is necessarily
-// found after them.
-__global__ void __launch_bounds__(64) dfs_replay(const ReplayTask* __restrict__ tasks, uint32_t T,
-                                                 uint32_t* __restrict__ queue, const uint32_t* __restrict__ cr,
-                                                 const uint32_t* __restrict__ plat,
-                                                 const unsigned long long* __restrict__ set_off,
-                                                 const uint32_t* __restrict__ set_len, const uint32_t* __restrict__ pool,
-                                                 const unsigned long long* __restrict__ rbase,
-                                                 const uint32_t* __restrict__ rec_node,
-                                                 const uint32_t* __restrict__ event_local, unsigned long long* vis_all,
-                                                 unsigned long long vis_words, unsigned long long* pent_all,
-                                                 unsigned long long pent_words, uint32_t* __restrict__ seen_words,
-                                                 uint32_t* __restrict__ stack_all, uint8_t* __restrict__ stack_k_all,
-                                                 unsigned long long stack_cap, uint32_t* __restrict__ rank,
-                                                 uint32_t* __restrict__ task_status,
-                                                 unsigned long long* __restrict__ task_steps, int X, int Y, int Z,
-                                                 int d3q7, ReplayBounds b, uint32_t* __restrict__ err) {
+// voxels, bitmaps over plateaus of entered, closed and traversal-marked plateaus, a bitmap over
+// the key events of ranked classes, the DFS stack and the traversal stack and list) takes pores
+// from a queue, largest reach first.  The code at the boundary of the loop over pores is executed
+// by every lane (the fetch adds 1 in lane 0 and 0 elsewhere, the last lane's value is read back):
+// with lane-0-only blocks there the compiler merged the end of one pore with the fetch of the next
+// and let lanes 1..63 start a pore while lane 0, which owns the stack, was still fetching.  Lanes
+// 0..26 test the 26 neighbours of the current voxel with independent loads (radius, plateau,
+// visited word); lane 0 owns the stack and the counters.  Frames are (voxel, next neighbour
+// index); a recursive DFS that tries neighbours in reverse loop order visits voxels in the
+// reference's stack-DFS order.  The replay of a pore stops once all but one of its classes are
+// ranked: the last one is necessarily found after them.  Words written by one lane and read by
+// another go through agent-scope atomics.
+__global__ void __launch_bounds__(64) dfs_replay(
+    const ReplayTask* __restrict__ tasks, uint32_t T, uint32_t* __restrict__ queue, const uint32_t* __restrict__ cr,
+    const uint32_t* __restrict__ plat, const unsigned long long* __restrict__ set_off,
+    const uint32_t* __restrict__ set_len, const uint32_t* __restrict__ pool, const uint32_t* __restrict__ cls_at,
+    const uint32_t* __restrict__ below, const uint32_t* __restrict__ out_start, const uint32_t* __restrict__ out_count,
+    const unsigned long long* __restrict__ oedges, unsigned long long* vis_all, unsigned long long vis_words,
+    unsigned long long* pbits_all, unsigned long long pent_words, uint32_t* seen_all, unsigned long long seen_words,
+    uint32_t* __restrict__ stack_all, uint8_t* __restrict__ stack_k_all, unsigned long long stack_cap,
+    uint32_t* trav_all, uint32_t* __restrict__ rank, uint32_t* __restrict__ task_status,
+    unsigned long long* __restrict__ task_steps, int X, int Y, int Z, int d3q7, ReplayBounds b,
+    uint32_t* __restrict__ err) {
+    __shared__ uint32_t tcount[2];  // traversal: stack length, list length
     const int lane = threadIdx.x;
     unsigned long long* vis = vis_all + (size_t)blockIdx.x * vis_words;
-    unsigned long long* pent = pent_all + (size_t)blockIdx.x * pent_words;
+    unsigned long long* pent = pbits_all + (size_t)blockIdx.x * 3 * pent_words;
+    unsigned long long* pclosed = pent + pent_words;
+    unsigned long long* ptmp = pclosed + pent_words;  // clear between traversals
+    uint32_t* seen = seen_all + (size_t)blockIdx.x * seen_words;
     uint32_t* sv = stack_all + (size_t)blockIdx.x * stack_cap;
     uint8_t* sk = stack_k_all + (size_t)blockIdx.x * stack_cap;
+    uint32_t* tstack = trav_all + (size_t)blockIdx.x * 2 * b.trav_cap;
+    uint32_t* tlist = tstack + b.trav_cap;
 
     while (true) {
         // every lane adds (lane 0 adds 1): the returned values are the old count and old count + 1
@@ -637,47 +862,35 @@ __global__ void __launch_bounds__(64) dfs_replay(const ReplayTask* __restrict__ 
         if (t >= T) return;
         const ReplayTask task = tasks[t];
         const uint32_t p = task.pore;
-        const uint32_t nsw = (task.ncls + 31) / 32;
-        if (task.seen_off + nsw > b.seen_words || task.seed >= b.n) {
+        if (task.seed >= b.n) {
             if (lane == 0) fail_check(err, kDfsReplay << 8 | 1);
             return;
         }
-        uint32_t* seen = seen_words + task.seen_off;
         for (unsigned long long i = lane; i < vis_words; i += 64) vis[i] = 0;
-        for (unsigned long long i = lane; i < pent_words; i += 64) pent[i] = 0;
-        for (uint32_t i = lane; i < nsw; i += 64) seen[i] = 0;
+        for (unsigned long long i = lane; i < 2 * pent_words; i += 64) pent[i] = 0;  // entered, closed
+        for (unsigned long long i = lane; i < seen_words; i += 64) seen[i] = 0;
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");  // the cleared words are in L2 before the atomics
         __builtin_amdgcn_wave_barrier();  // every lane has cleared its share before lane 0 marks the seed
 
-        // Lane 0: on the pore's first entry into plateau q, ranks q's class if it is new.
+        // Lane 0: ranks class event ev if it is new; returns the new count of ranked classes.
+        auto rank_event = [&](uint32_t ev, uint32_t found) -> uint32_t {
+            if (ev == kInv) return found;
+            const uint32_t old = atomicOr(&seen[ev >> 5], 1u << (ev & 31));
+            if (old & (1u << (ev & 31))) return found;
+            rank[ev] = found;
+            return found + 1;
+        };
+        // Lane 0: on the walk's first entry into plateau q, ranks q's class if it is new.
         // Returns the new count of ranked classes, or kInv after a failed check.
         auto enter = [&](uint32_t q, uint32_t found) -> uint32_t {
-            if (q >= b.P || set_off[q] + set_len[q] > b.pool_cap) {
-                fail_check(err, kDfsReplay << 8 | 2);
+            uint32_t ev;
+            if (!class_of(q, p, set_off, set_len, pool, cls_at, b, ev)) {
+                fail_check(err, kDfsReplay << 8 | 3);  // p does not reach a plateau its DFS entered
                 return kInv;
             }
             const unsigned long long pw = atomicOr(&pent[q >> 6], 1ull << (q & 63));
             if (pw & (1ull << (q & 63))) return found;
-            const int j = member_index(pool, set_off[q], set_len[q], p);
-            if (j < 0) {
-                fail_check(err, kDfsReplay << 8 | 3);  // p does not reach a plateau its DFS entered
-                return kInv;
-            }
-            if (j >= 1) {
-                const unsigned long long r = rbase[q] + (uint32_t)j - 1;
-                const uint32_t e = r < b.R ? rec_node[r] : kInv;
-                const uint32_t l = e < b.D ? event_local[e] : kInv;
-                if (l >= task.ncls) {
-                    fail_check(err, kDfsReplay << 8 | 4);
-                    return kInv;
-                }
-                if (!(seen[l >> 5] & (1u << (l & 31)))) {
-                    seen[l >> 5] |= 1u << (l & 31);
-                    rank[e] = found;
-                    ++found;
-                }
-            }
-            return found;
+            return rank_event(ev, found);
         };
 
         uint32_t u = task.seed;
@@ -701,14 +914,17 @@ __global__ void __launch_bounds__(64) dfs_replay(const ReplayTask* __restrict__ 
             if (lane < k && lane < 27 && in_stencil(lane, d3q7) && neighbour(lane, x, y, z, X, Y, Z, j)) {
                 cn = cr[j];
                 qn = plat[j];
-                const unsigned long long w =
-                    __hip_atomic_load(&vis[j >> 6], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const unsigned long long w = __hip_atomic_load(&vis[j >> 6], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 cand = cn != 0 && cn <= cu && !(w & (1ull << (j & 63)));
-                // the entered-plateau test, for the lane whose neighbour is taken
-                // (an index out of range reads as not entered, so that enter() reports it)
-                if (cand && qn != uq)
-                    entered = qn < b.P && ((__hip_atomic_load(&pent[qn >> 6], __ATOMIC_RELAXED,
-                                                              __HIP_MEMORY_SCOPE_AGENT) >> (qn & 63)) & 1ull);
+                // neighbours in other plateaus: closed plateaus are visited; the entered test is
+                // for the lane whose neighbour is taken (an index out of range reads as not entered,
+                // so that enter() reports it)
+                if (cand && qn != uq && qn < b.P) {
+                    entered = bit_of(pent, qn);
+                    if (bit_of(pclosed, qn)) cand = false;
+                } else if (cand && qn != uq) {
+                    entered = false;
+                }
             }
             const unsigned long long bal = __ballot(cand);
             if (!bal) {
@@ -729,15 +945,130 @@ __global__ void __launch_bounds__(64) dfs_replay(const ReplayTask* __restrict__ 
                 cu = cr[u];
                 continue;
             }
-            if (top + 1 >= stack_cap) {  // no room for this frame: the host repeats the pore
-                overflow = true;
-                break;
-            }
             const int kk = 63 - __clzll((long long)bal);  // highest neighbour index below k
             const uint32_t nvox = bcast(j, kk);
             const uint32_t nq = bcast(qn, kk);
             const uint32_t ncr = bcast(cn, kk);
             const bool is_new = !bcast(entered ? 1u : 0u, kk);  // first entry into plateau nq
+            if (is_new && nq < b.P) {
+                // an excursion into nq: skipped when it holds at most one unranked class
+                bool skip = false;
+                uint32_t skip_ev = kInv;
+                if (below[nq] > p) {
+                    uint32_t ev = kInv;
+                    bool ok = true;
+                    if (lane == 0) ok = class_of(nq, p, set_off, set_len, pool, cls_at, b, ev);
+                    if (!bcast(ok ? 1u : 0u, 0)) {
+                        if (lane == 0) fail_check(err, kDfsReplay << 8 | 7);
+                        return;
+                    }
+                    skip = true;
+                    skip_ev = bcast(ev, 0);
+                    if (lane == 0) {
+                        atomicOr(&pent[nq >> 6], 1ull << (nq & 63));
+                        atomicOr(&pclosed[nq >> 6], 1ull << (nq & 63));
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");  // marks land before the next loads
+                } else {
+                    // traversal: stack tstack[0 .. tcount[0]), every pushed plateau listed in tlist
+                    if (lane == 0) {
+                        atomicOr(&ptmp[nq >> 6], 1ull << (nq & 63));
+                        __hip_atomic_store(&tstack[0], nq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        __hip_atomic_store(&tlist[0], nq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        tcount[0] = 1;
+                        tcount[1] = 1;
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    uint32_t nun = 0, k1 = kInv;
+                    bool room = true;
+                    for (uint32_t ts = 1; ts > 0;) {
+                        const uint32_t nb = min(ts, 64u);
+                        uint32_t bq = kInv, ev = kInv;
+                        bool bad = false;
+                        if ((uint32_t)lane < nb) {
+                            bq = __hip_atomic_load(&tstack[ts - 1 - lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            bad = !class_of(bq, p, set_off, set_len, pool, cls_at, b, ev);
+                        }
+                        if (__ballot(bad)) {
+                            if (lane == 0) fail_check(err, kDfsReplay << 8 | 8);
+                            return;
+                        }
+                        const bool unr = ev != kInv && !((__hip_atomic_load(&seen[ev >> 5], __ATOMIC_RELAXED,
+                                                                              __HIP_MEMORY_SCOPE_AGENT) >> (ev & 31)) & 1u);
+                        const unsigned long long ub = __ballot(unr);
+                        if (ub) {
+                            if (k1 == kInv) k1 = bcast(ev, __ffsll((long long)ub) - 1);
+                            nun = __ballot(unr && ev != k1) ? 2u : 1u;
+                        }
+                        if (nun >= 2) break;
+                        if (lane == 0) tcount[0] = ts - nb;  // the popped entries are in registers
+                        __builtin_amdgcn_wave_barrier();
+                        bool lane_room = true;
+                        if ((uint32_t)lane < nb && below[bq] <= p) {
+                            const uint32_t e0 = out_start[bq], ne = out_count[bq];
+                            if ((unsigned long long)e0 + ne > b.E) {
+                                bad = true;
+                            } else {
+                                for (uint32_t e = e0; e < e0 + ne; ++e) {
+                                    const uint32_t d = (uint32_t)oedges[e];
+                                    if (d >= b.P) {
+                                        bad = true;
+                                        break;
+                                    }
+                                    if (bit_of(pent, d)) continue;
+                                    const unsigned long long m1 = 1ull << (d & 63);
+                                    if (atomicOr(&ptmp[d >> 6], m1) & m1) continue;
+                                    const uint32_t w = atomicAdd(&tcount[1], 1u);
+                                    if (w >= b.trav_cap) {  // not listed: take the mark back
+                                        atomicAnd(&ptmp[d >> 6], ~m1);
+                                        lane_room = false;
+                                        continue;
+                                    }
+                                    __hip_atomic_store(&tlist[w], d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                                    const uint32_t at = atomicAdd(&tcount[0], 1u);  // at <= w: every entry is listed
+                                    __hip_atomic_store(&tstack[at], d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                                }
+                            }
+                        }
+                        if (__ballot(bad)) {
+                            if (lane == 0) fail_check(err, kDfsReplay << 8 | 9);
+                            return;
+                        }
+                        __builtin_amdgcn_wave_barrier();
+                        if (__ballot(!lane_room)) {
+                            room = false;
+                            break;
+                        }
+                        ts = bcast(tcount[0], 0);
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    const uint32_t wn = min(bcast(tcount[1], 0), (uint32_t)b.trav_cap);
+                    skip = room && nun <= 1;
+                    for (uint32_t i = lane; i < wn; i += 64) {
+                        const uint32_t d = __hip_atomic_load(&tlist[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        if (d >= b.P) continue;
+                        const unsigned long long m1 = 1ull << (d & 63);
+                        atomicAnd(&ptmp[d >> 6], ~m1);
+                        if (skip) {
+                            atomicOr(&pent[d >> 6], m1);
+                            atomicOr(&pclosed[d >> 6], m1);
+                        }
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+                    __builtin_amdgcn_wave_barrier();
+                    skip_ev = nun == 1 ? k1 : kInv;
+                }
+                if (skip) {
+                    if (lane == 0) found = rank_event(skip_ev, found);
+                    found = bcast(found, 0);
+                    k = kk;  // go on with the neighbours below kk
+                    continue;
+                }
+            }
+            if (top + 1 >= stack_cap) {  // no room for this frame: the host repeats the pore
+                overflow = true;
+                break;
+            }
             if (lane == 0) {
                 sv[top] = u;  // save this frame, resuming below kk
                 sk[top] = (uint8_t)kk;
@@ -1028,10 +1359,20 @@ void run_pores(pma_ctx* c) {
         PMA_HIP(hipMemsetAsync(pool_top, 0, 8, c->stream));
         PMA_HIP(hipMemsetAsync(of, 0, 4, c->stream));
         const SetBounds sb{E, P, npores, pool_cap};
+        // one wave per plateau over an LDS bitmap of the pores while the bitmaps of a workgroup's
+        // waves fit in PMA_UNION_LDS_MAX bytes; the per-thread merge otherwise
+        const uint32_t words = (npores + 31) / 32;
+        const size_t union_lds = (size_t)kUnionWaves * std::max(words, 1u) * 4;
+        const bool bitmap = union_lds <= PMA_UNION_LDS_MAX;
         for (size_t l = 0; l + 1 < lstart_h.size(); ++l) {
             uint32_t lo = lstart_h[l], hi = lstart_h[l + 1];
-            {
-                PMA_TIMED(c, "set_union_level");
+            PMA_TIMED(c, "set_union_level");
+            if (bitmap) {
+                set_union_bitmap<<<grid_for(hi - lo, kUnionWaves), 64 * kUnionWaves, union_lds, c->stream>>>(
+                    order, lo, hi, in_start, in_count, edges, plat_pore, set_off, set_len, pool, pool_top, sb,
+                    std::max(words, 1u), of, err);
+                PMA_CHECKED(c, "set_union_bitmap", err);
+            } else {
                 set_union_level<<<grid_for(hi - lo, 64), 64, 0, c->stream>>>(
                     order, lo, hi, in_start, in_count, edges, plat_pore, set_off, set_len, pool, pool_top, cur, cend,
                     sb, of, err);
@@ -1065,11 +1406,11 @@ void run_pores(pma_ctx* c) {
                                  reinterpret_cast<const uint64_t*>(rbase), nullptr, plat_size, P, R, rd);
     // the pore that forms a list is its largest member, the last one in ascending order
     const uint32_t D = trie.distinct;
-    std::vector<uint32_t> ev_owner(D), ev_local(D), ncls(npores, 0);
+    std::vector<uint32_t> ev_owner(D), ncls(npores, 0);
     for (uint32_t d = 0; d < D; ++d) {
         ev_owner[d] = (uint32_t)trie.key[d];
         PMA_REQUIRE(ev_owner[d] < npores, PMA_ERR_INTERNAL, "pma_pores: list member out of range");
-        ev_local[d] = ncls[ev_owner[d]]++;
+        ncls[ev_owner[d]]++;
     }
 
     // reach sizes and boundary flags
@@ -1098,15 +1439,12 @@ void run_pores(pma_ctx* c) {
     uint64_t replay_tasks = 0, replay_steps = 0;
     {
         std::vector<ReplayTask> all;
-        unsigned long long seen_total = 0;
         for (uint32_t k = 0; k < npores; ++k) {
             if (ncls[k] < 2) continue;
             ReplayTask t{};
             t.pore = k;
             t.seed = (uint32_t)seeds_h[k];
             t.ncls = ncls[k];
-            t.seen_off = seen_total;
-            seen_total += (ncls[k] + 31) / 32;
             all.push_back(t);
         }
         // largest reach first, so that the longest replays start at once
@@ -1115,18 +1453,44 @@ void run_pores(pma_ctx* c) {
         replay_tasks = all.size();
         if (!all.empty()) {
             const size_t T = all.size();
-            uint32_t* d_local = c->buf<uint32_t>("pt_ev_local", std::max(D, 1u));
+            // skip data: out-edges by source, below[] (ascending CR), the class event of each set member
+            unsigned long long* oe_in = c->buf<unsigned long long>("pt_oedges_in", E1);
+            unsigned long long* oedges = c->buf<unsigned long long>("pt_oedges", E1);
+            uint32_t* out_start = c->buf<uint32_t>("pt_out_start", P1);
+            uint32_t* out_count = c->buf<uint32_t>("pt_out_count", P1);
+            PMA_HIP(hipMemsetAsync(out_start, 0, P1 * 4, c->stream));
+            PMA_HIP(hipMemsetAsync(out_count, 0, P1 * 4, c->stream));
+            if (E) {
+                swap_halves<<<grid_for(E, 256), 256, 0, c->stream>>>(edges, E, oe_in);
+                PMA_CHECKED(c, "swap_halves", err);
+                radix_sort_keys_u64(c, reinterpret_cast<uint64_t*>(oe_in), reinterpret_cast<uint64_t*>(oedges), E);
+                edge_ranges<<<grid_for(E, 256), 256, 0, c->stream>>>(oedges, E, P, out_start, out_count, err);
+                PMA_CHECKED(c, "edge_ranges", err);
+            }
+            uint32_t* below = c->buf<uint32_t>("pt_below", P1);
+            for (size_t l = lstart_h.size() - 1; l-- > 0;) {
+                const uint32_t lo = lstart_h[l], hi = lstart_h[l + 1];
+                PMA_TIMED(c, "below_min");
+                below_min<<<grid_for(hi - lo, 4), 256, 0, c->stream>>>(order, lo, hi, out_start, out_count, oedges,
+                                                                      set_off, set_len, pool, sb, below, err);
+                PMA_CHECKED(c, "below_min", err);
+            }
+            uint32_t* cls_at = c->buf<uint32_t>("pt_cls_at", pool_cap);
+            class_events<<<grid_for(P, 256), 256, 0, c->stream>>>(set_off, set_len, rbase, rd, sb, R, D, cls_at, err);
+            PMA_CHECKED(c, "class_events", err);
+            check_stage(c, err);
+
             uint32_t* d_rank = c->buf<uint32_t>("pt_rank", std::max(D, 1u));
-            uint32_t* seen = c->buf<uint32_t>("pt_seen", seen_total);
             uint32_t* status = c->buf<uint32_t>("pt_task_status", T);
             unsigned long long* steps = c->buf<unsigned long long>("pt_task_steps", T);
             uint32_t* queue = c->buf<uint32_t>("pt_queue", 1);
             ReplayTask* d_tasks = c->buf<ReplayTask>("pt_tasks", T);
-            upload(c, d_local, ev_local.data(), D * 4ull);
             PMA_HIP(hipMemsetAsync(d_rank, 0xFF, D * 4ull, c->stream));
-            // Per wave: a visited bitmap over all voxels, a bitmap of entered plateaus and a stack.
-            // The number of waves follows from the device memory free for them.
-            const unsigned long long vis_words = n / 64 + 1, pent_words = P / 64 + 1;
+            // Per wave: a visited bitmap over all voxels, three bitmaps over plateaus (entered,
+            // closed, traversal marks), a bitmap over key events, a stack and the traversal
+            // stack and list.  The number of waves follows from the device memory free for them.
+            const unsigned long long vis_words = n / 64 + 1, pent_words = P / 64 + 1, seen_words = D / 32 + 1;
+            const unsigned long long trav_cap = std::min<unsigned long long>(P + 1, PMA_PORES_TRAV_CAP);
             unsigned long long max_reach = 0;
             for (const ReplayTask& t : all) max_reach = std::max<unsigned long long>(max_reach, reach_h[t.pore]);
             size_t free_b = 0, total_b = 0;
@@ -1139,24 +1503,28 @@ void run_pores(pma_ctx* c) {
                 // pass 0: stacks of up to PMA_PORES_STACK_CAP frames; pass 1: room for the whole reach
                 const unsigned long long cap =
                     pass == 0 ? std::min<unsigned long long>(max_reach + 2, PMA_PORES_STACK_CAP) : max_reach + 2;
-                const unsigned long long per_wave = (vis_words + pent_words) * 8 + cap * 5;
+                const unsigned long long per_wave =
+                    (vis_words + 3 * pent_words) * 8 + seen_words * 4 + cap * 5 + trav_cap * 8;
                 const unsigned long long W =
                     std::max<unsigned long long>(1, std::min<unsigned long long>({pending.size(), budget / per_wave, 1024}));
                 unsigned long long* vis = c->buf<unsigned long long>("pt_replay_vis", W * vis_words);
-                unsigned long long* pent = c->buf<unsigned long long>("pt_replay_plateaus", W * pent_words);
+                unsigned long long* pbits = c->buf<unsigned long long>("pt_replay_plateaus", W * 3 * pent_words);
+                uint32_t* seen = c->buf<uint32_t>("pt_replay_seen", W * seen_words);
                 uint32_t* stack_voxel = c->buf<uint32_t>("pt_stack_voxel", W * cap);
                 uint8_t* stack_k = c->buf<uint8_t>("pt_stack_k", W * cap);
+                uint32_t* trav = c->buf<uint32_t>("pt_replay_trav", W * 2 * trav_cap);
+                PMA_HIP(hipMemsetAsync(pbits, 0, W * 3 * pent_words * 8, c->stream));  // traversal marks start clear
                 upload(c, d_tasks, pending.data(), pending.size() * sizeof(ReplayTask));
                 PMA_HIP(hipMemsetAsync(status, 0, pending.size() * 4, c->stream));
-                const ReplayBounds rb{n, P, pool_cap, R, D, seen_total};
+                const ReplayBounds rb{n, P, pool_cap, D, E, trav_cap};
                 PMA_HIP(hipMemsetAsync(queue, 0, 4, c->stream));
                 const auto t0 = std::chrono::steady_clock::now();
                 {
                     PMA_TIMED(c, "dfs_replay");
                     dfs_replay<<<(unsigned)W, 64, 0, c->stream>>>(
-                        d_tasks, (uint32_t)pending.size(), queue, cr, plat, set_off, set_len, pool, rbase, rd, d_local,
-                        vis, vis_words, pent, pent_words, seen, stack_voxel, stack_k, cap, d_rank, status, steps, X, Y,
-                        Z, d3q7, rb, err);
+                        d_tasks, (uint32_t)pending.size(), queue, cr, plat, set_off, set_len, pool, cls_at, below,
+                        out_start, out_count, oedges, vis, vis_words, pbits, pent_words, seen, seen_words,
+                        stack_voxel, stack_k, cap, trav, d_rank, status, steps, X, Y, Z, d3q7, rb, err);
                     PMA_CHECKED(c, "dfs_replay", err);
                 }
                 if (c->debug_sync)
@@ -1183,8 +1551,10 @@ void run_pores(pma_ctx* c) {
             }
             PMA_HIP(hipMemcpyAsync(rank_h.data(), d_rank, D * 4ull, hipMemcpyDeviceToHost, c->stream));
             PMA_HIP(hipStreamSynchronize(c->stream));
-            // the per-slot bitmaps and stacks are sized by the free memory: give it back
-            for (const char* name : {"pt_replay_vis", "pt_replay_plateaus", "pt_stack_voxel", "pt_stack_k"}) c->drop(name);
+            // the per-slot buffers are sized by the free memory: give it back
+            for (const char* name : {"pt_replay_vis", "pt_replay_plateaus", "pt_replay_seen", "pt_stack_voxel",
+                                     "pt_stack_k", "pt_replay_trav"})
+                c->drop(name);
         }
         c->stage[PMA_STAGE_PORES].work_items = replay_steps;
         (void)replay_tasks;
