@@ -43,10 +43,21 @@ namespace {
 
 constexpr uint32_t kInv = 0xFFFFFFFFu;
 #ifndef PMA_PORES_STACK_CAP
-#define PMA_PORES_STACK_CAP (1ull << 24)  // DFS frames per replaying wave in the first pass
+#define PMA_PORES_STACK_CAP (1ull << 21)  // DFS frames per replaying wave in the first pass
 #endif
 #ifndef PMA_PORES_TRAV_CAP
 #define PMA_PORES_TRAV_CAP (1ull << 22)  // plateaus one skip traversal may list (then it gives up)
+#endif
+#ifndef PMA_PORES_RING
+#define PMA_PORES_RING 1024u  // DFS frames of a replaying wave kept in LDS
+#endif
+#ifndef PMA_PORES_REPLAY_MEM
+#define PMA_PORES_REPLAY_MEM (128ull << 30)  // device bytes the replaying waves may hold
+#endif
+#ifndef PMA_PORES_POOL_MIN
+#define PMA_PORES_POOL_MIN 8ull     // pore-set pool entries per plateau, at least
+#define PMA_PORES_POOL_MAX 64ull    // ... and at most, while free memory allows
+#define PMA_PORES_POOL_FLOOR 1024ull
 #endif
 #ifndef PMA_UNION_LDS_MAX
 #define PMA_UNION_LDS_MAX 65536u  // LDS bytes a workgroup of the bitmap set union may use
@@ -56,13 +67,14 @@ constexpr uint32_t kInv = 0xFFFFFFFFu;
 enum : uint32_t {
     kWriteCentres = 1, kCcUnion, kCcFlatten, kCcNotMax, kFirstInSorted, kSeedFlags, kWriteSeeds, kPlateauIndex,
     kPlateauOfVoxels, kSeedPlateaus, kPlateauEdges, kWriteEdges, kEdgeRanges, kLevelStarts, kSetUnion,
-    kReachSizes, kDfsReplay, kWriteLabels, kSingleVolumes, kBelowMin, kClassEvents
+    kReachSizes, kDfsReplay, kWriteLabels, kSingleVolumes, kBelowMin, kPoreLists
 };
+constexpr uint32_t kNoClassU = 0xFFFF;  // class of a pore on a plateau whose first member it is
 const char* const kKernelNames[] = {"?", "write_centres", "cc_union", "cc_flatten", "cc_not_max",
                                     "first_in_sorted", "seed_flags", "write_seeds", "plateau_index",
                                     "plateau_of_voxels", "seed_plateaus", "plateau_edges", "write_edges",
                                     "edge_ranges", "level_starts", "set_union_level", "reach_sizes",
-                                    "dfs_replay", "write_labels", "single_volumes", "below_min", "class_events"};
+                                    "dfs_replay", "write_labels", "single_volumes", "below_min", "pore_lists"};
 
 __device__ __forceinline__ uint32_t ld_relaxed(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -573,6 +585,57 @@ __global__ void __launch_bounds__(64 * kUnionWaves)
         if (lane == 0) fail_check(err, kSetUnion << 8 | 2);
         return;
     }
+    if (ne <= 1) {
+        // the source's set with the own pore inserted: no bitmap needed
+        uint32_t src = kInv, len = 0;
+        unsigned long long off = 0;
+        if (ne == 1) {
+            src = (uint32_t)edges[e0];
+            if (src >= b.P || set_off[src] + set_len[src] > b.pool_cap) {
+                if (lane == 0) fail_check(err, kSetUnion << 8 | 3);
+                return;
+            }
+            off = set_off[src];
+            len = set_len[src];
+        }
+        // members below the own pore keep their place, the own pore follows them, the rest shift by one
+        uint32_t below_own = 0;
+        bool dup = false, bad = false;
+        for (uint32_t t = lane; t < len; t += 64) {
+            const uint32_t v = pool[off + t];
+            bad |= v >= b.npores;
+            below_own += v < own;
+            dup |= v == own;
+        }
+        if (__ballot(bad)) {
+            if (lane == 0) fail_check(err, kSetUnion << 8 | 3);
+            return;
+        }
+        for (int d = 32; d >= 1; d >>= 1) below_own += __shfl_xor(below_own, d);
+        const bool ins = own != kInv && !__ballot(dup);
+        const uint32_t total = len + (ins ? 1u : 0u);
+        unsigned long long o = 0;
+        if (lane == 0 && total) o = atomicAdd(pool_top, (unsigned long long)total);
+        o = __shfl(o, 0);
+        if (o + total > b.pool_cap) {
+            if (lane == 0) {
+                atomicOr(overflow, 2u);
+                set_off[q] = 0;
+                set_len[q] = 0;
+            }
+            return;
+        }
+        for (uint32_t t = lane; t < len; t += 64) {
+            const uint32_t v = pool[off + t];
+            pool[o + t + (ins && t >= below_own ? 1u : 0u)] = v;
+        }
+        if (lane == 0) {
+            if (ins) pool[o + below_own] = own;
+            set_off[q] = total ? o : 0;
+            set_len[q] = total;
+        }
+        return;
+    }
     for (uint32_t w = lane; w < words; w += 64) bm[w] = 0;
     __builtin_amdgcn_wave_barrier();
     bool bad = false;
@@ -699,29 +762,68 @@ __global__ void __launch_bounds__(256) below_min(const uint32_t* __restrict__ or
     if (lane == 0) below[q] = m;
 }
 
-// Per set member: the key-formation event of the plateau's class of that pore (the trie node of
-// the member's prefix), kInv for the first member (it forms no key on the plateau).
-__global__ void class_events(const unsigned long long* __restrict__ set_off, const uint32_t* __restrict__ set_len,
-                             const unsigned long long* __restrict__ rbase, const uint32_t* __restrict__ rec_node,
-                             SetBounds b, unsigned long long R, uint32_t D, uint32_t* __restrict__ cls_at,
-                             uint32_t* __restrict__ err) {
+// Per replayed pore, its (plateau << 32 | class) pairs: every plateau whose set holds the pore, with
+// the pore's class there (its position among the key events it forms, kNoClass where it is the
+// plateau's first member).  Counted, then written at per-pore offsets (any order).
+__global__ void list_counts(const unsigned long long* __restrict__ set_off, const uint32_t* __restrict__ set_len,
+                            const uint32_t* __restrict__ pool, SetBounds b, const uint32_t* __restrict__ replayed,
+                            unsigned long long* __restrict__ cnt, uint32_t* __restrict__ err) {
     size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= b.P) return;
     const unsigned long long off = set_off[q];
     const uint32_t len = set_len[q];
     if (off + len > b.pool_cap) {
-        fail_check(err, kClassEvents << 8 | 1);
+        fail_check(err, kPoreLists << 8 | 1);
         return;
     }
-    if (len) cls_at[off] = kInv;
-    for (uint32_t j = 1; j < len; ++j) {
-        const unsigned long long r = rbase[q] + j - 1;
-        const uint32_t d = r < R ? rec_node[r] : kInv;
-        if (d >= D) {
-            fail_check(err, kClassEvents << 8 | 2);
+    for (uint32_t j = 0; j < len; ++j) {
+        const uint32_t v = pool[off + j];
+        if (v >= b.npores) {
+            fail_check(err, kPoreLists << 8 | 2);
             return;
         }
-        cls_at[off + j] = d;
+        if (replayed[v]) atomicAdd(&cnt[v], 1ull);
+    }
+}
+
+__global__ void list_fill(const unsigned long long* __restrict__ set_off, const uint32_t* __restrict__ set_len,
+                          const uint32_t* __restrict__ pool, const unsigned long long* __restrict__ rbase,
+                          const uint32_t* __restrict__ rec_node, const uint32_t* __restrict__ ev_local, SetBounds b,
+                          unsigned long long R, uint32_t D, const uint32_t* __restrict__ replayed,
+                          const unsigned long long* __restrict__ list_off, unsigned long long* __restrict__ cursor,
+                          unsigned long long* __restrict__ lists, unsigned long long lists_len,
+                          uint32_t* __restrict__ err) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= b.P) return;
+    const unsigned long long off = set_off[q];
+    const uint32_t len = set_len[q];
+    if (off + len > b.pool_cap) {
+        fail_check(err, kPoreLists << 8 | 3);
+        return;
+    }
+    for (uint32_t j = 0; j < len; ++j) {
+        const uint32_t v = pool[off + j];
+        if (v >= b.npores) {
+            fail_check(err, kPoreLists << 8 | 4);
+            return;
+        }
+        if (!replayed[v]) continue;
+        uint32_t l = kNoClassU;
+        if (j >= 1) {
+            const unsigned long long r = rbase[q] + j - 1;
+            const uint32_t d = r < R ? rec_node[r] : kInv;
+            l = d < D ? ev_local[d] : kInv;
+            if (l >= kNoClassU) {
+                fail_check(err, kPoreLists << 8 | 5);
+                return;
+            }
+        }
+        const unsigned long long pos = list_off[v] + atomicAdd(&cursor[v], 1ull);
+        if (pos >= lists_len) {
+            fail_check(err, kPoreLists << 8 | 6);
+            return;
+        }
+        lists[pos] = ((unsigned long long)q << 32) | l;
     }
 }
 
@@ -772,26 +874,24 @@ __global__ void reach_sizes(const unsigned long long* __restrict__ set_off, cons
 //     descending below static plateaus, stops at the second distinct unranked class.  A
 //     traversal that runs out of room answers "two" (the step is taken: always exact).
 // Plateaus below a static plateau are left unmarked; when the walk meets one it is static too.
+//
+// Classes are numbered per pore (0 .. ncls - 1, kNoClass where the pore is the plateau's first
+// member).  A task's plateau classes come from its list of (plateau, class) pairs, scattered at
+// the start of the task into the slot's dense table, so a lookup is one load; the bitmap of
+// ranked classes lives in LDS.
+constexpr uint16_t kNoClass = (uint16_t)kNoClassU;
+constexpr uint32_t kRing = PMA_PORES_RING;  // DFS frames kept in LDS per wave (a power of two)
+
 struct ReplayTask {
-    uint32_t pore, seed, ncls, pad;
+    uint32_t pore, seed, ncls, rank_base;  // ranks of the pore's classes at rank_slot[rank_base + class]
+    unsigned long long list_off;           // the pore's (plateau << 32 | class) pairs
+    uint32_t list_len, pad;
 };
 
 // Lengths of the buffers the replay selects from with values read from memory.
 struct ReplayBounds {
-    unsigned long long n, P, pool_cap, D, E, trav_cap;
+    unsigned long long n, P, E, trav_cap, lists_len, ranks_len;
 };
-
-// Position of pore p in the ascending set pool[off .. off + len), or -1.
-__device__ __forceinline__ int member_index(const uint32_t* __restrict__ pool, unsigned long long off, uint32_t len,
-                                            uint32_t p) {
-    uint32_t lo = 0, hi = len;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (pool[off + mid] < p) lo = mid + 1;
-        else hi = mid;
-    }
-    return (lo < len && pool[off + lo] == p) ? (int)lo : -1;
-}
 
 // Lane `src`'s value as a wave-uniform scalar: the loop conditions built from broadcasts are then
 // uniform to the compiler as well.
@@ -803,55 +903,37 @@ __device__ __forceinline__ bool bit_of(const unsigned long long* words, uint32_t
     return (__hip_atomic_load(&words[i >> 6], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> (i & 63)) & 1ull;
 }
 
-// The class event of pore p on plateau q (kInv where p is q's first member); false when the
-// plateau index or p's membership is inconsistent.
-__device__ __forceinline__ bool class_of(uint32_t q, uint32_t p, const unsigned long long* __restrict__ set_off,
-                                         const uint32_t* __restrict__ set_len, const uint32_t* __restrict__ pool,
-                                         const uint32_t* __restrict__ cls_at, const ReplayBounds& b, uint32_t& ev) {
-    ev = kInv;
-    if (q >= b.P || set_off[q] + set_len[q] > b.pool_cap) return false;
-    const int j = member_index(pool, set_off[q], set_len[q], p);
-    if (j < 0) return false;
-    if (j >= 1) {
-        ev = cls_at[set_off[q] + j];
-        if (ev >= b.D) return false;
-    }
-    return true;
-}
-
 // A pool of waves (workgroup b uses slot b of the per-slot buffers: a visited bitmap over all
-// voxels, bitmaps over plateaus of entered, closed and traversal-marked plateaus, a bitmap over
-// the key events of ranked classes, the DFS stack and the traversal stack and list) takes pores
-// from a queue, largest reach first.  The code at the boundary of the loop over pores is executed
-// by every lane (the fetch adds 1 in lane 0 and 0 elsewhere, the last lane's value is read back):
-// with lane-0-only blocks there the compiler merged the end of one pore with the fetch of the next
-// and let lanes 1..63 start a pore while lane 0, which owns the stack, was still fetching.  Lanes
-// 0..26 test the 26 neighbours of the current voxel with independent loads (radius, plateau,
-// visited word); lane 0 owns the stack and the counters.  Frames are (voxel, next neighbour
-// index); a recursive DFS that tries neighbours in reverse loop order visits voxels in the
-// reference's stack-DFS order.  The replay of a pore stops once all but one of its classes are
-// ranked: the last one is necessarily found after them.  Words written by one lane and read by
-// another go through agent-scope atomics.
+// voxels, bitmaps over plateaus of entered and closed plateaus, the dense class table, the DFS stack and the traversal stack and list) takes pores from a queue, largest reach
+// first.  The code at the boundary of the loop over pores is executed by every lane (the fetch
+// adds 1 in lane 0 and 0 elsewhere, the last lane's value is read back): with lane-0-only blocks
+// there the compiler merged the end of one pore with the fetch of the next and let lanes 1..63
+// start a pore while lane 0, which owns the stack, was still fetching.  Lanes 0..26 test the 26
+// neighbours of the current voxel with independent loads (radius, plateau, visited word); lane 0
+// owns the stack and the counters.  Frames are (voxel, next neighbour index); a recursive DFS
+// that tries neighbours in reverse loop order visits voxels in the reference's stack-DFS order.
+// The replay of a pore stops once all but one of its classes are ranked: the last one is
+// necessarily found after them.  Words written by one lane and read by another go through
+// agent-scope atomics.  Dynamic LDS: the ranked-class bitmap, seen_words words.
 __global__ void __launch_bounds__(64) dfs_replay(
     const ReplayTask* __restrict__ tasks, uint32_t T, uint32_t* __restrict__ queue, const uint32_t* __restrict__ cr,
-    const uint32_t* __restrict__ plat, const unsigned long long* __restrict__ set_off,
-    const uint32_t* __restrict__ set_len, const uint32_t* __restrict__ pool, const uint32_t* __restrict__ cls_at,
+    const uint32_t* __restrict__ plat, const unsigned long long* __restrict__ lists,
     const uint32_t* __restrict__ below, const uint32_t* __restrict__ out_start, const uint32_t* __restrict__ out_count,
     const unsigned long long* __restrict__ oedges, unsigned long long* vis_all, unsigned long long vis_words,
-    unsigned long long* pbits_all, unsigned long long pent_words, uint32_t* seen_all, unsigned long long seen_words,
-    uint32_t* __restrict__ stack_all, uint8_t* __restrict__ stack_k_all, unsigned long long stack_cap,
-    uint32_t* trav_all, uint32_t* __restrict__ rank, uint32_t* __restrict__ task_status,
+    unsigned long long* pbits_all, unsigned long long pent_words, uint16_t* cls_all, uint32_t seen_words,
+    uint4* __restrict__ stack_all, unsigned long long stack_cap,
+    uint32_t* trav_all, uint32_t* __restrict__ rank_slot, uint32_t* __restrict__ task_status,
     unsigned long long* __restrict__ task_steps, int X, int Y, int Z, int d3q7, ReplayBounds b,
     uint32_t* __restrict__ err) {
-    __shared__ uint32_t tcount[2];  // traversal: stack length, list length
+    extern __shared__ uint32_t seen[];  // ranked classes of the current pore
+    __shared__ uint32_t tcount[2];      // traversal: stack length, list length
+    __shared__ uint4 ring[kRing];       // the newest DFS frames (voxel, plateau, radius, next index)
     const int lane = threadIdx.x;
     unsigned long long* vis = vis_all + (size_t)blockIdx.x * vis_words;
-    unsigned long long* pent = pbits_all + (size_t)blockIdx.x * 3 * pent_words;
+    unsigned long long* pent = pbits_all + (size_t)blockIdx.x * 2 * pent_words;
     unsigned long long* pclosed = pent + pent_words;
-    unsigned long long* ptmp = pclosed + pent_words;  // clear between traversals
-    uint32_t* seen = seen_all + (size_t)blockIdx.x * seen_words;
-    uint32_t* sv = stack_all + (size_t)blockIdx.x * stack_cap;
-    uint8_t* sk = stack_k_all + (size_t)blockIdx.x * stack_cap;
+    uint16_t* cls = cls_all + (size_t)blockIdx.x * b.P;
+    uint4* gstack = stack_all + (size_t)blockIdx.x * stack_cap;
     uint32_t* tstack = trav_all + (size_t)blockIdx.x * 2 * b.trav_cap;
     uint32_t* tlist = tstack + b.trav_cap;
 
@@ -862,87 +944,96 @@ __global__ void __launch_bounds__(64) dfs_replay(
         if (t >= T) return;
         const ReplayTask task = tasks[t];
         const uint32_t p = task.pore;
-        if (task.seed >= b.n) {
+        if (task.seed >= b.n || task.ncls > seen_words * 32 || task.list_off + task.list_len > b.lists_len ||
+            (unsigned long long)task.rank_base + task.ncls > b.ranks_len) {
             if (lane == 0) fail_check(err, kDfsReplay << 8 | 1);
             return;
         }
         for (unsigned long long i = lane; i < vis_words; i += 64) vis[i] = 0;
         for (unsigned long long i = lane; i < 2 * pent_words; i += 64) pent[i] = 0;  // entered, closed
-        for (unsigned long long i = lane; i < seen_words; i += 64) seen[i] = 0;
+        for (uint32_t i = lane; i < seen_words; i += 64) seen[i] = 0;
+        bool bad_list = false;
+        for (uint32_t i = lane; i < task.list_len; i += 64) {
+            const unsigned long long e = lists[task.list_off + i];
+            const uint32_t q = (uint32_t)(e >> 32), l = (uint32_t)e;
+            if (q >= b.P || (l != kNoClass && l >= task.ncls)) bad_list = true;
+            else cls[q] = (uint16_t)l;
+        }
+        if (__ballot(bad_list)) {
+            if (lane == 0) fail_check(err, kDfsReplay << 8 | 2);
+            return;
+        }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");  // the cleared words are in L2 before the atomics
         __builtin_amdgcn_wave_barrier();  // every lane has cleared its share before lane 0 marks the seed
 
-        // Lane 0: ranks class event ev if it is new; returns the new count of ranked classes.
-        auto rank_event = [&](uint32_t ev, uint32_t found) -> uint32_t {
-            if (ev == kInv) return found;
-            const uint32_t old = atomicOr(&seen[ev >> 5], 1u << (ev & 31));
-            if (old & (1u << (ev & 31))) return found;
-            rank[ev] = found;
+        // Lane 0: ranks class l if it is new; returns the new count of ranked classes.
+        auto rank_class = [&](uint32_t l, uint32_t found) -> uint32_t {
+            if (l == kNoClass) return found;
+            const uint32_t old = atomicOr(&seen[l >> 5], 1u << (l & 31));
+            if (old & (1u << (l & 31))) return found;
+            rank_slot[task.rank_base + l] = found;
             return found + 1;
         };
-        // Lane 0: on the walk's first entry into plateau q, ranks q's class if it is new.
-        // Returns the new count of ranked classes, or kInv after a failed check.
-        auto enter = [&](uint32_t q, uint32_t found) -> uint32_t {
-            uint32_t ev;
-            if (!class_of(q, p, set_off, set_len, pool, cls_at, b, ev)) {
-                fail_check(err, kDfsReplay << 8 | 3);  // p does not reach a plateau its DFS entered
-                return kInv;
-            }
-            const unsigned long long pw = atomicOr(&pent[q >> 6], 1ull << (q & 63));
-            if (pw & (1ull << (q & 63))) return found;
-            return rank_event(ev, found);
-        };
-
         uint32_t u = task.seed;
         uint32_t uq = plat[u];
         uint32_t cu = cr[u];
+        if (uq >= b.P) {
+            if (lane == 0) fail_check(err, kDfsReplay << 8 | 3);
+            return;
+        }
+        const uint32_t seed_cls = cls[uq];
         uint32_t found = 0;
         if (lane == 0) {
             atomicOr(&vis[u >> 6], 1ull << (u & 63));
-            found = enter(uq, 0);
+            atomicOr(&pent[uq >> 6], 1ull << (uq & 63));
+            found = rank_class(seed_cls, 0);
         }
         found = bcast(found, 0);
-        if (found == kInv) return;
-        unsigned long long top = 0, steps = 0;
+        // The stack: frames [0, gtop) in global memory, [gtop, top) in the LDS ring at index % kRing.
+        unsigned long long top = 0, gtop = 0, steps = 0;
         int k = 27;
         bool overflow = false;
         while (found + 1 < task.ncls) {
             int x, y, z;
             decode(u, X, Y, x, y, z);
             bool cand = false, entered = true;
-            uint32_t j = 0, qn = 0, cn = 0;
+            uint32_t j = 0, qn = 0, cn = 0, ql = kNoClass;
             if (lane < k && lane < 27 && in_stencil(lane, d3q7) && neighbour(lane, x, y, z, X, Y, Z, j)) {
                 cn = cr[j];
                 qn = plat[j];
                 const unsigned long long w = __hip_atomic_load(&vis[j >> 6], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 cand = cn != 0 && cn <= cu && !(w & (1ull << (j & 63)));
-                // neighbours in other plateaus: closed plateaus are visited; the entered test is
-                // for the lane whose neighbour is taken (an index out of range reads as not entered,
-                // so that enter() reports it)
+                // neighbours in other plateaus: closed plateaus are visited; the entered test and the
+                // class are for the lane whose neighbour is taken
                 if (cand && qn != uq && qn < b.P) {
                     entered = bit_of(pent, qn);
+                    ql = cls[qn];
                     if (bit_of(pclosed, qn)) cand = false;
-                } else if (cand && qn != uq) {
-                    entered = false;
                 }
             }
             const unsigned long long bal = __ballot(cand);
             if (!bal) {
                 if (top == 0) break;  // stack empty: the pore's reach is exhausted
-                --top;                // pop; the frame below is read back by lane 0
-                uint32_t fu = 0, fk = 0;
-                if (lane == 0) {
-                    fu = sv[top];
-                    fk = sk[top];
+                if (top - 1 < gtop) {
+                    // the ring is empty: bring back the newest half ring of frames from global memory
+                    const unsigned long long h = min(gtop, (unsigned long long)(kRing / 2));
+                    for (unsigned long long i = lane; i < h; i += 64) {
+                        const unsigned long long at = gtop - h + i;
+                        ring[at % kRing] = gstack[at];
+                    }
+                    gtop -= h;
+                    __builtin_amdgcn_wave_barrier();
                 }
-                u = bcast(fu, 0);
-                k = (int)bcast(fk, 0);
-                if (u >= b.n || k > 27) {
+                --top;  // pop: every lane reads the frame from LDS
+                const uint4 f = ring[top % kRing];
+                u = f.x;
+                uq = f.y;
+                cu = f.z;
+                k = (int)f.w;
+                if (u >= b.n || uq >= b.P || k > 27) {
                     if (lane == 0) fail_check(err, kDfsReplay << 8 | 5);
                     return;
                 }
-                uq = plat[u];
-                cu = cr[u];
                 continue;
             }
             const int kk = 63 - __clzll((long long)bal);  // highest neighbour index below k
@@ -950,90 +1041,120 @@ __global__ void __launch_bounds__(64) dfs_replay(
             const uint32_t nq = bcast(qn, kk);
             const uint32_t ncr = bcast(cn, kk);
             const bool is_new = !bcast(entered ? 1u : 0u, kk);  // first entry into plateau nq
-            if (is_new && nq < b.P) {
+            if (nq >= b.P) {
+                if (lane == 0) fail_check(err, kDfsReplay << 8 | 4);
+                return;
+            }
+            if (is_new) {
                 // an excursion into nq: skipped when it holds at most one unranked class
                 bool skip = false;
-                uint32_t skip_ev = kInv;
+                uint32_t skip_cls = kNoClass;
                 if (below[nq] > p) {
-                    uint32_t ev = kInv;
-                    bool ok = true;
-                    if (lane == 0) ok = class_of(nq, p, set_off, set_len, pool, cls_at, b, ev);
-                    if (!bcast(ok ? 1u : 0u, 0)) {
-                        if (lane == 0) fail_check(err, kDfsReplay << 8 | 7);
-                        return;
-                    }
                     skip = true;
-                    skip_ev = bcast(ev, 0);
+                    skip_cls = bcast(ql, kk);
                     if (lane == 0) {
                         atomicOr(&pent[nq >> 6], 1ull << (nq & 63));
                         atomicOr(&pclosed[nq >> 6], 1ull << (nq & 63));
                     }
                     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");  // marks land before the next loads
                 } else {
-                    // traversal: stack tstack[0 .. tcount[0]), every pushed plateau listed in tlist
+                    // traversal: stack tstack[0 .. tcount[0]), every plateau it marks listed in tlist.
+                    // It marks plateaus in the entered bitmap itself (the walk waits meanwhile), so one
+                    // atomic both tests "entered or already met" and marks; a skip keeps the marks, a
+                    // step takes them back.
                     if (lane == 0) {
-                        atomicOr(&ptmp[nq >> 6], 1ull << (nq & 63));
+                        atomicOr(&pent[nq >> 6], 1ull << (nq & 63));
                         __hip_atomic_store(&tstack[0], nq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         __hip_atomic_store(&tlist[0], nq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         tcount[0] = 1;
                         tcount[1] = 1;
                     }
+                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
                     __builtin_amdgcn_wave_barrier();
-                    uint32_t nun = 0, k1 = kInv;
+                    uint32_t nun = 0, k1 = kNoClass;
                     bool room = true;
                     for (uint32_t ts = 1; ts > 0;) {
+                        // pop up to 64 plateaus, one per lane
                         const uint32_t nb = min(ts, 64u);
-                        uint32_t bq = kInv, ev = kInv;
-                        bool bad = false;
+                        uint32_t bq = kInv, l = kNoClass, cnt = 0, st = 0;
+                        bool bad = false, expand = false;
                         if ((uint32_t)lane < nb) {
                             bq = __hip_atomic_load(&tstack[ts - 1 - lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                            bad = !class_of(bq, p, set_off, set_len, pool, cls_at, b, ev);
+                            if (bq >= b.P) {
+                                bad = true;
+                            } else {
+                                l = cls[bq];
+                                expand = below[bq] <= p;  // below a static plateau all share its class
+                                bad = l != kNoClass && l >= task.ncls;
+                                st = out_start[bq];
+                                cnt = out_count[bq];
+                                if ((unsigned long long)st + cnt > b.E) bad = true;
+                            }
                         }
                         if (__ballot(bad)) {
                             if (lane == 0) fail_check(err, kDfsReplay << 8 | 8);
                             return;
                         }
-                        const bool unr = ev != kInv && !((__hip_atomic_load(&seen[ev >> 5], __ATOMIC_RELAXED,
-                                                                              __HIP_MEMORY_SCOPE_AGENT) >> (ev & 31)) & 1u);
+                        const bool unr = l != kNoClass && !((seen[l >> 5] >> (l & 31)) & 1u);
                         const unsigned long long ub = __ballot(unr);
                         if (ub) {
-                            if (k1 == kInv) k1 = bcast(ev, __ffsll((long long)ub) - 1);
-                            nun = __ballot(unr && ev != k1) ? 2u : 1u;
+                            if (k1 == kNoClass) k1 = bcast(l, __ffsll((long long)ub) - 1);
+                            nun = __ballot(unr && l != k1) ? 2u : 1u;
                         }
                         if (nun >= 2) break;
+                        if (!expand) cnt = 0;
+                        // out-edges of the popped plateaus, four per lane per round: inclusive prefix of counts
+                        uint32_t incl = cnt;
+                        for (int d = 1; d < 64; d <<= 1) {
+                            const uint32_t o = __shfl_up(incl, d);
+                            if (lane >= d) incl += o;
+                        }
+                        const uint32_t total = bcast(incl, 63);
+                        const uint32_t first = incl - cnt;
                         if (lane == 0) tcount[0] = ts - nb;  // the popped entries are in registers
                         __builtin_amdgcn_wave_barrier();
                         bool lane_room = true;
-                        if ((uint32_t)lane < nb && below[bq] <= p) {
-                            const uint32_t e0 = out_start[bq], ne = out_count[bq];
-                            if ((unsigned long long)e0 + ne > b.E) {
-                                bad = true;
-                            } else {
-                                for (uint32_t e = e0; e < e0 + ne; ++e) {
-                                    const uint32_t d = (uint32_t)oedges[e];
-                                    if (d >= b.P) {
-                                        bad = true;
-                                        break;
-                                    }
-                                    if (bit_of(pent, d)) continue;
-                                    const unsigned long long m1 = 1ull << (d & 63);
-                                    if (atomicOr(&ptmp[d >> 6], m1) & m1) continue;
-                                    const uint32_t w = atomicAdd(&tcount[1], 1u);
-                                    if (w >= b.trav_cap) {  // not listed: take the mark back
-                                        atomicAnd(&ptmp[d >> 6], ~m1);
-                                        lane_room = false;
-                                        continue;
-                                    }
-                                    __hip_atomic_store(&tlist[w], d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                                    const uint32_t at = atomicAdd(&tcount[0], 1u);  // at <= w: every entry is listed
-                                    __hip_atomic_store(&tstack[at], d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        for (uint32_t base = 0; base < total; base += 256) {
+                            uint32_t dd[4];
+                            bool have[4];
+                            unsigned long long old[4];
+#pragma unroll
+                            for (int u = 0; u < 4; ++u) {
+                                const uint32_t i = base + 64 * u + lane;
+                                // owner lane of edge i: the number of lanes whose inclusive count is <= i
+                                int o = 0;
+                                for (int s = 32; s >= 1; s >>= 1)
+                                    if (__shfl(incl, o + s - 1) <= i) o += s;
+                                const uint32_t o_start = __shfl(st, o), o_first = __shfl(first, o);
+                                have[u] = i < total;
+                                dd[u] = have[u] ? (uint32_t)oedges[o_start + (i - o_first)] : kInv;
+                            }
+#pragma unroll
+                            for (int u = 0; u < 4; ++u) {
+                                old[u] = ~0ull;
+                                if (have[u] && dd[u] < b.P) old[u] = atomicOr(&pent[dd[u] >> 6], 1ull << (dd[u] & 63));
+                                else if (have[u]) bad = true;
+                            }
+#pragma unroll
+                            for (int u = 0; u < 4; ++u) {
+                                const uint32_t d = dd[u];
+                                if (!have[u] || d >= b.P || ((old[u] >> (d & 63)) & 1ull)) continue;  // entered or met
+                                const uint32_t w = atomicAdd(&tcount[1], 1u);
+                                if (w >= b.trav_cap) {  // not listed: take the mark back
+                                    atomicAnd(&pent[d >> 6], ~(1ull << (d & 63)));
+                                    lane_room = false;
+                                    continue;
                                 }
+                                __hip_atomic_store(&tlist[w], d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                                const uint32_t at = atomicAdd(&tcount[0], 1u);  // at <= w: every entry is listed
+                                __hip_atomic_store(&tstack[at], d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                             }
                         }
                         if (__ballot(bad)) {
-                            if (lane == 0) fail_check(err, kDfsReplay << 8 | 9);
+                            if (lane == 0) fail_check(err, kDfsReplay << 8 | 10);
                             return;
                         }
+                        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
                         __builtin_amdgcn_wave_barrier();
                         if (__ballot(!lane_room)) {
                             room = false;
@@ -1048,37 +1169,40 @@ __global__ void __launch_bounds__(64) dfs_replay(
                         const uint32_t d = __hip_atomic_load(&tlist[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         if (d >= b.P) continue;
                         const unsigned long long m1 = 1ull << (d & 63);
-                        atomicAnd(&ptmp[d >> 6], ~m1);
-                        if (skip) {
-                            atomicOr(&pent[d >> 6], m1);
-                            atomicOr(&pclosed[d >> 6], m1);
-                        }
+                        if (skip) atomicOr(&pclosed[d >> 6], m1);
+                        else atomicAnd(&pent[d >> 6], ~m1);
                     }
                     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
                     __builtin_amdgcn_wave_barrier();
-                    skip_ev = nun == 1 ? k1 : kInv;
+                    skip_cls = nun == 1 ? k1 : kNoClass;
                 }
                 if (skip) {
-                    if (lane == 0) found = rank_event(skip_ev, found);
+                    if (lane == 0) found = rank_class(skip_cls, found);
                     found = bcast(found, 0);
                     k = kk;  // go on with the neighbours below kk
                     continue;
                 }
             }
-            if (top + 1 >= stack_cap) {  // no room for this frame: the host repeats the pore
-                overflow = true;
-                break;
+            if (top - gtop == kRing) {
+                // the ring is full: move its oldest half to global memory
+                if (gtop + kRing / 2 > stack_cap) {  // no room: the host repeats the pore with a larger stack
+                    overflow = true;
+                    break;
+                }
+                for (unsigned long long i = lane; i < kRing / 2; i += 64) gstack[gtop + i] = ring[(gtop + i) % kRing];
+                gtop += kRing / 2;
+                __builtin_amdgcn_wave_barrier();
             }
+            const uint32_t ncls_new = bcast(ql, kk);
             if (lane == 0) {
-                sv[top] = u;  // save this frame, resuming below kk
-                sk[top] = (uint8_t)kk;
+                ring[top % kRing] = make_uint4(u, uq, cu, (uint32_t)kk);  // this frame resumes below kk
                 atomicOr(&vis[nvox >> 6], 1ull << (nvox & 63));
-                if (is_new) found = enter(nq, found);
+                if (is_new) {
+                    atomicOr(&pent[nq >> 6], 1ull << (nq & 63));
+                    found = rank_class(ncls_new, found);
+                }
             }
-            if (is_new) {
-                found = bcast(found, 0);
-                if (found == kInv) return;
-            }
+            if (is_new) found = bcast(found, 0);
             ++top;
             ++steps;
             u = nvox;
@@ -1351,7 +1475,16 @@ void run_pores(pma_ctx* c) {
         for (size_t l = 0; l < L; ++l)
             PMA_REQUIRE(lstart_h[l] < lstart_h[l + 1], PMA_ERR_INTERNAL, "pma_pores: level starts out of order");
     }
-    unsigned long long pool_cap = std::max<unsigned long long>(8 * P, 1024);
+    // Room for the sets: a run that overflows is repeated in full, so start with what the free
+    // device memory allows (up to PMA_PORES_POOL_MAX members per plateau), at least
+    // PMA_PORES_POOL_MIN per plateau.
+    unsigned long long pool_cap = std::max<unsigned long long>(PMA_PORES_POOL_MIN * P, PMA_PORES_POOL_FLOOR);
+    {
+        size_t free_b = 0, total_b = 0;
+        PMA_HIP(hipMemGetInfo(&free_b, &total_b));
+        const unsigned long long room = free_b / 8 / sizeof(uint32_t);
+        pool_cap = std::max<unsigned long long>(pool_cap, std::min<unsigned long long>(PMA_PORES_POOL_MAX * P, room));
+    }
     uint32_t* pool = nullptr;
     unsigned long long* pool_top = c->buf<unsigned long long>("pt_pool_top", 1);
     for (int attempt = 0;; ++attempt) {
@@ -1475,56 +1608,93 @@ void run_pores(pma_ctx* c) {
                                                                       set_off, set_len, pool, sb, below, err);
                 PMA_CHECKED(c, "below_min", err);
             }
-            uint32_t* cls_at = c->buf<uint32_t>("pt_cls_at", pool_cap);
-            class_events<<<grid_for(P, 256), 256, 0, c->stream>>>(set_off, set_len, rbase, rd, sb, R, D, cls_at, err);
-            PMA_CHECKED(c, "class_events", err);
+            // per replayed pore: its (plateau, class) pairs, and where its class ranks go
+            std::vector<uint32_t> ev_local(D), seen_cls(npores, 0), replayed_h(npores, 0);
+            for (uint32_t d = 0; d < D; ++d) ev_local[d] = seen_cls[ev_owner[d]]++;
+            std::vector<uint64_t> rank_base(npores, 0);
+            uint64_t ranks_len = 0;
+            uint32_t max_ncls = 1;
+            for (const ReplayTask& t : all) {
+                replayed_h[t.pore] = 1;
+                rank_base[t.pore] = ranks_len;
+                ranks_len += t.ncls;
+                max_ncls = std::max(max_ncls, t.ncls);
+            }
+            PMA_REQUIRE(max_ncls < kNoClassU && ranks_len < 0xFFFFFFFFull, PMA_ERR_CAPACITY,
+                        "pma_pores: a pore forms 65535 or more throat keys (the DFS replay numbers them in 16 bits)");
+            uint32_t* d_replayed = c->buf<uint32_t>("pt_replayed", npores);
+            uint32_t* d_evl = c->buf<uint32_t>("pt_ev_local", std::max(D, 1u));
+            unsigned long long* lcnt = c->buf<unsigned long long>("pt_list_count", npores);
+            unsigned long long* loff = c->buf<unsigned long long>("pt_list_off", npores);
+            unsigned long long* lcur = c->buf<unsigned long long>("pt_list_cursor", npores);
+            upload(c, d_replayed, replayed_h.data(), npores * 4ull);
+            upload(c, d_evl, ev_local.data(), D * 4ull);
+            PMA_HIP(hipMemsetAsync(lcnt, 0, npores * 8ull, c->stream));
+            PMA_HIP(hipMemsetAsync(lcur, 0, npores * 8ull, c->stream));
+            list_counts<<<grid_for(P, 256), 256, 0, c->stream>>>(set_off, set_len, pool, sb, d_replayed, lcnt, err);
+            PMA_CHECKED(c, "list_counts", err);
+            const uint64_t lists_len = exclusive_scan_u64(c, reinterpret_cast<const uint64_t*>(lcnt),
+                                                          reinterpret_cast<uint64_t*>(loff), npores);
+            unsigned long long* lists = c->buf<unsigned long long>("pt_lists", std::max<uint64_t>(lists_len, 1));
+            list_fill<<<grid_for(P, 256), 256, 0, c->stream>>>(set_off, set_len, pool, rbase, rd, d_evl, sb, R, D,
+                                                             d_replayed, loff, lcur, lists, lists_len, err);
+            PMA_CHECKED(c, "list_fill", err);
             check_stage(c, err);
+            std::vector<unsigned long long> lcnt_h(npores), loff_h(npores);
+            PMA_HIP(hipMemcpyAsync(lcnt_h.data(), lcnt, npores * 8ull, hipMemcpyDeviceToHost, c->stream));
+            PMA_HIP(hipMemcpyAsync(loff_h.data(), loff, npores * 8ull, hipMemcpyDeviceToHost, c->stream));
+            PMA_HIP(hipStreamSynchronize(c->stream));
+            for (ReplayTask& t : all) {
+                PMA_REQUIRE(lcnt_h[t.pore] < 0xFFFFFFFFull, PMA_ERR_INTERNAL, "pma_pores: pore list too long");
+                t.list_off = loff_h[t.pore];
+                t.list_len = (uint32_t)lcnt_h[t.pore];
+                t.rank_base = (uint32_t)rank_base[t.pore];
+            }
 
-            uint32_t* d_rank = c->buf<uint32_t>("pt_rank", std::max(D, 1u));
+            uint32_t* d_rank = c->buf<uint32_t>("pt_rank_slot", std::max<uint64_t>(ranks_len, 1));
             uint32_t* status = c->buf<uint32_t>("pt_task_status", T);
             unsigned long long* steps = c->buf<unsigned long long>("pt_task_steps", T);
             uint32_t* queue = c->buf<uint32_t>("pt_queue", 1);
             ReplayTask* d_tasks = c->buf<ReplayTask>("pt_tasks", T);
-            PMA_HIP(hipMemsetAsync(d_rank, 0xFF, D * 4ull, c->stream));
-            // Per wave: a visited bitmap over all voxels, three bitmaps over plateaus (entered,
-            // closed, traversal marks), a bitmap over key events, a stack and the traversal
-            // stack and list.  The number of waves follows from the device memory free for them.
-            const unsigned long long vis_words = n / 64 + 1, pent_words = P / 64 + 1, seen_words = D / 32 + 1;
+            PMA_HIP(hipMemsetAsync(d_rank, 0xFF, ranks_len * 4ull, c->stream));
+            // Per wave: a visited bitmap over all voxels, two bitmaps over plateaus (entered,
+            // closed), the dense class table, a stack and the traversal stack
+            // and list.  The number of waves follows from the device memory free for them.
+            const unsigned long long vis_words = n / 64 + 1, pent_words = P / 64 + 1;
+            const uint32_t seen_words = (max_ncls + 31) / 32;
             const unsigned long long trav_cap = std::min<unsigned long long>(P + 1, PMA_PORES_TRAV_CAP);
             unsigned long long max_reach = 0;
             for (const ReplayTask& t : all) max_reach = std::max<unsigned long long>(max_reach, reach_h[t.pore]);
             size_t free_b = 0, total_b = 0;
             PMA_HIP(hipMemGetInfo(&free_b, &total_b));
             const unsigned long long budget =
-                free_b > (40ull << 30) ? std::min<unsigned long long>(free_b - (24ull << 30), 64ull << 30) : free_b / 2;
+                free_b > (40ull << 30) ? std::min<unsigned long long>(free_b - (24ull << 30), PMA_PORES_REPLAY_MEM)
+                                       : free_b / 2;
             std::vector<ReplayTask> pending = all;
             for (int pass = 0; !pending.empty(); ++pass) {
                 PMA_REQUIRE(pass < 2, PMA_ERR_INTERNAL, "pma_pores: DFS replay stack overflow with full capacity");
                 // pass 0: stacks of up to PMA_PORES_STACK_CAP frames; pass 1: room for the whole reach
                 const unsigned long long cap =
                     pass == 0 ? std::min<unsigned long long>(max_reach + 2, PMA_PORES_STACK_CAP) : max_reach + 2;
-                const unsigned long long per_wave =
-                    (vis_words + 3 * pent_words) * 8 + seen_words * 4 + cap * 5 + trav_cap * 8;
+                const unsigned long long per_wave = (vis_words + 2 * pent_words) * 8 + P * 2 + cap * 16 + trav_cap * 8;
                 const unsigned long long W =
                     std::max<unsigned long long>(1, std::min<unsigned long long>({pending.size(), budget / per_wave, 1024}));
                 unsigned long long* vis = c->buf<unsigned long long>("pt_replay_vis", W * vis_words);
-                unsigned long long* pbits = c->buf<unsigned long long>("pt_replay_plateaus", W * 3 * pent_words);
-                uint32_t* seen = c->buf<uint32_t>("pt_replay_seen", W * seen_words);
-                uint32_t* stack_voxel = c->buf<uint32_t>("pt_stack_voxel", W * cap);
-                uint8_t* stack_k = c->buf<uint8_t>("pt_stack_k", W * cap);
+                unsigned long long* pbits = c->buf<unsigned long long>("pt_replay_plateaus", W * 2 * pent_words);
+                uint16_t* cls = c->buf<uint16_t>("pt_replay_cls", W * P);
+                uint4* stack = c->buf<uint4>("pt_stack", W * cap);
                 uint32_t* trav = c->buf<uint32_t>("pt_replay_trav", W * 2 * trav_cap);
-                PMA_HIP(hipMemsetAsync(pbits, 0, W * 3 * pent_words * 8, c->stream));  // traversal marks start clear
-                upload(c, d_tasks, pending.data(), pending.size() * sizeof(ReplayTask));
+                                upload(c, d_tasks, pending.data(), pending.size() * sizeof(ReplayTask));
                 PMA_HIP(hipMemsetAsync(status, 0, pending.size() * 4, c->stream));
-                const ReplayBounds rb{n, P, pool_cap, D, E, trav_cap};
+                const ReplayBounds rb{n, P, E, trav_cap, lists_len, ranks_len};
                 PMA_HIP(hipMemsetAsync(queue, 0, 4, c->stream));
                 const auto t0 = std::chrono::steady_clock::now();
                 {
                     PMA_TIMED(c, "dfs_replay");
-                    dfs_replay<<<(unsigned)W, 64, 0, c->stream>>>(
-                        d_tasks, (uint32_t)pending.size(), queue, cr, plat, set_off, set_len, pool, cls_at, below,
-                        out_start, out_count, oedges, vis, vis_words, pbits, pent_words, seen, seen_words,
-                        stack_voxel, stack_k, cap, trav, d_rank, status, steps, X, Y, Z, d3q7, rb, err);
+                    dfs_replay<<<(unsigned)W, 64, seen_words * 4, c->stream>>>(
+                        d_tasks, (uint32_t)pending.size(), queue, cr, plat, lists, below, out_start, out_count, oedges,
+                        vis, vis_words, pbits, pent_words, cls, seen_words, stack, cap, trav, d_rank,
+                        status, steps, X, Y, Z, d3q7, rb, err);
                     PMA_CHECKED(c, "dfs_replay", err);
                 }
                 if (c->debug_sync)
@@ -1549,11 +1719,14 @@ void run_pores(pma_ctx* c) {
                     if (st[i]) failed.push_back(pending[i]);
                 pending.swap(failed);
             }
-            PMA_HIP(hipMemcpyAsync(rank_h.data(), d_rank, D * 4ull, hipMemcpyDeviceToHost, c->stream));
+            std::vector<uint32_t> rank_slot(ranks_len);
+            PMA_HIP(hipMemcpyAsync(rank_slot.data(), d_rank, ranks_len * 4ull, hipMemcpyDeviceToHost, c->stream));
             PMA_HIP(hipStreamSynchronize(c->stream));
-            // the per-slot buffers are sized by the free memory: give it back
-            for (const char* name : {"pt_replay_vis", "pt_replay_plateaus", "pt_replay_seen", "pt_stack_voxel",
-                                     "pt_stack_k", "pt_replay_trav"})
+            for (uint32_t d = 0; d < D; ++d)
+                if (replayed_h[ev_owner[d]]) rank_h[d] = rank_slot[rank_base[ev_owner[d]] + ev_local[d]];
+            // the per-slot buffers are sized by the free memory: give them back
+            for (const char* name : {"pt_replay_vis", "pt_replay_plateaus", "pt_replay_cls", "pt_stack",
+                                     "pt_replay_trav", "pt_lists"})
                 c->drop(name);
         }
         c->stage[PMA_STAGE_PORES].work_items = replay_steps;
