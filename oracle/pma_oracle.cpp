@@ -13,23 +13,32 @@
 // (ImageProcessing/Services/Source/Serializer.cpp:139-221), so a stage can be fed the
 // previous stage's text exactly like the reference feeds it the previous stage's files.
 //
-// Parity pinning (what checks each function):
+// Parity pinning (what checks each function; tests/test_oracle_example.py and
+// tests/test_oracle_bruteforce.py):
 //   oracle_edt              : Docs/Example EuclideanDistanceSquares bit-exact, histogram and
 //                             max file bit-exact; brute force on tiny volumes; scipy EDT of
 //                             an x-padded array.
 //   oracle_containing_balls : Docs/Example ContainingBallRadiiSquares bit-exact; brute force
 //                             (max over covering balls, lexicographic centre) on tiny volumes.
 //   oracle_pores            : Docs/Example PoreThroatIds partition (up to relabelling, fed the
-//                             fixture's own mask/EDT/CR); seeds = regional-max plateaus and
-//                             pore sets = per-seed reachability by brute force on tiny volumes.
-//                             ID numbering follows libstdc++ std::sort tie order: parity
-//                             unpinned beyond the restatement itself (the only fixture was
-//                             made with another STL).
-//   oracle_remove_empty_pores : rules exercised on hand-made tables with known answers.
+//                             fixture's own mask/EDT/CR); on tiny volumes, both stencils: seeds =
+//                             the first libstdc++-sorted maximum ball of each regional-maximum CR
+//                             plateau (scipy labels) in pore-ID order; every voxel's list = the
+//                             pores whose seed reaches it by BFS, in [second, first, ...] order;
+//                             volumes and boundary flags by the same reachability; the DFS order
+//                             of key numbering on a row worked out by hand.  Key numbering on
+//                             larger inputs rests on the restatement (the only fixture was made
+//                             with another STL).
+//   oracle_remove_empty_pores : a hand-made table with known answers (drop, fold into the one
+//                             remaining pore, drop without remap, merge duplicates into the
+//                             largest ID).  The Docs/Example removes nothing.
 //   oracle_watershed        : Docs/Example PoreThroatIdsWatershed partition (up to
-//                             relabelling); key-ID numbering parity unpinned as above.
+//                             relabelling); the FIFO order and throat-key order on a row worked
+//                             out by hand, both stencils.
 //   oracle_geodesic         : Docs/Example ShortestPathDistances + CavityIds bit-exact,
-//                             cavityDescriptions.txt; float32 Bellman-Ford on tiny volumes.
+//                             cavityDescriptions.txt; float32 Bellman-Ford from every cavity's
+//                             seed on tiny volumes with up to 4 cavities, both stencils; box-shell
+//                             restarts worked out by hand (shell and split volumes).
 #include <algorithm>
 #include <array>
 #include <iterator>
