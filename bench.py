@@ -12,8 +12,8 @@ Prints one JSON line:
   stage_roofline  per stage: compulsory bytes (each input read once, each output written
              once, per voxel) / stage time, against the MI355X HBM peak of 8 TB/s,
   roofline   the dominant instrumented kernel: algorithmic bytes per launch / its average
-             launch time from HIP events recorded on the context's stream in the timed
-             region,
+             launch time from HIP events recorded on the context's stream, in one extra step
+             after the timed ones (the timed steps run without per-kernel events),
   cpu_baseline  the CPU oracle (a restatement of the reference: OpenMP in the EDT and
              containing-ball loops as in the reference, serial elsewhere) running all six
              stages on bounded samples of the same generators.
@@ -54,6 +54,10 @@ KERNEL_BYTES_PER_VOXEL = {
     "plateau_edges": 4 + 4,
     "write_labels": 4 + 4,      # u32 plateau index -> u32 PoreThroatIds
 }
+# Kernels that carry a whole stage: the stage's compulsory bytes per voxel, spread over the
+# kernel's launches in a step.  dfs_replay is the pore propagation (CR in, PT out: 8 B/voxel);
+# ws_resolve does the watershed's claims and labels (CR in, PTW out: 8 B/voxel).
+STAGE_KERNEL_BYTES_PER_VOXEL = {"dfs_replay": 8, "ws_resolve": 8}
 # Per block visit of one relax_blocks launch: the 8^3 distances read and written once (the
 # one-voxel halo is shared with the neighbouring blocks and not counted).
 RELAX_BYTES_PER_BLOCK = 512 * (4 + 4)
@@ -68,7 +72,8 @@ def parse():
     ap.add_argument("--sigma", type=float, default=8.0)
     ap.add_argument("--porosity", type=float, default=0.65)
     ap.add_argument("--seed", type=int, default=20261203)
-    ap.add_argument("--cpu-size", type=int, default=128)
+    ap.add_argument("--cpu-sizes", type=int, nargs="+", default=[128, 192])
+    ap.add_argument("--cpu-cap", type=float, default=30.0, help="wall seconds per CPU stage")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stencil", choices=["d3q27", "d3q7"], default="d3q27")
@@ -115,32 +120,48 @@ def oracle_stages(oracle, solid, d3q7, threads):
 
 
 def cpu_baseline(args):
-    """The oracle's six stages on a bounded sample of the benchmark's generator (the value) and
-    on the 128^3 sphere-union configuration, timed on this host."""
+    """The oracle's six stages, timed on this host: samples of the benchmark's generator (128^3,
+    then larger while every stage stays under the per-stage cap) and the 128^3 sphere-union
+    configuration.  The value is voxels/s on the largest sample that finished under the cap."""
     import oracle
     import porousmediaanalysis_amd as pma
 
-    n = args.cpu_size
     d3q7 = args.stencil == "d3q7"
-    with pma.Context(0) as ctx:
-        ctx.generate_gaussian((n, n, n), args.sigma, args.porosity, args.seed + 1)
-        gauss = ctx.read(pma.VOL_SOLID)
-        ctx.generate_spheres((128, 128, 128), 300, 6.0, 12.0, 2024)
-        spheres = ctx.read(pma.VOL_SOLID)
     # OpenMP threads as the host allows this process (OMP_NUM_THREADS, else every core)
     threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", 0) or os.cpu_count() or 1)
-    progress(f"cpu baseline: oracle, all six stages, {n}^3 Gaussian sample, {threads} threads")
-    tg = oracle_stages(oracle, gauss, d3q7, threads)
+    samples = {}
+    best = None
+    for n in args.cpu_sizes:
+        key = f"gaussian_{n}"
+        if best is not None and max(samples[best]["stages_s"].values()) * (n / samples[best]["size"]) ** 3 > args.cpu_cap:
+            samples[key] = {"size": n, "skipped": f"> cap: a stage would exceed {args.cpu_cap:g} s"}
+            continue
+        with pma.Context(0) as ctx:
+            ctx.generate_gaussian((n, n, n), args.sigma, args.porosity, args.seed + 1)
+            solid = ctx.read(pma.VOL_SOLID)
+        progress(f"cpu baseline: oracle, all six stages, {n}^3 Gaussian sample, {threads} threads")
+        t = oracle_stages(oracle, solid, d3q7, threads)
+        samples[key] = {"size": n, "voxels_per_s": n ** 3 / sum(t.values()),
+                        "stages_s": {k: round(v, 3) for k, v in t.items()}}
+        over = [k for k, v in t.items() if v > args.cpu_cap]
+        if over:
+            samples[key]["over_cap"] = over
+            break
+        best = key
+    with pma.Context(0) as ctx:
+        ctx.generate_spheres((128, 128, 128), 300, 6.0, 12.0, 2024)
+        spheres = ctx.read(pma.VOL_SOLID)
     progress("cpu baseline: oracle, all six stages, 128^3 sphere union")
     ts = oracle_stages(oracle, spheres, d3q7, threads)
-    total = sum(tg.values())
-    return {"value": n ** 3 / total, "unit": "voxels/s", "cores": threads, "kind": "port",
-            "sample": f"{n}^3 voxels of the benchmark's Gaussian-field generator (sigma={args.sigma:g}, "
+    samples["sphere_union_128"] = {"size": 128, "voxels_per_s": 128 ** 3 / sum(ts.values()),
+                                   "stages_s": {k: round(v, 3) for k, v in ts.items()}}
+    ref = samples[best] if best else samples["sphere_union_128"]
+    return {"value": ref["voxels_per_s"], "unit": "voxels/s", "cores": threads, "kind": "port",
+            "sample": f"{ref['size']}^3 voxels of the benchmark's Gaussian-field generator (sigma={args.sigma:g}, "
                       f"porosity={args.porosity:g}), all six stages: EDT and containing balls with OpenMP, "
-                      f"pores, removal, watershed and geodesic serial (as in the reference), {total:.1f} s",
-            "stages_s": {k: round(v, 3) for k, v in tg.items()},
-            "sphere_union_128": {"value": 128 ** 3 / sum(ts.values()), "unit": "voxels/s",
-                                 "stages_s": {k: round(v, 3) for k, v in ts.items()}},
+                      f"pores, removal, watershed and geodesic serial (as in the reference), "
+                      f"{sum(ref['stages_s'].values()):.1f} s; per-stage cap {args.cpu_cap:g} s",
+            "samples": samples,
             "host": {"cpu_model": cpu_model(), "os_cpu_count": os.cpu_count(),
                      "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS"), "threads_used": threads}}
 
@@ -205,7 +226,6 @@ def main():
             t0 = time.perf_counter()
             step(f"warmup {w + 1}/{args.warmup}")
             progress(f"warmup step {w + 1}/{args.warmup}: {time.perf_counter() - t0:.3f} s")
-        pma.lib().pma_set_kernel_timing(ctx._h, 1)
         stage_ms = {name: 0.0 for name in STAGES}
         stage_items = {name: 0 for name in STAGES}
         t0 = time.perf_counter()
@@ -218,8 +238,12 @@ def main():
                 stage_items[name] += info["work_items"]
             progress(f"step {s + 1}/{args.steps}: {time.perf_counter() - ts:.3f} s")
         elapsed = time.perf_counter() - t0
+        # one more step, outside the timed region, with HIP events around every instrumented kernel
+        pma.lib().pma_set_kernel_timing(ctx._h, 1)
+        step("kernel-timing step")
         kernels = kernel_times(pma, ctx)
         pma.lib().pma_set_kernel_timing(ctx._h, 0)
+        kstage_items = {name: ctx.stage_info(k)["work_items"] for k, name in enumerate(pma.STAGE_NAMES)}
         max_e, n_max = ctx.statistics()
         n_pores = ctx.text(pma.TEXT_PORES).count("\n") - 1
         n_throats = ctx.text(pma.TEXT_THROATS).count("\n") - 1
@@ -238,16 +262,20 @@ def main():
     if dom:
         name, (tot_ms, launches) = dom
         avg_ms = tot_ms / max(launches, 1)
-        bpv = KERNEL_BYTES_PER_VOXEL.get(name)
-        per_launch = bpv * nvox * args.steps / launches if bpv is not None and launches else None
-        if name == "relax_blocks" and launches:
-            per_launch = RELAX_BYTES_PER_BLOCK * stage_items["geodesic"] / launches
+        per_launch = None
+        if name in KERNEL_BYTES_PER_VOXEL:
+            per_launch = KERNEL_BYTES_PER_VOXEL[name] * nvox / max(launches, 1)
+        elif name in STAGE_KERNEL_BYTES_PER_VOXEL:
+            per_launch = STAGE_KERNEL_BYTES_PER_VOXEL[name] * nvox / max(launches, 1)
+        elif name == "relax_blocks" and launches:
+            per_launch = RELAX_BYTES_PER_BLOCK * kstage_items["geodesic"] / launches
         if per_launch is not None:
             achieved = per_launch / (avg_ms * 1e-3) / 1e9
-            roofline = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            roofline = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6),
                         "traffic": measured_traffic(name, S),
-                        "bytes_per_launch": round(per_launch), "avg_launch_ms": round(avg_ms, 4)}
+                        "bytes_per_launch": round(per_launch), "avg_launch_ms": round(avg_ms, 4),
+                        "launches_per_step": launches}
         else:
             roofline = {"bound": "hbm", "kernel": name, "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": None, "traffic": None, "avg_launch_ms": round(avg_ms, 4), "launches": launches,
@@ -278,7 +306,8 @@ def main():
         # EDT: voxels; containing balls: (ball, tile) pairs; pores: DFS replay frames;
         # geodesic: block relaxations
         "stage_work_items": {k: v // args.steps for k, v in stage_items.items()},
-        "kernels_ms": {k: {"total_ms_per_step": round(v[0] / args.steps, 3), "launches_per_step": v[1] / args.steps}
+        # from the one extra step with events around every instrumented kernel
+        "kernels_ms": {k: {"total_ms_per_step": round(v[0], 3), "launches_per_step": v[1]}
                        for k, v in sorted(kernels.items(), key=lambda kv: -kv[1][0])},
         "pipeline_algorithmic_GBs": round(pipeline_bytes / (ms_per_step * 1e-3) / 1e9, 1),
         "roofline": roofline,

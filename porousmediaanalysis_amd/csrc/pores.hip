@@ -43,10 +43,16 @@ namespace {
 
 constexpr uint32_t kInv = 0xFFFFFFFFu;
 #ifndef PMA_PORES_STACK_CAP
-#define PMA_PORES_STACK_CAP (1ull << 21)  // DFS frames per replaying wave in the first pass
+#define PMA_PORES_STACK_CAP (1ull << 19)  // DFS frames per replaying wave in the first pass
 #endif
 #ifndef PMA_PORES_TRAV_CAP
-#define PMA_PORES_TRAV_CAP (1ull << 22)  // plateaus one skip traversal may list (then it gives up)
+#define PMA_PORES_TRAV_CAP (1ull << 20)  // plateaus one skip traversal may list (then it gives up)
+#endif
+#ifndef PMA_PORES_VIS_PAGES
+#define PMA_PORES_VIS_PAGES 2048u  // visited-voxel pages per replaying wave in the first pass
+#endif
+#ifndef PMA_PORES_VIS_MIN_BITS
+#define PMA_PORES_VIS_MIN_BITS 16u  // voxels per visited-voxel page: 2^bits, at least
 #endif
 #ifndef PMA_PORES_RING
 #define PMA_PORES_RING 1024u  // DFS frames of a replaying wave kept in LDS
@@ -881,6 +887,7 @@ __global__ void reach_sizes(const unsigned long long* __restrict__ set_off, cons
 // ranked classes lives in LDS.
 constexpr uint16_t kNoClass = (uint16_t)kNoClassU;
 constexpr uint32_t kRing = PMA_PORES_RING;  // DFS frames kept in LDS per wave (a power of two)
+constexpr uint16_t kNoPage = 0xFFFF;
 
 struct ReplayTask {
     uint32_t pore, seed, ncls, rank_base;  // ranks of the pore's classes at rank_slot[rank_base + class]
@@ -914,12 +921,13 @@ __device__ __forceinline__ bool bit_of(const unsigned long long* words, uint32_t
 // that tries neighbours in reverse loop order visits voxels in the reference's stack-DFS order.
 // The replay of a pore stops once all but one of its classes are ranked: the last one is
 // necessarily found after them.  Words written by one lane and read by another go through
-// agent-scope atomics.  Dynamic LDS: the ranked-class bitmap, seen_words words.
+// agent-scope atomics.  Dynamic LDS: the ranked-class bitmap (seen_words words), then the
+// block-to-page table of the visited voxels.
 __global__ void __launch_bounds__(64) dfs_replay(
     const ReplayTask* __restrict__ tasks, uint32_t T, uint32_t* __restrict__ queue, const uint32_t* __restrict__ cr,
     const uint32_t* __restrict__ plat, const unsigned long long* __restrict__ lists,
     const uint32_t* __restrict__ below, const uint32_t* __restrict__ out_start, const uint32_t* __restrict__ out_count,
-    const unsigned long long* __restrict__ oedges, unsigned long long* vis_all, unsigned long long vis_words,
+    const unsigned long long* __restrict__ oedges, unsigned long long* vis_all, uint32_t vis_pages, uint32_t vis_bits,
     unsigned long long* pbits_all, unsigned long long pent_words, uint16_t* cls_all, uint32_t seen_words,
     uint4* __restrict__ stack_all, unsigned long long stack_cap,
     uint32_t* trav_all, uint32_t* __restrict__ rank_slot, uint32_t* __restrict__ task_status,
@@ -929,7 +937,11 @@ __global__ void __launch_bounds__(64) dfs_replay(
     __shared__ uint32_t tcount[2];      // traversal: stack length, list length
     __shared__ uint4 ring[kRing];       // the newest DFS frames (voxel, plateau, radius, next index)
     const int lane = threadIdx.x;
-    unsigned long long* vis = vis_all + (size_t)blockIdx.x * vis_words;
+    // visited voxels: blocks of 2^vis_bits voxels, each on an 8 KB-per-65536-voxel page of this
+    // slot's pool taken on the block's first visit (vtab in LDS: block -> page, kNoPage if none)
+    const uint32_t page_words = 1u << (vis_bits - 6), nblocks = (uint32_t)((b.n + (1ull << vis_bits) - 1) >> vis_bits);
+    unsigned long long* vpool = vis_all + (size_t)blockIdx.x * vis_pages * page_words;
+    uint16_t* vtab = reinterpret_cast<uint16_t*>(seen + seen_words);
     unsigned long long* pent = pbits_all + (size_t)blockIdx.x * 2 * pent_words;
     unsigned long long* pclosed = pent + pent_words;
     uint16_t* cls = cls_all + (size_t)blockIdx.x * b.P;
@@ -949,7 +961,8 @@ __global__ void __launch_bounds__(64) dfs_replay(
             if (lane == 0) fail_check(err, kDfsReplay << 8 | 1);
             return;
         }
-        for (unsigned long long i = lane; i < vis_words; i += 64) vis[i] = 0;
+        for (uint32_t i = lane; i < nblocks; i += 64) vtab[i] = kNoPage;
+        uint32_t npages = 0;
         for (unsigned long long i = lane; i < 2 * pent_words; i += 64) pent[i] = 0;  // entered, closed
         for (uint32_t i = lane; i < seen_words; i += 64) seen[i] = 0;
         bool bad_list = false;
@@ -974,6 +987,21 @@ __global__ void __launch_bounds__(64) dfs_replay(
             rank_slot[task.rank_base + l] = found;
             return found + 1;
         };
+        // Wave-wide: the page of voxel v's block, taken and zeroed on first use; false when the
+        // pool is full (the host repeats the pore with a larger pool).
+        auto ensure_page = [&](uint32_t v) -> bool {
+            const uint32_t blk = v >> vis_bits;
+            if (vtab[blk] != kNoPage) return true;
+            if (npages >= vis_pages) return false;
+            unsigned long long* pg = vpool + (size_t)npages * page_words;
+            for (uint32_t i = lane; i < page_words; i += 64) pg[i] = 0;
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");  // zeros land before the marks
+            __builtin_amdgcn_wave_barrier();
+            if (lane == 0) vtab[blk] = (uint16_t)npages;
+            ++npages;
+            __builtin_amdgcn_wave_barrier();
+            return true;
+        };
         uint32_t u = task.seed;
         uint32_t uq = plat[u];
         uint32_t cu = cr[u];
@@ -981,10 +1009,15 @@ __global__ void __launch_bounds__(64) dfs_replay(
             if (lane == 0) fail_check(err, kDfsReplay << 8 | 3);
             return;
         }
+        if (!ensure_page(u)) {  // a pool of one page at least: never taken
+            if (lane == 0) fail_check(err, kDfsReplay << 8 | 11);
+            return;
+        }
         const uint32_t seed_cls = cls[uq];
         uint32_t found = 0;
         if (lane == 0) {
-            atomicOr(&vis[u >> 6], 1ull << (u & 63));
+            atomicOr(&vpool[(size_t)vtab[u >> vis_bits] * page_words + ((u & ((1u << vis_bits) - 1)) >> 6)],
+                     1ull << (u & 63));
             atomicOr(&pent[uq >> 6], 1ull << (uq & 63));
             found = rank_class(seed_cls, 0);
         }
@@ -1001,7 +1034,11 @@ __global__ void __launch_bounds__(64) dfs_replay(
             if (lane < k && lane < 27 && in_stencil(lane, d3q7) && neighbour(lane, x, y, z, X, Y, Z, j)) {
                 cn = cr[j];
                 qn = plat[j];
-                const unsigned long long w = __hip_atomic_load(&vis[j >> 6], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const uint32_t pg = vtab[j >> vis_bits];
+                const unsigned long long w =
+                    pg == kNoPage ? 0ull
+                                  : __hip_atomic_load(&vpool[(size_t)pg * page_words + ((j & ((1u << vis_bits) - 1)) >> 6)],
+                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 cand = cn != 0 && cn <= cu && !(w & (1ull << (j & 63)));
                 // neighbours in other plateaus: closed plateaus are visited; the entered test and the
                 // class are for the lane whose neighbour is taken
@@ -1193,10 +1230,15 @@ __global__ void __launch_bounds__(64) dfs_replay(
                 gtop += kRing / 2;
                 __builtin_amdgcn_wave_barrier();
             }
+            if (!ensure_page(nvox)) {  // no page left: the host repeats the pore with a larger pool
+                overflow = true;
+                break;
+            }
             const uint32_t ncls_new = bcast(ql, kk);
             if (lane == 0) {
                 ring[top % kRing] = make_uint4(u, uq, cu, (uint32_t)kk);  // this frame resumes below kk
-                atomicOr(&vis[nvox >> 6], 1ull << (nvox & 63));
+                atomicOr(&vpool[(size_t)vtab[nvox >> vis_bits] * page_words + ((nvox & ((1u << vis_bits) - 1)) >> 6)],
+                         1ull << (nvox & 63));
                 if (is_new) {
                     atomicOr(&pent[nq >> 6], 1ull << (nq & 63));
                     found = rank_class(ncls_new, found);
@@ -1620,7 +1662,8 @@ void run_pores(pma_ctx* c) {
                 ranks_len += t.ncls;
                 max_ncls = std::max(max_ncls, t.ncls);
             }
-            PMA_REQUIRE(max_ncls < kNoClassU && ranks_len < 0xFFFFFFFFull, PMA_ERR_CAPACITY,
+            PMA_REQUIRE(max_ncls < kNoClassU && ranks_len < 0xFFFFFFFFull && (max_ncls + 31) / 32 * 4 <= 8192,
+                        PMA_ERR_CAPACITY,
                         "pma_pores: a pore forms 65535 or more throat keys (the DFS replay numbers them in 16 bits)");
             uint32_t* d_replayed = c->buf<uint32_t>("pt_replayed", npores);
             uint32_t* d_evl = c->buf<uint32_t>("pt_ev_local", std::max(D, 1u));
@@ -1657,10 +1700,15 @@ void run_pores(pma_ctx* c) {
             uint32_t* queue = c->buf<uint32_t>("pt_queue", 1);
             ReplayTask* d_tasks = c->buf<ReplayTask>("pt_tasks", T);
             PMA_HIP(hipMemsetAsync(d_rank, 0xFF, ranks_len * 4ull, c->stream));
-            // Per wave: a visited bitmap over all voxels, two bitmaps over plateaus (entered,
-            // closed), the dense class table, a stack and the traversal stack
+            // Per wave: pages of visited voxels, two bitmaps over plateaus (entered, closed), the
+            // dense class table, a stack and the traversal stack
             // and list.  The number of waves follows from the device memory free for them.
-            const unsigned long long vis_words = n / 64 + 1, pent_words = P / 64 + 1;
+            // visited voxels by blocks of 2^vis_bits (at most 16384 blocks: the table is in LDS)
+            uint32_t vis_bits = PMA_PORES_VIS_MIN_BITS;
+            while (vis_bits < 32 && ((n + (1ull << vis_bits) - 1) >> vis_bits) > 16384) ++vis_bits;
+            const uint32_t nblocks = (uint32_t)((n + (1ull << vis_bits) - 1) >> vis_bits);
+            const unsigned long long page_words = 1ull << (vis_bits - 6);
+            const unsigned long long pent_words = P / 64 + 1;
             const uint32_t seen_words = (max_ncls + 31) / 32;
             const unsigned long long trav_cap = std::min<unsigned long long>(P + 1, PMA_PORES_TRAV_CAP);
             unsigned long long max_reach = 0;
@@ -1676,10 +1724,13 @@ void run_pores(pma_ctx* c) {
                 // pass 0: stacks of up to PMA_PORES_STACK_CAP frames; pass 1: room for the whole reach
                 const unsigned long long cap =
                     pass == 0 ? std::min<unsigned long long>(max_reach + 2, PMA_PORES_STACK_CAP) : max_reach + 2;
-                const unsigned long long per_wave = (vis_words + 2 * pent_words) * 8 + P * 2 + cap * 16 + trav_cap * 8;
+                // pass 1: a page for every block
+                const uint32_t vis_pages = pass == 0 ? std::min<uint32_t>(nblocks, PMA_PORES_VIS_PAGES) : nblocks;
+                const unsigned long long per_wave =
+                    (vis_pages * page_words + 2 * pent_words) * 8 + P * 2 + cap * 16 + trav_cap * 8;
                 const unsigned long long W =
                     std::max<unsigned long long>(1, std::min<unsigned long long>({pending.size(), budget / per_wave, 1024}));
-                unsigned long long* vis = c->buf<unsigned long long>("pt_replay_vis", W * vis_words);
+                unsigned long long* vis = c->buf<unsigned long long>("pt_replay_vis", W * vis_pages * page_words);
                 unsigned long long* pbits = c->buf<unsigned long long>("pt_replay_plateaus", W * 2 * pent_words);
                 uint16_t* cls = c->buf<uint16_t>("pt_replay_cls", W * P);
                 uint4* stack = c->buf<uint4>("pt_stack", W * cap);
@@ -1691,9 +1742,9 @@ void run_pores(pma_ctx* c) {
                 const auto t0 = std::chrono::steady_clock::now();
                 {
                     PMA_TIMED(c, "dfs_replay");
-                    dfs_replay<<<(unsigned)W, 64, seen_words * 4, c->stream>>>(
+                    dfs_replay<<<(unsigned)W, 64, seen_words * 4 + (nblocks * 2 + 3) / 4 * 4, c->stream>>>(
                         d_tasks, (uint32_t)pending.size(), queue, cr, plat, lists, below, out_start, out_count, oedges,
-                        vis, vis_words, pbits, pent_words, cls, seen_words, stack, cap, trav, d_rank,
+                        vis, vis_pages, vis_bits, pbits, pent_words, cls, seen_words, stack, cap, trav, d_rank,
                         status, steps, X, Y, Z, d3q7, rb, err);
                     PMA_CHECKED(c, "dfs_replay", err);
                 }

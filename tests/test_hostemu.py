@@ -93,7 +93,8 @@ def hostemu_slabs(tmp_path_factory):
     4000 pairs, 32-bit winner slots, ranks (not packed balls) in the pair sort, far tables read
     from global memory by the centre prune, geodesic relaxation on 2 waves with 32-entry push
     buffers, DFS replay stacks of 8 frames in LDS and 64 in global memory (so frames move between
-    them and replays overflow and are repeated), the per-thread
+    them and replays overflow and are repeated), visited voxels on 4 pages of 64 voxels (so pages
+    run out and replays are repeated), the per-thread
     merge of pore sets starting from a pool of one entry per plateau (so it overflows and is
     repeated), and skip traversals that give up after listing 3 plateaus."""
     return _build(tmp_path_factory, SANITIZED + ["-DPMA_CB_SLAB_PAIRS=4000ull", "-DPMA_CB_WIN16_MAX=0u",
@@ -101,6 +102,7 @@ def hostemu_slabs(tmp_path_factory):
                                                  "-DPMA_GEO_RELAX_GRID=2u", "-DPMA_GEO_PUSH_BUF=32u",
                                                  "-DPMA_CB_PACKED=0", "-DPMA_CB_PRUNE_FAR_LDS=0u",
                                                  "-DPMA_PORES_STACK_CAP=64ull", "-DPMA_PORES_RING=8u", "-DPMA_UNION_LDS_MAX=0u",
+                                                 "-DPMA_PORES_VIS_MIN_BITS=6u", "-DPMA_PORES_VIS_PAGES=4u",
                                                  "-DPMA_PORES_TRAV_CAP=3ull", "-DPMA_PORES_POOL_MIN=1ull",
                                                  "-DPMA_PORES_POOL_MAX=1ull", "-DPMA_PORES_POOL_FLOOR=16ull"], "slabs")
 
