@@ -137,6 +137,35 @@ pma_status pma_geodesic(pma_ctx* ctx);
 /* The reference's -all mode followed by -spd (ImageProcessingManager.cpp:63-108). */
 pma_status pma_run_all(pma_ctx* ctx);
 
+/* ---- the reference's folders ------------------------------------------------------------ */
+/* Console modes (CommandLineDispatcher.cpp:80-106). */
+typedef enum pma_mode {
+    PMA_MODE_ALL = 0, /* -all: EDT, containing balls, pores, empty-pore removal, watershed */
+    PMA_MODE_EDT = 1, /* -edt: (resampling and) the Euclidean distance transform */
+    PMA_MODE_RIB = 2, /* -rib: containing balls from EuclideanDistanceSquares/ */
+    PMA_MODE_APT = 3, /* -apt: pores, removal, watershed from the -rib outputs */
+    PMA_MODE_SPD = 4  /* -spd: (resampling and) geodesic distances */
+} pma_mode;
+/* One console pass over a folder holding config.txt and Images/ (ImageProcessingManager.cpp:
+ * 44-108 with Serializer.cpp:31-64, 223-281 and ImageResampler.cpp:28-206): reads the binary TIFF
+ * stack (8-bit, non-zero = solid; files ordered by name), resamples it when the resolution in
+ * config.txt is anisotropic, runs the mode's stages on the GPU and writes the same sub-folders
+ * (one TIFF per slice, named and tagged like Images/; 32-bit volumes as 4-sample CMYK images of
+ * the values' little-endian bytes, LZW) and text files as the reference.  -rib and -apt read the
+ * earlier modes' folders and intermediateStatistics.txt.  Errors return a status (the reference
+ * printed them and exited 0, Main.cpp:62-98). */
+pma_status pma_run_folder(pma_ctx* ctx, const char* folder, pma_mode mode);
+/* Reads a folder of 8-bit or 32-bit (4-sample) TIFF slices, ordered by file name, into a host
+ * buffer in [z][x][y] order (DataArray.h:85-94; Image.h:189-214).  With dst == NULL it only
+ * reports the size (x = rows, y = columns, z = files) and the bytes per voxel (1 or 4). */
+pma_status pma_read_tiff_stack(const char* folder, void* dst, size_t bytes, pma_dims* dims, int* bytes_per_voxel);
+/* Writes a host volume ([z][x][y], 1 or 4 bytes per voxel) as a folder of TIFF slices named and
+ * tagged like the 8-bit slices in `like` (an Images/ folder), in the reference's output form
+ * (DataArray.h:226-252, Image.h:72-83): 8-bit with the slices' photometric interpretation,
+ * 32-bit as 4-sample CMYK of the little-endian bytes; LZW with the horizontal predictor. */
+pma_status pma_write_tiff_stack(const char* folder, const char* like, const void* src, pma_dims dims,
+                                int bytes_per_voxel);
+
 /* ---- allocation-free geodesic ---------------------------------------------------------- */
 /* One row of cavityDescriptions.txt (ShortestPathComputer.cpp:143-164, Serializer.cpp:205-221). */
 typedef struct pma_cavity {

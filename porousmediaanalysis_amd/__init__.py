@@ -19,6 +19,7 @@ VOL_SOLID, VOL_EDT2, VOL_CR2, VOL_CX, VOL_CY, VOL_CZ, VOL_MASK, VOL_PT, VOL_PTW,
 TEXT_STATS, TEXT_EDT_COUNTS, TEXT_PORES, TEXT_THROATS, TEXT_PORES_WS, TEXT_THROATS_WS, TEXT_CAVITIES = range(7)
 STAGE_EDT, STAGE_BALLS, STAGE_PORES, STAGE_REMOVE, STAGE_WATERSHED, STAGE_GEODESIC = range(6)
 STAGE_NAMES = ["edt", "containing_balls", "pores", "remove_empty_pores", "watershed", "geodesic"]
+MODE_ALL, MODE_EDT, MODE_RIB, MODE_APT, MODE_SPD = range(5)
 _DTYPES = {VOL_SOLID: np.uint8, VOL_MASK: np.uint8, VOL_SPD: np.float32}
 
 STATUS = {0: "PMA_OK", 1: "PMA_ERR_INVALID_ARGUMENT", 2: "PMA_ERR_HIP", 3: "PMA_ERR_OUT_OF_MEMORY",
@@ -31,7 +32,8 @@ EXPORTS = [
     "pma_pores", "pma_remove_empty_pores", "pma_watershed", "pma_geodesic", "pma_run_all",
     "pma_get_statistics", "pma_get_histogram", "pma_get_text", "pma_set_text", "pma_get_stage_info",
     "pma_generate_spheres", "pma_generate_gaussian", "pma_set_kernel_timing", "pma_get_kernel_time",
-    "pma_list_kernel_times", "pma_geodesic_noalloc",
+    "pma_list_kernel_times", "pma_geodesic_noalloc", "pma_run_folder", "pma_read_tiff_stack",
+    "pma_write_tiff_stack",
 ]
 
 
@@ -85,6 +87,10 @@ def lib():
         L.pma_get_stage_info.argtypes = [vp, ctypes.c_int, ctypes.POINTER(StageInfo)]
         L.pma_generate_spheres.argtypes = [vp, ctypes.c_int32, ctypes.c_float, ctypes.c_float, ctypes.c_uint64]
         L.pma_generate_gaussian.argtypes = [vp, ctypes.c_float, ctypes.c_float, ctypes.c_uint64]
+        L.pma_run_folder.argtypes = [vp, ctypes.c_char_p, ctypes.c_int]
+        L.pma_read_tiff_stack.argtypes = [ctypes.c_char_p, vp, ctypes.c_size_t, ctypes.POINTER(Dims),
+                                          ctypes.POINTER(ctypes.c_int)]
+        L.pma_write_tiff_stack.argtypes = [ctypes.c_char_p, ctypes.c_char_p, vp, Dims, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -92,6 +98,25 @@ def lib():
 def _check(status):
     if status != 0:
         raise PmaError(status, lib().pma_last_error().decode())
+
+
+def read_tiff_stack(folder):
+    """A folder of 8-bit or 32-bit TIFF slices (the reference's form) as a (Z, X, Y) array: uint8, or
+    uint32 for 32-bit slices (view a float volume with .view(np.float32))."""
+    d, bpv = Dims(), ctypes.c_int()
+    _check(lib().pma_read_tiff_stack(folder.encode(), None, 0, ctypes.byref(d), ctypes.byref(bpv)))
+    out = np.empty((d.z, d.x, d.y), np.uint8 if bpv.value == 1 else np.uint32)
+    _check(lib().pma_read_tiff_stack(folder.encode(), out.ctypes.data, out.nbytes, ctypes.byref(d), ctypes.byref(bpv)))
+    return out
+
+
+def write_tiff_stack(folder, like, array):
+    """Writes a (Z, X, Y) uint8 or 4-byte volume as TIFF slices named and tagged like folder `like`."""
+    a = np.ascontiguousarray(array)
+    if a.dtype.itemsize not in (1, 4):
+        raise ValueError("1 or 4 bytes per voxel")
+    Z, X, Y = a.shape
+    _check(lib().pma_write_tiff_stack(folder.encode(), like.encode(), a.ctypes.data, Dims(X, Y, Z), a.dtype.itemsize))
 
 
 class Context:
@@ -166,6 +191,13 @@ class Context:
 
     def geodesic(self):
         _check(lib().pma_geodesic(self._h))
+
+    def run_folder(self, folder, mode):
+        """One console pass (MODE_ALL, MODE_EDT, MODE_RIB, MODE_APT, MODE_SPD) over a reference folder."""
+        _check(lib().pma_run_folder(self._h, folder.encode(), mode))
+        d = Dims()
+        _check(lib().pma_get_dims(self._h, ctypes.byref(d)))
+        self.shape = (d.z, d.x, d.y)
 
     def run_all(self):
         _check(lib().pma_run_all(self._h))

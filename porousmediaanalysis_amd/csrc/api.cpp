@@ -61,6 +61,12 @@ pma_status guard(const std::function<void()>& f) {
     }
 }
 
+}  // namespace
+
+void set_last_error(const std::string& m) { g_last_error = m; }
+
+namespace {
+
 void check_ctx(pma_ctx* c) {
     PMA_REQUIRE(c != nullptr, PMA_ERR_INVALID_ARGUMENT, "null context");
     PMA_HIP(hipSetDevice(c->device));

@@ -21,6 +21,9 @@ struct Error : std::runtime_error {
     Error(pma_status s, const std::string& m) : std::runtime_error(m), status(s) {}
 };
 
+// The text pma_last_error returns on this thread (api.cpp).
+void set_last_error(const std::string& message);
+
 #define PMA_HIP(expr)                                                                         \
     do {                                                                                      \
         hipError_t e_ = (expr);                                                               \

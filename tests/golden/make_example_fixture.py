@@ -15,6 +15,8 @@ What is stored:
                        ShortestPathDistances, CavityIds) and of the first-occurrence
                        relabelling of PoreThroatIds / PoreThroatIdsWatershed (partitions).
   * *.txt           -- the fixture's text outputs, verbatim.
+  * Images/, config.txt -- the example's own input folder (five 8-bit LZW TIFF slices) and
+                       config, verbatim: the drop-in folder test runs on them.
 The 32-bit fixture volumes are CMYK TIFFs whose four byte samples are the little-endian
 bytes of the value.
 """
@@ -76,6 +78,10 @@ def main():
               "sharedPixelTypes.txt", "poresWatershed.txt", "sharedPixelTypesWatershed.txt"]:
         shutil.copyfile(os.path.join(a, t), os.path.join(OUT, t))
     shutil.copyfile(os.path.join(s, "cavityDescriptions.txt"), os.path.join(OUT, "cavityDescriptions.txt"))
+    os.makedirs(os.path.join(OUT, "Images"), exist_ok=True)
+    for f in os.listdir(os.path.join(REF, "Images")):
+        shutil.copyfile(os.path.join(REF, "Images", f), os.path.join(OUT, "Images", f))
+    shutil.copyfile(os.path.join(REF, "config.txt"), os.path.join(OUT, "config.txt"))
     print("wrote", OUT, dig)
 
 
