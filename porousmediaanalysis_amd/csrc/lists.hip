@@ -27,10 +27,22 @@ __global__ void length_keys(const uint32_t* __restrict__ len, size_t S, uint32_t
     atomicMax(max_len, len[s]);
 }
 
-__global__ void length_histogram(const uint32_t* __restrict__ len, size_t S, uint32_t max_len,
-                                 unsigned int* __restrict__ hist) {
-    size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < S && len[s] <= max_len) atomicAdd(&hist[len[s]], 1u);
+// Counts per length: each workgroup counts its grid-stride share in LDS (lengths below
+// `lds_bins`), then adds its non-zero bins once to the global histogram; longer lengths (rare) go
+// to the global histogram directly.
+__global__ void __launch_bounds__(256) length_histogram(const uint32_t* __restrict__ len, size_t S, uint32_t max_len,
+                                                        uint32_t lds_bins, unsigned int* __restrict__ hist) {
+    extern __shared__ uint32_t lhist[];
+    for (uint32_t i = threadIdx.x; i < lds_bins; i += blockDim.x) lhist[i] = 0;
+    __syncthreads();
+    for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < S; s += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t l = len[s];
+        if (l < lds_bins) atomicAdd(&lhist[l], 1u);
+        else if (l <= max_len) atomicAdd(&hist[l], 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < lds_bins; i += blockDim.x)
+        if (lhist[i]) atomicAdd(&hist[i], lhist[i]);
 }
 
 // Kernel ids of the in-kernel checks (code = id << 8 | check).
@@ -135,7 +147,9 @@ PrefixTrie build_prefix_trie(pma_ctx* c, const uint32_t* elems, uint64_t elem_ca
     const uint32_t max_len = read_scalar(c, dmax);
     unsigned int* dhist = c->buf<unsigned int>("trie_hist", (size_t)max_len + 1);
     PMA_HIP(hipMemsetAsync(dhist, 0, ((size_t)max_len + 1) * 4, c->stream));
-    length_histogram<<<grid_for(S, 256), 256, 0, c->stream>>>(seq_len, S, max_len, dhist);
+    const uint32_t lds_bins = std::min<uint32_t>(max_len + 1, 8192);
+    length_histogram<<<std::min(grid_for(S, 256), 1024u), 256, lds_bins * 4, c->stream>>>(seq_len, S, max_len,
+                                                                                          lds_bins, dhist);
     PMA_CHECKED(c, "length_histogram", err);
     std::vector<uint32_t> hist((size_t)max_len + 1);
     PMA_HIP(hipMemcpyAsync(hist.data(), dhist, hist.size() * 4, hipMemcpyDeviceToHost, c->stream));
