@@ -80,7 +80,7 @@ struct WsRound {
     uint32_t a, b;
 };
 constexpr uint32_t kWsGrid = 512;       // workgroups of the round's scan kernels (grid-stride over the wave)
-constexpr uint32_t kWsItemGrid = 1024;  // workgroups of ws_bid and ws_resolve (grid-stride over the lanes)
+constexpr uint32_t kWsItemGrid = 4096;  // workgroups of ws_bid and ws_resolve (grid-stride over the lanes)
 
 // A round's items are spread over lanes, one lane per (item, neighbour): the item's lanes form
 // an aligned group of 32 (the 26 neighbours of D3Q27) or 8 (the 6 of D3Q7), so a group never

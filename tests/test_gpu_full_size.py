@@ -5,7 +5,11 @@ D3Q27), checked through properties that hold at any size, on a seeded sample of 
   distance to a solid voxel or to the virtual planes x = -X, x = 2X - 1 (brute force over the
   cube that can hold it).
 * Containing balls: the voxel's ball covers it, its centre's EDT is the ball's r2, r2 >= the
-  voxel's own EDT, the centre is marked 255, and the 255 count is the maximum-ball count.
+  voxel's own EDT, the centre is marked 255, and the 255 count is the maximum-ball count; on a
+  smaller sample, brute force over every centre within ceil(sqrt(max EDT)) gives the same largest
+  covering r2 and the same centre (the first in x, y, z scan order among equals).
+* Pores and watershed: labels are 0 exactly on solid; every label is a listed pore or key; the
+  volumes add up to the void count; each pore's seed voxel holds the pore in the watershed.
 * Geodesic distances: at sampled reached void voxels other than a cavity start, the distance is
   the minimum over the 26 neighbours of fl32(d(u) + w) with w = 1, sqrt 2, sqrt 3 (the fixpoint
   equation, evaluated in float32); solid voxels hold -1.
@@ -32,11 +36,18 @@ def run():
         ctx.edt()
         ctx.containing_balls()
         ctx.geodesic()
+        log("full size: pores, removal, watershed")
+        ctx.pores()
+        ctx.remove_empty_pores()
+        ctx.watershed()
         max_e, n_max = ctx.statistics()
         out = {k: ctx.read(getattr(pma, k)) for k in
-               ["VOL_SOLID", "VOL_EDT2", "VOL_CR2", "VOL_CX", "VOL_CY", "VOL_CZ", "VOL_MASK", "VOL_SPD", "VOL_CAVITY"]}
+               ["VOL_SOLID", "VOL_EDT2", "VOL_CR2", "VOL_CX", "VOL_CY", "VOL_CZ", "VOL_MASK", "VOL_SPD", "VOL_CAVITY",
+                "VOL_PT", "VOL_PTW"]}
         out["max_e"], out["n_max"] = max_e, n_max
         out["cavities"] = ctx.text(pma.TEXT_CAVITIES)
+        for t in ["TEXT_PORES", "TEXT_THROATS", "TEXT_PORES_WS", "TEXT_THROATS_WS"]:
+            out[t] = ctx.text(getattr(pma, t))
     return out
 
 
@@ -74,6 +85,54 @@ def test_containing_balls(run):
         assert r2 >= int(e[z, x, y])
         assert mask[bz, bx, by] == 255
     assert np.all(cr[solid != 0] == 0)
+
+
+def test_containing_balls_are_the_largest_and_first(run):
+    """Brute force: the largest r2 among the balls covering the voxel, and among those the centre
+    first in the reference's x, y, z scan order."""
+    solid, e, cr = run["VOL_SOLID"], run["VOL_EDT2"], run["VOL_CR2"]
+    cx, cy, cz = run["VOL_CX"], run["VOL_CY"], run["VOL_CZ"]
+    Z, X, Y = solid.shape
+    R = int(np.ceil(np.sqrt(run["max_e"])))
+    rng = np.random.default_rng(4)
+    for z, x, y in sample_void(solid, rng)[:400]:
+        z0, z1, x0, x1, y0, y1 = max(z - R, 0), min(z + R + 1, Z), max(x - R, 0), min(x + R + 1, X), \
+            max(y - R, 0), min(y + R + 1, Y)
+        w = e[z0:z1, x0:x1, y0:y1].astype(np.int64)
+        zz, xx, yy = np.meshgrid(np.arange(z0, z1), np.arange(x0, x1), np.arange(y0, y1), indexing="ij")
+        cover = (w > 0) & ((zz - z) ** 2 + (xx - x) ** 2 + (yy - y) ** 2 <= w)
+        best = int(w[cover].max())
+        first = cover & (w == best)
+        scan = ((xx * Y + yy) * Z + zz)[first]
+        k = np.argmin(scan)
+        bz, bx, by = zz[first][k], xx[first][k], yy[first][k]
+        assert int(cr[z, x, y]) == best, f"CR at {(z, x, y)}: {cr[z, x, y]} vs {best}"
+        assert (int(cx[z, x, y]), int(cy[z, x, y]), int(cz[z, x, y])) == (bx, by, bz), (z, x, y)
+
+
+def _ids(text, key_rows=False):
+    rows = text.strip().splitlines()[1:]
+    if key_rows:
+        return {int(r.split(";")[0]): int(r.split(";")[1]) for r in rows}
+    return {int(r.split()[0]): (int(r.split()[6]), tuple(int(v) for v in r.split()[1:4])) for r in rows}
+
+
+@pytest.mark.parametrize("which", ["pt", "ptw"])
+def test_pore_and_watershed_labels(run, which):
+    solid = run["VOL_SOLID"]
+    lab = run["VOL_PT" if which == "pt" else "VOL_PTW"]
+    pores = _ids(run["TEXT_PORES" if which == "pt" else "TEXT_PORES_WS"])
+    keys = _ids(run["TEXT_THROATS" if which == "pt" else "TEXT_THROATS_WS"], key_rows=True)
+    assert np.array_equal(lab == 0, solid != 0)
+    ids, counts = np.unique(lab[lab != 0], return_counts=True)
+    vol = {pid: v for pid, (v, _) in pores.items()}
+    vol.update(keys)
+    for i, c in zip(ids.tolist(), counts.tolist()):
+        assert vol.get(i) == c, f"label {i}: {c} voxels, listed volume {vol.get(i)}"
+    assert sum(vol.values()) == int((solid == 0).sum())
+    if which == "ptw":
+        for pid, (_, (x, y, z)) in pores.items():
+            assert lab[z, x, y] == pid or lab[z, x, y] in keys
 
 
 def test_geodesic_fixpoint(run):
