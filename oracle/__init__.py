@@ -41,6 +41,9 @@ def lib():
                                               _u32p, _u32p, _u8p, _u64p, ctypes.c_int]
         L.oracle_pores.argtypes = [_u8p, _u32p, _u32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                    _u32p, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p)]
+        L.oracle_pores_in_order.argtypes = [_u32p, _u32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                            _u32p, ctypes.c_size_t, _u32p, ctypes.POINTER(ctypes.c_void_p),
+                                            ctypes.POINTER(ctypes.c_void_p)]
         L.oracle_remove_empty_pores.argtypes = [_u32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p,
                                                 ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p),
                                                 ctypes.POINTER(ctypes.c_void_p)]
@@ -111,6 +114,22 @@ def pores(mask, e, cr, d3q7=False):
     a, b = ctypes.c_void_p(), ctypes.c_void_p()
     lib().oracle_pores(_ptr(mask, _u8p), _ptr(e, _u32p), _ptr(cr, _u32p), X, Y, Z, int(d3q7), _ptr(pt, _u32p),
                        ctypes.byref(a), ctypes.byref(b))
+    return pt, _take_string(a.value), _take_string(b.value)
+
+
+def pores_in_order(e, cr, order, d3q7=False):
+    """Pore propagation over the maximum balls in the given order (flat [z][x][y] voxel indices)
+    instead of the std::sort permutation."""
+    e = np.ascontiguousarray(e, dtype=np.uint32)
+    cr = np.ascontiguousarray(cr, dtype=np.uint32)
+    order = np.ascontiguousarray(order, dtype=np.uint32)
+    X, Y, Z = _dims(e)
+    pt = np.empty(e.shape, np.uint32)
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    rc = lib().oracle_pores_in_order(_ptr(e, _u32p), _ptr(cr, _u32p), X, Y, Z, int(d3q7), _ptr(order, _u32p),
+                                     order.size, _ptr(pt, _u32p), ctypes.byref(a), ctypes.byref(b))
+    if rc != 0:
+        raise ValueError("an order entry lies outside the volume")
     return pt, _take_string(a.value), _take_string(b.value)
 
 

@@ -26,14 +26,18 @@
 //                             plateau (scipy labels) in pore-ID order; every voxel's list = the
 //                             pores whose seed reaches it by BFS, in [second, first, ...] order;
 //                             volumes and boundary flags by the same reachability; the DFS order
-//                             of key numbering on a row worked out by hand.  Key numbering on
-//                             larger inputs rests on the restatement (the only fixture was made
-//                             with another STL).
+//                             of key numbering on a row worked out by hand.  The sort itself is
+//                             libstdc++'s std::sort.
+//   oracle_pores_in_order   : the same loop in a given ball order.  Docs/Example PoreThroatIds,
+//                             pores.txt and sharedPixelTypes.txt bit-exact, throat-key IDs
+//                             included, in the fixture's seed order (the fixture was made with
+//                             MSVC's std::sort, whose tie order differs from libstdc++'s).
 //   oracle_remove_empty_pores : a hand-made table with known answers (drop, fold into the one
 //                             remaining pore, drop without remap, merge duplicates into the
-//                             largest ID).  The Docs/Example removes nothing.
-//   oracle_watershed        : Docs/Example PoreThroatIdsWatershed partition (up to
-//                             relabelling); the FIFO order and throat-key order on a row worked
+//                             largest ID); the Docs/Example check above runs through it.
+//   oracle_watershed        : Docs/Example PoreThroatIdsWatershed, poresWatershed.txt and
+//                             sharedPixelTypesWatershed.txt bit-exact from the fixture's
+//                             pores.txt; the FIFO order and throat-key order on a row worked
 //                             out by hand, both stencils.
 //   oracle_geodesic         : Docs/Example ShortestPathDistances + CavityIds bit-exact,
 //                             cavityDescriptions.txt; float32 Bellman-Ford from every cavity's
@@ -391,22 +395,13 @@ int oracle_containing_balls(const uint32_t* e, int X, int Y, int Z, uint32_t* cr
 // opened at still-unlabelled centres with IDs 127, +1 (+127 after a multiple of 255)
 // (:125-163), DFS over non-increasing containing radii (:192-269), empty keys erased after
 // every pore (:168-189).
-int oracle_pores(const uint8_t* mask, const uint32_t* e, const uint32_t* cr, int X, int Y, int Z,
-                 int d3q7, uint32_t* pt, char** pores_out, char** keys_out) {
-    Grid g{X, Y, Z};
-    size_t n = g.count();
-    std::fill(pt, pt + n, 0u);
-    struct Ball { int x, y, z; uint32_t r2; };
-    std::vector<Ball> balls;
-    for (int z = 0; z < Z; ++z)
-        for (int x = 0; x < X; ++x)
-            for (int y = 0; y < Y; ++y) {
-                size_t i = g.idx(x, y, z);
-                if (mask[i] != 255) continue;
-                balls.push_back(Ball{x, y, z, e[i]});
-            }
-    std::sort(balls.begin(), balls.end(), [](const Ball& a, const Ball& b) { return a.r2 > b.r2; });
+namespace {
+struct Ball { int x, y, z; uint32_t r2; };
 
+// The pore loop (:125-269) over maximum balls already in processing order.
+void pores_from_balls(const std::vector<Ball>& balls, const uint32_t* cr, const Grid& g, int d3q7,
+                      uint32_t* pt, char** pores_out, char** keys_out) {
+    std::fill(pt, pt + g.count(), 0u);
     Labeller L;
     uint32_t pore_id = 127;
     for (const Ball& b : balls) {
@@ -444,6 +439,39 @@ int oracle_pores(const uint8_t* mask, const uint32_t* e, const uint32_t* cr, int
     }
     *pores_out = dup_string(pores_text(L.pores));
     *keys_out = dup_string(keys_text(L.keys));
+}
+}  // namespace
+
+int oracle_pores(const uint8_t* mask, const uint32_t* e, const uint32_t* cr, int X, int Y, int Z,
+                 int d3q7, uint32_t* pt, char** pores_out, char** keys_out) {
+    Grid g{X, Y, Z};
+    std::vector<Ball> balls;
+    for (int z = 0; z < Z; ++z)
+        for (int x = 0; x < X; ++x)
+            for (int y = 0; y < Y; ++y) {
+                size_t i = g.idx(x, y, z);
+                if (mask[i] != 255) continue;
+                balls.push_back(Ball{x, y, z, e[i]});
+            }
+    std::sort(balls.begin(), balls.end(), [](const Ball& a, const Ball& b) { return a.r2 > b.r2; });
+    pores_from_balls(balls, cr, g, d3q7, pt, pores_out, keys_out);
+    return 0;
+}
+
+// The same pore loop over the maximum balls in a caller-given order (voxel indices), in place of
+// the std::sort permutation: lets a test replay the tie order another STL's sort produced. The
+// loop itself takes any order; the reference's is non-increasing in r2.
+int oracle_pores_in_order(const uint32_t* e, const uint32_t* cr, int X, int Y, int Z, int d3q7,
+                          const uint32_t* order, size_t m, uint32_t* pt, char** pores_out, char** keys_out) {
+    Grid g{X, Y, Z};
+    std::vector<Ball> balls(m);
+    for (size_t k = 0; k < m; ++k) {
+        size_t i = order[k];
+        if (i >= g.count()) return 1;
+        int y = (int)(i % (size_t)Y), x = (int)(i / (size_t)Y % (size_t)X), z = (int)(i / ((size_t)X * Y));
+        balls[k] = Ball{x, y, z, e[i]};
+    }
+    pores_from_balls(balls, cr, g, d3q7, pt, pores_out, keys_out);
     return 0;
 }
 

@@ -399,6 +399,15 @@ pma_status pma_containing_balls(pma_ctx* c) {
     return guard([&] { timed(c, PMA_STAGE_CONTAINING_BALLS, run_containing_balls); });
 }
 pma_status pma_pores(pma_ctx* c) { return guard([&] { timed(c, PMA_STAGE_PORES, run_pores); }); }
+// Test hook (not part of the public header): the next pma_pores takes the maximum balls in this
+// order (flat [z][x][y] voxel indices, every maximum ball once, r2 non-increasing) instead of
+// sorting them, so that a test can replay the tie order another STL's std::sort produced.
+pma_status pma_testing_set_ball_order(pma_ctx* c, const uint32_t* order, size_t m) {
+    return guard([&] {
+        PMA_REQUIRE(c && (order || m == 0), PMA_ERR_INVALID_ARGUMENT, "pma_testing_set_ball_order: null argument");
+        c->testing_ball_order.assign(order, order + m);
+    });
+}
 pma_status pma_remove_empty_pores(pma_ctx* c) {
     return guard([&] { timed(c, PMA_STAGE_REMOVE_EMPTY_PORES, run_remove_empty_pores); });
 }

@@ -113,6 +113,9 @@ struct pma_ctx {
 
     pma_stage_info stage[PMA_STAGE_COUNT] = {};
     uint64_t launches = 0;
+    // test hook: the next pma_pores processes the maximum balls in this order (voxel indices)
+    // instead of the sorted one, then clears it
+    std::vector<uint32_t> testing_ball_order;
     // PMA_DEBUG_SYNC=1 at pma_create: every checked launch is named on stderr, waited for and
     // its stage's error flag read, so that a failing kernel is identified.
     bool debug_sync = false;

@@ -73,14 +73,15 @@ constexpr uint32_t kInv = 0xFFFFFFFFu;
 enum : uint32_t {
     kWriteCentres = 1, kCcUnion, kCcFlatten, kCcNotMax, kFirstInSorted, kSeedFlags, kWriteSeeds, kPlateauIndex,
     kPlateauOfVoxels, kSeedPlateaus, kPlateauEdges, kWriteEdges, kEdgeRanges, kLevelStarts, kSetUnion,
-    kReachSizes, kDfsReplay, kWriteLabels, kSingleVolumes, kBelowMin, kPoreLists
+    kReachSizes, kDfsReplay, kWriteLabels, kSingleVolumes, kBelowMin, kPoreLists, kRecordsInOrder
 };
 constexpr uint32_t kNoClassU = 0xFFFF;  // class of a pore on a plateau whose first member it is
 const char* const kKernelNames[] = {"?", "write_centres", "cc_union", "cc_flatten", "cc_not_max",
                                     "first_in_sorted", "seed_flags", "write_seeds", "plateau_index",
                                     "plateau_of_voxels", "seed_plateaus", "plateau_edges", "write_edges",
                                     "edge_ranges", "level_starts", "set_union_level", "reach_sizes",
-                                    "dfs_replay", "write_labels", "single_volumes", "below_min", "pore_lists"};
+                                    "dfs_replay", "write_labels", "single_volumes", "below_min", "pore_lists",
+                                    "records_in_order"};
 
 __device__ __forceinline__ uint32_t ld_relaxed(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -158,6 +159,31 @@ __global__ void __launch_bounds__(256) write_centres(const uint8_t* __restrict__
             rec[o++] = ((unsigned long long)e[i] << 32) | (unsigned long long)i;
         }
     }
+}
+
+// Records (e << 32 | voxel) for maximum balls given in processing order (the test hook's order).
+// The order must be non-increasing in r2, as every std::sort by r2 leaves it: only the order of
+// equal radii may differ.
+__global__ void __launch_bounds__(256) records_in_order(const uint32_t* __restrict__ order, size_t M,
+                                                        const uint8_t* __restrict__ mask,
+                                                        const uint32_t* __restrict__ e, size_t n,
+                                                        unsigned long long* __restrict__ rec,
+                                                        uint32_t* __restrict__ err) {
+    const size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= M) return;
+    const uint32_t i = order[k];
+    if (i >= n || mask[i] != 255) {
+        fail_check(err, kRecordsInOrder << 8 | 1);
+        return;
+    }
+    if (k > 0) {
+        const uint32_t prev = order[k - 1];
+        if (prev < n && e[prev] < e[i]) {
+            fail_check(err, kRecordsInOrder << 8 | 2);
+            return;
+        }
+    }
+    rec[k] = ((unsigned long long)e[i] << 32) | (unsigned long long)i;
 }
 
 __global__ void cc_init(const uint32_t* __restrict__ cr, size_t n, uint32_t* __restrict__ parent) {
@@ -1359,7 +1385,20 @@ void run_pores(pma_ctx* c) {
     unsigned long long* rec = c->buf<unsigned long long>("pt_rec", std::max<uint64_t>(M, 1));
     write_centres<<<(unsigned)cblocks, 256, 0, c->stream>>>(mask, e, n, boff, rec, M, err);
     PMA_CHECKED(c, "write_centres", err);
-    sort_like_libstdcxx_desc(c, reinterpret_cast<uint64_t*>(rec), M);
+    if (c->testing_ball_order.empty()) {
+        sort_like_libstdcxx_desc(c, reinterpret_cast<uint64_t*>(rec), M);
+    } else {
+        std::vector<uint32_t> order;
+        order.swap(c->testing_ball_order);
+        PMA_REQUIRE(order.size() == M, PMA_ERR_INVALID_ARGUMENT,
+                    "pma_pores: the ball order does not list every maximum ball once");
+        uint32_t* d_order = c->buf<uint32_t>("pt_testing_order", std::max<uint64_t>(M, 1));
+        upload(c, d_order, order.data(), M * 4ull);
+        if (M) {
+            records_in_order<<<grid_for(M, 256), 256, 0, c->stream>>>(d_order, M, mask, e, n, rec, err);
+            PMA_CHECKED(c, "records_in_order", err);
+        }
+    }
 
     // 2. plateaus of CR and regional maxima.
     uint32_t* root = c->buf<uint32_t>("pt_root", n);

@@ -12,8 +12,9 @@ What is stored:
                        own maximum balls into pore propagation.
   * digests.json    -- SHA-256 of the fixture volumes that are bit-exact
                        (EuclideanDistanceSquares, ContainingBallRadiiSquares,
-                       ShortestPathDistances, CavityIds) and of the first-occurrence
-                       relabelling of PoreThroatIds / PoreThroatIdsWatershed (partitions).
+                       ShortestPathDistances, CavityIds, PoreThroatIds,
+                       PoreThroatIdsWatershed) and of the first-occurrence relabelling of
+                       PoreThroatIds / PoreThroatIdsWatershed (partitions).
   * *.txt           -- the fixture's text outputs, verbatim.
   * Images/, config.txt -- the example's own input folder (five 8-bit LZW TIFF slices) and
                        config, verbatim: the drop-in folder test runs on them.
@@ -68,6 +69,8 @@ def main():
         "cr2": sha256_array(stack(os.path.join(a, "ContainingBallRadiiSquares"), "u32")),
         "spd": sha256_array(stack(os.path.join(s, "ShortestPathDistances"), "f32")),
         "cavity": sha256_array(stack(os.path.join(s, "CavityIds"), "u32")),
+        "pt": sha256_array(stack(os.path.join(a, "PoreThroatIds"), "u32")),
+        "ptw": sha256_array(stack(os.path.join(a, "PoreThroatIdsWatershed"), "u32")),
         "pt_partition": sha256_array(canonical_relabel(stack(os.path.join(a, "PoreThroatIds"), "u32"))),
         "ptw_partition": sha256_array(canonical_relabel(stack(os.path.join(a, "PoreThroatIdsWatershed"), "u32"))),
         "mask_values": sorted(int(v) for v in np.unique(mask)),

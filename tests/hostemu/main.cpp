@@ -1,6 +1,7 @@
 // Driver of the host emulation: runs one stage of the library (translated kernels) on inputs
 // written by tests/test_hostemu.py and writes its volume and text outputs.
 //   hostemu pores     <in> <out_volume> <out_text>   in: X Y Z d3q7 | EDT2 u32 | CR2 u32 | mask u8
+//                                                    [| u64 m | m u32 maximum-ball order]
 //   hostemu watershed <in> <out_volume> <out_text>   in: X Y Z d3q7 maxE | CR2 u32 | np | np x 7 i32
 //   hostemu remove    <in> <out_volume> <out_text>   in: X Y Z | PT u32 | text (pores, "@@", throats)
 //   hostemu edt       <in> <out_volume> <out_text>   in: X Y Z | solid u8; out: EDT2 u32;
@@ -84,6 +85,12 @@ void read_pores_input(pma_ctx& c, FILE* f) {
     read_into(f, c.v<uint32_t>(PMA_VOL_CR2), c.n);
     read_into(f, c.v<uint8_t>(PMA_VOL_MASK), c.n);
     c.valid[PMA_VOL_EDT2] = c.valid[PMA_VOL_CR2] = c.valid[PMA_VOL_MASK] = true;
+    // optional: u64 count | u32 voxel indices, the maximum balls' processing order
+    uint64_t m = 0;
+    if (std::fread(&m, 8, 1, f) == 1) {
+        c.testing_ball_order.resize(m);
+        read_into(f, c.testing_ball_order.data(), m);
+    }
 }
 
 int main(int argc, char** argv) {

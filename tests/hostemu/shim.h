@@ -453,6 +453,7 @@ struct pma_ctx {
     std::vector<pma::CavityRow> cavities;
     bool have_cavities = false;
     StageInfo stage[PMA_STAGE_COUNT];
+    std::vector<uint32_t> testing_ball_order;
     bool debug_sync = false;
     template <class T> T* v(int w) { return (T*)vol[w].data(); }
     void ensure_volume(int w) {

@@ -124,10 +124,12 @@ def _run(exe, stage, payload, tmp_path, shape):
     return np.fromfile(vol, np.uint32).reshape(shape), pores, throats
 
 
-def emu_pores(exe, tmp_path, e, cr, mask, d3q7):
+def emu_pores(exe, tmp_path, e, cr, mask, d3q7, order=None):
     Z, X, Y = e.shape
     payload = np.array([X, Y, Z, int(d3q7)], np.int32).tobytes() + e.astype(np.uint32).tobytes() + \
         cr.astype(np.uint32).tobytes() + mask.astype(np.uint8).tobytes()
+    if order is not None:
+        payload += np.array([len(order)], np.uint64).tobytes() + np.asarray(order, np.uint32).tobytes()
     return _run(exe, "pores", payload, tmp_path, e.shape)
 
 
@@ -331,6 +333,27 @@ def test_example_partitions(hostemu_fast, tmp_path):
     gw = emu_watershed(hostemu, tmp_path, cr, max_e, o2[1], False)
     assert_same(gw, oracle.watershed(cr, max_e, o2[1], False), "watershed")
     assert sha256_array(canonical_relabel(gw[0])) == ex.digests()["ptw_partition"]
+
+
+def test_example_bit_exact_in_the_fixture_seed_order(hostemu_fast, tmp_path):
+    """With the fixture's seed order replayed, the pore labels and tables, throat-key IDs included,
+    are the reference's own files."""
+    e, max_e, cr, _ = front(ex.solid())
+    g = emu_pores(hostemu_fast, tmp_path, e, cr, ex.mask(), False, ex.ball_order(ex.mask(), e))
+    pt, ptxt, ktxt = emu_remove(hostemu_fast, tmp_path, *g)
+    assert ptxt == ex.text("pores.txt")
+    assert ktxt == ex.text("sharedPixelTypes.txt")
+    assert sha256_array(pt) == ex.digests()["pt"]
+
+
+@pytest.mark.parametrize("d3q7", [False, True])
+def test_pores_in_a_given_ball_order(hostemu, tmp_path, d3q7):
+    e, max_e, cr, mask = front(CASES["gauss_s3_low"])
+    balls = np.flatnonzero(mask.reshape(-1) == 255)
+    r2 = e.reshape(-1)[balls].astype(np.int64)
+    order = balls[np.lexsort((np.random.default_rng(5).random(balls.size), -r2))].astype(np.uint32)
+    assert_same(emu_pores(hostemu, tmp_path, e, cr, mask, d3q7, order), oracle.pores_in_order(e, cr, order, d3q7),
+                "pores")
 
 
 @pytest.mark.parametrize("name", sorted(removal_inputs.CASES))

@@ -4,7 +4,11 @@ Bit-exact: EuclideanDistanceSquares, euclideanDistanceSquareCounts.txt, max EDT 
 ContainingBallRadiiSquares, ShortestPathDistances, CavityIds, cavityDescriptions.txt.
 Partition (up to relabelling, fed the fixture's own maximum-ball mask): PoreThroatIds and
 PoreThroatIdsWatershed.  The fixture was made with 7 OpenMP threads and MSVC's STL, so
-coordinates, the maximum-ball mask and ID numbering are only invariants there.
+coordinates, the maximum-ball mask and the std::sort tie order (hence pore IDs) are only
+invariants there.  With the fixture's own seed order replayed, the rest is bit-exact:
+PoreThroatIds, pores.txt and sharedPixelTypes.txt (throat-key IDs from the DFS order and the
+empty-pore removal), and, from the fixture's pores.txt, PoreThroatIdsWatershed, poresWatershed.txt
+and sharedPixelTypesWatershed.txt (key IDs from the watershed's FIFO order).
 """
 import numpy as np
 import pytest
@@ -66,6 +70,24 @@ def test_pore_partition_matches_fixture(run):
     ours = sorted(tuple(l.split()[4:6]) for l in ptxt.splitlines()[1:])
     theirs = sorted(tuple(l.split()[4:6]) for l in ex.text("pores.txt").splitlines()[1:])
     assert ours == theirs
+
+
+def test_pores_bit_exact_in_the_fixture_seed_order(run):
+    """Seeds taken in the fixture's pore-ID order (in place of libstdc++'s tie order): labels,
+    pore table and throat-key table with their IDs equal the reference's output."""
+    mask = ex.mask()
+    pt, ptxt, ktxt = oracle.pores_in_order(run["e"], run["cr"], ex.ball_order(mask, run["e"]))
+    pt, ptxt, ktxt = oracle.remove_empty_pores(pt, ptxt, ktxt)
+    assert ptxt == ex.text("pores.txt")
+    assert ktxt == ex.text("sharedPixelTypes.txt")
+    assert sha256_array(pt) == ex.digests()["pt"]
+
+
+def test_watershed_bit_exact_from_the_fixture_pores(run):
+    ptw, wtxt, wktxt = oracle.watershed(run["cr"], run["max_e"], ex.text("pores.txt"))
+    assert wtxt == ex.text("poresWatershed.txt")
+    assert wktxt == ex.text("sharedPixelTypesWatershed.txt")
+    assert sha256_array(ptw) == ex.digests()["ptw"]
 
 
 def test_geodesic_bit_exact():
