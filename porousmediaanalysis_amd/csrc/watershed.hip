@@ -151,10 +151,10 @@ __global__ void __launch_bounds__(256) ws_resolve(const unsigned long long* __re
         const unsigned long long it = items[pos];
         const uint32_t p = (uint32_t)(it >> 32);
         const unsigned long long when = (unsigned long long)(pos) << 5 | (unsigned long long)(k < 0 ? 0 : k);
-        bool same = false, logged = false;
-        uint32_t j = 0;
+        bool same = false, logged = false, lower = false;
+        uint32_t j = 0, c = 0;
         if (k >= 0 && ws_neighbour((uint32_t)it, k, X, Y, Z, j)) {
-            const uint32_t c = cr[j];
+            c = cr[j];
             if (c != 0 && c <= d) {
                 const uint32_t lab = label[j];
                 uint32_t owner = p;
@@ -162,17 +162,8 @@ __global__ void __launch_bounds__(256) ws_resolve(const unsigned long long* __re
                     const unsigned long long w = claim[j];
                     if (w == (hi | when)) {
                         label[j] = p + 1;
-                        if (c == d) {
-                            same = true;
-                        } else {
-                            const uint32_t slot = atomicAdd(&qcount[c], 1u);
-                            if (qoff[c] + slot < qoff[c + 1]) {
-                                qitem[qoff[c] + slot] = (unsigned long long)j | ((unsigned long long)p << 32);
-                                qkey[qoff[c] + slot] = tlevel | when;
-                            } else {
-                                atomicOr(error, 1u);
-                            }
-                        }
+                        same = c == d;
+                        lower = c != d;
                     } else {
                         owner = (uint32_t)(items[(w >> 5) & kPosMask] >> 32);
                     }
@@ -181,6 +172,28 @@ __global__ void __launch_bounds__(256) ws_resolve(const unsigned long long* __re
                 }
                 logged = owner != p;
             }
+        }
+        // Lower-level claims take slots in their level's queue: one atomic per distinct level in
+        // the wave (the queue is sorted by push time before it is popped, so slot order is free).
+        unsigned long long pend = __ballot(lower);
+        while (pend) {
+            const int leader = __ffsll((long long)pend) - 1;
+            const uint32_t c0 = (uint32_t)__shfl((int)c, leader);
+            const unsigned long long peers = __ballot(lower && c == c0);
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(&qcount[c0], (uint32_t)__popcll(peers));
+            base = (unsigned long long)(uint32_t)__shfl((int)(uint32_t)base, leader);
+            if (lower && c == c0) {
+                const uint64_t slot = qoff[c] + base + (uint64_t)__popcll(peers & ((1ull << lane) - 1ull));
+                if (slot < qoff[c + 1]) {
+                    qitem[slot] = (unsigned long long)j | ((unsigned long long)p << 32);
+                    qkey[slot] = tlevel | when;
+                } else {
+                    atomicOr(error, 1u);
+                }
+                lower = false;
+            }
+            pend &= ~peers;
         }
         const unsigned long long logs = __ballot(logged);
         if (logs) {
