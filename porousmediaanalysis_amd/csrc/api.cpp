@@ -238,7 +238,53 @@ void* pma_ctx::host_staging(size_t bytes) {
     return pinned;
 }
 
-void pma_ctx::ensure_volume(pma_volume w) { vol[w].reserve(n * volume_element_size(w)); }
+namespace {
+// Runs an allocation; when the device is out of memory, frees the evictable buffers (other than
+// `keep`) and tries once more.
+template <class F> void with_eviction(pma_ctx* c, const std::string& keep, F&& alloc) {
+    try {
+        alloc();
+    } catch (const pma::Error& e) {
+        if (e.status != PMA_ERR_OUT_OF_MEMORY) throw;
+        (void)hipGetLastError();  // the failed hipMalloc must not surface at the next launch check
+        if (!c->evict(keep)) throw;
+        alloc();
+    }
+}
+}  // namespace
+
+void pma_ctx::ensure_volume(pma_volume w) {
+    with_eviction(this, std::string(), [&] { vol[w].reserve(n * volume_element_size(w)); });
+}
+
+void* pma_ctx::buf(const std::string& name, size_t bytes) {
+    pma::DeviceBuffer& b = scratch[name];
+    with_eviction(this, name, [&] { b.reserve(bytes); });
+    return b.ptr;
+}
+
+bool pma_ctx::evict(const std::string& keep) {
+    bool any = false;
+    for (const std::string& name : evictable) {
+        if (name == keep) continue;
+        auto it = scratch.find(name);
+        if (it == scratch.end() || !it->second.ptr) continue;
+        if (!any) PMA_HIP(hipDeviceSynchronize());
+        it->second.release();
+        scratch.erase(it);
+        any = true;
+    }
+    return any;
+}
+
+size_t pma_ctx::evictable_bytes() const {
+    size_t total = 0;
+    for (const std::string& name : evictable) {
+        auto it = scratch.find(name);
+        if (it != scratch.end()) total += it->second.bytes;
+    }
+    return total;
+}
 
 void pma_ctx::drop(const std::string& name) {
     auto it = scratch.find(name);

@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <map>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -134,17 +135,21 @@ struct pma_ctx {
     void kernel_done(const char* name, hipEvent_t start);  // records the end event
     void resolve_kernel_times();                   // after a synchronisation
 
+    // scratch buffers that a failing allocation may free (large ones sized from the free memory,
+    // kept between calls only to avoid allocating them again)
+    std::set<std::string> evictable;
+
     template <class T> T* v(pma_volume w) { return vol[w].as<T>(); }
-    void* buf(const std::string& name, size_t bytes) {
-        pma::DeviceBuffer& b = scratch[name];
-        b.reserve(bytes);
-        return b.ptr;
-    }
+    void* buf(const std::string& name, size_t bytes);
     template <class T> T* buf(const std::string& name, size_t count) {
         return static_cast<T*>(buf(name, count * sizeof(T)));
     }
     // Frees a named scratch buffer after the device is idle (for the large, short-lived ones).
     void drop(const std::string& name);
+    // Frees the evictable buffers other than `keep`; false if there was none.
+    bool evict(const std::string& keep);
+    // Bytes held by the evictable buffers.
+    size_t evictable_bytes() const;
     void* host_staging(size_t bytes);
     void ensure_volume(pma_volume w);
     void sync();

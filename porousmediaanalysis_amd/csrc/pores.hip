@@ -35,6 +35,7 @@
 #include "pma_internal.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cstdlib>
 #include <numeric>
 
@@ -1358,7 +1359,22 @@ void check_stage(pma_ctx* c, const uint32_t* err) {
 
 }  // namespace
 
+// With PMA_DEBUG_SYNC, prints the time of each phase of the stage (after waiting for the stream).
+struct PhaseClock {
+    pma_ctx* c;
+    std::chrono::steady_clock::time_point last = std::chrono::steady_clock::now();
+    void mark(const char* what) {
+        if (!c->debug_sync) return;
+        c->sync();
+        const auto now = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "[pma] pores phase %-34s %9.1f ms\n", what,
+                     std::chrono::duration<double, std::milli>(now - last).count());
+        last = now;
+    }
+};
+
 void run_pores(pma_ctx* c) {
+    PhaseClock phase{c};
     PMA_REQUIRE(c->valid[PMA_VOL_MASK] && c->valid[PMA_VOL_EDT2] && c->valid[PMA_VOL_CR2], PMA_ERR_STATE,
                 "pma_pores: the maximum-ball mask, EDT and containing radii are needed");
     const int X = c->dims.x, Y = c->dims.y, Z = c->dims.z;
@@ -1400,6 +1416,7 @@ void run_pores(pma_ctx* c) {
         }
     }
 
+    phase.mark("1 maximum balls and their order");
     // 2. plateaus of CR and regional maxima.
     uint32_t* root = c->buf<uint32_t>("pt_root", n);
     uint8_t* not_max = c->buf<uint8_t>("pt_not_max", n);
@@ -1421,6 +1438,7 @@ void run_pores(pma_ctx* c) {
         PMA_CHECKED(c, "cc_not_max", err);
     }
 
+    phase.mark("2 plateaus");
     // 3. seeds: the first sorted maximum ball of every regional-maximum plateau.
     uint32_t npores = 0;
     unsigned long long* seeds = c->buf<unsigned long long>("pt_seeds", std::max<uint64_t>(M, 1));
@@ -1445,6 +1463,7 @@ void run_pores(pma_ctx* c) {
     for (uint32_t k = 0; k < npores; ++k)
         PMA_REQUIRE((uint32_t)seeds_h[k] < n, PMA_ERR_INTERNAL, "pma_pores: a seed lies outside the volume");
 
+    phase.mark("3 seeds");
     // 4. plateau indices, sizes, border flags.
     uint32_t* rflag = c->buf<uint32_t>("pt_rflag", n);
     uint32_t* roff = c->buf<uint32_t>("pt_roff", n);
@@ -1481,6 +1500,7 @@ void run_pores(pma_ctx* c) {
     }
     check_stage(c, err);
 
+    phase.mark("4 plateau indices");
     // 5. plateau DAG edges (distinct), grouped by target.
     size_t cap = 1024;
     while (cap < P * 8) cap <<= 1;
@@ -1523,6 +1543,7 @@ void run_pores(pma_ctx* c) {
         PMA_CHECKED(c, "edge_ranges", err);
     }
 
+    phase.mark("5 plateau edges");
     // 6. pore sets, level by level in descending CR.
     unsigned long long* lkeys = c->buf<unsigned long long>("pt_lkeys", P1);
     unsigned long long* lkeys2 = c->buf<unsigned long long>("pt_lkeys2", P1);
@@ -1603,6 +1624,7 @@ void run_pores(pma_ctx* c) {
     }
     const SetBounds sb{E, P, npores, pool_cap};
 
+    phase.mark("6 pore sets");
     // 7. key formations (prefix records) and their distinct lists.
     unsigned long long* pcnt = c->buf<unsigned long long>("pt_pcnt", P1);
     unsigned long long* rbase = c->buf<unsigned long long>("pt_rbase", P1);
@@ -1648,6 +1670,7 @@ void run_pores(pma_ctx* c) {
     for (uint32_t k = 0; k < npores; ++k)
         PMA_REQUIRE(reach_h[k] >= 1 && reach_h[k] <= n, PMA_ERR_INTERNAL, "pma_pores: pore reach out of range");
 
+    phase.mark("7 key formations");
     // 8. DFS replays for pores that form two or more lists: rank of each list inside its pore.
     std::vector<uint32_t> rank_h(D, kInv);
     uint64_t replay_tasks = 0, replay_steps = 0;
@@ -1752,8 +1775,16 @@ void run_pores(pma_ctx* c) {
             const unsigned long long trav_cap = std::min<unsigned long long>(P + 1, PMA_PORES_TRAV_CAP);
             unsigned long long max_reach = 0;
             for (const ReplayTask& t : all) max_reach = std::max<unsigned long long>(max_reach, reach_h[t.pore]);
+            // the replay buffers of an earlier call are kept (allocating ~100 GB again costs more
+            // than a second) and count as free here; any allocation that finds the device full
+            // frees them first
+            static const char* const kReplayBuffers[] = {"pt_replay_vis", "pt_replay_plateaus", "pt_replay_cls",
+                                                         "pt_stack", "pt_replay_trav"};
+            for (const char* name : kReplayBuffers) c->evictable.insert(name);
             size_t free_b = 0, total_b = 0;
             PMA_HIP(hipMemGetInfo(&free_b, &total_b));
+            free_b += c->evictable_bytes();
+            for (const char* name : kReplayBuffers) c->evictable.erase(name);  // in use until the replays end
             const unsigned long long budget =
                 free_b > (40ull << 30) ? std::min<unsigned long long>(free_b - (24ull << 30), PMA_PORES_REPLAY_MEM)
                                        : free_b / 2;
@@ -1767,17 +1798,29 @@ void run_pores(pma_ctx* c) {
                 const uint32_t vis_pages = pass == 0 ? std::min<uint32_t>(nblocks, PMA_PORES_VIS_PAGES) : nblocks;
                 const unsigned long long per_wave =
                     (vis_pages * page_words + 2 * pent_words) * 8 + P * 2 + cap * 16 + trav_cap * 8;
-                const unsigned long long W =
+                unsigned long long W =
                     std::max<unsigned long long>(1, std::min<unsigned long long>({pending.size(), budget / per_wave, 1024}));
+                {
+                    // waves the kept buffers already hold: take them rather than grow by a little
+                    const unsigned long long need[5] = {vis_pages * page_words * 8, 2 * pent_words * 8, P * 2,
+                                                        cap * 16, trav_cap * 8};
+                    unsigned long long fit = ~0ull;
+                    for (int k = 0; k < 5; ++k) {
+                        auto it = c->scratch.find(kReplayBuffers[k]);
+                        fit = std::min(fit, it == c->scratch.end() ? 0ull : it->second.bytes / std::max(need[k], 1ull));
+                    }
+                    if (fit >= W - W / 4) W = std::max<unsigned long long>(1, std::min<unsigned long long>({fit, pending.size(), 1024}));
+                }
                 unsigned long long* vis = c->buf<unsigned long long>("pt_replay_vis", W * vis_pages * page_words);
                 unsigned long long* pbits = c->buf<unsigned long long>("pt_replay_plateaus", W * 2 * pent_words);
                 uint16_t* cls = c->buf<uint16_t>("pt_replay_cls", W * P);
                 uint4* stack = c->buf<uint4>("pt_stack", W * cap);
                 uint32_t* trav = c->buf<uint32_t>("pt_replay_trav", W * 2 * trav_cap);
-                                upload(c, d_tasks, pending.data(), pending.size() * sizeof(ReplayTask));
+                upload(c, d_tasks, pending.data(), pending.size() * sizeof(ReplayTask));
                 PMA_HIP(hipMemsetAsync(status, 0, pending.size() * 4, c->stream));
                 const ReplayBounds rb{n, P, E, trav_cap, lists_len, ranks_len};
                 PMA_HIP(hipMemsetAsync(queue, 0, 4, c->stream));
+                phase.mark("8 replay set-up and buffers");
                 const auto t0 = std::chrono::steady_clock::now();
                 {
                     PMA_TIMED(c, "dfs_replay");
@@ -1814,10 +1857,9 @@ void run_pores(pma_ctx* c) {
             PMA_HIP(hipStreamSynchronize(c->stream));
             for (uint32_t d = 0; d < D; ++d)
                 if (replayed_h[ev_owner[d]]) rank_h[d] = rank_slot[rank_base[ev_owner[d]] + ev_local[d]];
-            // the per-slot buffers are sized by the free memory: give them back
-            for (const char* name : {"pt_replay_vis", "pt_replay_plateaus", "pt_replay_cls", "pt_stack",
-                                     "pt_replay_trav", "pt_lists"})
-                c->drop(name);
+            for (const char* name : kReplayBuffers) c->evictable.insert(name);
+            c->drop("pt_lists");
+            phase.mark("8 replay passes and release");
         }
         c->stage[PMA_STAGE_PORES].work_items = replay_steps;
         (void)replay_tasks;
@@ -1842,6 +1884,7 @@ void run_pores(pma_ctx* c) {
     std::vector<uint32_t> key_id(std::max(D, 1u), 0);
     for (uint32_t r = 0; r < D; ++r) key_id[ord[r]] = 0xFFFFFFFFu - 1u - r;
 
+    phase.mark("8 DFS replays");
     // 9. labels.
     uint32_t* d_kid = c->buf<uint32_t>("pt_key_id", std::max(D, 1u));
     upload(c, d_kid, key_id.data(), D * 4ull);
@@ -1862,6 +1905,7 @@ void run_pores(pma_ctx* c) {
     if (npores) PMA_HIP(hipMemcpyAsync(pvol_h.data(), pvol, npores * 8ull, hipMemcpyDeviceToHost, c->stream));
     PMA_HIP(hipStreamSynchronize(c->stream));
 
+    phase.mark("9 labels");
     // 10. tables: pores in ascending ID, keys with volume in ascending ID, lists in the
     // reference's order [p2, p1, p3, ...].
     c->pores.assign(npores, PoreRow{});
@@ -1890,6 +1934,7 @@ void run_pores(pma_ctx* c) {
         for (size_t q = 2; q < mem.size(); ++q) t.pores.push_back(127u + 255u * (mem[q] / 129u) + (mem[q] % 129u));
         c->throats.push_back(t);
     }
+    phase.mark("10 tables");
     c->have_pores = true;
     c->valid[PMA_VOL_PT] = true;
 }

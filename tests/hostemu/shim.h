@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <set>
 #include <mutex>
 #include <numeric>
 #include <stdexcept>
@@ -483,6 +484,16 @@ struct pma_ctx {
         bufs.erase(it);
     }
     void* host_staging(size_t bytes) { return buf("host_staging", bytes); }
+    void sync() {}
+    std::set<std::string> evictable;
+    size_t evictable_bytes() const {
+        size_t total = 0;
+        for (const std::string& name : evictable) {
+            auto it = bufs.find(name);
+            if (it != bufs.end()) total += it->second.second;
+        }
+        return total;
+    }
 };
 
 namespace pma {
