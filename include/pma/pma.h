@@ -166,6 +166,48 @@ pma_status pma_read_tiff_stack(const char* folder, void* dst, size_t bytes, pma_
 pma_status pma_write_tiff_stack(const char* folder, const char* like, const void* src, pma_dims dims,
                                 int bytes_per_voxel);
 
+/* ---- tuning knobs and batches of subvolumes ---------------------------------------------- */
+/* No counterpart in the reference (it runs one volume per console pass).  Knobs that change how
+ * the work is cut, never a result: */
+typedef enum pma_tunable {
+    PMA_TUNE_GEODESIC_BUCKET = 0,  /* distance width of a geodesic round (1 .. 1024; default 16)   */
+    PMA_TUNE_FLOOD_GRID = 1,       /* workgroups of a watershed round (64 .. 65536; default 4096)  */
+    PMA_TUNE_REPLAY_MEMORY_MB = 2  /* device memory the pore stage's DFS replays may hold, in MiB
+                                      (256 .. 1048576; default 131072)                             */
+} pma_tunable;
+pma_status pma_set_tuning(pma_ctx* ctx, pma_tunable which, double value);
+pma_status pma_get_tuning(pma_ctx* ctx, pma_tunable which, double* value);
+
+/* One subvolume of a batch: a host solid volume ([z][x][y], non-zero = solid) and its size. */
+typedef struct pma_batch_volume {
+    pma_dims dims;
+    const uint8_t* solid;
+} pma_batch_volume;
+/* How a batch runs: `workers` host threads, each with its own context and stream, take the
+ * subvolumes largest first; the other fields are the knobs above (0 = default; replay memory
+ * defaults to the free device memory / (2 * workers), at most 128 GiB). */
+typedef struct pma_batch_config {
+    uint32_t workers;
+    double geodesic_bucket;
+    uint32_t flood_grid;
+    uint32_t replay_memory_mb;
+} pma_batch_config;
+/* Called on a worker's thread when subvolume `index` is done (status PMA_OK) or failed; `ctx`
+ * holds that subvolume's volumes and tables until the callback returns, for pma_read_volume and
+ * pma_get_text.  Calls for different subvolumes may run at the same time. */
+typedef void (*pma_batch_done)(void* user, size_t index, pma_ctx* ctx, pma_status status);
+/* Runs the whole pipeline (pma_run_all) on every subvolume.  Every subvolume is attempted; the
+ * return value is PMA_OK or the status of the first failure (pma_last_error names its index).
+ * `done` may be NULL. */
+pma_status pma_run_batch(int device, const pma_batch_volume* volumes, size_t count, pma_stencil stencil,
+                         const pma_batch_config* config, pma_batch_done done, void* user);
+/* The autotuner: times pma_run_batch on `sample` (after one untimed pass that allocates) over
+ * worker counts 1, 2, 4, 8, 12, 16, then geodesic buckets 8, 16, 32, then flood grids 1024, 4096,
+ * 16384, keeping the fastest of each in turn, and writes the fastest configuration and its
+ * voxels per second. */
+pma_status pma_tune_batch(int device, const pma_batch_volume* sample, size_t count, pma_stencil stencil,
+                          pma_batch_config* best, double* best_voxels_per_second);
+
 /* ---- allocation-free geodesic ---------------------------------------------------------- */
 /* One row of cavityDescriptions.txt (ShortestPathComputer.cpp:143-164, Serializer.cpp:205-221). */
 typedef struct pma_cavity {

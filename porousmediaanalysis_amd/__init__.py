@@ -33,8 +33,9 @@ EXPORTS = [
     "pma_get_statistics", "pma_get_histogram", "pma_get_text", "pma_set_text", "pma_get_stage_info",
     "pma_generate_spheres", "pma_generate_gaussian", "pma_set_kernel_timing", "pma_get_kernel_time",
     "pma_list_kernel_times", "pma_geodesic_noalloc", "pma_run_folder", "pma_read_tiff_stack",
-    "pma_write_tiff_stack",
+    "pma_write_tiff_stack", "pma_set_tuning", "pma_get_tuning", "pma_run_batch", "pma_tune_batch",
 ]
+TUNE_GEODESIC_BUCKET, TUNE_FLOOD_GRID, TUNE_REPLAY_MEMORY_MB = range(3)
 
 
 class PmaError(RuntimeError):
@@ -50,6 +51,21 @@ class Dims(ctypes.Structure):
 class StageInfo(ctypes.Structure):
     _fields_ = [("device_ms", ctypes.c_double), ("host_ms", ctypes.c_double), ("launches", ctypes.c_uint64),
                 ("work_items", ctypes.c_uint64)]
+
+
+class BatchVolume(ctypes.Structure):
+    _fields_ = [("dims", Dims), ("solid", ctypes.c_void_p)]
+
+
+class BatchConfig(ctypes.Structure):
+    _fields_ = [("workers", ctypes.c_uint32), ("geodesic_bucket", ctypes.c_double), ("flood_grid", ctypes.c_uint32),
+                ("replay_memory_mb", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+BATCH_DONE = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_int)
 
 
 def build(jobs=8):
@@ -91,6 +107,12 @@ def lib():
         L.pma_read_tiff_stack.argtypes = [ctypes.c_char_p, vp, ctypes.c_size_t, ctypes.POINTER(Dims),
                                           ctypes.POINTER(ctypes.c_int)]
         L.pma_write_tiff_stack.argtypes = [ctypes.c_char_p, ctypes.c_char_p, vp, Dims, ctypes.c_int]
+        L.pma_set_tuning.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+        L.pma_get_tuning.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+        L.pma_run_batch.argtypes = [ctypes.c_int, ctypes.POINTER(BatchVolume), ctypes.c_size_t, ctypes.c_int,
+                                    ctypes.POINTER(BatchConfig), BATCH_DONE, vp]
+        L.pma_tune_batch.argtypes = [ctypes.c_int, ctypes.POINTER(BatchVolume), ctypes.c_size_t, ctypes.c_int,
+                                     ctypes.POINTER(BatchConfig), ctypes.POINTER(ctypes.c_double)]
         _lib = L
     return _lib
 
@@ -117,6 +139,44 @@ def write_tiff_stack(folder, like, array):
         raise ValueError("1 or 4 bytes per voxel")
     Z, X, Y = a.shape
     _check(lib().pma_write_tiff_stack(folder.encode(), like.encode(), a.ctypes.data, Dims(X, Y, Z), a.dtype.itemsize))
+
+
+def _batch_volumes(volumes):
+    arrs = [np.ascontiguousarray(v, dtype=np.uint8) for v in volumes]
+    items = (BatchVolume * max(len(arrs), 1))()
+    for i, a in enumerate(arrs):
+        Z, X, Y = a.shape
+        items[i] = BatchVolume(Dims(X, Y, Z), a.ctypes.data)
+    return arrs, items
+
+
+def run_batch(volumes, d3q7=False, config=None, on_done=None, device=0):
+    """The whole pipeline on every (Z, X, Y) solid volume, on a pool of worker contexts.
+    on_done(index, ctx) runs on a worker thread for every finished subvolume, with a Context view of
+    the worker's context (read volumes and texts there); config: a BatchConfig or None (defaults)."""
+    arrs, items = _batch_volumes(volumes)
+    errors = []
+
+    def done(_user, index, h, status):
+        try:
+            if status == 0 and on_done is not None:
+                on_done(index, _Borrowed(h, arrs[index].shape))
+        except Exception as e:  # an exception must not cross into the worker thread
+            errors.append((index, e))
+
+    cb = BATCH_DONE(done) if on_done is not None else BATCH_DONE()
+    cfg = ctypes.byref(config) if config is not None else None
+    _check(lib().pma_run_batch(device, items, len(arrs), int(d3q7), cfg, cb, None))
+    if errors:
+        raise errors[0][1]
+
+
+def tune_batch(sample, d3q7=False, device=0):
+    """The autotuner on a sample of subvolumes: (BatchConfig, voxels per second)."""
+    arrs, items = _batch_volumes(sample)
+    best, vps = BatchConfig(), ctypes.c_double()
+    _check(lib().pma_tune_batch(device, items, len(arrs), int(d3q7), ctypes.byref(best), ctypes.byref(vps)))
+    return best, vps.value
 
 
 class Context:
@@ -146,6 +206,14 @@ class Context:
             pass
 
     # -- configuration and volumes --------------------------------------------------------
+    def set_tuning(self, which, value):
+        _check(lib().pma_set_tuning(self._h, which, float(value)))
+
+    def tuning(self, which):
+        v = ctypes.c_double()
+        _check(lib().pma_get_tuning(self._h, which, ctypes.byref(v)))
+        return v.value
+
     def set_stencil(self, d3q7):
         _check(lib().pma_set_stencil(self._h, 1 if d3q7 else 0))
 
@@ -242,3 +310,14 @@ class Context:
     def generate_gaussian(self, shape, sigma, porosity, seed):
         self.set_shape(shape)
         _check(lib().pma_generate_gaussian(self._h, sigma, porosity, seed))
+
+
+class _Borrowed(Context):
+    """A view of a context owned by the library (a batch worker's): never destroyed here."""
+
+    def __init__(self, h, shape):
+        self._h = ctypes.c_void_p(h)
+        self.shape = tuple(shape)
+
+    def close(self):
+        self._h = None

@@ -11,10 +11,10 @@
 
 namespace pma {
 
-void DeviceBuffer::reserve(size_t n) {
+void DeviceBuffer::reserve(size_t n, hipStream_t stream) {
     if (n <= bytes) return;
     if (ptr) {
-        PMA_HIP(hipDeviceSynchronize());
+        PMA_HIP(hipStreamSynchronize(stream));  // work queued on the old buffer has finished
         PMA_HIP(hipFree(ptr));
         ptr = nullptr;
         bytes = 0;
@@ -254,12 +254,12 @@ template <class F> void with_eviction(pma_ctx* c, const std::string& keep, F&& a
 }  // namespace
 
 void pma_ctx::ensure_volume(pma_volume w) {
-    with_eviction(this, std::string(), [&] { vol[w].reserve(n * volume_element_size(w)); });
+    with_eviction(this, std::string(), [&] { vol[w].reserve(n * volume_element_size(w), stream); });
 }
 
 void* pma_ctx::buf(const std::string& name, size_t bytes) {
     pma::DeviceBuffer& b = scratch[name];
-    with_eviction(this, name, [&] { b.reserve(bytes); });
+    with_eviction(this, name, [&] { b.reserve(bytes, stream); });
     return b.ptr;
 }
 
@@ -269,7 +269,7 @@ bool pma_ctx::evict(const std::string& keep) {
         if (name == keep) continue;
         auto it = scratch.find(name);
         if (it == scratch.end() || !it->second.ptr) continue;
-        if (!any) PMA_HIP(hipDeviceSynchronize());
+        if (!any) PMA_HIP(hipStreamSynchronize(stream));
         it->second.release();
         scratch.erase(it);
         any = true;
@@ -289,7 +289,7 @@ size_t pma_ctx::evictable_bytes() const {
 void pma_ctx::drop(const std::string& name) {
     auto it = scratch.find(name);
     if (it == scratch.end()) return;
-    PMA_HIP(hipDeviceSynchronize());
+    PMA_HIP(hipStreamSynchronize(stream));
     it->second.release();
     scratch.erase(it);
 }

@@ -104,7 +104,7 @@ __global__ void __launch_bounds__(256) find_start(const uint8_t* __restrict__ so
 
 
 constexpr int kB = 8;           // block edge
-constexpr float kDelta = 16.0f;  // bucket width of a round (distance units; 4..64 measured within 10%)
+constexpr float kDelta = 16.0f;  // default bucket width of a round (distance units; 4..64 measured within 10%)
 #ifndef PMA_GEO_RELAX_GRID
 // one-wave workgroups of a relax launch: 64 per CU, twice what fits at once, so that waves that
 // finish early are replaced (16384 measured best of 2048..16384 at 1024^3)
@@ -133,7 +133,7 @@ __global__ void fill_key(uint32_t* __restrict__ p, uint32_t v, size_t n) {
     if (i < n) p[i] = v;
 }
 
-// Splits the active list: keys within kDelta of its smallest key go to `proc` (their key is
+// Splits the active list: keys within `delta` of its smallest key go to `proc` (their key is
 // cleared, so that activations during the round start it afresh), the others are carried into
 // the next list (tagged so that a new activation does not list them twice).  Grid-stride over
 // the list, whose length is cur->count, 256 entries per workgroup step; each step reserves its
@@ -142,10 +142,10 @@ __global__ void __launch_bounds__(256) partition_active(const uint32_t* __restri
                                                         RoundState* __restrict__ cur, uint32_t* __restrict__ block_key,
                                                         uint32_t* __restrict__ proc, uint32_t* __restrict__ next,
                                                         uint32_t* __restrict__ block_tag, uint32_t tag,
-                                                        RoundState* __restrict__ nxt) {
+                                                        RoundState* __restrict__ nxt, float delta) {
     __shared__ uint32_t wave_now[4], wave_later[4], base_now, base_later;
     const uint32_t n = cur->count;
-    const float thr = __uint_as_float(cur->kmin) + kDelta;
+    const float thr = __uint_as_float(cur->kmin) + delta;
     const uint32_t threshold = isfinite(thr) ? __float_as_uint(thr) : 0xFFFFFFFFu;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const unsigned long long below = (1ull << lane) - 1;
@@ -581,7 +581,7 @@ unsigned long long geodesic_core(GeoHost& h, const uint8_t* solid, float* dist, 
             uint32_t* cur = (r & 1) ? g.fb : g.fa;
             uint32_t* nxt = (r & 1) ? g.fa : g.fb;
             partition_active<<<std::min<size_t>(grid_for(NB, 256), 256u), 256, 0, st>>>(
-                cur, cur_s, g.block_key, g.fp, nxt, g.block_tag, round, nxt_s);
+                cur, cur_s, g.block_key, g.fp, nxt, g.block_tag, round, nxt_s, h.ctx ? h.ctx->geo_delta : kDelta);
             h.launched("partition_active");
             {
                 PMA_TIMED(h.ctx, "relax_blocks");

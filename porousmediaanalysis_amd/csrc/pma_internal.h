@@ -73,11 +73,12 @@ struct CavityRow {
     bool reaches_all = false;
 };
 
-// A device allocation that only grows.
+// A device allocation that only grows.  Its work runs on one stream, which reserve() waits for
+// before it frees a buffer too small.
 struct DeviceBuffer {
     void* ptr = nullptr;
     size_t bytes = 0;
-    void reserve(size_t n);
+    void reserve(size_t n, hipStream_t stream);
     void release();
     template <class T> T* as() const { return static_cast<T*>(ptr); }
 };
@@ -135,6 +136,11 @@ struct pma_ctx {
     void kernel_done(const char* name, hipEvent_t start);  // records the end event
     void resolve_kernel_times();                   // after a synchronisation
 
+    // knobs of pma_set_tuning (none changes a result)
+    float geo_delta = 16.0f;                   // distance width of a geodesic round
+    uint32_t flood_grid = 4096;                // workgroups of ws_bid and ws_resolve
+    unsigned long long replay_mem = 128ull << 30;  // device bytes the DFS replays may hold
+
     // scratch buffers that a failing allocation may free (large ones sized from the free memory,
     // kept between calls only to avoid allocating them again)
     std::set<std::string> evictable;
@@ -144,7 +150,7 @@ struct pma_ctx {
     template <class T> T* buf(const std::string& name, size_t count) {
         return static_cast<T*>(buf(name, count * sizeof(T)));
     }
-    // Frees a named scratch buffer after the device is idle (for the large, short-lived ones).
+    // Frees a named scratch buffer once the context's stream is idle (for the large, short-lived ones).
     void drop(const std::string& name);
     // Frees the evictable buffers other than `keep`; false if there was none.
     bool evict(const std::string& keep);

@@ -1786,8 +1786,10 @@ void run_pores(pma_ctx* c) {
             free_b += c->evictable_bytes();
             for (const char* name : kReplayBuffers) c->evictable.erase(name);  // in use until the replays end
             const unsigned long long budget =
-                free_b > (40ull << 30) ? std::min<unsigned long long>(free_b - (24ull << 30), PMA_PORES_REPLAY_MEM)
-                                       : free_b / 2;
+                std::min<unsigned long long>(c->replay_mem, free_b > (40ull << 30)
+                                                                ? std::min<unsigned long long>(free_b - (24ull << 30),
+                                                                                               PMA_PORES_REPLAY_MEM)
+                                                                : free_b / 2);
             std::vector<ReplayTask> pending = all;
             for (int pass = 0; !pending.empty(); ++pass) {
                 PMA_REQUIRE(pass < 2, PMA_ERR_INTERNAL, "pma_pores: DFS replay stack overflow with full capacity");

@@ -80,7 +80,6 @@ struct WsRound {
     uint32_t a, b;
 };
 constexpr uint32_t kWsGrid = 512;       // workgroups of the round's scan kernels (grid-stride over the wave)
-constexpr uint32_t kWsItemGrid = 4096;  // workgroups of ws_bid and ws_resolve (grid-stride over the lanes)
 
 // A round's items are spread over lanes, one lane per (item, neighbour): the item's lanes form
 // an aligned group of 32 (the 26 neighbours of D3Q27) or 8 (the 6 of D3Q7), so a group never
@@ -579,7 +578,8 @@ bool watershed_once(pma_ctx* c, unsigned long long log_cap, unsigned long long* 
         const WsRound first{0u, cnt};
         upload(c, rstate, &first, sizeof first);
         const uint32_t g = std::min(grid_for(h[d], 256), kWsGrid);
-        const uint32_t gi = std::min(grid_for((size_t)h[d] << (d3q7 ? 3 : 5), 256), kWsItemGrid);
+        // ws_bid and ws_resolve: grid-stride over the lanes on at most c->flood_grid workgroups
+        const uint32_t gi = std::min(grid_for((size_t)h[d] << (d3q7 ? 3 : 5), 256), c->flood_grid);
         uint32_t r = 0, check_at = 1, step = 1;
         while (true) {
             PMA_REQUIRE(gr < (1u << 24) - 1, PMA_ERR_INTERNAL, "pma_watershed: too many rounds");

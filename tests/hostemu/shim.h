@@ -485,6 +485,9 @@ struct pma_ctx {
     }
     void* host_staging(size_t bytes) { return buf("host_staging", bytes); }
     void sync() {}
+    float geo_delta = 16.0f;
+    uint32_t flood_grid = 4096;
+    unsigned long long replay_mem = 128ull << 30;
     std::set<std::string> evictable;
     size_t evictable_bytes() const {
         size_t total = 0;
