@@ -393,11 +393,17 @@ __device__ __forceinline__ unsigned long long mix64(unsigned long long k) {
 }
 
 // Distinct plateau edges (higher CR -> lower CR) in an open-addressing set.  A voxel skips a
-// source plateau it has already inserted for its previous neighbours (at most 26).
+// source plateau it has already inserted for its previous neighbours (at most 26).  The set is
+// kept at most half full: the insertion that passes half the capacity (or a probe sequence longer
+// than kEdgeProbes) flags an overflow, every thread stops at its next edge, and the host retries
+// with a table four times larger -- so probe sequences stay short instead of sweeping a nearly
+// full table.
+constexpr unsigned long long kEdgeProbes = 1024;
 __global__ void __launch_bounds__(256) plateau_edges(const uint32_t* __restrict__ cr, const uint32_t* __restrict__ plat,
                                                      int X, int Y, int Z, int d3q7, size_t P,
                                                      unsigned long long* __restrict__ table, unsigned long long mask,
-                                                     uint32_t* __restrict__ overflow, uint32_t* __restrict__ err) {
+                                                     unsigned long long* __restrict__ fill, uint32_t* __restrict__ overflow,
+                                                     uint32_t* __restrict__ err) {
     const size_t n = (size_t)X * Y * Z;
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -426,10 +432,11 @@ __global__ void __launch_bounds__(256) plateau_edges(const uint32_t* __restrict_
         for (int q = 0; q < nd; ++q) seen |= done[q] == src;
         if (seen) continue;
         if (nd < 26) done[nd++] = src;
+        if (ld_relaxed(overflow)) return;  // the host retries with a larger table
         const unsigned long long key = ((unsigned long long)src << 32) | dst;
         unsigned long long h = mix64(key) & mask;
         for (unsigned long long probe = 0;; ++probe) {
-            if (probe > mask) {
+            if (probe > mask || probe >= kEdgeProbes) {
                 atomicOr(overflow, 1u);
                 break;
             }
@@ -437,6 +444,7 @@ __global__ void __launch_bounds__(256) plateau_edges(const uint32_t* __restrict_
             if (cur == key) break;
             if (cur == ~0ull) {
                 unsigned long long old = atomicCAS(&table[h], ~0ull, key);
+                if (old == ~0ull && atomicAdd(fill, 1ull) >= (mask + 1) / 2) atomicOr(overflow, 1u);
                 if (old == ~0ull || old == key) break;
             }
             h = (h + 1) & mask;
@@ -1503,20 +1511,22 @@ void run_pores(pma_ctx* c) {
     phase.mark("4 plateau indices");
     // 5. plateau DAG edges (distinct), grouped by target.
     size_t cap = 1024;
-    while (cap < P * 8) cap <<= 1;
+    while (cap < P * 16) cap <<= 1;
     unsigned long long* table = nullptr;
     for (int attempt = 0;; ++attempt) {
         table = c->buf<unsigned long long>("pt_edge_table", cap);
+        unsigned long long* fill = c->buf<unsigned long long>("pt_edge_fill", 1);
         PMA_HIP(hipMemsetAsync(table, 0xFF, cap * 8, c->stream));
         PMA_HIP(hipMemsetAsync(of, 0, 4, c->stream));
+        PMA_HIP(hipMemsetAsync(fill, 0, 8, c->stream));
         {
             PMA_TIMED(c, "plateau_edges");
-            plateau_edges<<<nb, 256, 0, c->stream>>>(cr, plat, X, Y, Z, d3q7, P, table, cap - 1, of, err);
+            plateau_edges<<<nb, 256, 0, c->stream>>>(cr, plat, X, Y, Z, d3q7, P, table, cap - 1, fill, of, err);
             PMA_CHECKED(c, "plateau_edges", err);
         }
         check_stage(c, err);
         if (read_scalar(c, of) == 0) break;
-        PMA_REQUIRE(attempt < 4, PMA_ERR_INTERNAL, "pma_pores: plateau edge table overflow");
+        PMA_REQUIRE(attempt < 8, PMA_ERR_INTERNAL, "pma_pores: plateau edge table overflow");
         cap <<= 2;
     }
     uint32_t* tflag = c->buf<uint32_t>("pt_tflag", cap);
