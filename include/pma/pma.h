@@ -208,6 +208,30 @@ pma_status pma_run_batch(int device, const pma_batch_volume* volumes, size_t cou
 pma_status pma_tune_batch(int device, const pma_batch_volume* sample, size_t count, pma_stencil stencil,
                           pma_batch_config* best, double* best_voxels_per_second);
 
+/* ---- learned screening ----------------------------------------------------------------------
+ * No counterpart in the reference.  A 3-D U-Net (3 levels, 16/32/64 channels, 3x3x3
+ * convolutions, ~0.36 M parameters, bf16 weights and activations, fp32 accumulation on MFMA)
+ * that predicts, per void voxel, the containing-ball radius and the ratio of the geodesic to the
+ * Euclidean distance from the voxel's cavity start, from the solid volume and its EDT.  Its
+ * fields are estimates for screening many subvolumes: they never replace a volume or table of the
+ * exact pipeline, which makes the training labels and is rerun where the estimates are off. */
+typedef struct pma_surrogate pma_surrogate;
+/* He-normal weights from `seed`; its own stream on `device`. */
+pma_status pma_surrogate_create(pma_surrogate** s, int device, uint64_t seed);
+void pma_surrogate_destroy(pma_surrogate* s);
+pma_status pma_surrogate_param_count(pma_surrogate* s, size_t* count);
+/* Copies the fp32 weights (which = 0) or the last step's gradients (which = 1) out; layers in
+ * order, each as weights [tap][out][in] then bias [out], with the padded channels. */
+pma_status pma_surrogate_params(pma_surrogate* s, int which, float* dst, size_t count);
+pma_status pma_surrogate_set_params(pma_surrogate* s, const float* src, size_t count);
+/* One step on the volume of `ctx`, which must hold SOLID, EDT2, CR2, SPD and CAVITY (pma_edt,
+ * pma_containing_balls and pma_geodesic run), every side a multiple of 4: forward pass, mean
+ * squared error against the exact fields (*loss), backward pass; with update != 0 an Adam step. */
+pma_status pma_surrogate_train_step(pma_surrogate* s, pma_ctx* ctx, float learning_rate, int update, float* loss);
+/* Predicted radius (voxels) and distance ratio of every voxel of `ctx` (0 on solid voxels), from
+ * SOLID and EDT2 only; host buffers of one float per voxel, or device buffers. */
+pma_status pma_surrogate_predict(pma_surrogate* s, pma_ctx* ctx, float* radius, float* ratio, int dst_on_device);
+
 /* ---- allocation-free geodesic ---------------------------------------------------------- */
 /* One row of cavityDescriptions.txt (ShortestPathComputer.cpp:143-164, Serializer.cpp:205-221). */
 typedef struct pma_cavity {

@@ -34,6 +34,8 @@ EXPORTS = [
     "pma_generate_spheres", "pma_generate_gaussian", "pma_set_kernel_timing", "pma_get_kernel_time",
     "pma_list_kernel_times", "pma_geodesic_noalloc", "pma_run_folder", "pma_read_tiff_stack",
     "pma_write_tiff_stack", "pma_set_tuning", "pma_get_tuning", "pma_run_batch", "pma_tune_batch",
+    "pma_surrogate_create", "pma_surrogate_destroy", "pma_surrogate_param_count", "pma_surrogate_params",
+    "pma_surrogate_set_params", "pma_surrogate_train_step", "pma_surrogate_predict",
 ]
 TUNE_GEODESIC_BUCKET, TUNE_FLOOD_GRID, TUNE_REPLAY_MEMORY_MB = range(3)
 
@@ -113,6 +115,14 @@ def lib():
                                     ctypes.POINTER(BatchConfig), BATCH_DONE, vp]
         L.pma_tune_batch.argtypes = [ctypes.c_int, ctypes.POINTER(BatchVolume), ctypes.c_size_t, ctypes.c_int,
                                      ctypes.POINTER(BatchConfig), ctypes.POINTER(ctypes.c_double)]
+        L.pma_surrogate_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_uint64]
+        L.pma_surrogate_destroy.argtypes = [vp]
+        L.pma_surrogate_destroy.restype = None
+        L.pma_surrogate_param_count.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
+        L.pma_surrogate_params.argtypes = [vp, ctypes.c_int, vp, ctypes.c_size_t]
+        L.pma_surrogate_set_params.argtypes = [vp, vp, ctypes.c_size_t]
+        L.pma_surrogate_train_step.argtypes = [vp, vp, ctypes.c_float, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
+        L.pma_surrogate_predict.argtypes = [vp, vp, vp, vp, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -321,3 +331,62 @@ class _Borrowed(Context):
 
     def close(self):
         self._h = None
+
+
+class Surrogate:
+    """The learned screening network (pma_surrogate_*): train on contexts that hold the exact
+    fields, predict radius and distance ratio fields from a context's solid volume and EDT."""
+
+    # (cin, cout, taps, cin_real, cout_real) of the layers, in the parameter vector's order
+    LAYERS = [(16, 16, 27, 2, 16), (16, 16, 27, 16, 16), (16, 32, 27, 16, 32), (32, 32, 27, 32, 32),
+              (32, 64, 27, 32, 64), (64, 64, 27, 64, 64), (96, 32, 27, 96, 32), (32, 32, 27, 32, 32),
+              (48, 16, 27, 48, 16), (16, 16, 27, 16, 16), (16, 16, 1, 16, 2)]
+
+    def __init__(self, device=0, seed=1):
+        h = ctypes.c_void_p()
+        _check(lib().pma_surrogate_create(ctypes.byref(h), device, seed))
+        self._h = h
+        n = ctypes.c_size_t()
+        _check(lib().pma_surrogate_param_count(self._h, ctypes.byref(n)))
+        self.nparam = n.value
+
+    def close(self):
+        if self._h:
+            lib().pma_surrogate_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def params(self, grads=False):
+        out = np.empty(self.nparam, np.float32)
+        _check(lib().pma_surrogate_params(self._h, 1 if grads else 0, out.ctypes.data, out.size))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(lib().pma_surrogate_set_params(self._h, p.ctypes.data, p.size))
+
+    def layer_views(self, flat):
+        """[(weights [tap][out][in], bias [out])] views of a flat parameter vector."""
+        out, off = [], 0
+        for cin, cout, taps, _, _ in self.LAYERS:
+            w = flat[off:off + taps * cout * cin].reshape(taps, cout, cin)
+            off += taps * cout * cin
+            out.append((w, flat[off:off + cout]))
+            off += cout
+        return out
+
+    def train_step(self, ctx, lr=1e-3, update=True):
+        loss = ctypes.c_float()
+        _check(lib().pma_surrogate_train_step(self._h, ctx._h, lr, 1 if update else 0, ctypes.byref(loss)))
+        return loss.value
+
+    def predict(self, ctx):
+        radius = np.empty(ctx.shape, np.float32)
+        ratio = np.empty(ctx.shape, np.float32)
+        _check(lib().pma_surrogate_predict(self._h, ctx._h, radius.ctypes.data, ratio.ctypes.data, 0))
+        return radius, ratio
