@@ -156,6 +156,11 @@ struct pma_ctx {
     bool evict(const std::string& keep);
     // Bytes held by the evictable buffers.
     size_t evictable_bytes() const;
+    // Bytes held by a named scratch buffer (0 if none).
+    size_t buffer_bytes(const std::string& name) const {
+        auto it = scratch.find(name);
+        return it == scratch.end() ? 0 : it->second.bytes;
+    }
     void* host_staging(size_t bytes);
     void ensure_volume(pma_volume w);
     void sync();

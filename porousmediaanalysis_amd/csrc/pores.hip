@@ -394,10 +394,10 @@ __device__ __forceinline__ unsigned long long mix64(unsigned long long k) {
 
 // Distinct plateau edges (higher CR -> lower CR) in an open-addressing set.  A voxel skips a
 // source plateau it has already inserted for its previous neighbours (at most 26).  The set is
-// kept at most half full: the insertion that passes half the capacity (or a probe sequence longer
-// than kEdgeProbes) flags an overflow, every thread stops at its next edge, and the host retries
-// with a table four times larger -- so probe sequences stay short instead of sweeping a nearly
-// full table.
+// kept about half full: a wave whose insertions take the fill count past half the capacity (or a
+// probe sequence longer than kEdgeProbes) flags an overflow, threads that start later skip their
+// work, and the host retries with a table four times larger -- so probe sequences stay short
+// instead of sweeping a nearly full table.
 constexpr unsigned long long kEdgeProbes = 1024;
 __global__ void __launch_bounds__(256) plateau_edges(const uint32_t* __restrict__ cr, const uint32_t* __restrict__ plat,
                                                      int X, int Y, int Z, int d3q7, size_t P,
@@ -405,50 +405,60 @@ __global__ void __launch_bounds__(256) plateau_edges(const uint32_t* __restrict_
                                                      unsigned long long* __restrict__ fill, uint32_t* __restrict__ overflow,
                                                      uint32_t* __restrict__ err) {
     const size_t n = (size_t)X * Y * Z;
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint32_t c = cr[i];
-    if (!c) return;
-    const uint32_t dst = plat[i];
-    if (dst >= P) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // every lane reaches the count at the end: no early return
+    uint32_t inserted = 0;
+    const uint32_t c = i < n ? cr[i] : 0u;
+    const uint32_t dst = c ? plat[i] : 0u;
+    bool go = c != 0 && !ld_relaxed(overflow);  // after an overflow the host retries with a larger table
+    if (go && dst >= P) {
         fail_check(err, kPlateauEdges << 8 | 1);
-        return;
+        go = false;
     }
-    int x, y, z;
-    decode((uint32_t)i, X, Y, x, y, z);
-    uint32_t done[26];
-    int nd = 0;
-    for (int k = 0; k < 27; ++k) {
-        if (!in_stencil(k, d3q7)) continue;
-        uint32_t j;
-        if (!neighbour(k, x, y, z, X, Y, Z, j)) continue;
-        if (cr[j] <= c) continue;
-        const uint32_t src = plat[j];
-        if (src >= P) {
-            fail_check(err, kPlateauEdges << 8 | 2);
-            return;
-        }
-        bool seen = false;
-        for (int q = 0; q < nd; ++q) seen |= done[q] == src;
-        if (seen) continue;
-        if (nd < 26) done[nd++] = src;
-        if (ld_relaxed(overflow)) return;  // the host retries with a larger table
-        const unsigned long long key = ((unsigned long long)src << 32) | dst;
-        unsigned long long h = mix64(key) & mask;
-        for (unsigned long long probe = 0;; ++probe) {
-            if (probe > mask || probe >= kEdgeProbes) {
-                atomicOr(overflow, 1u);
+    if (go) {
+        int x, y, z;
+        decode((uint32_t)i, X, Y, x, y, z);
+        uint32_t done[26];
+        int nd = 0;
+        for (int k = 0; k < 27 && go; ++k) {
+            if (!in_stencil(k, d3q7)) continue;
+            uint32_t j;
+            if (!neighbour(k, x, y, z, X, Y, Z, j)) continue;
+            if (cr[j] <= c) continue;
+            const uint32_t src = plat[j];
+            if (src >= P) {
+                fail_check(err, kPlateauEdges << 8 | 2);
+                go = false;
                 break;
             }
-            unsigned long long cur = table[h];
-            if (cur == key) break;
-            if (cur == ~0ull) {
-                unsigned long long old = atomicCAS(&table[h], ~0ull, key);
-                if (old == ~0ull && atomicAdd(fill, 1ull) >= (mask + 1) / 2) atomicOr(overflow, 1u);
-                if (old == ~0ull || old == key) break;
+            bool seen = false;
+            for (int q = 0; q < nd; ++q) seen |= done[q] == src;
+            if (seen) continue;
+            if (nd < 26) done[nd++] = src;
+            const unsigned long long key = ((unsigned long long)src << 32) | dst;
+            unsigned long long h = mix64(key) & mask;
+            for (unsigned long long probe = 0;; ++probe) {
+                if (probe > mask || probe >= kEdgeProbes) {
+                    atomicOr(overflow, 1u);
+                    go = false;
+                    break;
+                }
+                unsigned long long cur = table[h];
+                if (cur == key) break;
+                if (cur == ~0ull) {
+                    unsigned long long old = atomicCAS(&table[h], ~0ull, key);
+                    if (old == ~0ull) ++inserted;
+                    if (old == ~0ull || old == key) break;
+                }
+                h = (h + 1) & mask;
             }
-            h = (h + 1) & mask;
         }
+    }
+    // one atomic per wave on the fill count; passing half the capacity is an overflow
+    uint32_t sum = inserted;
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+    if ((threadIdx.x & 63) == 0 && sum) {
+        if (atomicAdd(fill, (unsigned long long)sum) + sum > (mask + 1) / 2) atomicOr(overflow, 1u);
     }
 }
 
@@ -1818,8 +1828,7 @@ void run_pores(pma_ctx* c) {
                                                         cap * 16, trav_cap * 8};
                     unsigned long long fit = ~0ull;
                     for (int k = 0; k < 5; ++k) {
-                        auto it = c->scratch.find(kReplayBuffers[k]);
-                        fit = std::min(fit, it == c->scratch.end() ? 0ull : it->second.bytes / std::max(need[k], 1ull));
+                        fit = std::min(fit, (unsigned long long)c->buffer_bytes(kReplayBuffers[k]) / std::max(need[k], 1ull));
                     }
                     if (fit >= W - W / 4) W = std::max<unsigned long long>(1, std::min<unsigned long long>({fit, pending.size(), 1024}));
                 }

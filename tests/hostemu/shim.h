@@ -485,6 +485,10 @@ struct pma_ctx {
     }
     void* host_staging(size_t bytes) { return buf("host_staging", bytes); }
     void sync() {}
+    size_t buffer_bytes(const std::string& name) const {
+        auto it = bufs.find(name);
+        return it == bufs.end() ? 0 : it->second.second;
+    }
     float geo_delta = 16.0f;
     uint32_t flood_grid = 4096;
     unsigned long long replay_mem = 128ull << 30;

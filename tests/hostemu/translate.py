@@ -11,7 +11,8 @@ WAVE_KERNELS = {"count_centres", "write_centres", "dfs_replay", "cr_histogram", 
                 "prune_centres", "emit_centres", "cover_tiles", "count_equal_u8", "far_prefix_max",
                 "relax_blocks", "find_start", "cavity_totals", "max_run", "partition_active", "next_min_key",
                 "edt_pass_y", "edt_pass_z", "owned_counts",
-                "ws_resolve", "ws_count_parts", "ws_scan_parts", "ws_append", "set_union_bitmap", "below_min", "length_histogram"}
+                "ws_resolve", "ws_count_parts", "ws_scan_parts", "ws_append", "set_union_bitmap", "below_min", "length_histogram",
+                "plateau_edges"}
 
 
 def split_args(s):
