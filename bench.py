@@ -25,8 +25,14 @@ one context and once through pma_run_batch with the configuration pma_tune_batch
 sample of 24 of them.  value = voxels per second of the batch run (fresh contexts, uploads and
 allocations included in both timings).
 
+With --workload surrogate: the screening network trained on --train-crops crops of --crop^3
+(half sphere unions, half Gaussian fields) labelled by the exact pipeline, then run over
+--test-crops fresh crops; the exact stages rerun on every 20th crop give the radius and ratio MAE
+and max error, and a sampled crop whose radius MAE exceeds --bound falls back to its exact fields.
+value = voxels per second of the screening (EDT, network, the sampled exact reruns).
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 1024] [--no-cpu]
-                       [--workload plain|batched]
+                       [--workload plain|batched|surrogate]
 """
 import argparse
 import json
@@ -85,8 +91,13 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stencil", choices=["d3q27", "d3q7"], default="d3q27")
-    ap.add_argument("--workload", choices=["plain", "batched"], default="plain")
+    ap.add_argument("--workload", choices=["plain", "batched", "surrogate"], default="plain")
     ap.add_argument("--batch-count", type=int, default=256)
+    ap.add_argument("--crop", type=int, default=64, help="surrogate crop side")
+    ap.add_argument("--train-crops", type=int, default=192)
+    ap.add_argument("--test-crops", type=int, default=256)
+    ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--bound", type=float, default=3.0, help="radius MAE bound (voxels) of a sampled crop")
     return ap.parse_args()
 
 
@@ -285,6 +296,141 @@ def batched(args, pma):
     }
 
 
+def surrogate(args, pma):
+    """Screening with the learned network: train on crops labelled by the exact pipeline, then
+    predict the containing radius and distance-ratio fields of fresh crops, rerun the exact stages on
+    a 5 % sample, and fall back to them for every sampled crop whose radius MAE exceeds the bound."""
+    import numpy as np
+
+    rng = np.random.default_rng(args.seed + 1)
+    S = args.crop
+    shape = (S, S, S)
+
+    def gen(ctx, i):
+        if i % 2 == 0:
+            ctx.generate_spheres(shape, max(1, round(300 * S ** 3 / 128 ** 3)), 6.0, 12.0, args.seed + 7919 * i)
+        else:
+            ctx.generate_gaussian(shape, float(rng.uniform(2, 6)), float(rng.uniform(0.5, 0.8)), args.seed + 7919 * i)
+
+    def exact(ctx):
+        ctx.edt()
+        ctx.containing_balls()
+        ctx.geodesic()
+
+    def fields(ctx):
+        solid = ctx.read(pma.VOL_SOLID)
+        void = solid == 0
+        radius = np.sqrt(ctx.read(pma.VOL_CR2).astype(np.float32))
+        dist = ctx.read(pma.VOL_SPD)
+        return void, radius, dist
+
+    progress(f"labelling {args.train_crops} training crops of {S}^3")
+    train = []
+    for i in range(args.train_crops):
+        c = pma.Context(0)
+        gen(c, i)
+        exact(c)
+        train.append(c)
+    net = pma.Surrogate(0, args.seed)
+    t0 = time.perf_counter()
+    losses = []
+    steps = args.epochs * len(train)
+    for it in range(steps):
+        lr = 2e-3 * (0.1 ** (it / max(steps - 1, 1)))
+        losses.append(net.train_step(train[int(rng.integers(len(train)))], lr=lr))
+        if (it + 1) % 128 == 0:
+            progress(f"training step {it + 1}/{steps}: loss {np.mean(losses[-128:]):.4f}")
+    train_s = time.perf_counter() - t0
+    for c in train:
+        c.close()
+
+    progress(f"screening {args.test_crops} fresh crops")
+    ctx = pma.Context(0)
+    sample = set(range(0, args.test_crops, 20))  # 5 %
+    t_screen = t_exact = 0.0
+    rad_err, rad_max, ratio_err, ratio_max, base_err, fallbacks = [], 0.0, [], 0.0, [], 0
+    for i in range(args.test_crops):
+        gen(ctx, args.train_crops + i)
+        t0 = time.perf_counter()
+        ctx.edt()
+        radius, ratio = net.predict(ctx)
+        t_screen += time.perf_counter() - t0
+        if i in sample:
+            t0 = time.perf_counter()
+            ctx.containing_balls()
+            ctx.geodesic()
+            t_exact += time.perf_counter() - t0
+            void, r_ex, dist = fields(ctx)
+            e = np.abs(radius[void] - r_ex[void])
+            rad_err.append(float(e.mean()))
+            rad_max = max(rad_max, float(e.max()))
+            base_err.append(float(np.abs(np.sqrt(ctx.read(pma.VOL_EDT2)[void].astype(np.float32)) - r_ex[void]).mean()))
+            cav = ctx.read(pma.VOL_CAVITY)
+            rows = [r.split() for r in ctx.text(pma.TEXT_CAVITIES).strip().splitlines()[1:]]
+            Z, X, Y = np.meshgrid(np.arange(S), np.arange(S), np.arange(S), indexing="ij")
+            for row in rows:
+                cid, sx, sy, sz = int(row[0]), int(row[1]), int(row[2]), int(row[3])
+                sel = void & (cav == cid) & (dist < 3e38)
+                eu = np.sqrt(((X - sx) ** 2 + (Y - sy) ** 2 + (Z - sz) ** 2).astype(np.float32))
+                sel &= eu > 0
+                if sel.any():
+                    q = np.abs(ratio[sel] - dist[sel] / eu[sel])
+                    ratio_err.append(float(q.mean()))
+                    ratio_max = max(ratio_max, float(q.max()))
+            if rad_err[-1] > args.bound:
+                fallbacks += 1  # the exact fields just computed replace the estimate
+        if (i + 1) % 64 == 0:
+            progress(f"screened {i + 1}/{args.test_crops}")
+    # the exact stages that produce the same fields, on a share of the crops, for the comparison
+    t0 = time.perf_counter()
+    n_ref = max(1, args.test_crops // 8)
+    for i in range(n_ref):
+        gen(ctx, args.train_crops + i)
+        exact(ctx)
+    exact_per_crop = (time.perf_counter() - t0) / n_ref
+    t0 = time.perf_counter()
+    for i in range(n_ref):
+        gen(ctx, args.train_crops + i)
+        ctx.run_all()
+    plain_per_crop = (time.perf_counter() - t0) / n_ref
+    ctx.close()
+    net.close()
+    vox = args.test_crops * S ** 3
+    screened_s = t_screen + t_exact
+    return {
+        "metric": "voxels/s, surrogate-screened containing radius and geodesic/Euclidean ratio of fresh crops "
+                  "(5 % exact reruns, fallback above the bound)",
+        "value": vox / screened_s,
+        "unit": "voxels/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": round(1000 * screened_s, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16 (fp32 accumulate)",
+        "data": "synthetic",
+        "config": {"workload": f"U-Net 16/32/64, {args.train_crops} training crops x {args.epochs} epochs, "
+                               f"{args.test_crops} fresh {S}^3 crops (half sphere unions, half Gaussian fields)",
+                   "crop": S, "train_crops": args.train_crops, "epochs": args.epochs, "test_crops": args.test_crops,
+                   "bound_radius_mae_voxels": args.bound, "seed": args.seed},
+        "train_s": round(train_s, 3),
+        "final_train_loss": float(np.mean(losses[-64:])),
+        "sampled_crops": len(sample),
+        "radius_mae": float(np.mean(rad_err)),
+        "radius_max_error": rad_max,
+        "radius_mae_without_network": float(np.mean(base_err)),
+        "ratio_mae": float(np.mean(ratio_err)) if ratio_err else None,
+        "ratio_max_error": ratio_max,
+        "fallbacks": fallbacks,
+        "exact_fields_voxels_per_s": S ** 3 / exact_per_crop,
+        "plain_pipeline_voxels_per_s": S ** 3 / plain_per_crop,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -295,6 +441,9 @@ def main():
 
     if args.workload == "batched":
         print(json.dumps(batched(args, pma)))
+        return
+    if args.workload == "surrogate":
+        print(json.dumps(surrogate(args, pma)))
         return
 
     S = args.size
