@@ -951,8 +951,20 @@ __device__ __forceinline__ uint32_t bcast(uint32_t v, int src) {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, src);
 }
 
+// The replaying wave's LDS copy of recently met plateaus' state: entries (plateau << 32) | class << 16
+// | closed << 1 | entered, one per slot, ~0 when empty.  The wave is the only writer of its
+// entered and closed bitmaps and of its class table, so the copy is updated wherever they are.
+#ifndef PMA_PORES_PCACHE
+#define PMA_PORES_PCACHE 256u
+#endif
+constexpr uint32_t kPCache = PMA_PORES_PCACHE;  // a power of two
+__device__ __forceinline__ uint32_t pcache_slot(uint32_t q) { return (q * 0x9E3779B1u) >> 24 & (kPCache - 1u); }
+__device__ __forceinline__ unsigned long long pcache_entry(uint32_t q, uint32_t cls, bool closed, bool entered) {
+    return ((unsigned long long)q << 32) | ((cls & 0xFFFFu) << 16) | (closed ? 2u : 0u) | (entered ? 1u : 0u);
+}
+
 __device__ __forceinline__ bool bit_of(const unsigned long long* words, uint32_t i) {
-    return (__hip_atomic_load(&words[i >> 6], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> (i & 63)) & 1ull;
+    return (__hip_atomic_load(&words[i >> 6], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >> (i & 63)) & 1ull;
 }
 
 // A pool of waves (workgroup b uses slot b of the per-slot buffers: a visited bitmap over all
@@ -966,7 +978,9 @@ __device__ __forceinline__ bool bit_of(const unsigned long long* words, uint32_t
 // that tries neighbours in reverse loop order visits voxels in the reference's stack-DFS order.
 // The replay of a pore stops once all but one of its classes are ranked: the last one is
 // necessarily found after them.  Words written by one lane and read by another go through
-// agent-scope atomics.  Dynamic LDS: the ranked-class bitmap (seen_words words), then the
+// workgroup-scope atomics: every per-slot buffer belongs to this one wave, so they need not be
+// coherent beyond its CU (an agent-scope fence writes back and invalidates the XCD's L2, and an
+// agent-scope load skips it -- both slow every wave on the XCD).  Dynamic LDS: the ranked-class bitmap (seen_words words), then the
 // block-to-page table of the visited voxels.
 __global__ void __launch_bounds__(64) dfs_replay(
     const ReplayTask* __restrict__ tasks, uint32_t T, uint32_t* __restrict__ queue, const uint32_t* __restrict__ cr,
@@ -981,6 +995,7 @@ __global__ void __launch_bounds__(64) dfs_replay(
     extern __shared__ uint32_t seen[];  // ranked classes of the current pore
     __shared__ uint32_t tcount[2];      // traversal: stack length, list length
     __shared__ uint4 ring[kRing];       // the newest DFS frames (voxel, plateau, radius, next index)
+    __shared__ unsigned long long pcache[kPCache];  // recently met plateaus' state
     const int lane = threadIdx.x;
     // visited voxels: blocks of 2^vis_bits voxels, each on an 8 KB-per-65536-voxel page of this
     // slot's pool taken on the block's first visit (vtab in LDS: block -> page, kNoPage if none)
@@ -1009,6 +1024,7 @@ __global__ void __launch_bounds__(64) dfs_replay(
         for (uint32_t i = lane; i < nblocks; i += 64) vtab[i] = kNoPage;
         uint32_t npages = 0;
         for (unsigned long long i = lane; i < 2 * pent_words; i += 64) pent[i] = 0;  // entered, closed
+        for (uint32_t i = lane; i < kPCache; i += 64) pcache[i] = ~0ull;
         for (uint32_t i = lane; i < seen_words; i += 64) seen[i] = 0;
         bool bad_list = false;
         for (uint32_t i = lane; i < task.list_len; i += 64) {
@@ -1021,7 +1037,7 @@ __global__ void __launch_bounds__(64) dfs_replay(
             if (lane == 0) fail_check(err, kDfsReplay << 8 | 2);
             return;
         }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");  // the cleared words are in L2 before the atomics
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");  // the cleared words precede the atomics
         __builtin_amdgcn_wave_barrier();  // every lane has cleared its share before lane 0 marks the seed
 
         // Lane 0: ranks class l if it is new; returns the new count of ranked classes.
@@ -1040,7 +1056,7 @@ __global__ void __launch_bounds__(64) dfs_replay(
             if (npages >= vis_pages) return false;
             unsigned long long* pg = vpool + (size_t)npages * page_words;
             for (uint32_t i = lane; i < page_words; i += 64) pg[i] = 0;
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");  // zeros land before the marks
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");  // zeros land before the marks
             __builtin_amdgcn_wave_barrier();
             if (lane == 0) vtab[blk] = (uint16_t)npages;
             ++npages;
@@ -1083,14 +1099,25 @@ __global__ void __launch_bounds__(64) dfs_replay(
                 const unsigned long long w =
                     pg == kNoPage ? 0ull
                                   : __hip_atomic_load(&vpool[(size_t)pg * page_words + ((j & ((1u << vis_bits) - 1)) >> 6)],
-                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 cand = cn != 0 && cn <= cu && !(w & (1ull << (j & 63)));
                 // neighbours in other plateaus: closed plateaus are visited; the entered test and the
                 // class are for the lane whose neighbour is taken
                 if (cand && qn != uq && qn < b.P) {
-                    entered = bit_of(pent, qn);
-                    ql = cls[qn];
-                    if (bit_of(pclosed, qn)) cand = false;
+                    const uint32_t slot = pcache_slot(qn);
+                    const unsigned long long e = pcache[slot];
+                    bool closed;
+                    if ((uint32_t)(e >> 32) == qn) {
+                        entered = e & 1ull;
+                        closed = (e >> 1) & 1ull;
+                        ql = (uint32_t)(e >> 16) & 0xFFFFu;
+                    } else {
+                        entered = bit_of(pent, qn);
+                        closed = bit_of(pclosed, qn);
+                        ql = cls[qn];
+                        pcache[slot] = pcache_entry(qn, ql, closed, entered);
+                    }
+                    if (closed) cand = false;
                 }
             }
             const unsigned long long bal = __ballot(cand);
@@ -1137,8 +1164,9 @@ __global__ void __launch_bounds__(64) dfs_replay(
                     if (lane == 0) {
                         atomicOr(&pent[nq >> 6], 1ull << (nq & 63));
                         atomicOr(&pclosed[nq >> 6], 1ull << (nq & 63));
+                        pcache[pcache_slot(nq)] = pcache_entry(nq, skip_cls, true, true);
                     }
-                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");  // marks land before the next loads
+                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");  // marks land before the next loads
                 } else {
                     // traversal: stack tstack[0 .. tcount[0]), every plateau it marks listed in tlist.
                     // It marks plateaus in the entered bitmap itself (the walk waits meanwhile), so one
@@ -1146,8 +1174,8 @@ __global__ void __launch_bounds__(64) dfs_replay(
                     // step takes them back.
                     if (lane == 0) {
                         atomicOr(&pent[nq >> 6], 1ull << (nq & 63));
-                        __hip_atomic_store(&tstack[0], nq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        __hip_atomic_store(&tlist[0], nq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        __hip_atomic_store(&tstack[0], nq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        __hip_atomic_store(&tlist[0], nq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                         tcount[0] = 1;
                         tcount[1] = 1;
                     }
@@ -1161,7 +1189,7 @@ __global__ void __launch_bounds__(64) dfs_replay(
                         uint32_t bq = kInv, l = kNoClass, cnt = 0, st = 0;
                         bool bad = false, expand = false;
                         if ((uint32_t)lane < nb) {
-                            bq = __hip_atomic_load(&tstack[ts - 1 - lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            bq = __hip_atomic_load(&tstack[ts - 1 - lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                             if (bq >= b.P) {
                                 bad = true;
                             } else {
@@ -1227,9 +1255,9 @@ __global__ void __launch_bounds__(64) dfs_replay(
                                     lane_room = false;
                                     continue;
                                 }
-                                __hip_atomic_store(&tlist[w], d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                                __hip_atomic_store(&tlist[w], d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                                 const uint32_t at = atomicAdd(&tcount[0], 1u);  // at <= w: every entry is listed
-                                __hip_atomic_store(&tstack[at], d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                                __hip_atomic_store(&tstack[at], d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                             }
                         }
                         if (__ballot(bad)) {
@@ -1248,13 +1276,21 @@ __global__ void __launch_bounds__(64) dfs_replay(
                     const uint32_t wn = min(bcast(tcount[1], 0), (uint32_t)b.trav_cap);
                     skip = room && nun <= 1;
                     for (uint32_t i = lane; i < wn; i += 64) {
-                        const uint32_t d = __hip_atomic_load(&tlist[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        const uint32_t d = __hip_atomic_load(&tlist[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                         if (d >= b.P) continue;
                         const unsigned long long m1 = 1ull << (d & 63);
-                        if (skip) atomicOr(&pclosed[d >> 6], m1);
-                        else atomicAnd(&pent[d >> 6], ~m1);
+                        if (skip) {
+                            atomicOr(&pclosed[d >> 6], m1);
+                            // a listed plateau was not entered before the walk: a copy of its state
+                            // says so and now becomes entered and closed (one lane per plateau)
+                            const uint32_t slot = pcache_slot(d);
+                            const unsigned long long e = pcache[slot];
+                            if ((uint32_t)(e >> 32) == d) pcache[slot] = e | 3ull;
+                        } else {
+                            atomicAnd(&pent[d >> 6], ~m1);
+                        }
                     }
-                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
                     __builtin_amdgcn_wave_barrier();
                     skip_cls = nun == 1 ? k1 : kNoClass;
                 }
@@ -1286,6 +1322,7 @@ __global__ void __launch_bounds__(64) dfs_replay(
                          1ull << (nvox & 63));
                 if (is_new) {
                     atomicOr(&pent[nq >> 6], 1ull << (nq & 63));
+                    pcache[pcache_slot(nq)] = pcache_entry(nq, ncls_new, false, true);
                     found = rank_class(ncls_new, found);
                 }
             }
@@ -1830,7 +1867,7 @@ void run_pores(pma_ctx* c) {
                     for (int k = 0; k < 5; ++k) {
                         fit = std::min(fit, (unsigned long long)c->buffer_bytes(kReplayBuffers[k]) / std::max(need[k], 1ull));
                     }
-                    if (fit >= W - W / 4) W = std::max<unsigned long long>(1, std::min<unsigned long long>({fit, pending.size(), 1024}));
+                    if (fit >= W - W / 4) W = std::max<unsigned long long>(1, std::min(W, fit));
                 }
                 unsigned long long* vis = c->buf<unsigned long long>("pt_replay_vis", W * vis_pages * page_words);
                 unsigned long long* pbits = c->buf<unsigned long long>("pt_replay_plateaus", W * 2 * pent_words);

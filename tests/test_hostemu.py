@@ -104,7 +104,8 @@ def hostemu_slabs(tmp_path_factory):
                                                  "-DPMA_PORES_STACK_CAP=64ull", "-DPMA_PORES_RING=8u", "-DPMA_UNION_LDS_MAX=0u",
                                                  "-DPMA_PORES_VIS_MIN_BITS=6u", "-DPMA_PORES_VIS_PAGES=4u",
                                                  "-DPMA_PORES_TRAV_CAP=3ull", "-DPMA_PORES_POOL_MIN=1ull",
-                                                 "-DPMA_PORES_POOL_MAX=1ull", "-DPMA_PORES_POOL_FLOOR=16ull"], "slabs")
+                                                 "-DPMA_PORES_POOL_MAX=1ull", "-DPMA_PORES_POOL_FLOOR=16ull",
+                                                 "-DPMA_PORES_PCACHE=4u"], "slabs")
 
 
 @pytest.fixture(scope="session")
