@@ -307,10 +307,12 @@ def surrogate(args, pma):
     shape = (S, S, S)
 
     def gen(ctx, i):
+        """Crop i: a sphere union (even i) or a Gaussian field with its own sigma and porosity."""
         if i % 2 == 0:
             ctx.generate_spheres(shape, max(1, round(300 * S ** 3 / 128 ** 3)), 6.0, 12.0, args.seed + 7919 * i)
         else:
-            ctx.generate_gaussian(shape, float(rng.uniform(2, 6)), float(rng.uniform(0.5, 0.8)), args.seed + 7919 * i)
+            r = np.random.default_rng(args.seed + 7919 * i)
+            ctx.generate_gaussian(shape, float(r.uniform(2, 6)), float(r.uniform(0.5, 0.8)), args.seed + 7919 * i)
 
     def exact(ctx):
         ctx.edt()
@@ -324,25 +326,23 @@ def surrogate(args, pma):
         dist = ctx.read(pma.VOL_SPD)
         return void, radius, dist
 
-    progress(f"labelling {args.train_crops} training crops of {S}^3")
-    train = []
-    for i in range(args.train_crops):
-        c = pma.Context(0)
-        gen(c, i)
-        exact(c)
-        train.append(c)
+    # training crops are regenerated and relabelled by the exact pipeline at every step (one
+    # context; the same seed gives the same crop), so memory does not grow with their number
     net = pma.Surrogate(0, args.seed)
+    lab = pma.Context(0)
     t0 = time.perf_counter()
     losses = []
-    steps = args.epochs * len(train)
-    for it in range(steps):
+    steps = args.epochs * args.train_crops
+    order = np.concatenate([rng.permutation(args.train_crops) for _ in range(args.epochs)])
+    for it, i in enumerate(order):
+        gen(lab, int(i))
+        exact(lab)
         lr = 2e-3 * (0.1 ** (it / max(steps - 1, 1)))
-        losses.append(net.train_step(train[int(rng.integers(len(train)))], lr=lr))
+        losses.append(net.train_step(lab, lr=lr))
         if (it + 1) % 128 == 0:
             progress(f"training step {it + 1}/{steps}: loss {np.mean(losses[-128:]):.4f}")
     train_s = time.perf_counter() - t0
-    for c in train:
-        c.close()
+    lab.close()
 
     progress(f"screening {args.test_crops} fresh crops")
     ctx = pma.Context(0)

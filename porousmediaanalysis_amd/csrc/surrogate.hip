@@ -99,6 +99,76 @@ __global__ void __launch_bounds__(256) conv3d(const uint16_t* __restrict__ X, co
     }
 }
 
+// The 3x3x3 convolution with the input staged in LDS: a workgroup computes 4 x-rows of 16 voxels
+// along y at one z (one row per wave, every output channel), and first copies the 3 x 6 x 18
+// voxels those outputs read (zero outside the volume) into LDS, each voxel's channels padded by 4
+// to spread the banks.  The 27 taps then read LDS instead of global memory.
+constexpr int kTileX = 4, kTileY = 16;
+constexpr int kHaloX = kTileX + 2, kHaloY = kTileY + 2;
+__global__ void __launch_bounds__(256) conv3d_lds(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Wt,
+                                                  const float* __restrict__ bias, uint16_t* __restrict__ ybf,
+                                                  float* __restrict__ yf, Grid3 g, int cin, int cout, int relu) {
+    extern __shared__ uint16_t halo[];  // [3][kHaloX][kHaloY][cin + 4]
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int segs = (g.W + kTileY - 1) / kTileY, xrows = (g.H + kTileX - 1) / kTileX;
+    const long tile = blockIdx.x;
+    const int ys = (int)(tile % segs) * kTileY;
+    const long r = tile / segs;
+    const int x0 = (int)(r % xrows) * kTileX, z = (int)(r / xrows);
+    const int cs = cin + 4;  // LDS stride of a voxel
+    // stage: chunks of 4 channels (8 bytes), zero outside the volume
+    const int cq = cin / 4, nchunks = 3 * kHaloX * kHaloY * cq;
+    for (int q = threadIdx.x; q < nchunks; q += 256) {
+        const int c4 = q % cq;
+        const int vox = q / cq;
+        const int hy = vox % kHaloY, hx = (vox / kHaloY) % kHaloX, hz = vox / (kHaloY * kHaloX);
+        const int zz = z + hz - 1, xx = x0 + hx - 1, yy = ys + hy - 1;
+        bf16x4 v = bf16x4{0, 0, 0, 0};
+        if (zz >= 0 && zz < g.D && xx >= 0 && xx < g.H && yy >= 0 && yy < g.W)
+            v = *reinterpret_cast<const bf16x4*>(X + (((long)zz * g.H + xx) * g.W + yy) * cin + 4 * c4);
+        *reinterpret_cast<bf16x4*>(halo + vox * cs + 4 * c4) = v;
+    }
+    __syncthreads();
+    const int x = x0 + wave;
+    const int m = lane & 15, kq = lane >> 4;
+    const int nsub = cout / 16;
+    f32x4 acc[6];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int t = 0; t < 27; ++t) {
+        const int dz = t / 9, dx = (t / 3) % 3, dy = t % 3;  // halo coordinates of the tap (offset + 1)
+        const uint16_t* arow = halo + ((dz * kHaloX + wave + dx) * kHaloY + m + dy) * cs + 4 * kq;
+        const uint16_t* wt = Wt + (long)t * cout * cin;
+        for (int c0 = 0; c0 < cin; c0 += 16) {
+            const bf16x4 a = *reinterpret_cast<const bf16x4*>(arow + c0);
+#pragma unroll
+            for (int j = 0; j < 6; ++j) {
+                if (j < nsub) {
+                    const bf16x4 b = *reinterpret_cast<const bf16x4*>(wt + (long)(16 * j + m) * cin + c0 + 4 * kq);
+                    acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[j], 0, 0, 0);
+                }
+            }
+        }
+    }
+    if (x >= g.H) return;  // after the barrier: a row past the volume computed zeros
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        if (j >= nsub) break;
+        const int co = 16 * j + m;
+        const float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int yv = ys + 4 * kq + i;
+            if (yv >= g.W) continue;
+            float v = acc[j][i] + bv;
+            if (relu) v = fmaxf(v, 0.f);
+            const long o = (((long)z * g.H + x) * g.W + yv) * cout + co;
+            if (ybf) ybf[o] = to_bf16(v);
+            else yf[o] = v;
+        }
+    }
+}
+
 // dW[t][co][ci] += sum over voxels v of dP[v][co] * X[v + offset(t)][ci]: an MFMA GEMM with
 // M = 16 output channels, N = 16 input channels, K = 16 voxels, over a chunk of voxels per
 // workgroup (one wave per 16 output channels), added to dW with atomics.
@@ -155,14 +225,32 @@ __global__ void __launch_bounds__(256) conv3d_wgrad(const uint16_t* __restrict__
     }
 }
 
-// db[c] += sum over voxels of dP[v][c].
-__global__ void bias_grad(const uint16_t* __restrict__ dP, long V, int C, float* __restrict__ db) {
+// db[c] += sum over voxels of dP[v][c]: per-thread sums over a grid stride, summed over the
+// workgroup's rows in LDS, one atomic per channel per workgroup.  blockDim.x = 256, a multiple of C.
+__global__ void __launch_bounds__(256) bias_grad(const uint16_t* __restrict__ dP, long V, int C, float* __restrict__ db) {
+    __shared__ float part[256];
     const int c = threadIdx.x % C;
     const int rows = blockDim.x / C;
     float s = 0.f;
     for (long v = (long)blockIdx.x * rows + threadIdx.x / C; v < V; v += (long)gridDim.x * rows)
         s += from_bf16(dP[v * C + c]);
-    atomicAdd(&db[c], s);
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x < C) {
+        float t = 0.f;
+        for (int r = 0; r < rows; ++r) t += part[r * C + threadIdx.x];
+        atomicAdd(&db[threadIdx.x], t);
+    }
+}
+
+// Sum of a value over a 256-thread workgroup (every thread calls it); valid in thread 0.
+__device__ __forceinline__ float block_sum(float v, float* lds4) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    if ((threadIdx.x & 63) == 0) lds4[threadIdx.x >> 6] = v;
+    __syncthreads();
+    const float t = lds4[0] + lds4[1] + lds4[2] + lds4[3];
+    __syncthreads();
+    return t;
 }
 
 // Backward-data weights: Wf[26 - t][ci][co] = W[t][co][ci] (taps mirrored, in and out swapped).
@@ -305,7 +393,7 @@ __global__ void make_targets(const uint8_t* __restrict__ solid, const uint32_t* 
 // Mean squared error over the weighted entries of the network's two output channels (out has 16
 // channels, 2 used); writes the gradient dOut (16 channels, 2 used) and adds to sums[0] (squared
 // error) and sums[1] (weight).
-__global__ void mse_loss(const float* __restrict__ out, const float* __restrict__ tgt, const float* __restrict__ wgt,
+__global__ void __launch_bounds__(256) mse_loss(const float* __restrict__ out, const float* __restrict__ tgt, const float* __restrict__ wgt,
                          long n, float inv_count, float* __restrict__ dout, float* __restrict__ sums) {
     float se = 0.f, sw = 0.f;
     for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (long)gridDim.x * blockDim.x) {
@@ -321,22 +409,22 @@ __global__ void mse_loss(const float* __restrict__ out, const float* __restrict_
             dout[v * 16 + c] = gval;
         }
     }
-    for (int off = 32; off > 0; off >>= 1) {
-        se += __shfl_down(se, off);
-        sw += __shfl_down(sw, off);
-    }
-    if ((threadIdx.x & 63) == 0) {
+    __shared__ float lds4[4];
+    se = block_sum(se, lds4);
+    sw = block_sum(sw, lds4);
+    if (threadIdx.x == 0) {
         atomicAdd(&sums[0], se);
         atomicAdd(&sums[1], sw);
     }
 }
 
 // Weighted count of the targets (for the mean).
-__global__ void weight_sum(const float* __restrict__ wgt, long n, float* __restrict__ s) {
+__global__ void __launch_bounds__(256) weight_sum(const float* __restrict__ wgt, long n, float* __restrict__ s) {
+    __shared__ float lds4[4];
     float a = 0.f;
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) a += wgt[i];
-    for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off);
-    if ((threadIdx.x & 63) == 0) atomicAdd(s, a);
+    a = block_sum(a, lds4);
+    if (threadIdx.x == 0) atomicAdd(s, a);
 }
 
 // Adam on the fp32 master weights, then their bf16 copies.
@@ -401,14 +489,26 @@ const int kLayerSpec[][4] = {  // cin, cout, taps, cin_real
     {96, 32, 27, 96}, {32, 32, 27, 32}, {48, 16, 27, 48}, {16, 16, 27, 16}, {16, 16, 1, 16}};
 constexpr int kLayers = 11;
 
+// One convolution: 3x3x3 ones through the LDS-staged kernel, the 1x1x1 one directly.
+void launch_conv(pma_surrogate* s, const uint16_t* X, const uint16_t* W, const float* b, uint16_t* ybf, float* yf,
+                 Grid3 g, int cin, int cout, int taps, bool relu) {
+    if (taps == 27) {
+        const long tiles = (long)g.D * ((g.H + pma::kTileX - 1) / pma::kTileX) * ((g.W + pma::kTileY - 1) / pma::kTileY);
+        const size_t lds = (size_t)3 * pma::kHaloX * pma::kHaloY * (cin + 4) * sizeof(uint16_t);
+        hipLaunchKernelGGL(pma::conv3d_lds, dim3((unsigned)tiles), dim3(256), lds, s->stream, X, W, b, ybf, yf, g, cin,
+                           cout, relu ? 1 : 0);
+    } else {
+        const long tiles = (long)g.D * g.H * ((g.W + 15) / 16);
+        hipLaunchKernelGGL(pma::conv3d, dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0, s->stream, X, W, b, ybf, yf,
+                           g, cin, cout, taps, relu ? 1 : 0);
+    }
+    PMA_HIP(hipGetLastError());
+}
+
 void conv(pma_surrogate* s, int li, const uint16_t* X, Grid3 g, uint16_t* ybf, float* yf, bool relu) {
     const pma_surrogate::Layer& L = s->layers[li];
-    const long tiles = (long)g.D * g.H * ((g.W + 15) / 16);
-    const uint16_t* W = s->wbf.as<uint16_t>() + L.w_off;
-    const float* b = s->w32.as<float>() + L.b_off;
-    hipLaunchKernelGGL(pma::conv3d, dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0, s->stream, X, W, b, ybf, yf, g,
-                       L.cin, L.cout, L.taps, relu ? 1 : 0);
-    PMA_HIP(hipGetLastError());
+    launch_conv(s, X, s->wbf.as<uint16_t>() + L.w_off, s->w32.as<float>() + L.b_off, ybf, yf, g, L.cin, L.cout, L.taps,
+                relu);
 }
 
 // Gradients of layer li: dW, db from (X, dP); dX = conv(dP, flipped W) when wanted (fp32).
@@ -420,9 +520,9 @@ void conv_back(pma_surrogate* s, int li, const uint16_t* X, const uint16_t* dP, 
     hipLaunchKernelGGL(pma::conv3d_wgrad, dim3((unsigned)((V + pma::kWgradChunk - 1) / pma::kWgradChunk), L.taps),
                        dim3(256), 0, s->stream, X, dP, dW, g, L.cin, L.cout, L.taps);
     PMA_HIP(hipGetLastError());
-    const int rows = 256 / L.cout;
-    hipLaunchKernelGGL(pma::bias_grad, dim3((unsigned)std::min<long>((V + rows - 1) / rows, 4096)), dim3(rows * L.cout),
-                       0, s->stream, dP, V, L.cout, db);
+    const int rows = 256 / L.cout;  // cout is 16, 32 or 64
+    hipLaunchKernelGGL(pma::bias_grad, dim3((unsigned)std::min<long>((V + rows - 1) / rows, 1024)), dim3(256), 0,
+                       s->stream, dP, V, L.cout, db);
     PMA_HIP(hipGetLastError());
     if (!dX) return;
     uint16_t* Wf = s->wflip.as<uint16_t>();
@@ -430,10 +530,7 @@ void conv_back(pma_surrogate* s, int li, const uint16_t* X, const uint16_t* dP, 
     hipLaunchKernelGGL(pma::flip_weights, dim3(pma::blocks_for(nw)), dim3(256), 0, s->stream,
                        s->wbf.as<uint16_t>() + L.w_off, Wf, L.cin, L.cout, L.taps);
     PMA_HIP(hipGetLastError());
-    const long tiles = (long)g.D * g.H * ((g.W + 15) / 16);
-    hipLaunchKernelGGL(pma::conv3d, dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0, s->stream, dP, Wf,
-                       (const float*)nullptr, (uint16_t*)nullptr, dX, g, L.cout, L.cin, L.taps, 0);
-    PMA_HIP(hipGetLastError());
+    launch_conv(s, dP, Wf, nullptr, nullptr, dX, g, L.cout, L.cin, L.taps, false);
 }
 
 template <class K, class... A> void launch1(pma_surrogate* s, long n, K kernel, A... args) {
