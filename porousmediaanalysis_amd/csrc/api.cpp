@@ -82,6 +82,7 @@ void timed(pma_ctx* c, pma_stage s, void (*fn)(pma_ctx*)) {
     PMA_REQUIRE(c->n > 0, PMA_ERR_STATE, "set the dimensions first");
     auto t0 = std::chrono::steady_clock::now();
     uint64_t l0 = c->launches;
+    c->arena_begin_stage();
     PMA_HIP(hipEventRecord(c->ev0, c->stream));
     fn(c);
     PMA_HIP(hipEventRecord(c->ev1, c->stream));
@@ -92,6 +93,16 @@ void timed(pma_ctx* c, pma_stage s, void (*fn)(pma_ctx*)) {
     c->stage[s].host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     c->stage[s].launches = c->launches - l0;
     c->resolve_kernel_times();
+    if (c->debug_sync) {  // the scratch held after the stage
+        size_t arena = 0;
+        for (const auto& b : c->arena_chunks) arena += b.bytes;
+        std::fprintf(stderr, "[pma] after stage %d: arena %.2f GiB in %zu chunk(s), stage took %.2f GiB\n", (int)s,
+                     arena / double(1ull << 30), c->arena_chunks.size(), c->arena_stage / double(1ull << 30));
+        for (const auto& kv : c->scratch)
+            if (kv.second.bytes >= (256ull << 20))
+                std::fprintf(stderr, "[pma] after stage %d: kept %-24s %8.2f GiB\n", (int)s, kv.first.c_str(),
+                             kv.second.bytes / double(1ull << 30));
+    }
 }
 
 std::string format_pores(const std::vector<PoreRow>& rows) {
@@ -258,9 +269,54 @@ void pma_ctx::ensure_volume(pma_volume w) {
 }
 
 void* pma_ctx::buf(const std::string& name, size_t bytes) {
-    pma::DeviceBuffer& b = scratch[name];
-    with_eviction(this, name, [&] { b.reserve(bytes, stream); });
-    return b.ptr;
+    if (persistent.count(name)) {
+        pma::DeviceBuffer& b = scratch[name];
+        with_eviction(this, name, [&] { b.reserve(bytes, stream); });
+        return b.ptr;
+    }
+    bytes = std::max<size_t>(bytes, 256);
+    auto it = arena_names.find(name);
+    if (it != arena_names.end() && bytes <= it->second.bytes) return it->second.ptr;
+    const size_t want = (bytes + 255) & ~size_t(255);
+    if (arena_chunks.empty() || arena_used + want > arena_chunks.back().bytes) {
+        // a new chunk: earlier regions of this stage stay where they are
+        const size_t chunk = std::max<size_t>(want, arena_chunks.empty() ? (size_t(1) << 30) : arena_chunks.back().bytes);
+        arena_chunks.emplace_back();
+        pma::DeviceBuffer& b = arena_chunks.back();
+        try {
+            with_eviction(this, std::string(), [&] { b.reserve(chunk, stream); });
+        } catch (...) {
+            arena_chunks.pop_back();
+            throw;
+        }
+        arena_used = 0;
+    }
+    void* p = static_cast<unsigned char*>(arena_chunks.back().ptr) + arena_used;
+    arena_used += want;
+    arena_stage += want;
+    arena_peak = std::max(arena_peak, arena_stage);
+    arena_names[name] = ArenaRegion{p, want};
+    return p;
+}
+
+void pma_ctx::arena_begin_stage() {
+    arena_names.clear();
+    arena_stage = 0;
+    arena_used = 0;
+    if (arena_chunks.size() > 1) {
+        // merge into one chunk of the largest stage's size
+        PMA_HIP(hipStreamSynchronize(stream));
+        for (pma::DeviceBuffer& b : arena_chunks) b.release();
+        arena_chunks.clear();
+        arena_chunks.emplace_back();
+        pma::DeviceBuffer& b = arena_chunks.back();
+        try {
+            with_eviction(this, std::string(), [&] { b.reserve(arena_peak, stream); });
+        } catch (...) {
+            arena_chunks.clear();
+            throw;
+        }
+    }
 }
 
 bool pma_ctx::evict(const std::string& keep) {
@@ -287,6 +343,7 @@ size_t pma_ctx::evictable_bytes() const {
 }
 
 void pma_ctx::drop(const std::string& name) {
+    arena_names.erase(name);  // an arena region is taken back when the next stage starts
     auto it = scratch.find(name);
     if (it == scratch.end()) return;
     PMA_HIP(hipStreamSynchronize(stream));
@@ -362,6 +419,7 @@ void pma_destroy(pma_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto& b : c->vol) b.release();
     for (auto& kv : c->scratch) kv.second.release();
+    for (auto& b : c->arena_chunks) b.release();
     if (c->pinned) (void)hipHostFree(c->pinned);
     for (auto& k : c->kernel_pending) {
         (void)hipEventDestroy(k.second.first);

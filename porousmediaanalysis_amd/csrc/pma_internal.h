@@ -141,6 +141,22 @@ struct pma_ctx {
     uint32_t flood_grid = 4096;                // workgroups of ws_bid and ws_resolve
     unsigned long long replay_mem = 128ull << 30;  // device bytes the DFS replays may hold
 
+    // Stage scratch: every stage's buffers come from one arena that the next stage reuses (no
+    // buffer of a stage is read by another), so the device holds the largest stage's scratch, not
+    // the sum.  Within a stage a name keeps its region; regions are bumped from chunks that never
+    // move, and a stage that outgrows the arena adds a chunk, merged into one at the next stage.
+    struct ArenaRegion {
+        void* ptr;
+        size_t bytes;
+    };
+    std::vector<pma::DeviceBuffer> arena_chunks;
+    size_t arena_used = 0;   // bytes bumped in the last chunk
+    size_t arena_stage = 0;  // bytes taken by the running stage
+    size_t arena_peak = 0;   // most bytes any stage took
+    std::map<std::string, ArenaRegion> arena_names;
+    void arena_begin_stage();
+    // buffers kept between calls outside the arena (the DFS replay's, sized from the free memory)
+    std::set<std::string> persistent;
     // scratch buffers that a failing allocation may free (large ones sized from the free memory,
     // kept between calls only to avoid allocating them again)
     std::set<std::string> evictable;

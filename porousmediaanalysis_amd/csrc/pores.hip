@@ -1837,7 +1837,10 @@ void run_pores(pma_ctx* c) {
             // frees them first
             static const char* const kReplayBuffers[] = {"pt_replay_vis", "pt_replay_plateaus", "pt_replay_cls",
                                                          "pt_stack", "pt_replay_trav"};
-            for (const char* name : kReplayBuffers) c->evictable.insert(name);
+            for (const char* name : kReplayBuffers) {
+                c->persistent.insert(name);
+                c->evictable.insert(name);
+            }
             size_t free_b = 0, total_b = 0;
             PMA_HIP(hipMemGetInfo(&free_b, &total_b));
             free_b += c->evictable_bytes();
@@ -1867,7 +1870,11 @@ void run_pores(pma_ctx* c) {
                     for (int k = 0; k < 5; ++k) {
                         fit = std::min(fit, (unsigned long long)c->buffer_bytes(kReplayBuffers[k]) / std::max(need[k], 1ull));
                     }
-                    if (fit >= W - W / 4) W = std::max<unsigned long long>(1, std::min(W, fit));
+                    // held buffers cost nothing to use: up to the knob's cap, even where the free
+                    // memory alone (other stages hold theirs by now) would allow fewer waves
+                    const unsigned long long cap_w = std::max<unsigned long long>(
+                        1, std::min<unsigned long long>({pending.size(), c->replay_mem / per_wave, 1024}));
+                    if (fit >= W - W / 4) W = std::max<unsigned long long>(1, std::min(fit, cap_w));
                 }
                 unsigned long long* vis = c->buf<unsigned long long>("pt_replay_vis", W * vis_pages * page_words);
                 unsigned long long* pbits = c->buf<unsigned long long>("pt_replay_plateaus", W * 2 * pent_words);

@@ -269,6 +269,7 @@ template <class T, class U> T atomicMax(T* p, U v) {
 }
 #define __ATOMIC_RELAXED_ 0
 #define __HIP_MEMORY_SCOPE_AGENT 0
+#define __HIP_MEMORY_SCOPE_WORKGROUP 1
 template <class T> T __hip_atomic_load(T* p, int, int) { return __atomic_load_n(p, __ATOMIC_SEQ_CST); }
 template <class T, class U> void __hip_atomic_store(T* p, U v, int, int) { __atomic_store_n(p, (T)v, __ATOMIC_SEQ_CST); }
 inline void fail_check(uint32_t* err, uint32_t code) { atomicCAS(err, 0u, code); }
@@ -493,6 +494,7 @@ struct pma_ctx {
     uint32_t flood_grid = 4096;
     unsigned long long replay_mem = 128ull << 30;
     std::set<std::string> evictable;
+    std::set<std::string> persistent;
     size_t evictable_bytes() const {
         size_t total = 0;
         for (const std::string& name : evictable) {
