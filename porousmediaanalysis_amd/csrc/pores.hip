@@ -519,8 +519,18 @@ __global__ void level_starts(const uint32_t* __restrict__ head, const uint32_t* 
 // Sizes the pore-set stage reads its inputs with (each index read from memory is tested
 // against one of them).
 struct SetBounds {
-    unsigned long long E, P, npores, pool_cap;
+    unsigned long long E, P, npores, pool_cap, nsub;
 };
+
+// Room for n members in the sub-pool of plateau q: the pool is cut into b.nsub equal sub-pools,
+// each with its own counter, so a level's allocations do not all meet on one address.  Returns
+// the offset, or ~0 when the sub-pool is full (the host repeats the run with a larger pool).
+__device__ __forceinline__ unsigned long long pool_take(unsigned long long* tops, uint32_t q, unsigned long long n,
+                                                        const SetBounds& b) {
+    const unsigned long long sub = b.pool_cap / b.nsub, sp = q % b.nsub;
+    const unsigned long long o = atomicAdd(&tops[sp], n);
+    return o + n > sub ? ~0ull : sp * sub + o;
+}
 
 // k-way merge of the sorted pore sets of a plateau's higher neighbours and its own pore.
 // cur/end hold one cursor per in-edge; out (if not null, with room for out_cap) receives the
@@ -589,8 +599,8 @@ __global__ void set_union_level(const uint32_t* __restrict__ order, uint32_t lo,
     }
     const uint32_t m = merge_sets(own, e0, ne, edges, set_off, set_len, pool, cur, end, nullptr, 0, b, err);
     if (m == kInv) return;
-    const unsigned long long o = atomicAdd(pool_top, (unsigned long long)m);
-    if (o + m > b.pool_cap) {
+    const unsigned long long o = m ? pool_take(pool_top, q, m, b) : 0ull;
+    if (o == ~0ull) {
         atomicOr(overflow, 2u);
         set_off[q] = 0;
         set_len[q] = 0;
@@ -666,9 +676,9 @@ __global__ void __launch_bounds__(64 * kUnionWaves)
         const bool ins = own != kInv && !__ballot(dup);
         const uint32_t total = len + (ins ? 1u : 0u);
         unsigned long long o = 0;
-        if (lane == 0 && total) o = atomicAdd(pool_top, (unsigned long long)total);
+        if (lane == 0 && total) o = pool_take(pool_top, q, total, b);
         o = __shfl(o, 0);
-        if (o + total > b.pool_cap) {
+        if (o == ~0ull) {
             if (lane == 0) {
                 atomicOr(overflow, 2u);
                 set_off[q] = 0;
@@ -725,9 +735,9 @@ __global__ void __launch_bounds__(64 * kUnionWaves)
     }
     const uint32_t total = __shfl(incl, 63);
     unsigned long long o = 0;
-    if (lane == 0 && total) o = atomicAdd(pool_top, (unsigned long long)total);
+    if (lane == 0 && total) o = pool_take(pool_top, q, total, b);
     o = __shfl(o, 0);
-    if (o + total > b.pool_cap) {
+    if (o == ~0ull) {
         if (lane == 0) {
             atomicOr(overflow, 2u);
             set_off[q] = 0;
@@ -1645,12 +1655,15 @@ void run_pores(pma_ctx* c) {
         pool_cap = std::max<unsigned long long>(pool_cap, std::min<unsigned long long>(PMA_PORES_POOL_MAX * P, room));
     }
     uint32_t* pool = nullptr;
-    unsigned long long* pool_top = c->buf<unsigned long long>("pt_pool_top", 1);
+    constexpr unsigned long long kSubPools = 64;
+    unsigned long long* pool_top = c->buf<unsigned long long>("pt_pool_top", kSubPools);
     for (int attempt = 0;; ++attempt) {
         pool = c->buf<uint32_t>("pt_pool", pool_cap);
-        PMA_HIP(hipMemsetAsync(pool_top, 0, 8, c->stream));
+        PMA_HIP(hipMemsetAsync(pool_top, 0, kSubPools * 8, c->stream));
         PMA_HIP(hipMemsetAsync(of, 0, 4, c->stream));
-        const SetBounds sb{E, P, npores, pool_cap};
+        // sub-pools of at least 4096 members
+        const unsigned long long nsub = std::max<unsigned long long>(1, std::min(kSubPools, pool_cap / 4096));
+        const SetBounds sb{E, P, npores, pool_cap, nsub};
         // one wave per plateau over an LDS bitmap of the pores while the bitmaps of a workgroup's
         // waves fit in PMA_UNION_LDS_MAX bytes; the per-thread merge otherwise
         const uint32_t words = (npores + 31) / 32;
@@ -1677,9 +1690,13 @@ void run_pores(pma_ctx* c) {
         // After an overflow the lower levels merged truncated sets, so the space taken is only a
         // lower bound of what is needed: grow geometrically and start again.
         PMA_REQUIRE(attempt < 8, PMA_ERR_INTERNAL, "pma_pores: pore set pool overflow");
-        pool_cap = std::max<unsigned long long>(4 * pool_cap, 2 * read_scalar(c, pool_top));
+        unsigned long long tops[kSubPools] = {}, taken = 0;
+        PMA_HIP(hipMemcpyAsync(tops, pool_top, sizeof tops, hipMemcpyDeviceToHost, c->stream));
+        PMA_HIP(hipStreamSynchronize(c->stream));
+        for (unsigned long long t : tops) taken += t;
+        pool_cap = std::max<unsigned long long>(4 * pool_cap, 2 * taken);
     }
-    const SetBounds sb{E, P, npores, pool_cap};
+    const SetBounds sb{E, P, npores, pool_cap, std::max<unsigned long long>(1, std::min(kSubPools, pool_cap / 4096))};
 
     phase.mark("6 pore sets");
     // 7. key formations (prefix records) and their distinct lists.
