@@ -70,9 +70,11 @@ class BatchConfig(ctypes.Structure):
 BATCH_DONE = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_int)
 
 
-def build(jobs=8):
-    """Compiles libpma.so for gfx950 with hipcc (Makefile in csrc/)."""
-    subprocess.run(["make", "-s", f"-j{jobs}", "-C", os.path.join(_HERE, "csrc")], check=True)
+def build(jobs=8, force=False):
+    """Compiles libpma.so for gfx950 with hipcc and the pma program (Makefile in csrc/); force
+    recompiles every source."""
+    subprocess.run(["make", "-s", f"-j{jobs}"] + (["-B"] if force else []) + ["-C", os.path.join(_HERE, "csrc")],
+                   check=True)
 
 
 _lib = None
