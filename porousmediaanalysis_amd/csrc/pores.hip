@@ -56,7 +56,7 @@ constexpr uint32_t kInv = 0xFFFFFFFFu;
 #define PMA_PORES_VIS_MIN_BITS 16u  // voxels per visited-voxel page: 2^bits, at least
 #endif
 #ifndef PMA_PORES_RING
-#define PMA_PORES_RING 1024u  // DFS frames of a replaying wave kept in LDS
+#define PMA_PORES_RING 512u  // DFS frames of a replaying wave kept in LDS
 #endif
 #ifndef PMA_PORES_REPLAY_MEM
 #define PMA_PORES_REPLAY_MEM (128ull << 30)  // device bytes the replaying waves may hold
@@ -973,6 +973,15 @@ __device__ __forceinline__ unsigned long long pcache_entry(uint32_t q, uint32_t 
     return ((unsigned long long)q << 32) | ((cls & 0xFFFFu) << 16) | (closed ? 2u : 0u) | (entered ? 1u : 0u);
 }
 
+// The replaying wave's LDS copy of recently read visited words, by halves: entries
+// (half-word index << 32) | the 32 visited bits, ~0 when empty.  Lane 0 sets a voxel's bit in the
+// copy when it marks the voxel, so the copy is never behind the wave's own bitmap.
+#ifndef PMA_PORES_VCACHE
+#define PMA_PORES_VCACHE 512u
+#endif
+constexpr uint32_t kVCache = PMA_PORES_VCACHE;  // a power of two
+__device__ __forceinline__ uint32_t vcache_slot(uint32_t h) { return (h * 0x9E3779B1u) >> 16 & (kVCache - 1u); }
+
 __device__ __forceinline__ bool bit_of(const unsigned long long* words, uint32_t i) {
     return (__hip_atomic_load(&words[i >> 6], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >> (i & 63)) & 1ull;
 }
@@ -1006,6 +1015,7 @@ __global__ void __launch_bounds__(64) dfs_replay(
     __shared__ uint32_t tcount[2];      // traversal: stack length, list length
     __shared__ uint4 ring[kRing];       // the newest DFS frames (voxel, plateau, radius, next index)
     __shared__ unsigned long long pcache[kPCache];  // recently met plateaus' state
+    __shared__ unsigned long long vcache[kVCache];  // recently read visited half-words
     const int lane = threadIdx.x;
     // visited voxels: blocks of 2^vis_bits voxels, each on an 8 KB-per-65536-voxel page of this
     // slot's pool taken on the block's first visit (vtab in LDS: block -> page, kNoPage if none)
@@ -1035,6 +1045,7 @@ __global__ void __launch_bounds__(64) dfs_replay(
         uint32_t npages = 0;
         for (unsigned long long i = lane; i < 2 * pent_words; i += 64) pent[i] = 0;  // entered, closed
         for (uint32_t i = lane; i < kPCache; i += 64) pcache[i] = ~0ull;
+        for (uint32_t i = lane; i < kVCache; i += 64) vcache[i] = ~0ull;
         for (uint32_t i = lane; i < seen_words; i += 64) seen[i] = 0;
         bool bad_list = false;
         for (uint32_t i = lane; i < task.list_len; i += 64) {
@@ -1106,11 +1117,24 @@ __global__ void __launch_bounds__(64) dfs_replay(
                 cn = cr[j];
                 qn = plat[j];
                 const uint32_t pg = vtab[j >> vis_bits];
-                const unsigned long long w =
-                    pg == kNoPage ? 0ull
-                                  : __hip_atomic_load(&vpool[(size_t)pg * page_words + ((j & ((1u << vis_bits) - 1)) >> 6)],
-                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                cand = cn != 0 && cn <= cu && !(w & (1ull << (j & 63)));
+                bool visited = false;
+                if (pg != kNoPage) {
+                    const uint32_t off = j & ((1u << vis_bits) - 1);
+                    const uint32_t hw = pg * (page_words * 2) + (off >> 5);
+                    const uint32_t slot = vcache_slot(hw);
+                    const unsigned long long e = vcache[slot];
+                    uint32_t bits;
+                    if ((uint32_t)(e >> 32) == hw) {
+                        bits = (uint32_t)e;
+                    } else {
+                        const unsigned long long w = __hip_atomic_load(&vpool[(size_t)pg * page_words + (off >> 6)],
+                                                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        bits = (uint32_t)(w >> ((off >> 5) & 1u) * 32u);
+                        vcache[slot] = ((unsigned long long)hw << 32) | bits;
+                    }
+                    visited = (bits >> (j & 31)) & 1u;
+                }
+                cand = cn != 0 && cn <= cu && !visited;
                 // neighbours in other plateaus: closed plateaus are visited; the entered test and the
                 // class are for the lane whose neighbour is taken
                 if (cand && qn != uq && qn < b.P) {
@@ -1330,6 +1354,12 @@ __global__ void __launch_bounds__(64) dfs_replay(
                 ring[top % kRing] = make_uint4(u, uq, cu, (uint32_t)kk);  // this frame resumes below kk
                 atomicOr(&vpool[(size_t)vtab[nvox >> vis_bits] * page_words + ((nvox & ((1u << vis_bits) - 1)) >> 6)],
                          1ull << (nvox & 63));
+                {
+                    const uint32_t hw = vtab[nvox >> vis_bits] * (page_words * 2) + ((nvox & ((1u << vis_bits) - 1)) >> 5);
+                    const uint32_t slot = vcache_slot(hw);
+                    const unsigned long long e = vcache[slot];
+                    if ((uint32_t)(e >> 32) == hw) vcache[slot] = e | (1ull << (nvox & 31));
+                }
                 if (is_new) {
                     atomicOr(&pent[nq >> 6], 1ull << (nq & 63));
                     pcache[pcache_slot(nq)] = pcache_entry(nq, ncls_new, false, true);

@@ -105,7 +105,7 @@ def hostemu_slabs(tmp_path_factory):
                                                  "-DPMA_PORES_VIS_MIN_BITS=6u", "-DPMA_PORES_VIS_PAGES=4u",
                                                  "-DPMA_PORES_TRAV_CAP=3ull", "-DPMA_PORES_POOL_MIN=1ull",
                                                  "-DPMA_PORES_POOL_MAX=1ull", "-DPMA_PORES_POOL_FLOOR=16ull",
-                                                 "-DPMA_PORES_PCACHE=4u"], "slabs")
+                                                 "-DPMA_PORES_PCACHE=4u", "-DPMA_PORES_VCACHE=4u"], "slabs")
 
 
 @pytest.fixture(scope="session")
