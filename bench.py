@@ -31,7 +31,10 @@ With --workload surrogate: the screening network trained on --train-crops crops 
 and max error, and a sampled crop whose radius MAE exceeds --bound falls back to its exact fields.
 value = voxels per second of the screening (EDT, network, the sampled exact reruns).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 1024] [--no-cpu]
+The plain run's line also carries "batched" (64 subvolumes) and "surrogate" (96 training and 120
+fresh 64^3 crops) objects from reduced runs of those workloads (--no-extras skips them).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 1024] [--no-cpu] [--no-extras]
                        [--workload plain|batched|surrogate]
 """
 import argparse
@@ -98,6 +101,8 @@ def parse():
     ap.add_argument("--test-crops", type=int, default=256)
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--bound", type=float, default=3.0, help="radius MAE bound (voxels) of a sampled crop")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="plain workload: skip the reduced batched and screening runs added to its line")
     return ap.parse_args()
 
 
@@ -267,7 +272,7 @@ def batched(args, pma):
     default_s = time.perf_counter() - t0
     progress(f"batch with the default configuration: {default_s:.2f} s")
     return {
-        "metric": "voxels/s, batched: 256 uneven subvolumes, whole pipeline each (pma_run_batch, tuned)",
+        "metric": f"voxels/s, batched: {len(vols)} uneven subvolumes, whole pipeline each (pma_run_batch, tuned)",
         "value": nvox / batch_s,
         "unit": "voxels/s",
         "n_gpus": 1,
@@ -555,6 +560,22 @@ def main():
     }
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(args)
+    if not args.no_extras:
+        # reduced samples of the batched and screening workloads, so that one run reports both
+        # against the plain pipeline (the full configurations: --workload batched / surrogate)
+        ex = argparse.Namespace(**vars(args))
+        ex.batch_count = 64
+        b = batched(ex, pma)
+        result["batched"] = {k: b[k] for k in ["value", "unit", "per_volume_loop_voxels_per_s", "speedup_vs_loop",
+                                               "batch_s", "per_volume_loop_s", "autotuner_s"]}
+        result["batched"]["config"] = b["config"]
+        ex.crop, ex.train_crops, ex.epochs, ex.test_crops = 64, 96, 4, 120
+        sres = surrogate(ex, pma)
+        result["surrogate"] = {k: sres[k] for k in ["value", "unit", "train_s", "radius_mae", "radius_max_error",
+                                                   "radius_mae_without_network", "ratio_mae", "ratio_max_error",
+                                                   "sampled_crops", "fallbacks", "exact_fields_voxels_per_s",
+                                                   "plain_pipeline_voxels_per_s"]}
+        result["surrogate"]["config"] = sres["config"]
     print(json.dumps(result))
 
 
