@@ -13,14 +13,16 @@
 //     popped node's position in FIFO d and k its neighbour index; T increases exactly in the
 //     reference's event order (pushes into FIFO d all happen before d is popped, and FIFO d
 //     is popped in position order).
-//   * FIFO d's content pushed from higher depths is sorted by push time when d starts; the
-//     pops of FIFO d are split into rounds (round r+1 = the nodes pushed into d by round r).
+//   * FIFO d holds its seeds (ascending k) and then the pushes from higher depths in push
+//     time order; its pops are split into rounds (round r+1 = the nodes pushed into d by
+//     round r).
 //   * in a round, every qualifying contact to an unlabelled voxel bids its time with a 64-bit
 //     atomicMin (round-stamped so older bids lose); the minimum wins the claim.  All other
 //     contacts whose pore differs from the owner are logged (voxel, time, pore).
-//   * claims into the same depth are appended in (pos, k) order with a prefix sum; claims into
-//     lower depths are appended in any order with their push time (sorted when that depth
-//     starts).
+//   * claims into the same depth are appended to FIFO d in (pos, k) order with a prefix sum;
+//     claims into lower depths are staged the same way, so the level's staged list is in push
+//     time order; when the level ends, a stable sort by target depth moves each depth's run to
+//     the end of its FIFO (no queue atomics, and no sort by time).
 //   * after the flood the log is sorted by (voxel, time): the distinct foreign pores in order
 //     of first contact, c2, c3, ..., give the list [c2, owner, c3, ...] and its prefixes, each
 //     formed at the first contact of its last pore; keys are numbered by formation time.
@@ -61,24 +63,28 @@ __global__ void cr_histogram(const uint32_t* __restrict__ cr, size_t n, uint32_t
         if (lh[i]) atomicAdd(&hist[i], lh[i]);
 }
 
-__global__ void push_seeds(const uint32_t* __restrict__ seed_voxel, const uint32_t* __restrict__ seed_r2,
-                           uint32_t np, uint32_t* __restrict__ label, const uint64_t* __restrict__ qoff,
-                           uint32_t* __restrict__ qcount, unsigned long long* __restrict__ qitem,
-                           unsigned long long* __restrict__ qkey) {
+// The initial pushes: seed k at the slot the host gave it (its level's seeds in ascending k, the
+// first entries of their FIFOs).
+__global__ void push_seeds(const uint32_t* __restrict__ seed_voxel, const uint64_t* __restrict__ seed_slot,
+                           uint32_t np, uint32_t* __restrict__ label, unsigned long long* __restrict__ qitem) {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= np) return;
-    uint32_t v = seed_voxel[k], lvl = seed_r2[k];
+    uint32_t v = seed_voxel[k];
     label[v] = k + 1;
-    uint32_t slot = atomicAdd(&qcount[lvl], 1u);
-    qitem[qoff[lvl] + slot] = (unsigned long long)v | ((unsigned long long)k << 32);
-    qkey[qoff[lvl] + slot] = k;  // initial pushes precede every flood push (top field 0)
+    qitem[seed_slot[k]] = (unsigned long long)v | ((unsigned long long)k << 32);
 }
 
-// The bounds [a, b) of a round's FIFO wave, kept on the device: the kernels of a round read
-// them, and ws_scan_parts writes the next round's (rounds alternate between two states).
+// The bounds [a, b) of a round's FIFO wave and the number of lower-level pushes the level has
+// staged before the round, kept on the device: the kernels of a round read them, and
+// ws_scan_parts writes the next round's (rounds alternate between two states).
 struct WsRound {
-    uint32_t a, b;
+    uint32_t a, b, staged;
 };
+
+// A level starts with its FIFO as the earlier levels left it.
+__global__ void ws_level_start(const uint32_t* __restrict__ qcount, WsRound* __restrict__ state) {
+    if (threadIdx.x == 0) *state = WsRound{0u, *qcount, 0u};
+}
 constexpr uint32_t kWsGrid = 512;       // workgroups of the round's scan kernels (grid-stride over the wave)
 
 // A round's items are spread over lanes, one lane per (item, neighbour): the item's lanes form
@@ -123,21 +129,19 @@ __global__ void __launch_bounds__(256) ws_bid(const unsigned long long* __restri
     }
 }
 
-// Each contact of a round: a won claim labels the neighbour and queues it (same level: marked in
-// the item's mask, appended in order by ws_append; lower level: queued with its push time); a
-// contact with another pore's voxel is logged.  A group's same-level claims become its item's
-// mask over the 27 offsets (one ballot), and a wave's log entries take one atomic.
+// Each contact of a round: a won claim labels the neighbour and marks it in one of its item's
+// two masks over the 27 offsets (same level: appended to this level's FIFO by ws_append; lower
+// level: staged by ws_append); a contact with another pore's voxel is logged.  A group's claims
+// become its item's masks with one ballot each, and a wave's log entries take one atomic.
+// same_cnt holds an item's same-level claims in its low 16 bits and its lower-level ones above.
 __global__ void __launch_bounds__(256) ws_resolve(const unsigned long long* __restrict__ items,
                                                   const WsRound* __restrict__ round_state,
                                                   const uint32_t* __restrict__ cr, uint32_t* __restrict__ label,
                                                   const unsigned long long* __restrict__ claim, unsigned long long hi,
                                                   unsigned long long tlevel, uint32_t d, int X, int Y, int Z, int d3q7,
-                                                  uint32_t* __restrict__ same_mask, uint32_t* __restrict__ same_cnt,
-                                                  const uint64_t* __restrict__ qoff, uint32_t* __restrict__ qcount,
-                                                  unsigned long long* __restrict__ qitem,
-                                                  unsigned long long* __restrict__ qkey, LogEntry* __restrict__ log,
-                                                  unsigned long long* __restrict__ log_count, unsigned long long log_cap,
-                                                  uint32_t* __restrict__ error) {
+                                                  uint32_t* __restrict__ same_mask, uint32_t* __restrict__ lower_mask,
+                                                  uint32_t* __restrict__ same_cnt, LogEntry* __restrict__ log,
+                                                  unsigned long long* __restrict__ log_count, unsigned long long log_cap) {
     const uint32_t a = round_state->a, b = round_state->b;
     const uint32_t sh = ws_group_shift(d3q7), span = 1u << sh;
     const int lane = (int)(threadIdx.x & 63);
@@ -151,11 +155,11 @@ __global__ void __launch_bounds__(256) ws_resolve(const unsigned long long* __re
         const uint32_t p = (uint32_t)(it >> 32);
         const unsigned long long when = (unsigned long long)(pos) << 5 | (unsigned long long)(k < 0 ? 0 : k);
         bool same = false, logged = false, lower = false;
-        uint32_t j = 0, c = 0;
+        uint32_t j = 0;
         if (k >= 0 && ws_neighbour((uint32_t)it, k, X, Y, Z, j)) {
-            c = cr[j];
+            const uint32_t c = cr[j];
+            const uint32_t lab = label[j];  // loaded beside c: the voxel is in the image either way
             if (c != 0 && c <= d) {
-                const uint32_t lab = label[j];
                 uint32_t owner = p;
                 if (lab == 0) {
                     const unsigned long long w = claim[j];
@@ -172,28 +176,6 @@ __global__ void __launch_bounds__(256) ws_resolve(const unsigned long long* __re
                 logged = owner != p;
             }
         }
-        // Lower-level claims take slots in their level's queue: one atomic per distinct level in
-        // the wave (the queue is sorted by push time before it is popped, so slot order is free).
-        unsigned long long pend = __ballot(lower);
-        while (pend) {
-            const int leader = __ffsll((long long)pend) - 1;
-            const uint32_t c0 = (uint32_t)__shfl((int)c, leader);
-            const unsigned long long peers = __ballot(lower && c == c0);
-            unsigned long long base = 0;
-            if (lane == leader) base = atomicAdd(&qcount[c0], (uint32_t)__popcll(peers));
-            base = (unsigned long long)(uint32_t)__shfl((int)(uint32_t)base, leader);
-            if (lower && c == c0) {
-                const uint64_t slot = qoff[c] + base + (uint64_t)__popcll(peers & ((1ull << lane) - 1ull));
-                if (slot < qoff[c + 1]) {
-                    qitem[slot] = (unsigned long long)j | ((unsigned long long)p << 32);
-                    qkey[slot] = tlevel | when;
-                } else {
-                    atomicOr(error, 1u);
-                }
-                lower = false;
-            }
-            pend &= ~peers;
-        }
         const unsigned long long logs = __ballot(logged);
         if (logs) {
             const int leader = __ffsll((long long)logs) - 1;
@@ -205,19 +187,25 @@ __global__ void __launch_bounds__(256) ws_resolve(const unsigned long long* __re
                 if (s < log_cap) log[s] = LogEntry{tlevel | when, j, p};
             }
         }
-        const unsigned long long sames = __ballot(same);
+        const unsigned long long sames = __ballot(same), lowers = __ballot(lower);
         if (kk == 0) {
-            const uint32_t g = (uint32_t)((sames >> (lane & ~(int)(span - 1u))) & ((1ull << span) - 1ull));
-            uint32_t mask;
+            const int first = lane & ~(int)(span - 1u);
+            const unsigned long long gm = (1ull << span) - 1ull;
+            const uint32_t gs = (uint32_t)((sames >> first) & gm), gl = (uint32_t)((lowers >> first) & gm);
+            uint32_t ms, ml;
             if (d3q7) {
-                mask = 0;
-                for (uint32_t i = 0; i < 6; ++i)
-                    if (g >> i & 1u) mask |= 1u << ws_offset(i, 1);
+                ms = ml = 0;
+                for (uint32_t i = 0; i < 6; ++i) {
+                    if (gs >> i & 1u) ms |= 1u << ws_offset(i, 1);
+                    if (gl >> i & 1u) ml |= 1u << ws_offset(i, 1);
+                }
             } else {
-                mask = (g & 0x1FFFu) | ((g >> 13) << 14);
+                ms = (gs & 0x1FFFu) | ((gs >> 13) << 14);
+                ml = (gl & 0x1FFFu) | ((gl >> 13) << 14);
             }
-            same_mask[pos - a] = mask;
-            same_cnt[pos - a] = (uint32_t)__popc(mask);
+            same_mask[pos - a] = ms;
+            lower_mask[pos - a] = ml;
+            same_cnt[pos - a] = (uint32_t)__popc(ms) | (uint32_t)__popc(ml) << 16;
         }
     }
 }
@@ -250,73 +238,96 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t& e
     return total;
 }
 
-// Appended items per workgroup slice.
+// Appended and staged items per workgroup slice (part, part_low).
 __global__ void __launch_bounds__(256) ws_count_parts(const WsRound* __restrict__ round_state,
                                                       const uint32_t* __restrict__ same_cnt,
-                                                      uint32_t* __restrict__ part) {
+                                                      uint32_t* __restrict__ part, uint32_t* __restrict__ part_low) {
     __shared__ uint32_t lds[4];
     uint32_t lo, hi;
     ws_slice(round_state->b - round_state->a, lo, hi);
-    uint32_t sum = 0;
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) sum += same_cnt[i];
+    uint32_t sum = 0, sum_low = 0;
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        const uint32_t v = same_cnt[i];
+        sum += v & 0xFFFFu;
+        sum_low += v >> 16;
+    }
     uint32_t excl;
     const uint32_t total = block_exclusive_scan(sum, excl, lds);
-    if (threadIdx.x == 0) part[blockIdx.x] = total;
+    const uint32_t total_low = block_exclusive_scan(sum_low, excl, lds);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = total;
+        part_low[blockIdx.x] = total_low;
+    }
 }
 
-// One workgroup: the slices' offsets (exclusive scan of part, in place) and the next round's
-// bounds [b, b + appended).  A wave that would overflow its level's queue (capacity cap) is
+// One workgroup: the slices' offsets (exclusive scans of part and part_low, in place), the next
+// round's bounds [b, b + appended) and the level's staged count after this round.  A round that
+// would overflow its level's FIFO (capacity cap) or the staging list (capacity stage_cap) is
 // flagged, and ws_append skips what does not fit.
 __global__ void __launch_bounds__(256) ws_scan_parts(const WsRound* __restrict__ round_state,
                                                      WsRound* __restrict__ next_state, uint32_t* __restrict__ part,
-                                                     uint32_t nparts, uint32_t cap, uint32_t* __restrict__ error) {
+                                                     uint32_t* __restrict__ part_low, uint32_t nparts, uint32_t cap,
+                                                     uint32_t stage_cap, uint32_t* __restrict__ error) {
     __shared__ uint32_t lds[4];
-    uint32_t carry = 0;
+    uint32_t carry = 0, carry_low = 0;
     for (uint32_t base = 0; base < nparts; base += blockDim.x) {  // the same trips on every thread
         const uint32_t i = base + threadIdx.x;
-        const uint32_t v = i < nparts ? part[i] : 0u;
-        uint32_t excl;
+        const uint32_t v = i < nparts ? part[i] : 0u, w = i < nparts ? part_low[i] : 0u;
+        uint32_t excl, excl_low;
         const uint32_t total = block_exclusive_scan(v, excl, lds);
-        if (i < nparts) part[i] = carry + excl;
+        const uint32_t total_low = block_exclusive_scan(w, excl_low, lds);
+        if (i < nparts) {
+            part[i] = carry + excl;
+            part_low[i] = carry_low + excl_low;
+        }
         carry += total;
+        carry_low += total_low;
     }
     if (threadIdx.x == 0) {
-        const uint32_t b = round_state->b;
-        uint64_t nb = (uint64_t)b + carry;
-        if (nb > cap) {
+        const uint32_t b = round_state->b, staged = round_state->staged;
+        uint64_t nb = (uint64_t)b + carry, ns = (uint64_t)staged + carry_low;
+        if (nb > cap || ns > stage_cap) {
             atomicOr(error, 1u);
             nb = b;  // stop the level; the stage fails at its end
+            ns = staged;
         }
         next_state->a = b;
         next_state->b = (uint32_t)nb;
+        next_state->staged = (uint32_t)ns;
     }
 }
 
-// Appends each item's same-level claims after the wave, in item order (slice offset from
-// ws_scan_parts plus a scan within the slice), skipping places past the queue's capacity.
+// Appends each item's same-level claims after the wave and its lower-level claims after the
+// level's staged list (target depth CR(j) beside each), both in item order, then offset order
+// (slice offsets from ws_scan_parts plus scans within the slice), skipping places past the
+// capacities.
 __global__ void __launch_bounds__(256) ws_append(unsigned long long* __restrict__ items,
                                                  const WsRound* __restrict__ round_state,
-                                                 const uint32_t* __restrict__ part,
+                                                 const uint32_t* __restrict__ part, const uint32_t* __restrict__ part_low,
                                                  const uint32_t* __restrict__ same_mask,
-                                                 const uint32_t* __restrict__ same_cnt, uint32_t cap, int X, int Y) {
+                                                 const uint32_t* __restrict__ lower_mask,
+                                                 const uint32_t* __restrict__ same_cnt, uint32_t cap,
+                                                 const uint32_t* __restrict__ cr, unsigned long long* __restrict__ stage_item,
+                                                 uint32_t* __restrict__ stage_level, uint32_t stage_cap, int X, int Y) {
     __shared__ uint32_t lds[4];
     const uint32_t a = round_state->a, b = round_state->b;
     uint32_t lo, hi;
     ws_slice(b - a, lo, hi);
     uint32_t carry = b + part[blockIdx.x];
+    uint32_t carry_low = round_state->staged + part_low[blockIdx.x];
     for (uint32_t base = lo; base < hi; base += blockDim.x) {  // the same trips on every thread
         const uint32_t t = base + threadIdx.x;
         const uint32_t cnt = t < hi ? same_cnt[t] : 0u;
-        uint32_t excl;
-        const uint32_t total = block_exclusive_scan(cnt, excl, lds);
+        uint32_t excl, excl_low;
+        const uint32_t total = block_exclusive_scan(cnt & 0xFFFFu, excl, lds);
+        const uint32_t total_low = block_exclusive_scan(cnt >> 16, excl_low, lds);
         if (cnt) {
-            uint32_t mask = same_mask[t];
             const unsigned long long it = items[a + t];
             const uint32_t v = (uint32_t)it;
             const unsigned long long p = it >> 32;
             int x, y, z;
             decode(v, X, Y, x, y, z);
-            uint32_t o = carry + excl;
+            uint32_t mask = same_mask[t], o = carry + excl;
             while (mask) {
                 const int k = __ffs(mask) - 1;
                 mask &= mask - 1;
@@ -326,9 +337,55 @@ __global__ void __launch_bounds__(256) ws_append(unsigned long long* __restrict_
                 if (o < cap) items[o] = (unsigned long long)j | (p << 32);
                 ++o;
             }
+            mask = lower_mask[t];
+            o = carry_low + excl_low;
+            while (mask) {
+                const int k = __ffs(mask) - 1;
+                mask &= mask - 1;
+                const int dx = k / 9 - 1, dy = (k / 3) % 3 - 1, dz = k % 3 - 1;
+                const uint32_t j = ((uint32_t)(z + dz) * (uint32_t)X + (uint32_t)(x + dx)) * (uint32_t)Y +
+                                   (uint32_t)(y + dy);
+                if (o < stage_cap) {
+                    stage_item[o] = (unsigned long long)j | (p << 32);
+                    stage_level[o] = cr[j];
+                }
+                ++o;
+            }
         }
         carry += total;
+        carry_low += total_low;
     }
+}
+
+// The level's staged pushes, stably sorted by target depth, go to the ends of their FIFOs: run
+// heads record where each depth's run starts, every push takes FIFO slot count + (i - start),
+// and run tails then advance the counts.
+__global__ void ws_run_first(const uint32_t* __restrict__ lvl, uint32_t n, uint32_t* __restrict__ first) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && (i == 0 || lvl[i - 1] != lvl[i])) first[lvl[i]] = i;
+}
+
+__global__ void ws_distribute(const uint32_t* __restrict__ lvl, const unsigned long long* __restrict__ item, uint32_t n,
+                              const uint32_t* __restrict__ first, const uint64_t* __restrict__ qoff,
+                              const uint32_t* __restrict__ qcount, unsigned long long* __restrict__ qitem,
+                              uint32_t* __restrict__ error) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t c = lvl[i];
+    const uint64_t slot = qoff[c] + qcount[c] + (i - first[c]);
+    if (slot < qoff[c + 1]) qitem[slot] = item[i];
+    else atomicOr(error, 1u);
+}
+
+// A count never passes its FIFO's capacity (ws_distribute has flagged what did not fit), so the
+// rounds of that depth read only its own slots.
+__global__ void ws_run_commit(const uint32_t* __restrict__ lvl, uint32_t n, const uint32_t* __restrict__ first,
+                              const uint64_t* __restrict__ qoff, uint32_t* __restrict__ qcount) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || (i != n - 1 && lvl[i + 1] == lvl[i])) return;
+    const uint32_t c = lvl[i];
+    const uint64_t grown = (uint64_t)qcount[c] + (i - first[c] + 1), cap = qoff[c + 1] - qoff[c];
+    qcount[c] = (uint32_t)(grown < cap ? grown : cap);
 }
 
 __global__ void log_split(const LogEntry* __restrict__ log, size_t n, unsigned long long* __restrict__ t,
@@ -514,37 +571,43 @@ bool watershed_once(pma_ctx* c, unsigned long long log_cap, unsigned long long* 
     std::vector<uint32_t> h(L);
     PMA_HIP(hipMemcpyAsync(h.data(), hist, L * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
     PMA_HIP(hipStreamSynchronize(c->stream));
-    std::vector<uint32_t> seed_voxel(np), seed_r2(np), pore_id(np);
+    std::vector<uint32_t> seed_voxel(np), pore_id(np), seeds_at(L, 0);
     for (uint32_t k = 0; k < np; ++k) {
         const PoreRow& p = c->pores[k];
         PMA_REQUIRE(p.x >= 0 && p.x < X && p.y >= 0 && p.y < Y && p.z >= 0 && p.z < Z, PMA_ERR_INVALID_ARGUMENT,
                     "pma_watershed: a pore seed lies outside the image");
         PMA_REQUIRE(p.r2 <= maxE, PMA_ERR_INVALID_ARGUMENT, "pma_watershed: a seed radius exceeds the maximum EDT square");
         seed_voxel[k] = (uint32_t)(((size_t)p.z * X + p.x) * Y + p.y);
-        seed_r2[k] = p.r2;
         pore_id[k] = p.id;
         h[p.r2] += 1;
     }
     std::vector<uint64_t> qoff_h(L + 1, 0);
     for (size_t d = 0; d < L; ++d) qoff_h[d + 1] = qoff_h[d] + h[d];
     const uint64_t qtotal = qoff_h[L];
+    // the initial pushes: each level's seeds first in its FIFO, in ascending k
+    std::vector<uint64_t> seed_slot(np);
+    for (uint32_t k = 0; k < np; ++k) seed_slot[k] = qoff_h[c->pores[k].r2] + seeds_at[c->pores[k].r2]++;
     uint64_t* qoff = c->buf<uint64_t>("ws_qoff", L + 1);
     uint32_t* qcount = c->buf<uint32_t>("ws_qcount", L);
     unsigned long long* qitem = c->buf<unsigned long long>("ws_qitem", std::max<uint64_t>(qtotal, 1));
-    unsigned long long* qkey = c->buf<unsigned long long>("ws_qkey", std::max<uint64_t>(qtotal, 1));
-    unsigned long long* sort_k = c->buf<unsigned long long>("ws_sort_k", std::max<uint64_t>(qtotal, 1));
-    unsigned long long* sort_v = c->buf<unsigned long long>("ws_sort_v", std::max<uint64_t>(qtotal, 1));
     upload(c, qoff, qoff_h.data(), (L + 1) * sizeof(uint64_t));
-    PMA_HIP(hipMemsetAsync(qcount, 0, L * sizeof(uint32_t), c->stream));
+    upload(c, qcount, seeds_at.data(), L * sizeof(uint32_t));
+    // a level's lower-level pushes, staged in push time order and then sorted by target depth
+    const uint32_t stage_cap = (uint32_t)std::max<uint64_t>(qtotal, 1);
+    unsigned long long* st_item = c->buf<unsigned long long>("ws_stage_item", stage_cap);
+    uint32_t* st_lvl = c->buf<uint32_t>("ws_stage_level", stage_cap);
+    unsigned long long* st_item2 = c->buf<unsigned long long>("ws_stage_item2", stage_cap);
+    uint32_t* st_lvl2 = c->buf<uint32_t>("ws_stage_level2", stage_cap);
+    uint32_t* run_first = c->buf<uint32_t>("ws_run_first", L);
     PMA_HIP(hipMemsetAsync(label, 0, n * sizeof(uint32_t), c->stream));
     unsigned long long* claim = c->buf<unsigned long long>("ws_claim", n);
     PMA_HIP(hipMemsetAsync(claim, 0xFF, n * sizeof(unsigned long long), c->stream));
     uint32_t* d_seed_voxel = c->buf<uint32_t>("ws_seed_voxel", std::max(np, 1u));
-    uint32_t* d_seed_r2 = c->buf<uint32_t>("ws_seed_r2", std::max(np, 1u));
+    uint64_t* d_seed_slot = c->buf<uint64_t>("ws_seed_slot", std::max(np, 1u));
     if (np) {
         upload(c, d_seed_voxel, seed_voxel.data(), np * 4ull);
-        upload(c, d_seed_r2, seed_r2.data(), np * 4ull);
-        push_seeds<<<grid_for(np, 256), 256, 0, c->stream>>>(d_seed_voxel, d_seed_r2, np, label, qoff, qcount, qitem, qkey);
+        upload(c, d_seed_slot, seed_slot.data(), np * 8ull);
+        push_seeds<<<grid_for(np, 256), 256, 0, c->stream>>>(d_seed_voxel, d_seed_slot, np, label, qitem);
         PMA_LAUNCHED(c);
     }
     LogEntry* log = c->buf<LogEntry>("ws_log", log_cap);
@@ -553,34 +616,27 @@ bool watershed_once(pma_ctx* c, unsigned long long log_cap, unsigned long long* 
     uint32_t* ws_err = c->buf<uint32_t>("ws_error", 1);
     PMA_HIP(hipMemsetAsync(ws_err, 0, 4, c->stream));
     uint32_t* same_mask = c->buf<uint32_t>("ws_same_mask", std::max<uint64_t>(qtotal, 1));
+    uint32_t* lower_mask = c->buf<uint32_t>("ws_lower_mask", std::max<uint64_t>(qtotal, 1));
     uint32_t* same_cnt = c->buf<uint32_t>("ws_same_cnt", std::max<uint64_t>(qtotal, 1));
     uint32_t* part = c->buf<uint32_t>("ws_parts", kWsGrid);
+    uint32_t* part_low = c->buf<uint32_t>("ws_parts_low", kWsGrid);
     WsRound* rstate = c->buf<WsRound>("ws_round_state", 2);
 
     uint64_t rounds = 0;
     uint32_t gr = 0;  // global round counter for claim stamps
     for (long d = (long)maxE; d >= 1; --d) {
-        uint32_t cnt = read_scalar(c, qcount + d);
-        if (cnt == 0) continue;
-        // the kernels count past a full queue (and flag it) without writing past it
-        PMA_REQUIRE(cnt <= h[d], PMA_ERR_INTERNAL, "pma_watershed: FIFO capacity exceeded");
+        if (h[d] == 0) continue;  // no voxel and no seed at this depth: its FIFO stays empty
         unsigned long long* items = qitem + qoff_h[d];
-        if (cnt > 1) {
-            radix_sort_pairs_u64_u64(c, reinterpret_cast<const uint64_t*>(qkey + qoff_h[d]),
-                                     reinterpret_cast<uint64_t*>(sort_k), reinterpret_cast<const uint64_t*>(items),
-                                     reinterpret_cast<uint64_t*>(sort_v), cnt);
-            PMA_HIP(hipMemcpyAsync(items, sort_v, cnt * sizeof(uint64_t), hipMemcpyDeviceToDevice, c->stream));
-        }
         const unsigned long long tlevel = (unsigned long long)(maxE - d + 1) << 40;
         // the level's rounds run back to back on the device, each reading its bounds from the
         // previous one's; the host looks for an empty round only after rounds 1, 2, 4, 8 and
         // then every 8 (the rounds after the last are empty and change nothing)
-        const WsRound first{0u, cnt};
-        upload(c, rstate, &first, sizeof first);
+        ws_level_start<<<1, 64, 0, c->stream>>>(qcount + d, rstate);
+        PMA_LAUNCHED(c);
         const uint32_t g = std::min(grid_for(h[d], 256), kWsGrid);
         // ws_bid and ws_resolve: grid-stride over the lanes on at most c->flood_grid workgroups
         const uint32_t gi = std::min(grid_for((size_t)h[d] << (d3q7 ? 3 : 5), 256), c->flood_grid);
-        uint32_t r = 0, check_at = 1, step = 1;
+        uint32_t r = 0, check_at = 1, step = 1, staged = 0;
         while (true) {
             PMA_REQUIRE(gr < (1u << 24) - 1, PMA_ERR_INTERNAL, "pma_watershed: too many rounds");
             const unsigned long long hi = (unsigned long long)((1u << 24) - 1 - gr) << 40;
@@ -596,26 +652,50 @@ bool watershed_once(pma_ctx* c, unsigned long long log_cap, unsigned long long* 
             {
                 PMA_TIMED(c, "ws_resolve");
                 ws_resolve<<<gi, 256, 0, c->stream>>>(items, cur, cr, label, claim, hi, tlevel, (uint32_t)d, X, Y, Z,
-                                                     d3q7, same_mask, same_cnt, qoff, qcount, qitem, qkey, log,
-                                                     log_count, log_cap, ws_err);
+                                                     d3q7, same_mask, lower_mask, same_cnt, log, log_count, log_cap);
                 PMA_LAUNCHED(c);
             }
-            ws_count_parts<<<g, 256, 0, c->stream>>>(cur, same_cnt, part);
+            ws_count_parts<<<g, 256, 0, c->stream>>>(cur, same_cnt, part, part_low);
             PMA_LAUNCHED(c);
-            ws_scan_parts<<<1, 256, 0, c->stream>>>(cur, nxt, part, g, h[d], ws_err);
+            ws_scan_parts<<<1, 256, 0, c->stream>>>(cur, nxt, part, part_low, g, h[d], stage_cap, ws_err);
             PMA_LAUNCHED(c);
-            ws_append<<<g, 256, 0, c->stream>>>(items, cur, part, same_mask, same_cnt, h[d], X, Y);
+            ws_append<<<g, 256, 0, c->stream>>>(items, cur, part, part_low, same_mask, lower_mask, same_cnt, h[d], cr,
+                                                st_item, st_lvl, stage_cap, X, Y);
             PMA_LAUNCHED(c);
             ++r;
             if (r == check_at) {
                 const WsRound next = read_scalar(c, nxt);
-                if (next.a == next.b) break;
+                if (next.a == next.b) {
+                    staged = next.staged;
+                    break;
+                }
                 step = std::min(step * 2, 8u);
                 check_at += step;
             }
         }
+        if (staged == 0) continue;
+        // the level's pushes into lower depths join their FIFOs in push time order
+        PMA_TIMED(c, "ws_distribute");
+        const uint32_t* lv = st_lvl;
+        const unsigned long long* iv = st_item;
+        if (staged > 1) {
+            int end_bit = 1;
+            while (end_bit < 32 && ((unsigned long)d - 1) >> end_bit) ++end_bit;
+            radix_sort_pairs_u32_u64(c, st_lvl, st_lvl2, reinterpret_cast<const uint64_t*>(st_item),
+                                     reinterpret_cast<uint64_t*>(st_item2), staged, 0, end_bit);
+            lv = st_lvl2;
+            iv = st_item2;
+        }
+        const uint32_t gs = grid_for(staged, 256);
+        ws_run_first<<<gs, 256, 0, c->stream>>>(lv, staged, run_first);
+        PMA_LAUNCHED(c);
+        ws_distribute<<<gs, 256, 0, c->stream>>>(lv, iv, staged, run_first, qoff, qcount, qitem, ws_err);
+        PMA_LAUNCHED(c);
+        ws_run_commit<<<gs, 256, 0, c->stream>>>(lv, staged, run_first, qoff, qcount);
+        PMA_LAUNCHED(c);
     }
-    // a queue that filled up was flagged by ws_resolve or ws_scan_parts (neither writes past it)
+    // a FIFO or the staging list that filled up was flagged by ws_scan_parts or ws_distribute
+    // (neither they nor ws_append write past it)
     PMA_REQUIRE(read_scalar(c, ws_err) == 0, PMA_ERR_INTERNAL, "pma_watershed: FIFO capacity exceeded");
     const unsigned long long nlog = read_scalar(c, log_count);
     *needed = nlog;
