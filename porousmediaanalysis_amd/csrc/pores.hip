@@ -918,6 +918,153 @@ __global__ void reach_sizes(const unsigned long long* __restrict__ set_off, cons
     }
 }
 
+// ---- per-pore sums with counters in LDS ---------------------------------------------------
+// The same three walks over the plateaus' pore sets as list_counts, list_fill and reach_sizes, for
+// up to kLdsPores pores: each workgroup adds into its own LDS counters (one slot per pore) and
+// adds its non-zero totals to the global ones once at the end, instead of one global atomic per
+// (plateau, pore) pair on a few thousand counters.  Workgroups take plateaus grid-stride.  A bad
+// index fails the stage (the walk of that plateau stops; the workgroup still reaches its barriers).
+constexpr uint32_t kLdsPores = 8064;  // u64 counters plus a flag bit per pore within 64 KB
+constexpr uint32_t kLdsSumGrid = 512;  // workgroups of the LDS walks (two fit a CU)
+
+__device__ __forceinline__ bool set_in_pool(const unsigned long long* set_off, const uint32_t* set_len, size_t q,
+                                            const SetBounds& b) {
+    return set_off[q] + set_len[q] <= b.pool_cap;
+}
+
+__global__ void __launch_bounds__(256) reach_sizes_lds(const unsigned long long* __restrict__ set_off,
+                                                       const uint32_t* __restrict__ set_len,
+                                                       const uint32_t* __restrict__ pool,
+                                                       const uint32_t* __restrict__ plat_size,
+                                                       const uint32_t* __restrict__ plat_border, SetBounds b,
+                                                       unsigned long long* __restrict__ reach,
+                                                       uint32_t* __restrict__ boundary, uint32_t* __restrict__ err) {
+    extern __shared__ unsigned long long lsum[];  // npores sums, then (npores + 31) / 32 flag words
+    const uint32_t np = (uint32_t)b.npores;
+    uint32_t* lflag = reinterpret_cast<uint32_t*>(lsum + np);
+    for (uint32_t i = threadIdx.x; i < np; i += blockDim.x) lsum[i] = 0;
+    for (uint32_t i = threadIdx.x; i < (np + 31) / 32; i += blockDim.x) lflag[i] = 0;
+    __syncthreads();
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < b.P; q += (size_t)gridDim.x * blockDim.x) {
+        if (!set_in_pool(set_off, set_len, q, b)) {
+            fail_check(err, kReachSizes << 8 | 1);
+            continue;
+        }
+        const uint32_t* s = pool + set_off[q];
+        const unsigned long long size = plat_size[q];
+        const bool border = plat_border[q] != 0;
+        for (uint32_t t = 0; t < set_len[q]; ++t) {
+            const uint32_t v = s[t];
+            if (v >= np) {
+                fail_check(err, kReachSizes << 8 | 2);
+                break;
+            }
+            atomicAdd(&lsum[v], size);
+            if (border) atomicOr(&lflag[v >> 5], 1u << (v & 31));
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < np; i += blockDim.x) {
+        if (lsum[i]) atomicAdd(&reach[i], lsum[i]);
+        if ((lflag[i >> 5] >> (i & 31)) & 1u) boundary[i] = 1;
+    }
+}
+
+__global__ void __launch_bounds__(256) list_counts_lds(const unsigned long long* __restrict__ set_off,
+                                                       const uint32_t* __restrict__ set_len,
+                                                       const uint32_t* __restrict__ pool, SetBounds b,
+                                                       const uint32_t* __restrict__ replayed,
+                                                       unsigned long long* __restrict__ cnt, uint32_t* __restrict__ err) {
+    extern __shared__ unsigned long long lcount[];  // npores counts
+    const uint32_t np = (uint32_t)b.npores;
+    for (uint32_t i = threadIdx.x; i < np; i += blockDim.x) lcount[i] = 0;
+    __syncthreads();
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < b.P; q += (size_t)gridDim.x * blockDim.x) {
+        if (!set_in_pool(set_off, set_len, q, b)) {
+            fail_check(err, kPoreLists << 8 | 1);
+            continue;
+        }
+        const uint32_t* s = pool + set_off[q];
+        for (uint32_t j = 0; j < set_len[q]; ++j) {
+            const uint32_t v = s[j];
+            if (v >= np) {
+                fail_check(err, kPoreLists << 8 | 2);
+                break;
+            }
+            if (replayed[v]) atomicAdd(&lcount[v], 1ull);
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < np; i += blockDim.x)
+        if (lcount[i]) atomicAdd(&cnt[i], lcount[i]);
+}
+
+// The workgroup counts its entries per pore, reserves that many places in each pore's list with
+// one global atomic, then walks its plateaus again and fills the places (the order of a pore's
+// entries is free: the replay scatters them into a table by plateau).
+__global__ void __launch_bounds__(256) list_fill_lds(const unsigned long long* __restrict__ set_off,
+                                                     const uint32_t* __restrict__ set_len,
+                                                     const uint32_t* __restrict__ pool,
+                                                     const unsigned long long* __restrict__ rbase,
+                                                     const uint32_t* __restrict__ rec_node,
+                                                     const uint32_t* __restrict__ ev_local, SetBounds b,
+                                                     unsigned long long R, uint32_t D,
+                                                     const uint32_t* __restrict__ replayed,
+                                                     const unsigned long long* __restrict__ list_off,
+                                                     unsigned long long* __restrict__ cursor,
+                                                     unsigned long long* __restrict__ lists,
+                                                     unsigned long long lists_len, uint32_t* __restrict__ err) {
+    extern __shared__ unsigned long long lpos[];  // per pore: the count, then the next place
+    const uint32_t np = (uint32_t)b.npores;
+    for (uint32_t i = threadIdx.x; i < np; i += blockDim.x) lpos[i] = 0;
+    __syncthreads();
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < b.P; q += stride) {
+        if (!set_in_pool(set_off, set_len, q, b)) {
+            fail_check(err, kPoreLists << 8 | 3);
+            continue;
+        }
+        const uint32_t* s = pool + set_off[q];
+        for (uint32_t j = 0; j < set_len[q]; ++j) {
+            const uint32_t v = s[j];
+            if (v >= np) {
+                fail_check(err, kPoreLists << 8 | 4);
+                break;
+            }
+            if (replayed[v]) atomicAdd(&lpos[v], 1ull);
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < np; i += blockDim.x)
+        if (lpos[i]) lpos[i] = list_off[i] + atomicAdd(&cursor[i], lpos[i]);
+    __syncthreads();
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < b.P; q += stride) {
+        if (!set_in_pool(set_off, set_len, q, b)) continue;  // flagged above
+        const uint32_t* s = pool + set_off[q];
+        for (uint32_t j = 0; j < set_len[q]; ++j) {
+            const uint32_t v = s[j];
+            if (v >= np) break;  // flagged above
+            if (!replayed[v]) continue;
+            uint32_t l = kNoClassU;
+            if (j >= 1) {
+                const unsigned long long r = rbase[q] + j - 1;
+                const uint32_t d = r < R ? rec_node[r] : kInv;
+                l = d < D ? ev_local[d] : kInv;
+                if (l >= kNoClassU) {
+                    fail_check(err, kPoreLists << 8 | 5);
+                    break;
+                }
+            }
+            const unsigned long long pos = atomicAdd(&lpos[v], 1ull);
+            if (pos >= lists_len) {
+                fail_check(err, kPoreLists << 8 | 6);
+                break;
+            }
+            lists[pos] = ((unsigned long long)q << 32) | l;
+        }
+    }
+}
+
 // ---- DFS replay: a persistent pool of waves, one pore at a time per wave --------------------
 // What the replay reproduces.  Plateaus (connected sets of equal CR) are strongly connected and
 // edges between plateaus lead to lower CR, so the pore's DFS enters each plateau it reaches once,
@@ -943,6 +1090,7 @@ __global__ void reach_sizes(const unsigned long long* __restrict__ set_off, cons
 constexpr uint16_t kNoClass = (uint16_t)kNoClassU;
 constexpr uint32_t kRing = PMA_PORES_RING;  // DFS frames kept in LDS per wave (a power of two)
 constexpr uint16_t kNoPage = 0xFFFF;
+constexpr uint32_t kAllTests = (1u << 27) - 1u;  // neighbour indices 0..26 (13, the voxel itself, fails the stencil)
 
 struct ReplayTask {
     uint32_t pore, seed, ncls, rank_base;  // ranks of the pore's classes at rank_slot[rank_base + class]
@@ -1105,34 +1253,36 @@ __global__ void __launch_bounds__(64) dfs_replay(
         }
         found = bcast(found, 0);
         // The stack: frames [0, gtop) in global memory, [gtop, top) in the LDS ring at index % kRing.
+        // `tests` holds the neighbour indices still to test at u: all 27 at a new voxel; at a resumed
+        // frame (or after a skip) only those below the taken index that were candidates then -- a
+        // candidate can only stop being one (visited and closed marks are never cleared).
         unsigned long long top = 0, gtop = 0, steps = 0;
-        int k = 27;
+        uint32_t tests = kAllTests;
         bool overflow = false;
         while (found + 1 < task.ncls) {
             int x, y, z;
             decode(u, X, Y, x, y, z);
             bool cand = false, entered = true;
             uint32_t j = 0, qn = 0, cn = 0, ql = kNoClass;
-            if (lane < k && lane < 27 && in_stencil(lane, d3q7) && neighbour(lane, x, y, z, X, Y, Z, j)) {
-                cn = cr[j];
-                qn = plat[j];
+            if (lane < 27 && ((tests >> lane) & 1u) && in_stencil(lane, d3q7) && neighbour(lane, x, y, z, X, Y, Z, j)) {
+                // visited first: a neighbour the LDS copy shows as visited needs no load at all
                 const uint32_t pg = vtab[j >> vis_bits];
-                bool visited = false;
-                if (pg != kNoPage) {
-                    const uint32_t off = j & ((1u << vis_bits) - 1);
-                    const uint32_t hw = pg * (page_words * 2) + (off >> 5);
-                    const uint32_t slot = vcache_slot(hw);
-                    const unsigned long long e = vcache[slot];
-                    uint32_t bits;
-                    if ((uint32_t)(e >> 32) == hw) {
-                        bits = (uint32_t)e;
-                    } else {
+                const uint32_t off = j & ((1u << vis_bits) - 1);
+                const uint32_t hw = pg * (page_words * 2) + (off >> 5);
+                const uint32_t slot = vcache_slot(hw);
+                const unsigned long long e = pg != kNoPage ? vcache[slot] : 0ull;
+                const bool hit = pg == kNoPage || (uint32_t)(e >> 32) == hw;  // no page: nothing visited there
+                bool visited = pg != kNoPage && hit && (((uint32_t)e >> (j & 31)) & 1u);
+                if (!visited) {
+                    cn = cr[j];
+                    qn = plat[j];
+                    if (!hit) {
                         const unsigned long long w = __hip_atomic_load(&vpool[(size_t)pg * page_words + (off >> 6)],
                                                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                        bits = (uint32_t)(w >> ((off >> 5) & 1u) * 32u);
+                        const uint32_t bits = (uint32_t)(w >> ((off >> 5) & 1u) * 32u);
                         vcache[slot] = ((unsigned long long)hw << 32) | bits;
+                        visited = (bits >> (j & 31)) & 1u;
                     }
-                    visited = (bits >> (j & 31)) & 1u;
                 }
                 cand = cn != 0 && cn <= cu && !visited;
                 // neighbours in other plateaus: closed plateaus are visited; the entered test and the
@@ -1172,14 +1322,14 @@ __global__ void __launch_bounds__(64) dfs_replay(
                 u = f.x;
                 uq = f.y;
                 cu = f.z;
-                k = (int)f.w;
-                if (u >= b.n || uq >= b.P || k > 27) {
+                tests = f.w;
+                if (u >= b.n || uq >= b.P || (tests & ~kAllTests)) {
                     if (lane == 0) fail_check(err, kDfsReplay << 8 | 5);
                     return;
                 }
                 continue;
             }
-            const int kk = 63 - __clzll((long long)bal);  // highest neighbour index below k
+            const int kk = 63 - __clzll((long long)bal);  // the highest candidate index
             const uint32_t nvox = bcast(j, kk);
             const uint32_t nq = bcast(qn, kk);
             const uint32_t ncr = bcast(cn, kk);
@@ -1331,7 +1481,7 @@ __global__ void __launch_bounds__(64) dfs_replay(
                 if (skip) {
                     if (lane == 0) found = rank_class(skip_cls, found);
                     found = bcast(found, 0);
-                    k = kk;  // go on with the neighbours below kk
+                    tests = (uint32_t)bal & ((1u << kk) - 1u);  // go on with the candidates below kk
                     continue;
                 }
             }
@@ -1351,7 +1501,7 @@ __global__ void __launch_bounds__(64) dfs_replay(
             }
             const uint32_t ncls_new = bcast(ql, kk);
             if (lane == 0) {
-                ring[top % kRing] = make_uint4(u, uq, cu, (uint32_t)kk);  // this frame resumes below kk
+                ring[top % kRing] = make_uint4(u, uq, cu, (uint32_t)bal & ((1u << kk) - 1u));  // resumes below kk
                 atomicOr(&vpool[(size_t)vtab[nvox >> vis_bits] * page_words + ((nvox & ((1u << vis_bits) - 1)) >> 6)],
                          1ull << (nvox & 63));
                 {
@@ -1372,7 +1522,7 @@ __global__ void __launch_bounds__(64) dfs_replay(
             u = nvox;
             uq = nq;
             cu = ncr;
-            k = 27;
+            tests = kAllTests;
         }
         // every lane stores the same values
         task_steps[t] = steps;
@@ -1759,8 +1909,13 @@ void run_pores(pma_ctx* c) {
     PMA_HIP(hipMemsetAsync(reach, 0, std::max(npores, 1u) * 8ull, c->stream));
     PMA_HIP(hipMemsetAsync(bflag, 0, std::max(npores, 1u) * 4ull, c->stream));
     if (P) {
-        reach_sizes<<<grid_for(P, 256), 256, 0, c->stream>>>(set_off, set_len, pool, plat_size, plat_border, sb, reach,
-                                                             bflag, err);
+        if (npores <= kLdsPores) {
+            reach_sizes_lds<<<std::min(grid_for(P, 256), kLdsSumGrid), 256, npores * 8 + (npores + 31) / 32 * 4,
+                              c->stream>>>(set_off, set_len, pool, plat_size, plat_border, sb, reach, bflag, err);
+        } else {
+            reach_sizes<<<grid_for(P, 256), 256, 0, c->stream>>>(set_off, set_len, pool, plat_size, plat_border, sb,
+                                                                 reach, bflag, err);
+        }
         PMA_CHECKED(c, "reach_sizes", err);
     }
     check_stage(c, err);
@@ -1840,13 +1995,21 @@ void run_pores(pma_ctx* c) {
             upload(c, d_evl, ev_local.data(), D * 4ull);
             PMA_HIP(hipMemsetAsync(lcnt, 0, npores * 8ull, c->stream));
             PMA_HIP(hipMemsetAsync(lcur, 0, npores * 8ull, c->stream));
-            list_counts<<<grid_for(P, 256), 256, 0, c->stream>>>(set_off, set_len, pool, sb, d_replayed, lcnt, err);
+            if (npores <= kLdsPores)
+                list_counts_lds<<<std::min(grid_for(P, 256), kLdsSumGrid), 256, npores * 8, c->stream>>>(
+                    set_off, set_len, pool, sb, d_replayed, lcnt, err);
+            else
+                list_counts<<<grid_for(P, 256), 256, 0, c->stream>>>(set_off, set_len, pool, sb, d_replayed, lcnt, err);
             PMA_CHECKED(c, "list_counts", err);
             const uint64_t lists_len = exclusive_scan_u64(c, reinterpret_cast<const uint64_t*>(lcnt),
                                                           reinterpret_cast<uint64_t*>(loff), npores);
             unsigned long long* lists = c->buf<unsigned long long>("pt_lists", std::max<uint64_t>(lists_len, 1));
-            list_fill<<<grid_for(P, 256), 256, 0, c->stream>>>(set_off, set_len, pool, rbase, rd, d_evl, sb, R, D,
-                                                             d_replayed, loff, lcur, lists, lists_len, err);
+            if (npores <= kLdsPores)
+                list_fill_lds<<<std::min(grid_for(P, 256), kLdsSumGrid), 256, npores * 8, c->stream>>>(
+                    set_off, set_len, pool, rbase, rd, d_evl, sb, R, D, d_replayed, loff, lcur, lists, lists_len, err);
+            else
+                list_fill<<<grid_for(P, 256), 256, 0, c->stream>>>(set_off, set_len, pool, rbase, rd, d_evl, sb, R, D,
+                                                                 d_replayed, loff, lcur, lists, lists_len, err);
             PMA_CHECKED(c, "list_fill", err);
             check_stage(c, err);
             std::vector<unsigned long long> lcnt_h(npores), loff_h(npores);

@@ -12,7 +12,7 @@ WAVE_KERNELS = {"count_centres", "write_centres", "dfs_replay", "cr_histogram", 
                 "relax_blocks", "find_start", "cavity_totals", "max_run", "partition_active", "next_min_key",
                 "edt_pass_y", "edt_pass_z", "owned_counts",
                 "ws_resolve", "ws_count_parts", "ws_scan_parts", "ws_append", "set_union_bitmap", "below_min", "length_histogram",
-                "plateau_edges"}
+                "plateau_edges", "reach_sizes_lds", "list_counts_lds", "list_fill_lds"}
 
 
 def split_args(s):
@@ -45,7 +45,7 @@ def launch(m):
 def translate(text):
     text = text.replace('#include "pma_internal.h"', '#include "shim.h"')
     text = re.sub(r"#include <rocprim/.*?>", "", text)
-    text = re.sub(r"extern __shared__ (\w+) (\w+)\[\];", r"\1* \2 = (\1*)hostemu_dyn_shared();", text)
+    text = re.sub(r"extern __shared__ (\w+(?: \w+)*?) (\w+)\[\];", r"\1* \2 = (\1*)hostemu_dyn_shared();", text)
     return re.sub(r"(\w+(?:<[\w, ]*>)?)<<<(.*?)>>>\((.*?)\);", launch, text, flags=re.S)
 
 
