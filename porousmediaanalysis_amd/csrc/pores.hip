@@ -1260,10 +1260,10 @@ __global__ void __launch_bounds__(64) dfs_replay(
         uint32_t tests = kAllTests;
         bool overflow = false;
         while (found + 1 < task.ncls) {
-            int x, y, z;
-            decode(u, X, Y, x, y, z);
             bool cand = false, entered = true;
             uint32_t j = 0, qn = 0, cn = 0, ql = kNoClass;
+            int x = 0, y = 0, z = 0;
+            if (tests != 0) decode(u, X, Y, x, y, z);  // a frame with nothing left to test pops at once
             if (lane < 27 && ((tests >> lane) & 1u) && in_stencil(lane, d3q7) && neighbour(lane, x, y, z, X, Y, Z, j)) {
                 // visited first: a neighbour the LDS copy shows as visited needs no load at all
                 const uint32_t pg = vtab[j >> vis_bits];
