@@ -1099,8 +1099,10 @@ struct ReplayTask {
 };
 
 // Lengths of the buffers the replay selects from with values read from memory.
-struct ReplayBounds {
-    unsigned long long n, P, E, trav_cap, lists_len, ranks_len;
+struct ReplayBounds {  // 32-bit where the host has checked the value fits: fewer live scalar registers
+    uint32_t n, P, E, trav_cap;
+    unsigned long long lists_len;
+    uint32_t ranks_len;
 };
 
 // Lane `src`'s value as a wave-uniform scalar: the loop conditions built from broadcasts are then
@@ -1256,7 +1258,7 @@ __global__ void __launch_bounds__(64) dfs_replay(
         // `tests` holds the neighbour indices still to test at u: all 27 at a new voxel; at a resumed
         // frame (or after a skip) only those below the taken index that were candidates then -- a
         // candidate can only stop being one (visited and closed marks are never cleared).
-        unsigned long long top = 0, gtop = 0, steps = 0;
+        uint32_t top = 0, gtop = 0, steps = 0;  // at most the pore's reach, below 2^32
         uint32_t tests = kAllTests;
         bool overflow = false;
         while (found + 1 < task.ncls) {
@@ -1309,7 +1311,7 @@ __global__ void __launch_bounds__(64) dfs_replay(
                 if (top == 0) break;  // stack empty: the pore's reach is exhausted
                 if (top - 1 < gtop) {
                     // the ring is empty: bring back the newest half ring of frames from global memory
-                    const unsigned long long h = min(gtop, (unsigned long long)(kRing / 2));
+                    const uint32_t h = min(gtop, kRing / 2);
                     for (unsigned long long i = lane; i < h; i += 64) {
                         const unsigned long long at = gtop - h + i;
                         ring[at % kRing] = gstack[at];
@@ -2093,7 +2095,10 @@ void run_pores(pma_ctx* c) {
                 uint32_t* trav = c->buf<uint32_t>("pt_replay_trav", W * 2 * trav_cap);
                 upload(c, d_tasks, pending.data(), pending.size() * sizeof(ReplayTask));
                 PMA_HIP(hipMemsetAsync(status, 0, pending.size() * 4, c->stream));
-                const ReplayBounds rb{n, P, E, trav_cap, lists_len, ranks_len};
+                PMA_REQUIRE(E < 0xFFFFFFFFull && trav_cap < 0xFFFFFFFFull && ranks_len < 0xFFFFFFFFull, PMA_ERR_CAPACITY,
+                            "pma_pores: plateau edges, traversal room or class ranks beyond 32 bits");
+                const ReplayBounds rb{(uint32_t)n, (uint32_t)P, (uint32_t)E, (uint32_t)trav_cap, lists_len,
+                                      (uint32_t)ranks_len};
                 PMA_HIP(hipMemsetAsync(queue, 0, 4, c->stream));
                 phase.mark("8 replay set-up and buffers");
                 const auto t0 = std::chrono::steady_clock::now();
