@@ -584,6 +584,7 @@ bool watershed_once(pma_ctx* c, unsigned long long log_cap, unsigned long long* 
     std::vector<uint64_t> qoff_h(L + 1, 0);
     for (size_t d = 0; d < L; ++d) qoff_h[d + 1] = qoff_h[d] + h[d];
     const uint64_t qtotal = qoff_h[L];
+    PMA_REQUIRE(qtotal < 0xFFFFFFFFull, PMA_ERR_CAPACITY, "pma_watershed: 2^32 or more FIFO entries (void voxels and seeds)");
     // the initial pushes: each level's seeds first in its FIFO, in ascending k
     std::vector<uint64_t> seed_slot(np);
     for (uint32_t k = 0; k < np; ++k) seed_slot[k] = qoff_h[c->pores[k].r2] + seeds_at[c->pores[k].r2]++;
