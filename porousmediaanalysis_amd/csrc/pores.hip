@@ -1264,6 +1264,7 @@ __global__ void __launch_bounds__(64) dfs_replay(
         while (found + 1 < task.ncls) {
             bool cand = false, entered = true;
             uint32_t j = 0, qn = 0, cn = 0, ql = kNoClass;
+            uint32_t vpg = kNoPage, vbits = 0;  // the neighbour's visited page and its current half-word
             int x = 0, y = 0, z = 0;
             if (tests != 0) decode(u, X, Y, x, y, z);  // a frame with nothing left to test pops at once
             if (lane < 27 && ((tests >> lane) & 1u) && in_stencil(lane, d3q7) && neighbour(lane, x, y, z, X, Y, Z, j)) {
@@ -1275,6 +1276,8 @@ __global__ void __launch_bounds__(64) dfs_replay(
                 const unsigned long long e = pg != kNoPage ? vcache[slot] : 0ull;
                 const bool hit = pg == kNoPage || (uint32_t)(e >> 32) == hw;  // no page: nothing visited there
                 bool visited = pg != kNoPage && hit && (((uint32_t)e >> (j & 31)) & 1u);
+                vpg = pg;
+                vbits = (uint32_t)e;
                 if (!visited) {
                     cn = cr[j];
                     qn = plat[j];
@@ -1284,6 +1287,7 @@ __global__ void __launch_bounds__(64) dfs_replay(
                         const uint32_t bits = (uint32_t)(w >> ((off >> 5) & 1u) * 32u);
                         vcache[slot] = ((unsigned long long)hw << 32) | bits;
                         visited = (bits >> (j & 31)) & 1u;
+                        vbits = bits;
                     }
                 }
                 cand = cn != 0 && cn <= cu && !visited;
@@ -1497,21 +1501,34 @@ __global__ void __launch_bounds__(64) dfs_replay(
                 gtop += kRing / 2;
                 __builtin_amdgcn_wave_barrier();
             }
-            if (!ensure_page(nvox)) {  // no page left: the host repeats the pore with a larger pool
-                overflow = true;
-                break;
+            // The lane that tested nvox holds its page and its visited half-word as they are now (no
+            // visited bit changes between the test and here), so the mark needs no LDS reads; a
+            // block without a page yet takes one, all zero.
+            uint32_t page = bcast(vpg, kk), nbits = bcast(vbits, kk);
+            if (page == kNoPage) {
+                const uint32_t taken = npages;
+                if (!ensure_page(nvox)) {  // no page left: the host repeats the pore with a larger pool
+                    overflow = true;
+                    break;
+                }
+                if (npages != taken) {  // a new page, all zero
+                    page = taken;
+                    nbits = 0;
+                } else {  // the block had a page: read the word
+                    page = vtab[nvox >> vis_bits];
+                    const uint32_t off = nvox & ((1u << vis_bits) - 1);
+                    const unsigned long long w = __hip_atomic_load(&vpool[(size_t)page * page_words + (off >> 6)],
+                                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    nbits = (uint32_t)(w >> ((off >> 5) & 1u) * 32u);
+                }
             }
             const uint32_t ncls_new = bcast(ql, kk);
             if (lane == 0) {
                 ring[top % kRing] = make_uint4(u, uq, cu, (uint32_t)bal & ((1u << kk) - 1u));  // resumes below kk
-                atomicOr(&vpool[(size_t)vtab[nvox >> vis_bits] * page_words + ((nvox & ((1u << vis_bits) - 1)) >> 6)],
-                         1ull << (nvox & 63));
-                {
-                    const uint32_t hw = vtab[nvox >> vis_bits] * (page_words * 2) + ((nvox & ((1u << vis_bits) - 1)) >> 5);
-                    const uint32_t slot = vcache_slot(hw);
-                    const unsigned long long e = vcache[slot];
-                    if ((uint32_t)(e >> 32) == hw) vcache[slot] = e | (1ull << (nvox & 31));
-                }
+                const uint32_t off = nvox & ((1u << vis_bits) - 1);
+                atomicOr(&vpool[(size_t)page * page_words + (off >> 6)], 1ull << (nvox & 63));
+                const uint32_t hw = page * (page_words * 2) + (off >> 5);
+                vcache[vcache_slot(hw)] = ((unsigned long long)hw << 32) | nbits | (1u << (nvox & 31));
                 if (is_new) {
                     atomicOr(&pent[nq >> 6], 1ull << (nq & 63));
                     pcache[pcache_slot(nq)] = pcache_entry(nq, ncls_new, false, true);
