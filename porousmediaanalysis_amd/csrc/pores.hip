@@ -1124,7 +1124,7 @@ __device__ __forceinline__ unsigned long long pcache_entry(uint32_t q, uint32_t 
 }
 
 // The replaying wave's LDS copy of recently read visited words, by halves: entries
-// (half-word index << 32) | the 32 visited bits, ~0 when empty.  Lane 0 sets a voxel's bit in the
+// (voxel index / 32 << 32) | the 32 visited bits of those voxels, ~0 when empty.  Lane 0 sets a voxel's bit in the
 // copy when it marks the voxel, so the copy is never behind the wave's own bitmap.
 #ifndef PMA_PORES_VCACHE
 #define PMA_PORES_VCACHE 512u
@@ -1264,28 +1264,31 @@ __global__ void __launch_bounds__(64) dfs_replay(
         while (found + 1 < task.ncls) {
             bool cand = false, entered = true;
             uint32_t j = 0, qn = 0, cn = 0, ql = kNoClass;
-            uint32_t vpg = kNoPage, vbits = 0;  // the neighbour's visited page and its current half-word
+            uint32_t vbits = 0;  // the neighbour's visited half-word as it is now
             int x = 0, y = 0, z = 0;
             if (tests != 0) decode(u, X, Y, x, y, z);  // a frame with nothing left to test pops at once
             if (lane < 27 && ((tests >> lane) & 1u) && in_stencil(lane, d3q7) && neighbour(lane, x, y, z, X, Y, Z, j)) {
-                // visited first: a neighbour the LDS copy shows as visited needs no load at all
-                const uint32_t pg = vtab[j >> vis_bits];
-                const uint32_t off = j & ((1u << vis_bits) - 1);
-                const uint32_t hw = pg * (page_words * 2) + (off >> 5);
-                const uint32_t slot = vcache_slot(hw);
-                const unsigned long long e = pg != kNoPage ? vcache[slot] : 0ull;
-                const bool hit = pg == kNoPage || (uint32_t)(e >> 32) == hw;  // no page: nothing visited there
-                bool visited = pg != kNoPage && hit && (((uint32_t)e >> (j & 31)) & 1u);
-                vpg = pg;
+                // visited first: a neighbour the LDS copy shows as visited needs no load at all (the
+                // copy is keyed by the voxel's half-word index, so a hit needs no page lookup)
+                const uint32_t vh = j >> 5;
+                const uint32_t slot = vcache_slot(vh);
+                const unsigned long long e = vcache[slot];
+                const bool hit = (uint32_t)(e >> 32) == vh;
+                bool visited = hit && (((uint32_t)e >> (j & 31)) & 1u);
                 vbits = (uint32_t)e;
                 if (!visited) {
                     cn = cr[j];
                     qn = plat[j];
                     if (!hit) {
-                        const unsigned long long w = __hip_atomic_load(&vpool[(size_t)pg * page_words + (off >> 6)],
-                                                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                        const uint32_t bits = (uint32_t)(w >> ((off >> 5) & 1u) * 32u);
-                        vcache[slot] = ((unsigned long long)hw << 32) | bits;
+                        const uint32_t pg = vtab[j >> vis_bits];
+                        uint32_t bits = 0;  // a block without a page has nothing visited
+                        if (pg != kNoPage) {
+                            const uint32_t off = j & ((1u << vis_bits) - 1);
+                            const unsigned long long w = __hip_atomic_load(
+                                &vpool[(size_t)pg * page_words + (off >> 6)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                            bits = (uint32_t)(w >> ((off >> 5) & 1u) * 32u);
+                        }
+                        vcache[slot] = ((unsigned long long)vh << 32) | bits;
                         visited = (bits >> (j & 31)) & 1u;
                         vbits = bits;
                     }
@@ -1501,10 +1504,9 @@ __global__ void __launch_bounds__(64) dfs_replay(
                 gtop += kRing / 2;
                 __builtin_amdgcn_wave_barrier();
             }
-            // The lane that tested nvox holds its page and its visited half-word as they are now (no
-            // visited bit changes between the test and here), so the mark needs no LDS reads; a
-            // block without a page yet takes one, all zero.
-            uint32_t page = bcast(vpg, kk), nbits = bcast(vbits, kk);
+            // The lane that tested nvox holds its visited half-word as it is now (no visited bit
+            // changes between the test and here); a block without a page yet takes one, all zero.
+            uint32_t page = vtab[nvox >> vis_bits], nbits = bcast(vbits, kk);
             if (page == kNoPage) {
                 const uint32_t taken = npages;
                 if (!ensure_page(nvox)) {  // no page left: the host repeats the pore with a larger pool
@@ -1527,8 +1529,7 @@ __global__ void __launch_bounds__(64) dfs_replay(
                 ring[top % kRing] = make_uint4(u, uq, cu, (uint32_t)bal & ((1u << kk) - 1u));  // resumes below kk
                 const uint32_t off = nvox & ((1u << vis_bits) - 1);
                 atomicOr(&vpool[(size_t)page * page_words + (off >> 6)], 1ull << (nvox & 63));
-                const uint32_t hw = page * (page_words * 2) + (off >> 5);
-                vcache[vcache_slot(hw)] = ((unsigned long long)hw << 32) | nbits | (1u << (nvox & 31));
+                vcache[vcache_slot(nvox >> 5)] = ((unsigned long long)(nvox >> 5) << 32) | nbits | (1u << (nvox & 31));
                 if (is_new) {
                     atomicOr(&pent[nq >> 6], 1ull << (nq & 63));
                     pcache[pcache_slot(nq)] = pcache_entry(nq, ncls_new, false, true);
